@@ -1,0 +1,759 @@
+// -----------------------------------------------------------------------------
+// rd_oracle.cpp — CPU restatement of the reference's bit-exact radar chain.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: tests/, the smoke entry
+// point and the CPU-baseline leg of bench.py load it to produce expected values
+// and to time a CPU run. Nothing in the product library links, loads or calls
+// it, and the product must never fall back to it.
+//
+// Every function restates one reference function of `model/cpp` from its
+// semantics (file:line cited on each). It shares no code with the HIP library
+// in agilex-7-radar-detect_amd/csrc: the two are independent descriptions of
+// the same arithmetic and the tests compare them.
+//
+// Pinning (what checks each part against something other than itself):
+//   fxp primitives ........ model/vectors/fxp_ops.vec, fxp_accum.vec, cmult.vec
+//   twiddle table ......... the 1024-entry table digest c756048fbc00d64f on every
+//                           4th entry of the 4096 table; exact axis points;
+//                           a high-precision tie-distance check
+//   PFB ................... 10 pfb_*.vec/.coeff sets; reference header (oracle/_ref)
+//   FFT ................... fft64.vec (36 records); reference header N<=1024;
+//                           impulse/DC closed forms and a float-DFT bound at 4096
+//   beamformer ............ reference header; unit/zero/permutation closed forms
+//   power/covariance ...... reference header (power, cross_power, Integrator);
+//                           Hermitian symmetry; brute-force sums
+//   CFAR .................. reference header (cfar::run_frame); flat/zero closed
+//                           forms; edge-suppression counts
+//   event pack ............ field round trip against the layout offsets of
+//                           model/cpp/cfar/cfar_model.hpp:75-84
+//
+// Build: g++ -std=c++17 -O3 -fPIC -shared (oracle/Makefile).
+// -----------------------------------------------------------------------------
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace rdo {
+
+using i16 = std::int16_t;
+using i32 = std::int32_t;
+using i64 = std::int64_t;
+using u16 = std::uint16_t;
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+
+// ---------------------------------------------------------------------------
+// Integer primitives — model/cpp/fxp/fxp.hpp:87-231
+// All arithmetic on the 64-bit working type goes through unsigned operations so
+// that wrap-around is defined (fxp.hpp:79-110).
+// ---------------------------------------------------------------------------
+inline i64 wadd(i64 a, i64 b) { return (i64)((u64)a + (u64)b); }
+
+// Planted-mistake switch for the tests that prove each check bites. 0 = none.
+//   1 RNE replaced by round-half-up        2 FFT bypasses twiddles W^0
+//   3 BF2II -j negate wraps (no saturate)  4 PFB taps applied in reverse order
+//   5 CFAR lead/lag sides swapped          6 cross power conjugates X not Y
+inline int& fault() {
+  static int f = 0;
+  return f;
+}
+inline i64 wsub(i64 a, i64 b) { return (i64)((u64)a - (u64)b); }
+inline i64 wneg(i64 a) { return (i64)(0ULL - (u64)a); }
+
+// floor(v / 2^s) — fxp.hpp:104-110
+inline i64 floor_shift(i64 v, unsigned s) {
+  if (s == 0) return v;
+  if (s >= 64) return v < 0 ? -1 : 0;
+  u64 r = (u64)v >> s;
+  if (v < 0) r |= ~0ULL << (64 - s);
+  return (i64)r;
+}
+
+// Signed w-bit range endpoints — fxp.hpp:116-126
+inline i64 top_of(unsigned w) {
+  if (w >= 64) return INT64_MAX;
+  if (w < 2) return 0;
+  return (i64)((1ULL << (w - 1)) - 1ULL);
+}
+inline i64 bottom_of(unsigned w) {
+  if (w >= 64) return INT64_MIN;
+  if (w < 2) return 0;
+  return wneg((i64)(1ULL << (w - 1)));
+}
+
+// 2-bit flag word {sat_pos, sat_neg} — fxp.hpp:136-165
+inline unsigned flags_of(i64 v, unsigned w) {
+  return (v > top_of(w) ? 2u : 0u) | (v < bottom_of(w) ? 1u : 0u);
+}
+inline i64 clamp_w(i64 v, unsigned w) {
+  if (v > top_of(w)) return top_of(w);
+  if (v < bottom_of(w)) return bottom_of(w);
+  return v;
+}
+
+// Round to nearest, ties to even, using the floor remainder — fxp.hpp:198-208
+inline i64 rhu(i64 v, unsigned s);
+inline i64 rne(i64 v, unsigned s) {
+  if (s == 0) return v;
+  if (s >= 64) return 0;
+  if (fault() == 1) return rhu(v, s);
+  const i64 q = floor_shift(v, s);
+  const u64 rem = (u64)v & ((1ULL << s) - 1ULL);
+  const u64 half = 1ULL << (s - 1);
+  if (rem > half) return wadd(q, 1);
+  if (rem < half) return q;
+  return wadd(q, (i64)((u64)q & 1ULL));
+}
+
+// Round half up — fxp.hpp:191-196 (testable comparison only)
+inline i64 rhu(i64 v, unsigned s) {
+  if (s == 0) return v;
+  if (s >= 64) return 0;
+  return floor_shift(wadd(v, (i64)(1ULL << (s - 1))), s);
+}
+
+// Round first, saturate second; flags taken on the rounded value — fxp.hpp:221-231
+inline i64 rsat(i64 v, unsigned s, unsigned w) { return clamp_w(rne(v, s), w); }
+inline unsigned rsat_flags(i64 v, unsigned s, unsigned w) { return flags_of(rne(v, s), w); }
+
+// Q1.15 complex in one 32-bit word, real part in the low half — fxp.hpp:279-298
+struct Cx {
+  i16 re;
+  i16 im;
+};
+inline Cx unpack(u32 p) { return Cx{(i16)(u16)(p & 0xFFFFu), (i16)(u16)(p >> 16)}; }
+inline u32 pack(Cx c) { return ((u32)(u16)c.im << 16) | (u32)(u16)c.re; }
+inline u32 pack2(i64 re, i64 im) { return pack(Cx{(i16)re, (i16)im}); }
+
+// Exact Q2.30 product partials — fxp.hpp:261-263, 303-311
+inline i64 prod(i16 a, i16 b) { return (i64)a * (i64)b; }
+inline i64 cm_re(Cx a, Cx b) { return wsub(prod(a.re, b.re), prod(a.im, b.im)); }
+inline i64 cm_im(Cx a, Cx b) { return wadd(prod(a.re, b.im), prod(a.im, b.re)); }
+
+// Karatsuba three-multiply form — model/cpp/fxp/cmult.hpp:97-117
+inline void cm3(Cx a, Cx b, i64* re, i64* im) {
+  const i64 k1 = (i64)a.re * wadd(b.re, b.im);
+  const i64 k2 = (i64)b.im * wadd(a.re, a.im);
+  const i64 k3 = (i64)b.re * wsub(a.im, a.re);
+  *re = wsub(k1, k2);
+  *im = wadd(k1, k3);
+}
+
+// Complex multiply rounded once at shift 15 — fxp.hpp:314-325
+inline Cx cmul(Cx a, Cx b, unsigned* fl) {
+  const i64 re = cm_re(a, b), im = cm_im(a, b);
+  *fl |= rsat_flags(re, 15, 16) | rsat_flags(im, 15, 16);
+  return Cx{(i16)rsat(re, 15, 16), (i16)rsat(im, 15, 16)};
+}
+
+// Accumulator growth — fxp.hpp:334-352
+inline unsigned growth(unsigned n) {
+  if (n <= 1) return 0;
+  unsigned b = 0;
+  for (unsigned v = n - 1; v != 0; v >>= 1) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Twiddle table — rule of model/python/gen_fft_twiddles.py:97-159, generalised
+// from 1024 to any power-of-two size M (the 4096-point FFT needs M = 4096).
+// ---------------------------------------------------------------------------
+inline i64 q15_real(double x) {
+  const double v = std::nearbyint(x * 32768.0);  // ties to even (default mode)
+  if (v > 32767.0) return 32767;
+  if (v < -32768.0) return -32768;
+  return (i64)v;
+}
+
+inline void twiddle_table(unsigned M, u32* out) {
+  const unsigned q4 = M / 4;
+  for (unsigned e = 0; e < M; ++e) {
+    const unsigned quad = e / q4, rem = e % q4;
+    double c0 = 1.0, s0 = 0.0;
+    if (rem != 0) {
+      const double th = 2.0 * M_PI * (double)rem / (double)M;
+      c0 = std::cos(th);
+      s0 = std::sin(th);
+    }
+    double c, s;
+    switch (quad) {
+      case 0: c = c0; s = s0; break;
+      case 1: c = -s0; s = c0; break;
+      case 2: c = -c0; s = -s0; break;
+      default: c = s0; s = -c0; break;
+    }
+    out[e] = pack2(q15_real(c), q15_real(-s));
+  }
+}
+
+constexpr unsigned kTwLog = 12;
+constexpr unsigned kTwM = 1u << kTwLog;  // 4096-entry master table
+
+struct TwTable {
+  u32 w[kTwM];
+  TwTable() { twiddle_table(kTwM, w); }
+};
+inline const TwTable& tw_table() {
+  static const TwTable t;
+  return t;
+}
+// W_{2^l2l}^e — fft_ref.hpp:131-133 (table stride generalised to 4096)
+inline Cx tw(unsigned l2l, unsigned e) { return unpack(tw_table().w[e << (kTwLog - l2l)]); }
+
+inline unsigned bit_reverse(unsigned v, unsigned w) {
+  unsigned r = 0;
+  for (unsigned i = 0; i < w; ++i)
+    if ((v >> i) & 1u) r |= 1u << (w - 1 - i);
+  return r;
+}
+inline unsigned ilog2(unsigned v) {
+  unsigned r = 0;
+  while ((1u << r) < v) ++r;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// PFB — model/cpp/pfb/pfb_model.hpp:117-257
+// Branch p filters x_p[m] = x[m*P + p]; y_p[m] = round_sat(sum_k h_p[k] x_p[m-k]).
+// `hist` (T-1 beats x P, most recent first) seeds the delay line and receives
+// its final state; null means a zero history and no export.
+// ---------------------------------------------------------------------------
+void pfb_channel(const u32* x, const u32* coeff, unsigned P, unsigned T, unsigned n_beats,
+                 u32* hist, u32* y, std::uint8_t* flags) {
+  std::vector<Cx> line((size_t)P * (T > 0 ? T : 1), Cx{0, 0});  // line[p*T + k] = x_p[m-k]
+  if (hist) {
+    for (unsigned k = 1; k < T; ++k)
+      for (unsigned p = 0; p < P; ++p) line[(size_t)p * T + k] = unpack(hist[(size_t)(k - 1) * P + p]);
+  }
+  for (unsigned m = 0; m < n_beats; ++m) {
+    for (unsigned p = 0; p < P; ++p) {
+      Cx* l = &line[(size_t)p * T];
+      l[0] = unpack(x[(size_t)m * P + p]);
+      i64 are = 0, aim = 0;
+      for (unsigned k = 0; k < T; ++k) {
+        const Cx h = unpack(coeff[(size_t)p * T + (fault() == 4 ? T - 1 - k : k)]);  // phase-major, tap-minor (pfb_model.hpp:108-111)
+        are = wadd(are, cm_re(l[k], h));
+        aim = wadd(aim, cm_im(l[k], h));
+      }
+      y[(size_t)m * P + p] = pack2(rsat(are, 15, 16), rsat(aim, 15, 16));
+      if (flags) flags[(size_t)m * P + p] = (std::uint8_t)(rsat_flags(are, 15, 16) | (rsat_flags(aim, 15, 16) << 2));
+      // advance after evaluation (pfb_model.hpp:170-175)
+      for (unsigned k = T - 1; k > 0; --k) l[k] = l[k - 1];
+    }
+  }
+  if (hist) {
+    for (unsigned k = 1; k < T; ++k)
+      for (unsigned p = 0; p < P; ++p) hist[(size_t)(k - 1) * P + p] = pack(line[(size_t)p * T + k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT — model/cpp/fft/fft_ref.hpp:250-407
+// Radix-2^2 DIF on each of `spc` interleaved lanes, then (spc = 2) one radix-2
+// DIT merge. Every sub-stage rounds by its schedule bit and saturates to 16 bits;
+// flags are the OR over the frame per sub-stage.
+// ---------------------------------------------------------------------------
+struct FftGeom {
+  unsigned n, spc, m, nl;
+};
+
+inline bool fft_valid(unsigned n, unsigned spc) {
+  const bool p2 = n != 0 && (n & (n - 1)) == 0;
+  return p2 && (spc == 1 || spc == 2) && n >= 8 && n <= kTwM && n / spc >= 4;
+}
+
+// quantise one butterfly output component (fft_ref.hpp:315-319)
+inline i16 fq(i64 v, unsigned sh, unsigned* fl) {
+  const i64 r = rne(v, sh);
+  *fl |= flags_of(r, 16);
+  return (i16)clamp_w(r, 16);
+}
+
+// one frame; flags[g] for sub-stage g
+void fft_frame(const u32* in, unsigned n, unsigned spc, u32 sched, int reorder, u32* out, u32* bins,
+               u32* stage_flags_word) {
+  const unsigned m = n / spc, nl = ilog2(m), stages = ilog2(n);
+  std::vector<unsigned> fl(stages, 0u);
+  std::vector<Cx> lane((size_t)spc * m);
+  for (unsigned t = 0; t < m; ++t)
+    for (unsigned q = 0; q < spc; ++q) lane[(size_t)q * m + t] = unpack(in[(size_t)t * spc + q]);
+
+  for (unsigned q = 0; q < spc; ++q) {
+    Cx* v = &lane[(size_t)q * m];
+    for (unsigned s = 0; s < nl; ++s) {
+      const unsigned l = 1u << (nl - s), half = l / 2, sh = (sched >> s) & 1u;
+      const bool odd = (s & 1u) != 0;
+      for (unsigned b = 0; b < m; b += l) {
+        const bool rot = odd && ((b / l) & 1u);  // BF2II trivial -j (fft_ref.hpp:341-348)
+        for (unsigned i = 0; i < half; ++i) {
+          Cx a = v[b + i], t = v[b + half + i];
+          if (rot) {
+            const i64 nre = wneg((i64)t.re);
+            if (fault() == 3) {
+              t = Cx{t.im, (i16)nre};
+            } else {
+              fl[s] |= flags_of(nre, 16);
+              t = Cx{t.im, (i16)clamp_w(nre, 16)};
+            }
+          }
+          v[b + i] = Cx{fq((i64)a.re + t.re, sh, &fl[s]), fq((i64)a.im + t.im, sh, &fl[s])};
+          v[b + half + i] = Cx{fq((i64)a.re - t.re, sh, &fl[s]), fq((i64)a.im - t.im, sh, &fl[s])};
+        }
+      }
+      if (odd && s + 1 < nl) {  // twiddle closes the radix-2^2 group (fft_ref.hpp:368-377)
+        const unsigned l2l = nl - s + 1;
+        const unsigned mask = (1u << l2l) - 1u;
+        for (unsigned i = 0; i < m; ++i) {
+          const unsigned qq = i & mask;
+          const unsigned k1 = (qq >> (l2l - 1)) & 1u, k2 = (qq >> (l2l - 2)) & 1u;
+          const unsigned n3 = qq & ((1u << (l2l - 2)) - 1u);
+          const unsigned e = n3 * (k1 + 2u * k2);
+          if (fault() == 2 && e == 0) continue;
+          v[i] = cmul(v[i], tw(l2l, e), &fl[s]);
+        }
+      }
+    }
+  }
+
+  std::vector<Cx> beats(n);
+  if (spc == 1) {
+    for (unsigned j = 0; j < m; ++j) beats[j] = lane[j];
+  } else {  // DIT merge (fft_ref.hpp:380-406)
+    const unsigned sh = (sched >> nl) & 1u;
+    for (unsigned j = 0; j < m; ++j) {
+      const Cx w = tw(nl + 1, bit_reverse(j, nl));
+      const Cx e = lane[j];
+      const Cx t = cmul(lane[(size_t)m + j], w, &fl[nl]);
+      beats[2 * j] = Cx{fq((i64)e.re + t.re, sh, &fl[nl]), fq((i64)e.im + t.im, sh, &fl[nl])};
+      beats[2 * j + 1] = Cx{fq((i64)e.re - t.re, sh, &fl[nl]), fq((i64)e.im - t.im, sh, &fl[nl])};
+    }
+  }
+  if (bins) {
+    for (unsigned j = 0; j < m; ++j) {
+      const unsigned mj = bit_reverse(j, nl);
+      for (unsigned q = 0; q < spc; ++q) bins[mj + q * m] = pack(beats[(size_t)j * spc + q]);
+    }
+  }
+  if (out) {
+    for (unsigned j = 0; j < m; ++j) {
+      const unsigned src = reorder ? bit_reverse(j, nl) : j;
+      for (unsigned q = 0; q < spc; ++q) out[(size_t)j * spc + q] = pack(beats[(size_t)src * spc + q]);
+    }
+  }
+  if (stage_flags_word) {
+    u32 w = 0;
+    for (unsigned g = 0; g < stages; ++g) w |= (u32)fl[g] << (2 * g);
+    *stage_flags_word = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Beamformer — model/cpp/beamformer/beamformer_model.hpp:162-179
+// y = round_sat(sum_a x[a] * w[a]) with an exact int64 accumulator, no conjugate.
+// ---------------------------------------------------------------------------
+inline void bf_dot(const u32* x, const u32* w, unsigned n_ant, i64* are, i64* aim) {
+  i64 r = 0, i = 0;
+  for (unsigned a = 0; a < n_ant; ++a) {
+    const Cx xa = unpack(x[a]), wa = unpack(w[a]);
+    r = wadd(r, cm_re(xa, wa));
+    i = wadd(i, cm_im(xa, wa));
+  }
+  *are = r;
+  *aim = i;
+}
+
+// power = I^2 + Q^2 — model/cpp/covariance/covar_model.hpp:99-102
+inline i64 power_of(Cx s) { return prod(s.re, s.re) + prod(s.im, s.im); }
+
+// Rxy = X * conj(Y), operand-swap wiring — covar_model.hpp:137-151
+inline void cross_swap(Cx x, Cx y, i64* re, i64* im) {
+  if (fault() == 6) std::swap(x, y);
+  const Cx b{y.im, y.re};
+  *re = cm_im(x, b);
+  *im = wneg(cm_re(x, b));
+}
+// direct definition — covar_model.hpp:119-126
+inline void cross_direct(Cx x, Cx y, i64* re, i64* im) {
+  *re = prod(x.re, y.re) + prod(x.im, y.im);
+  *im = prod(x.im, y.re) - prod(x.re, y.im);
+}
+
+// Integrator, block mode, one window — covar_model.hpp:186-288. Adds then
+// saturates to acc_w every step, with sticky flags and a saturating counter.
+struct Integ {
+  unsigned acc_w;
+  i64 acc = 0;
+  unsigned sticky = 0;
+  u32 count = 0;
+  void add(i64 d) {
+    const i64 raw = wadd(acc, d);
+    const unsigned f = flags_of(raw, acc_w);
+    acc = clamp_w(raw, acc_w);
+    sticky |= f;
+    if (f && count != 0xFFFFFFFFu) ++count;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// CFAR — model/cpp/cfar/cfar_model.hpp:112-340
+// ---------------------------------------------------------------------------
+struct CfarCfg {
+  u32 enable, mode, out_mode, guard_lead, guard_lag, ref_lead, ref_lag, alpha;
+};
+
+struct Ev {
+  unsigned kind, bin, frame_id, ref_count, alpha, det, sup;
+  u64 cut, sum;
+  bool sof, eof;
+};
+
+// 176-bit event, LSB first (cfar_model.hpp:75-84); sof at bit 176, eof at 177
+// (the spare bits of the third 64-bit word; framing, not part of the event).
+constexpr unsigned kFieldLsb[9] = {0, 2, 18, 34, 41, 57, 73, 89, 129};
+constexpr unsigned kFieldW[9] = {2, 16, 16, 7, 16, 16, 16, 40, 47};
+
+inline void put_bits(u64* w, unsigned lsb, unsigned n, u64 v) {
+  if (n < 64) v &= (1ULL << n) - 1ULL;
+  for (unsigned i = 0; i < n; ++i)
+    if ((v >> i) & 1ULL) w[(lsb + i) / 64] |= 1ULL << ((lsb + i) % 64);
+}
+inline void event_pack(const Ev& e, u64* w) {
+  w[0] = w[1] = w[2] = 0;
+  const u64 f[9] = {e.kind, e.bin, e.frame_id, e.ref_count, e.alpha, e.det, e.sup, e.cut, e.sum};
+  for (int i = 0; i < 9; ++i) put_bits(w, kFieldLsb[i], kFieldW[i], f[i]);
+  put_bits(w, 176, 1, e.sof ? 1 : 0);
+  put_bits(w, 177, 1, e.eof ? 1 : 0);
+}
+
+// One frame -> ordered events (cfar_model.hpp:295-340). Returns the count.
+size_t cfar_frame(const u64* bins, unsigned n, const CfarCfg& written, unsigned max_guard, unsigned max_ref,
+                  unsigned frame_id, Ev* out) {
+  CfarCfg c = written;  // geometry clamp (cfar_model.hpp:134-141)
+  c.guard_lead = std::min(c.guard_lead, max_guard);
+  c.guard_lag = std::min(c.guard_lag, max_guard);
+  c.ref_lead = std::min(c.ref_lead, max_ref);
+  c.ref_lag = std::min(c.ref_lag, max_ref);
+  const bool go = c.mode == 1;
+  const bool refs_ok = go ? (c.ref_lead != 0 && c.ref_lag != 0) : (c.ref_lead + c.ref_lag) != 0;
+  size_t k = 0;
+  unsigned det = 0, sup = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    // lead cells at higher indices, lag at lower, guards skipped (cfar_model.hpp:249-267)
+    u64 slead = 0, slag = 0;
+    bool ok = true;
+    const bool swap_sides = fault() == 5;
+    for (unsigned j = 1; j <= c.ref_lead; ++j) {
+      const u64 idx = swap_sides ? (u64)((i64)i - (i64)(c.guard_lead + j)) : (u64)i + c.guard_lead + j;
+      if (idx >= n) { ok = false; break; }
+      slead += bins[idx];
+    }
+    if (ok) {
+      for (unsigned j = 1; j <= c.ref_lag; ++j) {
+        const i64 idx = swap_sides ? (i64)(i + c.guard_lag + j) >= (i64)n ? -1 : (i64)(i + c.guard_lag + j)
+                                   : (i64)i - (i64)(c.guard_lag + j);
+        if (idx < 0) { ok = false; break; }
+        slag += bins[idx];
+      }
+    }
+    u64 ssel;
+    unsigned nsel;
+    if (go) {  // greater mean by cross-multiply, ties to lead (cfar_model.hpp:268-273)
+      const bool lead = slead * c.ref_lag >= slag * c.ref_lead;
+      ssel = lead ? slead : slag;
+      nsel = lead ? c.ref_lead : c.ref_lag;
+    } else {
+      ssel = slead + slag;
+      nsel = c.ref_lead + c.ref_lag;
+    }
+    const bool evaluable = ok && c.enable && refs_ok;
+    const bool hit = evaluable && (((bins[i] * (u64)nsel) << 8) > (u64)c.alpha * ssel);
+    if (hit) ++det;
+    if (!evaluable) ++sup;
+    if (!hit && c.out_mode != 1) continue;
+    Ev e{};
+    e.kind = hit ? 0u : (!evaluable ? 2u : 1u);
+    e.bin = i;
+    e.frame_id = frame_id;
+    e.alpha = c.alpha;
+    e.cut = bins[i];
+    e.ref_count = evaluable ? nsel : 0u;
+    e.sum = evaluable ? ssel : 0u;
+    out[k++] = e;
+  }
+  Ev s{};
+  s.kind = 3;
+  s.bin = n;
+  s.frame_id = frame_id;
+  s.alpha = c.alpha;
+  s.det = det;
+  s.sup = sup;
+  s.eof = true;
+  out[k++] = s;
+  out[0].sof = true;
+  return k;
+}
+
+template <class F>
+void parallel_for(size_t n, unsigned threads, F f) {
+  if (threads <= 1 || n <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const unsigned nt = (unsigned)std::min<size_t>(threads, n);
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t] {
+      for (size_t i = t; i < n; i += nt) f(i);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace rdo
+
+// =============================================================================
+// C entry points (loaded with ctypes by the tests and bench.py's CPU leg)
+// =============================================================================
+using namespace rdo;
+
+extern "C" {
+
+void rdo_set_fault(int f) { fault() = f; }
+
+// fxp op table (model/cpp/fxp/fxp_ops.hpp:34-177). Returns 0 on an unknown op.
+int rdo_fxp_op(const char* op, int64_t a, int64_t b, unsigned sh, unsigned w, int64_t* y, unsigned* flags) {
+  auto low16 = [](i64 v) { return (i16)(u16)((u64)v & 0xFFFFu); };
+  auto low32 = [](i64 v) { return (u32)((u64)v & 0xFFFFFFFFu); };
+  *flags = 0;
+  if (!std::strcmp(op, "sat")) { *y = clamp_w(a, w); *flags = flags_of(a, w); }
+  else if (!std::strcmp(op, "trunc")) { *y = floor_shift(a, sh); }
+  else if (!std::strcmp(op, "round_even") || !std::strcmp(op, "round")) { *y = rne(a, sh); }
+  else if (!std::strcmp(op, "round_half_up")) { *y = rhu(a, sh); }
+  else if (!std::strcmp(op, "round_sat")) { *y = rsat(a, sh, w); *flags = rsat_flags(a, sh, w); }
+  else if (!std::strcmp(op, "add_sat")) { *y = clamp_w(wadd(a, b), w); *flags = flags_of(wadd(a, b), w); }
+  else if (!std::strcmp(op, "sub_sat")) { *y = clamp_w(wsub(a, b), w); *flags = flags_of(wsub(a, b), w); }
+  else if (!std::strcmp(op, "neg_sat")) { *y = clamp_w(wneg(a), w); *flags = flags_of(wneg(a), w); }
+  else if (!std::strcmp(op, "mul_q15")) { *y = prod(low16(a), low16(b)); }
+  else if (!std::strcmp(op, "mul_q15_rs")) {
+    const i64 p = prod(low16(a), low16(b));
+    *y = rsat(p, 15, 16);
+    *flags = rsat_flags(p, 15, 16);
+  } else if (!std::strcmp(op, "cmul_re") || !std::strcmp(op, "cmul_im")) {
+    const Cx ca = unpack(low32(a)), cb = unpack(low32(b));
+    const i64 v = op[5] == 'r' ? cm_re(ca, cb) : cm_im(ca, cb);
+    *y = rsat(v, 15, 16);
+    *flags = rsat_flags(v, 15, 16);
+  } else if (!std::strcmp(op, "acc_w")) { *y = (i64)(w + growth(low32(a))); }
+  else if (!std::strcmp(op, "growth_bits")) { *y = (i64)growth(low32(a)); }
+  else return 0;
+  return 1;
+}
+
+// Saturating accumulator run from a clear (fxp.hpp:392-418, StickyFlags :358-381).
+void rdo_acc_run(unsigned acc_w, const int64_t* x, unsigned n, int64_t* y, unsigned* step_flags, unsigned* sticky,
+                 uint32_t* count) {
+  Integ a{acc_w};
+  for (unsigned i = 0; i < n; ++i) {
+    const unsigned f = flags_of(wadd(a.acc, x[i]), acc_w);
+    a.add(x[i]);
+    y[i] = a.acc;
+    step_flags[i] = f;
+    sticky[i] = a.sticky;
+    count[i] = a.count;
+  }
+}
+
+// Complex multiply, both forms (cmult.hpp:85-168).
+void rdo_cmult(uint32_t a, uint32_t b, int64_t* p_re, int64_t* p_im, int64_t* p3_re, int64_t* p3_im, uint32_t* y,
+               unsigned* f_re, unsigned* f_im) {
+  const Cx ca = unpack(a), cb = unpack(b);
+  *p_re = cm_re(ca, cb);
+  *p_im = cm_im(ca, cb);
+  cm3(ca, cb, p3_re, p3_im);
+  *y = pack2(rsat(*p_re, 15, 16), rsat(*p_im, 15, 16));
+  *f_re = rsat_flags(*p_re, 15, 16);
+  *f_im = rsat_flags(*p_im, 15, 16);
+}
+
+void rdo_twiddle_table(unsigned M, uint32_t* out) { twiddle_table(M, out); }
+
+// PFB over n_chan channels: x[ch][n_beats*P] -> y[ch][n_beats][P], flags byte
+// (f_re | f_im << 2). `hist` is [ch][T-1][P] or null.
+void rdo_pfb_run(const uint32_t* x, const uint32_t* coeff, unsigned P, unsigned T, unsigned n_chan, unsigned n_beats,
+                 uint32_t* hist, uint32_t* y, uint8_t* flags, unsigned threads) {
+  const size_t span = (size_t)n_beats * P;
+  parallel_for(n_chan, threads, [&](size_t c) {
+    pfb_channel(x + c * span, coeff, P, T, n_beats, hist ? hist + c * (size_t)(T > 1 ? T - 1 : 0) * P : nullptr,
+                y + c * span, flags ? flags + c * span : nullptr);
+  });
+}
+
+// FFT over n_frames frames of n samples. Any of out / bins / stage_flags may be null.
+int rdo_fft_run(const uint32_t* in, unsigned n, unsigned spc, uint32_t sched, int reorder, unsigned n_frames,
+                uint32_t* out, uint32_t* bins, uint32_t* stage_flags, unsigned threads) {
+  if (!fft_valid(n, spc)) return -1;
+  parallel_for(n_frames, threads, [&](size_t f) {
+    fft_frame(in + f * n, n, spc, sched, reorder, out ? out + f * n : nullptr, bins ? bins + f * n : nullptr,
+              stage_flags ? stage_flags + f : nullptr);
+  });
+  return 0;
+}
+
+// Corner turn X[ch][frame][bin] -> Xa[frame][bin][ch] (align_model.hpp:172-186).
+void rdo_align_run(const uint32_t* X, unsigned n_chan, unsigned n_frames, unsigned n_bins, uint32_t* Xa) {
+  const size_t cols = (size_t)n_frames * n_bins;
+  for (unsigned c = 0; c < n_chan; ++c)
+    for (size_t col = 0; col < cols; ++col) Xa[col * n_chan + c] = X[(size_t)c * cols + col];
+}
+
+// Beamformer + power over aligned columns Xa[col][CH]; W[B][CH] beam-major.
+// y/flags/power are [B][cols]; acc (nullable) is [B][cols][2] exact int64.
+void rdo_bf_run(const uint32_t* Xa, const uint32_t* W, unsigned CH, unsigned B, size_t cols, uint32_t* y,
+                uint8_t* flags, uint32_t* power, int64_t* acc, unsigned threads) {
+  parallel_for(B, threads, [&](size_t b) {
+    for (size_t col = 0; col < cols; ++col) {
+      i64 re, im;
+      bf_dot(Xa + col * CH, W + b * CH, CH, &re, &im);
+      const size_t o = b * cols + col;
+      const Cx yy{(i16)rsat(re, 15, 16), (i16)rsat(im, 15, 16)};
+      if (y) y[o] = pack(yy);
+      if (flags) flags[o] = (uint8_t)(rsat_flags(re, 15, 16) | (rsat_flags(im, 15, 16) << 2));
+      if (power) power[o] = (u32)power_of(yy);
+      if (acc) {
+        acc[2 * o] = re;
+        acc[2 * o + 1] = im;
+      }
+    }
+  });
+}
+
+// Covariance: R[bin][pair][2], pairs (i<=j) row-major over the upper triangle,
+// R_ij = sum_frames cross_power(X_i, X_j), each component integrated by a
+// block-mode Integrator of width acc_w over one window of n_frames.
+// sat_flags (nullable) receives the sticky flags per [bin][pair] (re | im<<2).
+void rdo_cov_run(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w, int64_t* R,
+                 uint8_t* sat_flags, unsigned threads) {
+  const size_t npairs = (size_t)CH * (CH + 1) / 2;
+  parallel_for(N, threads, [&](size_t k) {
+    size_t pi = 0;
+    for (unsigned i = 0; i < CH; ++i) {
+      for (unsigned j = i; j < CH; ++j, ++pi) {
+        Integ ire{acc_w}, iim{acc_w};
+        for (unsigned m = 0; m < F; ++m) {
+          const size_t base = ((size_t)m * N + k) * CH;
+          i64 re, im;
+          cross_swap(unpack(Xa[base + i]), unpack(Xa[base + j]), &re, &im);
+          ire.add(re);
+          iim.add(im);
+        }
+        R[(k * npairs + pi) * 2] = ire.acc;
+        R[(k * npairs + pi) * 2 + 1] = iim.acc;
+        if (sat_flags) sat_flags[k * npairs + pi] = (uint8_t)(ire.sticky | (iim.sticky << 2));
+      }
+    }
+  });
+}
+
+void rdo_cross_power(uint32_t x, uint32_t y, int64_t* sre, int64_t* sim, int64_t* dre, int64_t* dim) {
+  cross_swap(unpack(x), unpack(y), sre, sim);
+  cross_direct(unpack(x), unpack(y), dre, dim);
+}
+
+int64_t rdo_power(uint32_t s) { return power_of(unpack(s)); }
+
+// CFAR over rows of n bins (u64 cells). Writes packed events (3 words each) for
+// every row in order into `events` (capacity events_cap) and the per-row event
+// count into row_count. Returns the total number of events, or -1 on overflow.
+int64_t rdo_cfar_run(const uint64_t* bins, unsigned n, unsigned n_rows, const uint32_t* cfg8, unsigned max_guard,
+                     unsigned max_ref, const uint32_t* frame_ids, uint64_t* events, int64_t events_cap,
+                     uint32_t* row_count) {
+  const CfarCfg c{cfg8[0], cfg8[1], cfg8[2], cfg8[3], cfg8[4], cfg8[5], cfg8[6], cfg8[7]};
+  std::vector<Ev> tmp((size_t)n + 1);
+  int64_t total = 0;
+  for (unsigned r = 0; r < n_rows; ++r) {
+    const size_t k = cfar_frame(bins + (size_t)r * n, n, c, max_guard, max_ref, frame_ids ? frame_ids[r] : r,
+                                tmp.data());
+    if (row_count) row_count[r] = (u32)k;
+    if (total + (int64_t)k > events_cap) return -1;
+    for (size_t e = 0; e < k; ++e) event_pack(tmp[e], events + 3 * (size_t)(total + (int64_t)e));
+    total += (int64_t)k;
+  }
+  return total;
+}
+
+// Unpacked event fields -> packed words (for the pack round-trip test).
+void rdo_event_pack(const uint64_t* fields11, uint64_t* words) {
+  Ev e{};
+  e.kind = (unsigned)fields11[0];
+  e.bin = (unsigned)fields11[1];
+  e.frame_id = (unsigned)fields11[2];
+  e.ref_count = (unsigned)fields11[3];
+  e.alpha = (unsigned)fields11[4];
+  e.det = (unsigned)fields11[5];
+  e.sup = (unsigned)fields11[6];
+  e.cut = fields11[7];
+  e.sum = fields11[8];
+  e.sof = fields11[9] != 0;
+  e.eof = fields11[10] != 0;
+  event_pack(e, words);
+}
+
+// ---------------------------------------------------------------------------
+// The composed chain for one CPI (builder-defined glue; per-stage semantics as
+// above). Outputs that are null are not kept. Returns the number of events, or
+// a negative value on invalid geometry / event overflow.
+//   iq      [CH][F*N]       u   [CH][F][N]      pfb_flags [CH][F][N]
+//   X       [CH][F][N]      fft_flags [CH][F]   Xa [F][N][CH]
+//   y/yflags/power [B][F][N]                    R  [N][CH(CH+1)/2][2]
+//   events  3 words each, rows ordered (beam, frame); row_count [B][F]
+// ---------------------------------------------------------------------------
+int64_t rdo_chain_run(const uint32_t* iq, const uint32_t* taps, const uint32_t* W, unsigned CH, unsigned N, unsigned T,
+                      unsigned F, unsigned B, unsigned spc, uint32_t sched, const uint32_t* cfar_cfg8,
+                      unsigned max_guard, unsigned max_ref, uint32_t* u, uint8_t* pfb_flags, uint32_t* X,
+                      uint32_t* fft_flags, uint32_t* Xa, uint32_t* y, uint8_t* yflags, uint32_t* power, int64_t* R,
+                      uint64_t* events, int64_t events_cap, uint32_t* row_count, unsigned threads) {
+  if (!fft_valid(N, spc)) return -2;
+  const size_t S = (size_t)CH * F * N, cols = (size_t)F * N;
+  std::vector<u32> u_l, X_l, Xa_l, pw_l;
+  if (!u) { u_l.resize(S); u = u_l.data(); }
+  if (!X) { X_l.resize(S); X = X_l.data(); }
+  if (!Xa) { Xa_l.resize(S); Xa = Xa_l.data(); }
+  if (!power) { pw_l.resize((size_t)B * cols); power = pw_l.data(); }
+  rdo_pfb_run(iq, taps, N, T, CH, F, nullptr, u, pfb_flags, threads);
+  rdo_fft_run(u, N, spc, sched, 1, CH * F, nullptr, X, fft_flags, threads);
+  rdo_align_run(X, CH, F, N, Xa);
+  rdo_bf_run(Xa, W, CH, B, cols, y, yflags, power, nullptr, threads);
+  if (R) rdo_cov_run(Xa, CH, F, N, 64, R, nullptr, threads);
+  // CFAR input: clamp_cell(power) (cfar_model.hpp:238-240); rows (beam, frame)
+  const size_t rows = (size_t)B * F;
+  std::vector<u64> cells(N);
+  std::vector<u32> fid(1);
+  std::vector<std::vector<Ev>> per_row(rows);
+  const CfarCfg c{cfar_cfg8[0], cfar_cfg8[1], cfar_cfg8[2], cfar_cfg8[3],
+                  cfar_cfg8[4], cfar_cfg8[5], cfar_cfg8[6], cfar_cfg8[7]};
+  parallel_for(rows, threads, [&](size_t r) {
+    std::vector<u64> cl(N);
+    for (unsigned i = 0; i < N; ++i) cl[i] = (u64)power[r * N + i];
+    std::vector<Ev> tmp((size_t)N + 1);
+    const size_t k = cfar_frame(cl.data(), N, c, max_guard, max_ref, (unsigned)(r % F), tmp.data());
+    tmp.resize(k);
+    per_row[r] = std::move(tmp);
+  });
+  int64_t total = 0;
+  for (size_t r = 0; r < rows; ++r) {
+    if (row_count) row_count[r] = (u32)per_row[r].size();
+    if (events) {
+      if (total + (int64_t)per_row[r].size() > events_cap) return -1;
+      for (size_t e = 0; e < per_row[r].size(); ++e)
+        event_pack(per_row[r][e], events + 3 * (size_t)(total + (int64_t)e));
+    }
+    total += (int64_t)per_row[r].size();
+  }
+  return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// -----------------------------------------------------------------------------
+// ref_shim.cpp — C entry points over the REFERENCE's own header-only models.
+//
+// TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile against the reference
+// sources where they lie (-I <reference>/model/cpp) into oracle/_ref/, which is
+// never committed. It is used to check the CPU restatement (rd_oracle.cpp)
+// against the real models, and optionally as the timed CPU baseline. Nothing
+// here restates arithmetic: every value comes from a call into the reference.
+//
+// The reference caps the FFT at N <= 1024 (fft_ref.hpp:216-219), so this shim
+// covers the chain only up to that size.
+// -----------------------------------------------------------------------------
+#include <cctype>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "beamformer/beamformer_model.hpp"
+#include "cfar/cfar_model.hpp"
+#include "covariance/covar_model.hpp"
+#include "fft/fft_ref.hpp"
+#include "fxp/cmult.hpp"
+#include "fxp/fxp.hpp"
+#include "fxp/fxp_ops.hpp"
+#include "pfb/pfb_model.hpp"
+
+namespace {
+
+template <class F>
+void par(size_t n, unsigned threads, F f) {
+  if (threads <= 1 || n <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const unsigned nt = (unsigned)(threads < n ? threads : n);
+  for (unsigned t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      for (size_t i = t; i < n; i += nt) f(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
+std::vector<fxp::Complex> to_cx(const uint32_t* p, size_t n) {
+  std::vector<fxp::Complex> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = fxp::Complex::from_packed(p[i]);
+  return v;
+}
+
+// Pack one reference event by the reference's own layout constants
+// (cfar_model.hpp:75-84); framing bits sof/eof at 176/177.
+void put(uint64_t* w, unsigned lsb, unsigned n, uint64_t v) {
+  if (n < 64) v &= (1ULL << n) - 1ULL;
+  for (unsigned i = 0; i < n; ++i)
+    if ((v >> i) & 1ULL) w[(lsb + i) / 64] |= 1ULL << ((lsb + i) % 64);
+}
+void pack_ref(const cfar::Event& e, uint64_t* w) {
+  w[0] = w[1] = w[2] = 0;
+  put(w, cfar::kEvKindLsb, cfar::kEvKindW, e.kind);
+  put(w, cfar::kEvBinLsb, cfar::kBinW, e.bin);
+  put(w, cfar::kEvFrameIdLsb, cfar::kFrameIdW, e.frame_id);
+  put(w, cfar::kEvRefCntLsb, cfar::kTotRefW, e.ref_count);
+  put(w, cfar::kEvAlphaLsb, cfar::kAlphaW, e.alpha);
+  put(w, cfar::kEvDetLsb, cfar::kCntW, e.det_count);
+  put(w, cfar::kEvSupLsb, cfar::kCntW, e.sup_count);
+  put(w, cfar::kEvCellLsb, cfar::kPowerW, e.cut_power);
+  put(w, cfar::kEvSumLsb, cfar::kSumW, e.noise_sum);
+  put(w, cfar::kEventW, 1, e.sof ? 1 : 0);
+  put(w, cfar::kEventW + 1, 1, e.eof ? 1 : 0);
+}
+
+cfar::Config cfg_of(const uint32_t* c) {
+  cfar::Config o;
+  o.enable = c[0] != 0;
+  o.mode = c[1] ? cfar::Mode::kGO : cfar::Mode::kCA;
+  o.out_mode = c[2] ? cfar::OutMode::kDense : cfar::OutMode::kEvents;
+  o.guard_lead = c[3];
+  o.guard_lag = c[4];
+  o.ref_lead = c[5];
+  o.ref_lag = c[6];
+  o.alpha = c[7];
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdr_fxp_op(const char* op, int64_t a, int64_t b, unsigned sh, unsigned w, int64_t* y, unsigned* flags) {
+  const unsigned code = fxp::ops::code_of(op);
+  if (code >= fxp::ops::kOpCount) return 0;
+  const fxp::ops::Result r = fxp::ops::apply(code, a, b, sh, w);
+  *y = r.y;
+  *flags = r.flags.packed();
+  return 1;
+}
+
+void rdr_cmult(uint32_t a, uint32_t b, int64_t* p_re, int64_t* p_im, int64_t* p3_re, int64_t* p3_im, uint32_t* y,
+               unsigned* f_re, unsigned* f_im) {
+  const fxp::Complex ca = fxp::Complex::from_packed(a), cb = fxp::Complex::from_packed(b);
+  const auto r4 = fxp::cmult::eval(fxp::cmult::Variant::kMult4, ca, cb);
+  const auto r3 = fxp::cmult::raw_mult3(ca, cb);
+  *p_re = r4.p_re;
+  *p_im = r4.p_im;
+  *p3_re = r3.re;
+  *p3_im = r3.im;
+  *y = fxp::Complex{r4.y_re, r4.y_im}.packed();
+  *f_re = r4.f_re.packed();
+  *f_im = r4.f_im.packed();
+}
+
+void rdr_twiddle_table(uint32_t* out) {
+  for (unsigned e = 0; e < fft::kTwN; ++e) out[e] = fft::kTw[e];
+}
+
+void rdr_pfb_run(const uint32_t* x, const uint32_t* coeff, unsigned P, unsigned T, unsigned n_chan,
+                 unsigned n_beats, uint32_t* y, uint8_t* flags, unsigned threads) {
+  const std::vector<fxp::Complex> h = to_cx(coeff, (size_t)P * T);
+  const size_t span = (size_t)n_beats * P;
+  par(n_chan, threads, [&](size_t c) {
+    pfb::PfbModel m(P, T, h);
+    m.reset();
+    for (unsigned b = 0; b < n_beats; ++b) {
+      const pfb::BeatResult r = m.step(to_cx(x + c * span + (size_t)b * P, P));
+      for (unsigned p = 0; p < P; ++p) {
+        const size_t o = c * span + (size_t)b * P + p;
+        y[o] = r.y[p].packed();
+        if (flags) flags[o] = (uint8_t)(r.flags_re[p].packed() | (r.flags_im[p].packed() << 2));
+      }
+    }
+  });
+}
+
+int rdr_fft_run(const uint32_t* in, unsigned n, unsigned spc, uint32_t sched, int reorder, unsigned n_frames,
+                uint32_t* out, uint32_t* bins, uint32_t* stage_flags, unsigned threads) {
+  fft::Config cfg;
+  cfg.n_fft = n;
+  cfg.spc = spc;
+  cfg.scale_sched = sched;
+  cfg.reorder = reorder != 0;
+  if (!cfg.valid()) return -1;
+  par(n_frames, threads, [&](size_t f) {
+    const fft::Result r = fft::transform(cfg, to_cx(in + f * n, n));
+    for (unsigned i = 0; i < n; ++i) {
+      if (out) out[f * n + i] = r.out[i].packed();
+      if (bins) bins[f * n + i] = r.bins[i].packed();
+    }
+    if (stage_flags) {
+      uint32_t w = 0;
+      for (size_t g = 0; g < r.stage_flags.size(); ++g) w |= r.stage_flags[g].packed() << (2 * g);
+      stage_flags[f] = w;
+    }
+  });
+  return 0;
+}
+
+void rdr_bf_run(const uint32_t* Xa, const uint32_t* W, unsigned CH, unsigned B, size_t cols, uint32_t* y,
+                uint8_t* flags, uint32_t* power, int64_t* acc, unsigned threads) {
+  const std::vector<fxp::Complex> w = to_cx(W, (size_t)B * CH);
+  par(B, threads, [&](size_t b) {
+    std::vector<fxp::Complex> x(CH);
+    for (size_t col = 0; col < cols; ++col) {
+      for (unsigned a = 0; a < CH; ++a) x[a] = fxp::Complex::from_packed(Xa[col * CH + a]);
+      const bf::DotResult d = bf::dot(x.data(), &w[bf::weight_index(CH, (unsigned)b, 0)], CH);
+      const size_t o = b * cols + col;
+      if (y) y[o] = d.y.packed();
+      if (flags) flags[o] = (uint8_t)(d.f_re.packed() | (d.f_im.packed() << 2));
+      if (power) power[o] = (uint32_t)cfar::clamp_cell(covar::power(d.y));
+      if (acc) {
+        acc[2 * o] = d.acc_re;
+        acc[2 * o + 1] = d.acc_im;
+      }
+    }
+  });
+}
+
+// Covariance through the reference Integrator: block mode, window = F frames,
+// one accumulator per component; the result is what the window close emits.
+void rdr_cov_run(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w, int64_t* R,
+                 uint8_t* sat_flags, unsigned threads) {
+  const size_t npairs = (size_t)CH * (CH + 1) / 2;
+  covar::Config cfg;
+  cfg.window_len = F;
+  cfg.mode = covar::Mode::kBlock;
+  par(N, threads, [&](size_t k) {
+    size_t pi = 0;
+    for (unsigned i = 0; i < CH; ++i) {
+      for (unsigned j = i; j < CH; ++j, ++pi) {
+        covar::Integrator ire(acc_w, 16), iim(acc_w, 16);
+        ire.reset(cfg);
+        iim.reset(cfg);
+        covar::Result rre{}, rim{};
+        for (unsigned m = 0; m < F; ++m) {
+          const size_t base = ((size_t)m * N + k) * CH;
+          const covar::Cross c = covar::cross_power(fxp::Complex::from_packed(Xa[base + i]),
+                                                    fxp::Complex::from_packed(Xa[base + j]));
+          ire.step(cfg, true, c.re, false, false, &rre);
+          iim.step(cfg, true, c.im, false, false, &rim);
+          if (m + 1 == F) {
+            // sticky flags of the closing window, read before the next edge
+            if (sat_flags) sat_flags[k * npairs + pi] = (uint8_t)(ire.sat_flags().packed() | (iim.sat_flags().packed() << 2));
+          }
+        }
+        R[(k * npairs + pi) * 2] = rre.acc;
+        R[(k * npairs + pi) * 2 + 1] = rim.acc;
+      }
+    }
+  });
+}
+
+void rdr_cross_power(uint32_t x, uint32_t y, int64_t* sre, int64_t* sim, int64_t* dre, int64_t* dim) {
+  const covar::Cross s = covar::cross_power_swap(fxp::Complex::from_packed(x), fxp::Complex::from_packed(y));
+  const covar::Cross d = covar::cross_power_direct(fxp::Complex::from_packed(x), fxp::Complex::from_packed(y));
+  *sre = s.re;
+  *sim = s.im;
+  *dre = d.re;
+  *dim = d.im;
+}
+
+int64_t rdr_power(uint32_t s) { return covar::power(fxp::Complex::from_packed(s)); }
+
+int64_t rdr_cfar_run(const uint64_t* bins, unsigned n, unsigned n_rows, const uint32_t* cfg8, unsigned max_guard,
+                     unsigned max_ref, const uint32_t* frame_ids, uint64_t* events, int64_t events_cap,
+                     uint32_t* row_count) {
+  const cfar::Config c = cfg_of(cfg8);
+  int64_t total = 0;
+  for (unsigned r = 0; r < n_rows; ++r) {
+    std::vector<uint64_t> row(bins + (size_t)r * n, bins + (size_t)(r + 1) * n);
+    const std::vector<cfar::Event> ev = cfar::run_frame(row, c, max_guard, max_ref, frame_ids ? frame_ids[r] : r);
+    if (row_count) row_count[r] = (uint32_t)ev.size();
+    if (total + (int64_t)ev.size() > events_cap) return -1;
+    for (size_t e = 0; e < ev.size(); ++e) pack_ref(ev[e], events + 3 * (size_t)(total + (int64_t)e));
+    total += (int64_t)ev.size();
+  }
+  return total;
+}
+
+// The composed chain for one CPI with the reference models (N <= 1024), same
+// glue and output layout as rdo_chain_run.
+int64_t rdr_chain_run(const uint32_t* iq, const uint32_t* taps, const uint32_t* W, unsigned CH, unsigned N, unsigned T,
+                      unsigned F, unsigned B, unsigned spc, uint32_t sched, const uint32_t* cfar_cfg8,
+                      unsigned max_guard, unsigned max_ref, uint32_t* u, uint8_t* pfb_flags, uint32_t* X,
+                      uint32_t* fft_flags, uint32_t* Xa, uint32_t* y, uint8_t* yflags, uint32_t* power, int64_t* R,
+                      uint64_t* events, int64_t events_cap, uint32_t* row_count, unsigned threads) {
+  const size_t S = (size_t)CH * F * N, cols = (size_t)F * N;
+  std::vector<uint32_t> u_l, X_l, Xa_l, pw_l;
+  if (!u) { u_l.resize(S); u = u_l.data(); }
+  if (!X) { X_l.resize(S); X = X_l.data(); }
+  if (!Xa) { Xa_l.resize(S); Xa = Xa_l.data(); }
+  if (!power) { pw_l.resize((size_t)B * cols); power = pw_l.data(); }
+  rdr_pfb_run(iq, taps, N, T, CH, F, u, pfb_flags, threads);
+  if (rdr_fft_run(u, N, spc, sched, 1, CH * F, nullptr, X, fft_flags, threads) != 0) return -2;
+  for (unsigned c = 0; c < CH; ++c)  // alignment: bin-major, antenna-minor (align_model.hpp:172-186)
+    for (size_t col = 0; col < cols; ++col) Xa[col * CH + c] = X[(size_t)c * cols + col];
+  rdr_bf_run(Xa, W, CH, B, cols, y, yflags, power, nullptr, threads);
+  if (R) rdr_cov_run(Xa, CH, F, N, 64, R, nullptr, threads);
+  const size_t rows = (size_t)B * F;
+  const cfar::Config c = cfg_of(cfar_cfg8);
+  std::vector<std::vector<cfar::Event>> per_row(rows);
+  par(rows, threads, [&](size_t r) {
+    std::vector<uint64_t> cells(N);
+    for (unsigned i = 0; i < N; ++i) cells[i] = cfar::clamp_cell((int64_t)power[r * N + i]);
+    per_row[r] = cfar::run_frame(cells, c, max_guard, max_ref, (unsigned)(r % F));
+  });
+  int64_t total = 0;
+  for (size_t r = 0; r < rows; ++r) {
+    if (row_count) row_count[r] = (uint32_t)per_row[r].size();
+    if (events) {
+      if (total + (int64_t)per_row[r].size() > events_cap) return -1;
+      for (size_t e = 0; e < per_row[r].size(); ++e) pack_ref(per_row[r][e], events + 3 * (size_t)(total + (int64_t)e));
+    }
+    total += (int64_t)per_row[r].size();
+  }
+  return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+"""Oracle stages pinned by the reference's vectors, closed forms and invariants.
+
+Mirrors: the PFB vector check (pfb_model.hpp loaders + test_pfb_bank.cpp),
+model/cpp/test/test_fft_ref.cpp (vectors, permutation property, axis twiddles),
+test_beamformer.cpp's directed cases (unit/zero/permutation), the covariance
+identities of DECISIONS.md (X=Y, swap==direct, Hermitian), and test_cfar.cpp's
+flat-spectrum and edge-suppression cases.
+"""
+import numpy as np
+import pytest
+
+import stim
+import vecio
+
+
+@pytest.mark.parametrize("set_name,geom", vecio.PFB_SETS)
+def test_pfb_golden_vectors(orc, set_name, geom):
+    P, T, coeff, v = vecio.load_pfb(set_name, geom)
+    y, fl = orc.pfb(v["x"], coeff, P, T, 1, v["beats"])
+    assert np.array_equal(y, v["y"])
+    assert np.array_equal(fl & 3, v["f_re"])
+    assert np.array_equal(fl >> 2, v["f_im"])
+
+
+def test_pfb_planted_tap_order_fault_is_caught(orc):
+    P, T, coeff, v = vecio.load_pfb("random", "p8t16")
+    orc.lib.rdo_set_fault(4)
+    try:
+        y, _ = orc.pfb(v["x"], coeff, P, T, 1, v["beats"])
+    finally:
+        orc.lib.rdo_set_fault(0)
+    assert not np.array_equal(y, v["y"])
+
+
+def test_pfb_history_split_equals_one_run(orc):
+    rng = np.random.default_rng(3)
+    P, T, nb = 16, 8, 24
+    x = stim.rand_cx(rng, nb * P)
+    h = stim.rand_cx(rng, P * T)
+    y1, f1 = orc.pfb(x, h, P, T, 1, nb)
+    hist = np.zeros((T - 1) * P, np.uint32)
+    ya, fa = orc.pfb(x[: 10 * P], h, P, T, 1, 10, hist=hist)
+    yb, fb = orc.pfb(x[10 * P:], h, P, T, 1, nb - 10, hist=hist)
+    assert np.array_equal(np.concatenate([ya, yb]), y1)
+    assert np.array_equal(np.concatenate([fa, fb]), f1)
+
+
+def test_fft_golden_vectors(orc):
+    hdr, recs = vecio.load_fft()
+    assert len(recs) == 36 and hdr["n"] == 64 and hdr["spc"] == 2
+    for r in recs:
+        out, bins, sf = orc.fft(r["inp"], hdr["n"], hdr["spc"], r["sched"], r["reorder"])
+        assert np.array_equal(out, r["out"]), r["id"]
+        assert int(sf[0]) == r["flags"], r["id"]
+
+
+@pytest.mark.parametrize("fault", [1, 2, 3])
+def test_fft_planted_faults_are_caught(orc, fault):
+    hdr, recs = vecio.load_fft()
+    orc.lib.rdo_set_fault(fault)
+    try:
+        bad = 0
+        for r in recs:
+            out, _, sf = orc.fft(r["inp"], hdr["n"], hdr["spc"], r["sched"], r["reorder"])
+            bad += (not np.array_equal(out, r["out"])) or int(sf[0]) != r["flags"]
+    finally:
+        orc.lib.rdo_set_fault(0)
+    assert bad > 0
+
+
+@pytest.mark.parametrize("n,spc", [(64, 2), (256, 1), (1024, 2), (4096, 2), (4096, 1)])
+def test_fft_permutation_and_bins(orc, n, spc):
+    rng = np.random.default_rng(n + spc)
+    x = stim.rand_cx(rng, n)
+    nat, bins_a, _ = orc.fft(x, n, spc, reorder=1)
+    rev, bins_b, _ = orc.fft(x, n, spc, reorder=0)
+    m, nl = n // spc, (n // spc).bit_length() - 1
+    br = np.array([int(format(j, f"0{nl}b")[::-1], 2) for j in range(m)])
+    for q in range(spc):
+        assert np.array_equal(nat[np.arange(m) * spc + q], rev[br * spc + q])
+        assert np.array_equal(bins_a[np.arange(m) + q * m], nat[np.arange(m) * spc + q])
+    assert np.array_equal(bins_a, bins_b)
+
+
+@pytest.mark.parametrize("n,spc", [(64, 2), (1024, 2), (1024, 1), (4096, 2), (4096, 1)])
+def test_fft_impulse_and_dc_closed_forms(orc, n, spc):
+    """x = 32767*delta -> every bin 32767/N (scaled 1/N by the all-ones schedule),
+    rounded; DC -> only bin 0."""
+    x = np.zeros(n, np.uint32)
+    x[0] = vecio.pack(32767, 0)
+    _, bins, sf = orc.fft(x, n, spc)
+    vals = {vecio.unpack(int(b)) for b in bins}
+    assert len(vals) == 1 and sf[0] == 0
+    (re, im), = vals
+    assert im == 0 and abs(re - 32767 / n) <= 1.0
+    d = np.full(n, vecio.pack(16384, 0), np.uint32)
+    _, bins, _ = orc.fft(d, n, spc)
+    b = np.array([vecio.unpack(int(v)) for v in bins])
+    assert abs(b[0, 0] - 16384 * 0.999) < 64 and np.all(np.abs(b[1:]) <= 2)
+
+
+@pytest.mark.parametrize("n,spc", [(64, 2), (1024, 2), (4096, 2), (4096, 1)])
+def test_fft_float_dft_bound(orc, n, spc):
+    """|fixed - DFT/N| within 16 LSB on non-saturating random records (the reference's
+    NumPy cross-check gate, DECISIONS.md:1886-1892) — the only pin for N = 4096."""
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        x = stim.rand_cx(rng, n, scale=12000)
+        _, bins, sf = orc.fft(x, n, spc)
+        assert sf[0] == 0
+        xc = stim.to_complex(x)
+        ref = np.fft.fft(xc) / n
+        got = stim.to_complex(bins)
+        assert np.max(np.abs(got - ref)) <= 16.0
+
+
+def test_bf_unit_zero_and_permutation(orc):
+    rng = np.random.default_rng(5)
+    CH, B, cols = 8, 8, 64
+    Xa = stim.rand_cx(rng, cols * CH)
+    sel = [3, 1, 4, 1, 5, 2, 6, 7]
+    Wu = np.zeros(B * CH, np.uint32)
+    for b in range(B):
+        Wu[b * CH + sel[b]] = vecio.pack(32767, 0)
+    y, fl, pw, acc = orc.bf(Xa, Wu, CH, B)
+    for b in range(B):
+        for c in range(cols):
+            xr, xi = vecio.unpack(int(Xa[c * CH + sel[b]]))
+            exp = vecio.pack(stim.rne_int(xr * 32767, 15), stim.rne_int(xi * 32767, 15))
+            assert y[b * cols + c] == exp
+    y0, fl0, pw0, _ = orc.bf(Xa, np.zeros(B * CH, np.uint32), CH, B)
+    assert not y0.any() and not fl0.any() and not pw0.any()
+    W = stim.rand_cx(rng, B * CH)
+    perm = rng.permutation(CH)
+    Wp = np.zeros_like(W)
+    Xp = np.zeros_like(Xa)
+    for b in range(B):
+        Wp[b * CH + perm] = W[b * CH: (b + 1) * CH]
+    for c in range(cols):
+        Xp[c * CH + perm] = Xa[c * CH: (c + 1) * CH]
+    ya, fa, _, aa = orc.bf(Xa, W, CH, B)
+    yb, fb, _, ab = orc.bf(Xp, Wp, CH, B)
+    assert np.array_equal(ya, yb) and np.array_equal(aa, ab) and np.array_equal(fa, fb)
+
+
+def test_bf_float_cross_check(orc):
+    """bf::dot_float analogue: the exact accumulator equals the double-precision dot."""
+    rng = np.random.default_rng(6)
+    CH, B, cols = 16, 4, 32
+    Xa = stim.rand_cx(rng, cols * CH)
+    W = stim.rand_cx(rng, B * CH)
+    _, _, _, acc = orc.bf(Xa, W, CH, B)
+    x = stim.to_complex(Xa).reshape(cols, CH)
+    w = stim.to_complex(W).reshape(B, CH)
+    ref = w @ x.T  # [B][cols]
+    acc = acc.reshape(B, cols, 2)
+    assert np.array_equal(acc[..., 0], ref.real.astype(np.int64))
+    assert np.array_equal(acc[..., 1], ref.imag.astype(np.int64))
+
+
+def test_power_and_cross_power_identities(orc):
+    m = vecio.pack(-32768, -32768)
+    assert orc.power(m) == 2**31
+    rng = np.random.default_rng(8)
+    for v in stim.rand_cx(rng, 300).tolist() + [m, vecio.pack(32767, -32768)]:
+        sre, sim, dre, dim = orc.cross_power(v, v)
+        assert (sre, sim) == (dre, dim) == (orc.power(v), 0)
+    a, b = stim.rand_cx(rng, 200), stim.rand_cx(rng, 200)
+    for x, y in zip(a.tolist(), b.tolist()):
+        sre, sim, dre, dim = orc.cross_power(x, y)
+        assert (sre, sim) == (dre, dim)
+        xc, yc = stim.to_complex(np.array([x], np.uint32))[0], stim.to_complex(np.array([y], np.uint32))[0]
+        z = xc * np.conj(yc)
+        assert (sre, sim) == (int(z.real), int(z.imag))
+
+
+def test_cov_brute_force_and_integrator_width(orc):
+    rng = np.random.default_rng(9)
+    CH, F, N = 4, 12, 8
+    Xa = stim.rand_cx(rng, F * N * CH)
+    R, sf = orc.cov(Xa, CH, F, N, acc_w=64)
+    x = stim.to_complex(Xa).reshape(F, N, CH)
+    R = R.reshape(N, -1, 2)
+    for k in range(N):
+        G = x[:, k, :].T @ np.conj(x[:, k, :])  # G[i][j] = sum_m x_i conj(x_j)
+        p = 0
+        for i in range(CH):
+            for j in range(i, CH):
+                assert R[k, p, 0] == int(G[i, j].real) and R[k, p, 1] == int(G[i, j].imag)
+                p += 1
+    assert not sf.any()
+    # 255 extreme terms are exact at acc_w = 40; the 256th saturates (DECISIONS.md:2030-2041)
+    m = np.full(256, vecio.pack(-32768, -32768), np.uint32)
+    R40, sf40 = orc.cov(m.reshape(256, 1, 1), 1, 256, 1, acc_w=40)
+    assert R40[0] == 2**39 - 1 and (sf40[0] & 3) == 2
+    R40b, sf40b = orc.cov(m[:255].reshape(255, 1, 1), 1, 255, 1, acc_w=40)
+    assert R40b[0] == 255 * 2**31 and sf40b[0] == 0
+
+
+CA = [1, 0, 0, 2, 2, 8, 8, 0x100]
+
+
+def test_cfar_flat_spectrum_alpha_edges(orc):
+    n = 64
+    flat = np.full(n, 1000, np.uint64)
+    cfg = list(CA)
+    ev, rc = orc.cfar(flat, n, cfg, 31, 63)
+    assert rc[0] == 1  # summary only
+    cfg[7] = 0xFF
+    ev, rc = orc.cfar(flat, n, cfg, 31, 63)
+    evaluable = n - 2 * (2 + 8)
+    assert rc[0] == evaluable + 1
+    zero = np.zeros(n, np.uint64)
+    for a in (0, 1, 0xFFFF):
+        cfg[7] = a
+        _, rc = orc.cfar(zero, n, cfg, 31, 63)
+        assert rc[0] == 1
+
+
+def test_cfar_planted_side_swap_is_caught(orc, ref):
+    rng = np.random.default_rng(12)
+    n = 128
+    bins = rng.integers(0, 2**20, n).astype(np.uint64)
+    bins[[10, 40, 90]] = 2**28
+    cfg = [1, 1, 0, 2, 3, 8, 4, 0x300]  # GO, asymmetric geometry
+    e_ref, _ = ref.cfar(bins, n, cfg, 31, 63)
+    orc.lib.rdo_set_fault(5)
+    try:
+        e_bad, _ = orc.cfar(bins, n, cfg, 31, 63)
+    finally:
+        orc.lib.rdo_set_fault(0)
+    e_ok, _ = orc.cfar(bins, n, cfg, 31, 63)
+    assert np.array_equal(e_ok, e_ref)
+    assert e_bad.shape != e_ref.shape or not np.array_equal(e_bad, e_ref)
+
+
+def test_event_pack_round_trip(orc):
+    rng = np.random.default_rng(13)
+    widths = [2, 16, 16, 7, 16, 16, 16, 40, 47]
+    lsbs = [0, 2, 18, 34, 41, 57, 73, 89, 129]
+    assert lsbs[-1] + widths[-1] == 176
+    for _ in range(200):
+        f = [int(rng.integers(0, 2**w)) for w in widths] + [int(rng.integers(0, 2)), int(rng.integers(0, 2))]
+        w = orc.event_pack(np.array(f, np.uint64))
+        big = int(w[0]) | (int(w[1]) << 64) | (int(w[2]) << 128)
+        for v, lsb, wd in zip(f, lsbs, widths):
+            assert (big >> lsb) & ((1 << wd) - 1) == v
+        assert (big >> 176) & 1 == f[9] and (big >> 177) & 1 == f[10]
+        assert big >> 178 == 0
+
+
+def test_event_detect_recheck(orc):
+    """Re-running C*N*2^8 > A*S on unpacked DETECT fields reproduces the decision."""
+    rng = np.random.default_rng(14)
+    n = 256
+    bins = rng.integers(0, 2**24, n).astype(np.uint64)
+    bins[rng.integers(0, n, 12)] = 2**30
+    ev, rc = orc.cfar(bins, n, [1, 0, 0, 2, 2, 16, 16, 0x400], 31, 63)
+    for w in ev:
+        big = int(w[0]) | (int(w[1]) << 64) | (int(w[2]) << 128)
+        kind = big & 3
+        if kind != 0:
+            continue
+        refc = (big >> 34) & 0x7F
+        alpha = (big >> 41) & 0xFFFF
+        cut = (big >> 89) & ((1 << 40) - 1)
+        s = (big >> 129) & ((1 << 47) - 1)
+        assert (cut * refc) << 8 > alpha * s

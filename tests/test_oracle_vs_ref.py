@@ -1,0 +1,129 @@
+"""The CPU restatement against the reference's own header models (oracle/_ref).
+
+Skipped where the reference tree was absent at build time (e.g. on a GPU box
+that received no prebuilt oracle/_ref).  Covers every stage on seeded random
+and corner-value inputs inside the reference's domain (FFT N <= 1024), and the
+composed chain on the tiny workload and a frame-reduced medium workload.
+"""
+import numpy as np
+import pytest
+
+import rd_scenario
+import stim
+
+
+def test_fxp_ops_random(orc, ref):
+    rng = np.random.default_rng(21)
+    ops = ["sat", "trunc", "round_even", "round_half_up", "round", "round_sat", "add_sat", "sub_sat", "neg_sat",
+           "mul_q15", "mul_q15_rs", "cmul_re", "cmul_im", "acc_w", "growth_bits"]
+    for op in ops:
+        for _ in range(300):
+            a = int(rng.integers(-(2**40), 2**40))
+            b = int(rng.integers(-(2**40), 2**40))
+            sh = int(rng.integers(0, 40))
+            w = int(rng.integers(2, 64))
+            assert orc.fxp_op(op, a, b, sh, w) == ref.fxp_op(op, a, b, sh, w), (op, a, b, sh, w)
+
+
+def test_cmult_random_and_corners(orc, ref):
+    rng = np.random.default_rng(22)
+    vals = np.concatenate([stim.rand_cx(rng, 2000), stim.rand_extreme(rng, 2000)])
+    for a, b in zip(vals[::2].tolist(), vals[1::2].tolist()):
+        assert orc.cmult(a, b) == ref.cmult(a, b)
+
+
+@pytest.mark.parametrize("P,T", [(4, 8), (64, 8), (1024, 16), (16, 1), (8, 3)])
+def test_pfb_vs_ref(orc, ref, P, T):
+    rng = np.random.default_rng(P * 100 + T)
+    nb = 20
+    for x, h in [(stim.rand_cx(rng, 2 * nb * P), stim.rand_cx(rng, P * T)),
+                 (stim.rand_extreme(rng, 2 * nb * P), stim.rand_extreme(rng, P * T))]:
+        yo, fo = orc.pfb(x, h, P, T, 2, nb)
+        yr, fr = ref.pfb(x, h, P, T, 2, nb)
+        assert np.array_equal(yo, yr) and np.array_equal(fo, fr)
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("spc", [1, 2])
+def test_fft_vs_ref(orc, ref, n, spc):
+    if n // spc < 4:
+        pytest.skip("invalid geometry")
+    rng = np.random.default_rng(n * 3 + spc)
+    frames = 6
+    xs = [stim.rand_cx(rng, frames * n), stim.rand_extreme(rng, frames * n), stim.rand_cx(rng, frames * n, 3000)]
+    for x in xs:
+        for sched in (0xFFFFFFFF, 0, int(rng.integers(0, 2**32)), 0x3):
+            for reorder in (0, 1):
+                o = orc.fft(x, n, spc, sched, reorder)
+                r = ref.fft(x, n, spc, sched, reorder)
+                for a, b in zip(o, r):
+                    assert np.array_equal(a, b), (n, spc, hex(sched), reorder)
+
+
+@pytest.mark.parametrize("CH,B", [(4, 4), (16, 16), (7, 3), (64, 8)])
+def test_bf_vs_ref(orc, ref, CH, B):
+    rng = np.random.default_rng(CH * 10 + B)
+    cols = 96
+    for Xa, W in [(stim.rand_cx(rng, cols * CH), stim.rand_cx(rng, B * CH)),
+                  (stim.rand_extreme(rng, cols * CH), stim.rand_extreme(rng, B * CH))]:
+        o = orc.bf(Xa, W, CH, B)
+        r = ref.bf(Xa, W, CH, B)
+        for a, b in zip(o, r):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("acc_w", [64, 40, 36])
+def test_cov_vs_ref(orc, ref, acc_w):
+    rng = np.random.default_rng(acc_w)
+    CH, F, N = 5, 40, 6
+    for Xa in (stim.rand_cx(rng, F * N * CH), stim.rand_extreme(rng, F * N * CH)):
+        Ro, so = orc.cov(Xa, CH, F, N, acc_w)
+        Rr, sr = ref.cov(Xa, CH, F, N, acc_w)
+        assert np.array_equal(Ro, Rr) and np.array_equal(so, sr)
+
+
+def test_power_cross_vs_ref(orc, ref):
+    rng = np.random.default_rng(23)
+    v = np.concatenate([stim.rand_cx(rng, 500), stim.rand_extreme(rng, 500)]).tolist()
+    for a, b in zip(v[::2], v[1::2]):
+        assert orc.power(a) == ref.power(a)
+        assert orc.cross_power(a, b) == ref.cross_power(a, b)
+
+
+CFAR_CFGS = [
+    [1, 0, 0, 2, 2, 8, 8, 0x800],
+    [1, 1, 0, 2, 3, 8, 4, 0x300],   # GO, asymmetric
+    [1, 0, 1, 1, 1, 4, 4, 0x200],   # dense
+    [1, 1, 1, 0, 0, 1, 0, 0x100],   # GO with an empty side: unusable geometry
+    [0, 0, 0, 2, 2, 8, 8, 0x800],   # disabled
+    [1, 0, 0, 40, 2, 80, 8, 0xFFFF],  # clamped geometry
+    [1, 0, 0, 4, 4, 32, 32, 0x0800],
+]
+
+
+@pytest.mark.parametrize("cfg", CFAR_CFGS)
+def test_cfar_vs_ref(orc, ref, cfg):
+    rng = np.random.default_rng(sum(cfg))
+    n, rows = 256, 6
+    bins = rng.integers(0, 2**31, rows * n).astype(np.uint64)
+    bins[rng.integers(0, rows * n, 40)] = 2**39
+    bins[rng.integers(0, rows * n, 10)] = (1 << 63) + 12345  # u64 wrap-around paths
+    eo, ro = orc.cfar(bins, n, cfg, 31, 63, frame_ids=np.arange(rows) * 7)
+    er, rr = ref.cfar(bins, n, cfg, 31, 63, frame_ids=np.arange(rows) * 7)
+    assert np.array_equal(ro, rr) and np.array_equal(eo, er)
+
+
+@pytest.mark.parametrize("name,frames", [("tiny", None), ("medium", 12)])
+def test_chain_vs_ref(orc, ref, name, frames):
+    sc = rd_scenario.config(name)
+    F = frames or sc.F
+    sc.F = F
+    iq = sc.iq()
+    args = (iq, sc.taps(), sc.weights(), sc.CH, sc.N, sc.T, F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
+            rd_scenario.MAX_REF)
+    keep = ("u", "pfb_flags", "X", "fft_flags", "Xa", "y", "yflags", "power", "R", "events")
+    o = orc.chain(*args, keep=keep, threads=8)
+    r = ref.chain(*args, keep=keep, threads=8)
+    for k in keep + ("row_count",):
+        assert np.array_equal(o[k], r[k]), k
+    assert o["n_events"] == r["n_events"] >= sc.B * F
