@@ -1,0 +1,297 @@
+// rd_capi.hip — the C ABI (include/rd_amd.h): context, memory helpers,
+// argument validation, and the per-stage and composed-chain entry points.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+
+#include "rd_fxp.h"
+#include "rd_internal.h"
+
+struct rd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t* tw = nullptr;      // 4096-entry twiddle table (device)
+  void* prep = nullptr;        // beamformer weight digits
+  size_t prep_bytes = 0;
+  void* scratch = nullptr;     // CFAR event offsets
+  size_t scratch_bytes = 0;
+  uint64_t* total_dev = nullptr;
+  uint64_t* total_host = nullptr;  // pinned
+};
+
+namespace {
+
+#define RD_TRY(expr)                              \
+  do {                                            \
+    if ((expr) != hipSuccess) return RD_E_HIP;    \
+  } while (0)
+
+// The twiddle rule of model/python/gen_fft_twiddles.py:97-159 for a master table
+// of M = 4096 entries: W^e = (q15(cos), q15(-sin)) of 2*pi*e/M, evaluated on the
+// first quadrant and mapped to the others by exact sign/swap symmetry; q15 is
+// round-to-nearest-even of x * 2^15 then saturation to int16.
+int32_t q15_real(double x) {
+  const double v = nearbyint(x * 32768.0);
+  return v > 32767.0 ? 32767 : (v < -32768.0 ? -32768 : (int32_t)v);
+}
+
+void make_twiddles(uint32_t* out) {
+  const uint32_t M = 4096, q4 = M / 4;
+  for (uint32_t e = 0; e < M; ++e) {
+    const uint32_t quad = e / q4, rem = e % q4;
+    double c0 = 1.0, s0 = 0.0;
+    if (rem) {
+      const double th = 2.0 * M_PI * (double)rem / (double)M;
+      c0 = cos(th);
+      s0 = sin(th);
+    }
+    const double c = quad == 0 ? c0 : quad == 1 ? -s0 : quad == 2 ? -c0 : s0;
+    const double s = quad == 0 ? s0 : quad == 1 ? c0 : quad == 2 ? -s0 : -c0;
+    out[e] = rd::cpack(q15_real(c), q15_real(-s));
+  }
+}
+
+const uint32_t* host_twiddles() {
+  static uint32_t table[4096];
+  static std::once_flag once;
+  std::call_once(once, [] { make_twiddles(table); });
+  return table;
+}
+
+rd_status grow(void** buf, size_t* have, size_t need) {
+  if (*have >= need) return RD_OK;
+  if (*buf) RD_TRY(hipFree(*buf));
+  *buf = nullptr;
+  *have = 0;
+  RD_TRY(hipMalloc(buf, need));
+  *have = need;
+  return RD_OK;
+}
+
+bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+rd_status enter(rd_ctx* ctx) {
+  if (!ctx) return RD_E_ARG;
+  RD_TRY(hipSetDevice(ctx->device));
+  return RD_OK;
+}
+
+rd_status fin(hipError_t e) { return e == hipSuccess ? RD_OK : RD_E_HIP; }
+
+rd_status check_cpi(const rd_cpi_desc* d) {
+  if (!d) return RD_E_ARG;
+  if (d->n_chan < 1 || d->n_chan > 64 || d->n_beams < 1 || d->n_beams > 128) return RD_E_ARG;
+  if (d->taps < 1 || d->taps > 64 || d->n_frames < 1 || d->n_frames > 65535) return RD_E_ARG;
+  if (!pow2(d->n_fft) || d->n_fft < 8 || d->n_fft > 4096 || (d->spc != 1 && d->spc != 2)) return RD_E_GEOM;
+  if (d->n_fft / d->spc < 4) return RD_E_GEOM;
+  return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* rd_status_str(rd_status s) {
+  switch (s) {
+    case RD_OK: return "ok";
+    case RD_E_ARG: return "invalid argument";
+    case RD_E_GEOM: return "geometry rejected";
+    case RD_E_HIP: return "HIP runtime error";
+    case RD_E_CAPACITY: return "event buffer too small";
+    case RD_E_NODEV: return "no such device";
+  }
+  return "unknown";
+}
+
+rd_status rd_create(int device, rd_ctx** out) {
+  if (!out) return RD_E_ARG;
+  *out = nullptr;
+  if (device < 0 || device >= rd_device_count()) return RD_E_NODEV;
+  rd_ctx* c = new (std::nothrow) rd_ctx();
+  if (!c) return RD_E_ARG;
+  c->device = device;
+  rd_status st = RD_OK;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->tw, 4096 * sizeof(uint32_t)) != hipSuccess ||
+      hipMemcpy(c->tw, host_twiddles(), 4096 * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&c->total_dev, sizeof(uint64_t)) != hipSuccess ||
+      hipHostMalloc(&c->total_host, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+    st = RD_E_HIP;
+  if (st != RD_OK) {
+    rd_destroy(c);
+    return st;
+  }
+  *out = c;
+  return RD_OK;
+}
+
+rd_status rd_destroy(rd_ctx* c) {
+  if (!c) return RD_E_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->tw) (void)hipFree(c->tw);
+  if (c->prep) (void)hipFree(c->prep);
+  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->total_dev) (void)hipFree(c->total_dev);
+  if (c->total_host) (void)hipHostFree(c->total_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RD_OK;
+}
+
+void* rd_stream(rd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+rd_status rd_sync(rd_ctx* c) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  return fin(hipStreamSynchronize(c->stream));
+}
+
+rd_status rd_malloc(rd_ctx* c, size_t bytes, void** dptr) {
+  rd_status st = enter(c);
+  if (st != RD_OK || !dptr) return st != RD_OK ? st : RD_E_ARG;
+  return fin(hipMalloc(dptr, bytes ? bytes : 4));
+}
+
+rd_status rd_free(rd_ctx* c, void* dptr) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  return fin(hipFree(dptr));
+}
+
+rd_status rd_h2d(rd_ctx* c, void* dst, const void* src, size_t bytes) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  RD_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return fin(hipStreamSynchronize(c->stream));
+}
+
+rd_status rd_d2h(rd_ctx* c, void* dst, const void* src, size_t bytes) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  RD_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return fin(hipStreamSynchronize(c->stream));
+}
+
+rd_status rd_memset(rd_ctx* c, void* dst, int value, size_t bytes) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  return fin(hipMemsetAsync(dst, value, bytes, c->stream));
+}
+
+rd_status rd_twiddle_table(uint32_t* host_out4096) {
+  if (!host_out4096) return RD_E_ARG;
+  memcpy(host_out4096, host_twiddles(), 4096 * sizeof(uint32_t));
+  return RD_OK;
+}
+
+rd_status rd_pfb_run(rd_ctx* c, const uint32_t* x, const uint32_t* coeff, uint32_t phases, uint32_t taps,
+                     uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!x || !coeff || !y || phases < 1 || taps < 1 || taps > 64 || n_chan > 65535) return RD_E_ARG;
+  return fin(rd::launch_pfb(c->stream, x, coeff, phases, taps, n_chan, n_beats, hist, y, flags));
+}
+
+rd_status rd_fft_run(rd_ctx* c, const uint32_t* in, uint32_t n_fft, uint32_t spc, uint32_t scale_sched, int reorder,
+                     uint32_t n_frames, uint32_t* out, uint32_t* bins, uint32_t* stage_flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!in || ((uintptr_t)in & 15u)) return RD_E_ARG;
+  if (!pow2(n_fft) || n_fft < 8 || n_fft > 4096 || (spc != 1 && spc != 2) || n_fft / spc < 4) return RD_E_GEOM;
+  return fin(rd::launch_fft(c->stream, c->tw, in, n_fft, spc, scale_sched, reorder, n_frames, out, bins,
+                            stage_flags));
+}
+
+rd_status rd_align_run(rd_ctx* c, const uint32_t* X, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                       uint32_t* Xa) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!X || !Xa || n_chan < 1 || n_chan > 64) return RD_E_ARG;
+  return fin(rd::launch_align(c->stream, X, n_chan, (uint64_t)n_frames * n_bins, Xa));
+}
+
+rd_status rd_bf_run(rd_ctx* c, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                    uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
+  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(n_ant, n_beams));
+  if (st != RD_OK) return st;
+  return fin(rd::launch_bf(c->stream, Xa, W, n_ant, n_beams, n_cols, y, flags, power, acc, c->prep));
+}
+
+rd_status rd_cov_run(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                     int64_t* R) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !R || n_chan < 1 || n_chan > 64 || n_frames < 1 || n_frames > 65535) return RD_E_ARG;
+  return fin(rd::launch_cov(c->stream, Xa, n_chan, n_frames, n_bins, R));
+}
+
+rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
+                      const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
+                      uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
+                      uint64_t* n_events_host) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!cells || !cfg || !row_count || (cell_bytes != 4 && cell_bytes != 8) || n_bins < 1 || n_bins > 8191)
+    return RD_E_ARG;
+  if (!events && events_cap) return RD_E_ARG;
+  st = grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_rows));
+  if (st != RD_OK) return st;
+  RD_TRY(rd::launch_cfar(c->stream, cells, cell_bytes, n_bins, n_rows, *cfg, max_guard, max_ref, frame_ids,
+                         frame_mod, events, events_cap, row_count, c->scratch, c->total_dev));
+  RD_TRY(hipMemcpyAsync(c->total_host, c->total_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  RD_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = n_rows ? *c->total_host : 0;
+  if (n_events_host) *n_events_host = tot;
+  return tot > events_cap ? RD_E_CAPACITY : RD_OK;
+}
+
+rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  st = check_cpi(d);
+  if (st != RD_OK) return st;
+  if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
+    return RD_E_ARG;
+  if ((uintptr_t)b->u & 15u) return RD_E_ARG;
+  const uint32_t CH = d->n_chan, N = d->n_fft, F = d->n_frames, B = d->n_beams;
+  const uint64_t cols = (uint64_t)F * N;
+  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(CH, B));
+  if (st != RD_OK) return st;
+  st = grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(B * F));
+  if (st != RD_OK) return st;
+  hipStream_t s = c->stream;
+  RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
+  RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
+  RD_TRY(rd::launch_align(s, b->X, CH, cols, b->Xa));
+  RD_TRY(rd::launch_bf(s, b->Xa, b->weights, CH, B, cols, b->y, b->yflags, b->power, nullptr, c->prep));
+  if (b->R) RD_TRY(rd::launch_cov(s, b->Xa, CH, F, N, b->R));
+  RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
+                         b->events ? b->events_cap : 0, b->row_count, c->scratch,
+                         n_events_dev ? n_events_dev : c->total_dev));
+  return RD_OK;
+}
+
+rd_status rd_chain_run(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_host) {
+  rd_status st = rd_chain_enqueue(c, d, b, c ? c->total_dev : nullptr);
+  if (st != RD_OK) return st;
+  RD_TRY(hipMemcpyAsync(c->total_host, c->total_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  RD_TRY(hipStreamSynchronize(c->stream));
+  if (n_events_host) *n_events_host = *c->total_host;
+  return (b->events && *c->total_host > b->events_cap) ? RD_E_CAPACITY : RD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+// rd_cfar.hip — CA/GO-CFAR with ordered event compaction and 176-bit packing.
+//
+// Semantics of cfar::run_frame (model/cpp/cfar/cfar_model.hpp:243-340):
+//   * geometry clamped to the elaboration maxima (cfar_model.hpp:134-141);
+//   * lead reference cells at higher bin indices, lag at lower, guards skipped;
+//     a bin whose full window does not fit is suppressed;
+//   * CA: S = lead + lag, N = ref_lead + ref_lag; GO: the greater mean by one
+//     cross-multiply, ties to lead;
+//   * detect iff (C * N) << 8 > alpha * S in u64 (strict);
+//   * events in bin order, then one SUMMARY{det, sup, bin = n}; sof on the first
+//     event, eof on the summary.
+//
+// The window sums come from an exclusive prefix sum of the row in LDS, so every
+// bin costs two differences whatever the reference length; u64 wrap-around of
+// the prefix differences equals the reference's u64 running sums modulo 2^64.
+//
+// Three launches: per-row counts, a scan of counts over rows into event
+// offsets, and the per-row emit with an in-block ordered compaction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rd_internal.h"
+
+namespace rd {
+
+struct CfarGeom {
+  uint32_t enable, go, dense, gl, gg, rl, rg, alpha, refs_ok;
+};
+
+__host__ __device__ inline CfarGeom clamp_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
+  CfarGeom g;
+  g.enable = c.enable != 0;
+  g.go = c.mode == 1;
+  g.dense = c.out_mode == 1;
+  g.gl = c.guard_lead < max_guard ? c.guard_lead : max_guard;
+  g.gg = c.guard_lag < max_guard ? c.guard_lag : max_guard;
+  g.rl = c.ref_lead < max_ref ? c.ref_lead : max_ref;
+  g.rg = c.ref_lag < max_ref ? c.ref_lag : max_ref;
+  g.alpha = c.alpha;
+  g.refs_ok = g.go ? (g.rl != 0 && g.rg != 0) : (g.rl + g.rg) != 0;
+  return g;
+}
+
+constexpr int kCfarThreads = 256;
+
+// Block-wide exclusive scan of one uint64 per thread; returns the block total.
+__device__ uint64_t block_exscan(uint64_t v, uint64_t* out_prefix, uint64_t* wsum /* [kCfarThreads/64] */) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, total = 0;
+  for (uint32_t w = 0; w < kCfarThreads / 64; ++w) {
+    if (w < wave) before += wsum[w];
+    total += wsum[w];
+  }
+  *out_prefix = before + x - v;
+  __syncthreads();
+  return total;
+}
+
+struct Decision {
+  bool hit, eval;
+  uint64_t cut, sum;
+  uint32_t n;
+};
+
+__device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32_t i, const CfarGeom& g) {
+  Decision d;
+  d.cut = P[i + 1] - P[i];
+  const bool lead_ok = g.rl == 0 || (uint64_t)i + g.gl + g.rl < n;
+  const bool lag_ok = g.rg == 0 || i >= g.gg + g.rg;
+  const bool ok = lead_ok && lag_ok;
+  uint64_t slead = 0, slag = 0;
+  if (ok) {
+    if (g.rl) slead = P[i + g.gl + g.rl + 1] - P[i + g.gl + 1];
+    if (g.rg) slag = P[i - g.gg] - P[i - g.gg - g.rg];
+  }
+  if (g.go) {
+    const bool lead = slead * g.rg >= slag * g.rl;
+    d.sum = lead ? slead : slag;
+    d.n = lead ? g.rl : g.rg;
+  } else {
+    d.sum = slead + slag;
+    d.n = g.rl + g.rg;
+  }
+  d.eval = ok && g.enable && g.refs_ok;
+  d.hit = d.eval && ((d.cut * (uint64_t)d.n) << 8) > (uint64_t)g.alpha * d.sum;
+  return d;
+}
+
+// Loads the row into LDS as an exclusive prefix P[0..n]; each thread owns a
+// contiguous chunk of `per` bins starting at t*per (read twice from global
+// memory: once for the chunk total, once to write the prefixes).
+template <typename Cell>
+__device__ void load_prefix(const Cell* row, uint32_t n, uint32_t per, uint64_t* P, uint64_t* wsum) {
+  const uint32_t b0 = threadIdx.x * per;
+  uint64_t local = 0;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i < n) local += (uint64_t)row[i];
+  }
+  uint64_t before;
+  const uint64_t total = block_exscan(local, &before, wsum);
+  uint64_t run = before;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i < n) {
+      P[i] = run;
+      run += (uint64_t)row[i];
+    }
+  }
+  if (threadIdx.x == 0) P[n] = total;
+  __syncthreads();
+}
+
+template <typename Cell>
+__global__ __launch_bounds__(kCfarThreads) void cfar_count_kernel(const Cell* __restrict__ cells, uint32_t n,
+                                                                  CfarGeom g, uint32_t* __restrict__ row_count) {
+  extern __shared__ uint64_t P[];  // [n + 1]
+  __shared__ uint64_t wsum[kCfarThreads / 64];
+  const uint32_t r = blockIdx.x;
+  const uint32_t per = (n + kCfarThreads - 1) / kCfarThreads;
+  load_prefix(cells + (size_t)r * n, n, per, P, wsum);
+  uint64_t det = 0;
+  const uint32_t b0 = threadIdx.x * per;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i < n) det += decide(P, n, i, g).hit ? 1 : 0;
+  }
+  uint64_t before;
+  const uint64_t tot = block_exscan(det, &before, wsum);
+  if (threadIdx.x == 0) row_count[r] = (uint32_t)((g.dense ? n : tot) + 1);
+}
+
+// Exclusive scan of row counts into 64-bit offsets; offs[n_rows] = total.
+__global__ __launch_bounds__(1024) void cfar_scan_kernel(const uint32_t* __restrict__ row_count, uint32_t n_rows,
+                                                         uint64_t* __restrict__ offs, uint64_t* total_out) {
+  __shared__ uint64_t part[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (n_rows + 1023) / 1024;
+  const uint32_t r0 = t * per;
+  uint64_t s = 0;
+  for (uint32_t q = 0; q < per; ++q)
+    if (r0 + q < n_rows) s += row_count[r0 + q];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+    const uint64_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[t] - s;
+  for (uint32_t q = 0; q < per; ++q) {
+    if (r0 + q < n_rows) {
+      offs[r0 + q] = run;
+      run += row_count[r0 + q];
+    }
+  }
+  if (t == 1023) {
+    offs[n_rows] = part[1023];
+    if (total_out) *total_out = part[1023];
+  }
+}
+
+// 176-bit event, LSB first: kind 2 | bin 16 | frame 16 | ref 7 | alpha 16 |
+// det 16 | sup 16 | cut 40 | sum 47 (cfar_model.hpp:75-84); sof bit 176, eof 177.
+__device__ __forceinline__ void put_event(uint64_t* w, uint32_t kind, uint32_t bin, uint32_t frame, uint32_t ref,
+                                          uint32_t alpha, uint32_t det, uint32_t sup, uint64_t cut, uint64_t sum,
+                                          bool sof, bool eof) {
+  const uint64_t det16 = det & 0xFFFFu;
+  const uint64_t cut40 = cut & ((1ull << 40) - 1);
+  const uint64_t sum47 = sum & ((1ull << 47) - 1);
+  w[0] = (uint64_t)(kind & 3u) | ((uint64_t)(bin & 0xFFFFu) << 2) | ((uint64_t)(frame & 0xFFFFu) << 18) |
+         ((uint64_t)(ref & 0x7Fu) << 34) | ((uint64_t)(alpha & 0xFFFFu) << 41) | (det16 << 57);
+  w[1] = (det16 >> 7) | ((uint64_t)(sup & 0xFFFFu) << 9) | (cut40 << 25);
+  w[2] = (cut40 >> 39) | (sum47 << 1) | ((uint64_t)sof << 48) | ((uint64_t)eof << 49);
+}
+
+template <typename Cell>
+__global__ __launch_bounds__(kCfarThreads) void cfar_emit_kernel(const Cell* __restrict__ cells, uint32_t n,
+                                                                 CfarGeom g, const uint32_t* __restrict__ frame_ids,
+                                                                 uint32_t frame_mod, const uint64_t* __restrict__ offs,
+                                                                 uint64_t* __restrict__ events, uint64_t cap) {
+  extern __shared__ uint64_t P[];
+  __shared__ uint64_t wsum[kCfarThreads / 64];
+  const uint32_t r = blockIdx.x;
+  const uint32_t per = (n + kCfarThreads - 1) / kCfarThreads;
+  load_prefix(cells + (size_t)r * n, n, per, P, wsum);
+  const uint32_t frame = frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
+  const uint32_t b0 = threadIdx.x * per;
+  uint64_t nemit = 0, ndet = 0, nsup = 0;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i >= n) break;
+    const Decision d = decide(P, n, i, g);
+    ndet += d.hit;
+    nsup += !d.eval;
+    nemit += (d.hit || g.dense) ? 1 : 0;
+  }
+  uint64_t before, bdet, bsup;
+  const uint64_t emitted = block_exscan(nemit, &before, wsum);
+  const uint64_t tdet = block_exscan(ndet, &bdet, wsum);
+  const uint64_t tsup = block_exscan(nsup, &bsup, wsum);
+  const uint64_t base = offs[r];
+  uint64_t k = before;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i >= n) break;
+    const Decision d = decide(P, n, i, g);
+    if (!(d.hit || g.dense)) continue;
+    const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
+    if (base + k < cap)
+      put_event(events + 3 * (base + k), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
+                d.eval ? d.sum : 0ull, k == 0, false);
+    ++k;
+  }
+  if (threadIdx.x == 0 && base + emitted < cap)
+    put_event(events + 3 * (base + emitted), 3u, n, frame, 0, g.alpha, (uint32_t)tdet, (uint32_t)tsup, 0, 0,
+              emitted == 0, true);
+}
+
+size_t cfar_scratch_bytes(uint32_t n_rows) { return ((size_t)n_rows + 1) * sizeof(uint64_t); }
+
+hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n, uint32_t n_rows,
+                       const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
+                       uint32_t frame_mod, uint64_t* events, uint64_t cap, uint32_t* row_count, void* scratch,
+                       uint64_t* total_dev) {
+  if (n_rows == 0) return hipSuccess;
+  const CfarGeom g = clamp_geom(cfg, max_guard, max_ref);
+  const size_t lds = ((size_t)n + 1) * sizeof(uint64_t);
+  uint64_t* offs = static_cast<uint64_t*>(scratch);
+  if (cell_bytes == 4)
+    cfar_count_kernel<uint32_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint32_t*>(cells), n, g,
+                                                                  row_count);
+  else
+    cfar_count_kernel<uint64_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint64_t*>(cells), n, g,
+                                                                  row_count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  cfar_scan_kernel<<<1, 1024, 0, s>>>(row_count, n_rows, offs, total_dev);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (cell_bytes == 4)
+    cfar_emit_kernel<uint32_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint32_t*>(cells), n, g,
+                                                                 frame_ids, frame_mod, offs, events, cap);
+  else
+    cfar_emit_kernel<uint64_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint64_t*>(cells), n, g,
+                                                                 frame_ids, frame_mod, offs, events, cap);
+  return hipGetLastError();
+}
+
+}  // namespace rd

@@ -1,0 +1,395 @@
+// rd_mfma.hip — the two dense contractions of the chain on gfx950 matrix cores:
+// the complex beamformer (+ power) and the Hermitian covariance.
+//
+// Both are exact int16 x int16 sums computed from int8 partial products with
+// v_mfma_i32_16x16x32_i8 and recombined in int64.
+//
+// Digit form. Every int16 value a is written a = 256*h + l + 128 with
+//   h = a >> 8        in [-128, 127]
+//   l = (a & 255) - 128 in [-128, 127]
+// so both digits are int8. For K-term sums of products a_k * b_k:
+//   sum a*b = 65536*HH + 256*(HL + LH) + LL + 128*(Sa + Sb) - 16384*K
+// where HH = sum h_a h_b etc. are four int8 GEMMs with int32 accumulation
+// (|HH| <= K * 2^14, no overflow for K < 2^17) and Sa, Sb are the plain sums of
+// the operands over k. Padding entries carry zero digits and are not counted.
+//
+// Negation without overflow. -x of an int16 can be +32768; instead
+// -x = ~x + 1, and the digits of ~x are the bitwise complements of the digits of
+// x (~x = 256*(~h) + (~l) + 128). A negated operand therefore costs one NOT and
+// a constant correction.
+//
+// Operand layout of v_mfma_i32_16x16x32_i8 (per lane l, 8 int8 per operand):
+//   A[i][k]: i = l % 16, k = 8*(l/16) + 0..7
+//   B[k][j]: j = l % 16, k = 8*(l/16) + 0..7
+//   C[i][j]: j = l % 16, i = 4*(l/16) + r for register r = 0..3
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rd_fxp.h"
+#include "rd_internal.h"
+
+namespace rd {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4i mfma8(uint64_t a, uint64_t b, v4i c) {
+  return __builtin_amdgcn_mfma_i32_16x16x32_i8((long)a, (long)b, c, 0, 0, 0);
+}
+
+// Bytes `byte` of four words, gathered into one word (word q -> byte q).
+__device__ __forceinline__ uint32_t gather4(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, uint32_t byte) {
+  const uint32_t s01 = 0x0C0C0000u | ((byte + 4) << 8) | byte;        // [p0.b, p1.b, 0, 0]
+  const uint32_t s23 = ((byte + 4) << 24) | (byte << 16) | 0x0C0Cu;   // [0, 0, p2.b, p3.b]
+  return __builtin_amdgcn_perm(p1, p0, s01) | __builtin_amdgcn_perm(p3, p2, s23);
+}
+
+// Digits of 8 packed complex values: H/L of the real part (bytes 1/0) and of the
+// imaginary part (bytes 3/2), one int8 per value. `vmask` zeroes padded entries.
+struct Dig8 {
+  uint64_t hr, lr, hi, li;
+};
+__device__ __forceinline__ Dig8 digits8(const uint32_t* p, uint64_t vmask) {
+  const uint64_t hr = (uint64_t)gather4(p[0], p[1], p[2], p[3], 1) | ((uint64_t)gather4(p[4], p[5], p[6], p[7], 1) << 32);
+  const uint64_t lr = (uint64_t)gather4(p[0], p[1], p[2], p[3], 0) | ((uint64_t)gather4(p[4], p[5], p[6], p[7], 0) << 32);
+  const uint64_t hi = (uint64_t)gather4(p[0], p[1], p[2], p[3], 3) | ((uint64_t)gather4(p[4], p[5], p[6], p[7], 3) << 32);
+  const uint64_t li = (uint64_t)gather4(p[0], p[1], p[2], p[3], 2) | ((uint64_t)gather4(p[4], p[5], p[6], p[7], 2) << 32);
+  const uint64_t flip = 0x8080808080808080ull;
+  return Dig8{hr & vmask, (lr ^ flip) & vmask, hi & vmask, (li ^ flip) & vmask};
+}
+
+__device__ __forceinline__ int64_t combine(v4i hh, v4i x, v4i ll, int r) {
+  return (int64_t)hh[r] * 65536 + (int64_t)x[r] * 256 + (int64_t)ll[r];
+}
+
+// ============================================================================
+// Beamformer: y[b][col] = round_sat(sum_a x[col][a] * W[b][a], 15, 16), no
+// conjugate (beamformer_model.hpp:162-179), and p = yr^2 + yi^2
+// (covar_model.hpp:99-102). As two real GEMMs over K = 2*CH:
+//   Im: A = [Wi | Wr], B = [Xr ; Xi]
+//   Re: A = [Wr | Wi], B = [Xr ; ~Xi], plus sum_a Wi   (-wi*xi = wi*~xi + wi)
+// K index k < CHp is the real part of channel k, k >= CHp the imaginary part of
+// channel k - CHp (CHp = CH rounded up to 16).
+// ============================================================================
+
+// Prepared weights: per beam tile, step, lane: {Are_h, Are_l, Aim_h, Aim_l};
+// then per beam: Sa[b] = sum_a (wr + wi), Wis[b] = sum_a wi.
+struct BfPrep {
+  static __host__ __device__ size_t frag_words(uint32_t bt, uint32_t steps) { return (size_t)bt * steps * 64 * 8; }
+};
+
+size_t bf_prep_bytes(uint32_t n_ant, uint32_t n_beams) {
+  const uint32_t chp = (n_ant + 15) / 16 * 16, steps = chp / 16, bt = (n_beams + 15) / 16;
+  return BfPrep::frag_words(bt, steps) * 4 + (size_t)bt * 16 * 2 * 4;
+}
+
+__global__ void bf_prep_kernel(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
+                               uint32_t steps, uint32_t bt_n, uint32_t* __restrict__ prep) {
+  const uint32_t lane = threadIdx.x;  // 64
+  const uint32_t s = blockIdx.x, bt = blockIdx.y;
+  const uint32_t b = bt * 16 + lane % 16;
+  const uint32_t k0 = 32 * s + 8 * (lane / 16);
+  const bool imag = k0 >= chp;
+  const uint32_t a0 = imag ? k0 - chp : k0;
+  uint32_t p[8];
+  uint64_t vmask = 0;
+  for (int t = 0; t < 8; ++t) {
+    const bool ok = b < B && a0 + t < CH;
+    p[t] = ok ? W[(size_t)b * CH + a0 + t] : 0u;
+    if (ok) vmask |= 0xFFull << (8 * t);
+  }
+  const Dig8 d = digits8(p, vmask);
+  // Are = [Wr | Wi], Aim = [Wi | Wr]
+  const uint64_t re_h = imag ? d.hi : d.hr, re_l = imag ? d.li : d.lr;
+  const uint64_t im_h = imag ? d.hr : d.hi, im_l = imag ? d.lr : d.li;
+  uint32_t* o = prep + (((size_t)bt * steps + s) * 64 + lane) * 8;
+  o[0] = (uint32_t)re_h; o[1] = (uint32_t)(re_h >> 32);
+  o[2] = (uint32_t)re_l; o[3] = (uint32_t)(re_l >> 32);
+  o[4] = (uint32_t)im_h; o[5] = (uint32_t)(im_h >> 32);
+  o[6] = (uint32_t)im_l; o[7] = (uint32_t)(im_l >> 32);
+  if (s == 0 && lane < 16) {
+    int32_t sa = 0, wis = 0;
+    if (b < B)
+      for (uint32_t a = 0; a < CH; ++a) {
+        const uint32_t w = W[(size_t)b * CH + a];
+        sa += cre(w) + cim(w);
+        wis += cim(w);
+      }
+    int32_t* tail = reinterpret_cast<int32_t*>(prep + BfPrep::frag_words(bt_n, steps));
+    tail[2 * b] = sa;
+    tail[2 * b + 1] = wis;
+  }
+}
+
+constexpr int kBfColTilesPerWave = 4;
+
+template <int STEPS>
+__global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ Xa, const uint32_t* __restrict__ prep,
+                                                 uint32_t CH, uint32_t B, uint32_t bt_n, uint64_t cols,
+                                                 uint32_t* __restrict__ y, uint8_t* __restrict__ flags,
+                                                 uint32_t* __restrict__ power, int64_t* __restrict__ acc) {
+  constexpr uint32_t chp = STEPS * 16;
+  extern __shared__ uint32_t lds[];
+  const size_t fw = BfPrep::frag_words(bt_n, STEPS);
+  const size_t total = fw + (size_t)bt_n * 32;
+  for (size_t i = threadIdx.x; i < total; i += blockDim.x) lds[i] = prep[i];
+  __syncthreads();
+  const int32_t* tail = reinterpret_cast<const int32_t*>(lds + fw);
+
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+  const int64_t kconst = 16384LL * 2 * CH;
+
+  for (int t = 0; t < kBfColTilesPerWave; ++t) {
+    const uint64_t col0 = ((uint64_t)blockIdx.x * 4 * kBfColTilesPerWave + wave * kBfColTilesPerWave + t) * 16;
+    if (col0 >= cols) break;
+    const uint64_t col = col0 + cl;
+    const bool col_ok = col < cols;
+    // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
+    uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
+    int32_t ps_r = 0, ps_i = 0;
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const uint32_t k0 = 32 * s + 8 * grp;
+      const bool imag = k0 >= chp;
+      const uint32_t a0 = imag ? k0 - chp : k0;
+      uint32_t p[8];
+      uint64_t vmask = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool ok = col_ok && a0 + q < CH;
+        p[q] = ok ? Xa[col * CH + a0 + q] : 0u;
+        vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
+      }
+      const Dig8 d = digits8(p, vmask);
+      int32_t sr = 0, si = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        sr += cre(p[q]);
+        si += cim(p[q]);
+      }
+      if (imag) {
+        ps_i += si;
+        bih[s] = d.hi;
+        bil[s] = d.li;
+        brh[s] = ~d.hi & vmask;
+        brl[s] = ~d.li & vmask;
+      } else {
+        ps_r += sr;
+        bih[s] = d.hr;
+        bil[s] = d.lr;
+        brh[s] = d.hr;
+        brl[s] = d.lr;
+      }
+    }
+    // column sums over all channels: the four lane groups of a column
+    ps_r += __shfl_xor(ps_r, 16);
+    ps_r += __shfl_xor(ps_r, 32);
+    ps_i += __shfl_xor(ps_i, 16);
+    ps_i += __shfl_xor(ps_i, 32);
+    const int64_t sb_im = (int64_t)ps_r + ps_i;
+    const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
+
+    for (uint32_t bt = 0; bt < bt_n; ++bt) {
+      v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
+      v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < STEPS; ++s) {
+        const uint64_t* a = reinterpret_cast<const uint64_t*>(lds + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
+        const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
+        rhh = mfma8(are_h, brh[s], rhh);
+        rx = mfma8(are_h, brl[s], rx);
+        rx = mfma8(are_l, brh[s], rx);
+        rll = mfma8(are_l, brl[s], rll);
+        ihh = mfma8(aim_h, bih[s], ihh);
+        ix = mfma8(aim_h, bil[s], ix);
+        ix = mfma8(aim_l, bih[s], ix);
+        ill = mfma8(aim_l, bil[s], ill);
+      }
+      if (!col_ok) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t b = bt * 16 + 4 * grp + r;
+        if (b >= B) continue;
+        const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
+        const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
+        const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
+        uint32_t fr = 0, fi = 0;
+        const int32_t yr = sat16_i64(rne15_i64(are), fr);
+        const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+        const size_t o = (size_t)b * cols + col;
+        if (y) y[o] = cpack(yr, yi);
+        if (flags) flags[o] = (uint8_t)(fr | (fi << 2));
+        if (power) power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+        if (acc) {
+          acc[2 * o] = are;
+          acc[2 * o + 1] = aim;
+        }
+      }
+    }
+  }
+}
+
+hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                     uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc, void* prep) {
+  const uint32_t chp = (n_ant + 15) / 16 * 16, steps = chp / 16, bt = (n_beams + 15) / 16;
+  uint32_t* pw = static_cast<uint32_t*>(prep);
+  bf_prep_kernel<<<dim3(steps, bt), 64, 0, s>>>(W, n_ant, n_beams, chp, steps, bt, pw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || n_cols == 0) return e;
+  const uint64_t cols_per_block = 16ull * 4 * kBfColTilesPerWave;
+  const dim3 grid((uint32_t)((n_cols + cols_per_block - 1) / cols_per_block));
+  const size_t lds = bf_prep_bytes(n_ant, n_beams);
+  switch (steps) {
+    case 1: bf_kernel<1><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
+    case 2: bf_kernel<2><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
+    case 3: bf_kernel<3><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
+    default: bf_kernel<4><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
+  }
+  return hipGetLastError();
+}
+
+// ============================================================================
+// Covariance: R[k][i<=j] = sum_m cross_power(x_i[m][k], x_j[m][k]) with
+// cross_power(x, y) = x * conj(y) (covar_model.hpp:119-151), as two real GEMMs
+// per bin over K = 2F (frames, real then imaginary part):
+//   Re: A_i = [Xr_i | Xi_i],  B_j = [Xr_j | Xi_j]
+//   Im: A_i = [Xi_i | ~Xr_i], B_j = [Xr_j | Xi_j], plus sum_m xi_j
+// A block owns NB bins (NB * CHp = 64) and their upper-triangle 16x16 channel
+// tiles; each wave takes a few (bin, tile) jobs and keeps their six
+// accumulators in registers across all frames. Frames are staged through LDS
+// 16 at a time: one row [NB bins][CH] per frame is NB*CH*4 contiguous bytes.
+// ============================================================================
+constexpr int kCovMaxJobsPerWave = 4;
+
+__global__ __launch_bounds__(256) void cov_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
+                                                  uint32_t N, uint32_t nb, uint32_t nt, int64_t* __restrict__ R) {
+  __shared__ uint32_t tile[16 * 64];  // [16 frames][nb * CH]
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+  const uint32_t k_base = blockIdx.x * nb;
+  const uint32_t row_words = nb * CH;
+  const uint32_t ut = nt * (nt + 1) / 2;
+  const uint32_t jobs = nb * ut;
+
+  // job -> (bin, ti, tj), upper triangle ti <= tj
+  uint32_t jb[kCovMaxJobsPerWave], jti[kCovMaxJobsPerWave], jtj[kCovMaxJobsPerWave];
+  bool jok[kCovMaxJobsPerWave];
+#pragma unroll
+  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
+    const uint32_t job = wave + 4 * q;
+    jok[q] = job < jobs && k_base + job / ut < N;
+    const uint32_t bin = job / ut;
+    uint32_t u = job % ut, ti = 0;
+    while (u >= nt - ti) {
+      u -= nt - ti;
+      ++ti;
+    }
+    jb[q] = bin;
+    jti[q] = ti;
+    jtj[q] = ti + u;
+  }
+
+  v4i acc[kCovMaxJobsPerWave][6];
+  int32_t sra[kCovMaxJobsPerWave], sia[kCovMaxJobsPerWave], srb[kCovMaxJobsPerWave], sib[kCovMaxJobsPerWave];
+#pragma unroll
+  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
+    for (int z = 0; z < 6; ++z) acc[q][z] = v4i{0, 0, 0, 0};
+    sra[q] = sia[q] = srb[q] = sib[q] = 0;
+  }
+
+  const uint32_t bins_here = min(nb, N - k_base);
+  for (uint32_t m0 = 0; m0 < F; m0 += 16) {
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < 16 * row_words; idx += blockDim.x) {
+      const uint32_t f = idx / row_words, w = idx % row_words;
+      const uint32_t bin = w / CH;
+      tile[idx] = (m0 + f < F && bin < bins_here) ? Xa[((size_t)(m0 + f) * N + k_base) * CH + w] : 0u;
+    }
+    __syncthreads();
+    // this lane's 8 frames: f = 8*(grp & 1) + t; part = grp >> 1
+    const uint32_t f0 = 8 * (grp & 1);
+    uint64_t fmask = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) fmask |= (m0 + f0 + t < F) ? (0xFFull << (8 * t)) : 0ull;
+    const bool imag = (grp >> 1) != 0;
+#pragma unroll
+    for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
+      if (!jok[q]) continue;
+      const uint32_t ci = 16 * jti[q] + cl, cj = 16 * jtj[q] + cl;
+      uint32_t pa[8], pb[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const uint32_t base = (f0 + t) * row_words + jb[q] * CH;
+        pa[t] = ci < CH ? tile[base + ci] : 0u;
+        pb[t] = cj < CH ? tile[base + cj] : 0u;
+      }
+      const Dig8 da = digits8(pa, ci < CH ? fmask : 0ull);
+      const Dig8 db = digits8(pb, cj < CH ? fmask : 0ull);
+      if (!imag) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          sra[q] += cre(pa[t]);
+          sia[q] += cim(pa[t]);
+          srb[q] += cre(pb[t]);
+          sib[q] += cim(pb[t]);
+        }
+      }
+      const uint64_t amask = ci < CH ? fmask : 0ull;
+      // Re: A = [Xr | Xi], B = [Xr | Xi];  Im: A = [Xi | ~Xr]
+      const uint64_t reah = imag ? da.hi : da.hr, real_ = imag ? da.li : da.lr;
+      const uint64_t imah = imag ? (~da.hr & amask) : da.hi, imal = imag ? (~da.lr & amask) : da.li;
+      const uint64_t bh = imag ? db.hi : db.hr, bl = imag ? db.li : db.lr;
+      acc[q][0] = mfma8(reah, bh, acc[q][0]);
+      acc[q][1] = mfma8(reah, bl, acc[q][1]);
+      acc[q][1] = mfma8(real_, bh, acc[q][1]);
+      acc[q][2] = mfma8(real_, bl, acc[q][2]);
+      acc[q][3] = mfma8(imah, bh, acc[q][3]);
+      acc[q][4] = mfma8(imah, bl, acc[q][4]);
+      acc[q][4] = mfma8(imal, bh, acc[q][4]);
+      acc[q][5] = mfma8(imal, bl, acc[q][5]);
+    }
+  }
+
+  const size_t npairs = (size_t)CH * (CH + 1) / 2;
+  const int64_t kconst = 16384LL * 2 * F;
+#pragma unroll
+  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
+    // per-channel frame sums: lanes of groups 0/1 hold the partials
+    int32_t a_r = sra[q] + __shfl_xor(sra[q], 16), a_i = sia[q] + __shfl_xor(sia[q], 16);
+    int32_t b_r = srb[q] + __shfl_xor(srb[q], 16), b_i = sib[q] + __shfl_xor(sib[q], 16);
+    a_r += __shfl_xor(a_r, 32);
+    a_i += __shfl_xor(a_i, 32);
+    b_r += __shfl_xor(b_r, 32);
+    b_i += __shfl_xor(b_i, 32);
+    if (!jok[q]) continue;
+    const uint32_t k = k_base + jb[q];
+    const uint32_t cj = 16 * jtj[q] + cl;
+    const int64_t tj = (int64_t)b_r + b_i;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t il = 4 * grp + r;
+      const int32_t ir = __shfl(a_r, il), ii = __shfl(a_i, il);
+      const uint32_t ci = 16 * jti[q] + il;
+      if (ci >= CH || cj >= CH || ci > cj) continue;
+      const int64_t ti = (int64_t)ir + ii, ui = (int64_t)ii - ir - (int64_t)F;
+      const int64_t re = combine(acc[q][0], acc[q][1], acc[q][2], r) + 128 * (ti + tj) - kconst;
+      const int64_t im = combine(acc[q][3], acc[q][4], acc[q][5], r) + 128 * (ui + tj) - kconst + b_i;
+      const size_t pair = (size_t)ci * CH - (size_t)ci * (ci - 1) / 2 + (cj - ci);
+      int64_t* o = R + ((size_t)k * npairs + pair) * 2;
+      o[0] = re;
+      o[1] = im;
+    }
+  }
+}
+
+hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                      int64_t* R) {
+  if (n_bins == 0) return hipSuccess;
+  const uint32_t chp = (n_chan + 15) / 16 * 16, nt = chp / 16;
+  const uint32_t nb = 64 / chp;
+  const dim3 grid((n_bins + nb - 1) / nb);
+  cov_kernel<<<grid, 256, 0, s>>>(Xa, n_chan, n_frames, n_bins, nb, nt, R);
+  return hipGetLastError();
+}
+
+}  // namespace rd

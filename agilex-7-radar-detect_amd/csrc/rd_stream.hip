@@ -1,0 +1,445 @@
+// rd_stream.hip — HBM-streaming stages of the radar chain on gfx950:
+// polyphase FIR bank, radix-2^2 FFT with per-sub-stage rounding, corner turn.
+//
+// Data are packed complex int16 words {im, re} (real low). All arithmetic is
+// integer and reproduces the reference models bit for bit:
+//   PFB  model/cpp/pfb/pfb_model.hpp:117-257
+//   FFT  model/cpp/fft/fft_ref.hpp:250-407
+//   turn model/cpp/align/align_model.hpp:172-186
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rd_fxp.h"
+#include "rd_internal.h"
+
+namespace rd {
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int32_t dot2(uint32_t a, uint32_t b, int32_t c) {
+  return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, a), __builtin_bit_cast(v2s, b), c, false);
+}
+__device__ __forceinline__ uint32_t pk16(int32_t lo, int32_t hi) {
+  return ((uint32_t)hi << 16) | ((uint32_t)lo & 0xFFFFu);
+}
+
+// ============================================================================
+// PFB
+// ----------------------------------------------------------------------------
+// Branch p of channel c owns the decimated stream x_p[m] = x[c][m*P + p] and
+// computes y_p[m] = round_sat(sum_k h_p[k] x_p[m-k], 15, 16) per component with
+// an exact accumulator (pfb_model.hpp:139-168).
+//
+// One thread per branch keeps the last T samples in registers, so each input
+// word is read once (plus T-1 warm-up words per beat chunk) with coalesced
+// 4-byte-per-lane loads along p. The taps stay in registers for the whole run.
+//
+// Exact products without 64-bit multiplies: each sample component is split
+// x = 256*xh + xl (xh in [-128,127], xl in [0,255]), and
+//   re = 256*sum(xrh*hr - xih*hi) + sum(xrl*hr - xil*hi)
+//   im = 256*sum(xrh*hi + xih*hr) + sum(xrl*hi + xil*hr)
+// Each bracket is one packed int16 dot product (v_dot2c_i32_i16) per tap into
+// an int32 partial that cannot overflow for T <= 64 (|partial| < 2^30); the two
+// partials are recombined exactly in int64 before the single rounding.
+// ============================================================================
+
+struct Digits {
+  uint32_t rh, ih, rl, il;  // (xrh,-xih) (xrh,xih) (xrl,-xil) (xrl,xil)
+};
+
+__device__ __forceinline__ Digits split(uint32_t v) {
+  const int32_t xr = cre(v), xi = cim(v);
+  const int32_t xrh = xr >> 8, xih = xi >> 8, xrl = xr & 255, xil = xi & 255;
+  return Digits{pk16(xrh, -xih), pk16(xrh, xih), pk16(xrl, -xil), pk16(xrl, xil)};
+}
+
+__device__ __forceinline__ uint32_t pfb_finish(int32_t arh, int32_t arl, int32_t aih, int32_t ail, uint8_t* fl) {
+  const int64_t are = (int64_t)arh * 256 + arl;
+  const int64_t aim = (int64_t)aih * 256 + ail;
+  uint32_t fr = 0, fi = 0;
+  const int32_t yr = sat16_i64(rne15_i64(are), fr);
+  const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+  *fl = (uint8_t)(fr | (fi << 2));
+  return cpack(yr, yi);
+}
+
+// x_p[m] for m in [-(T-1), n_beats): negative beats come from the imported
+// history [T-1][P] (most recent first) or are zero.
+__device__ __forceinline__ uint32_t pfb_sample(const uint32_t* xc, const uint32_t* hc, uint32_t P, uint32_t p,
+                                               int64_t m) {
+  if (m >= 0) return xc[(size_t)m * P + p];
+  return hc ? hc[(size_t)(-m - 1) * P + p] : 0u;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void pfb_reg_kernel(const uint32_t* __restrict__ x,
+                                                      const uint32_t* __restrict__ coeff, uint32_t P,
+                                                      uint32_t n_beats, uint32_t chunk,
+                                                      const uint32_t* __restrict__ hist, uint32_t* __restrict__ y,
+                                                      uint8_t* __restrict__ flags) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const uint32_t c = blockIdx.y;
+  const uint32_t m0 = blockIdx.z * chunk;
+  const uint32_t m1 = min(m0 + chunk, n_beats);
+  const size_t span = (size_t)n_beats * P;
+  const uint32_t* xc = x + (size_t)c * span;
+  const uint32_t* hc = hist ? hist + (size_t)c * (T - 1) * P : nullptr;
+  uint32_t* yc = y + (size_t)c * span;
+  uint8_t* fc = flags ? flags + (size_t)c * span : nullptr;
+
+  uint32_t ta[T], tb[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    ta[k] = coeff[(size_t)p * T + k];          // (hr, hi)
+    tb[k] = (ta[k] >> 16) | (ta[k] << 16);     // (hi, hr)
+  }
+  // Window slot (m - m0) mod T holds the digits of x_p[m].
+  Digits w[T];
+#pragma unroll
+  for (int k = 1; k < T; ++k) w[T - k] = split(pfb_sample(xc, hc, P, p, (int64_t)m0 - k));
+
+  for (uint32_t mb = m0; mb < m1; mb += T) {
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+      const uint32_t m = mb + j;
+      if (m >= m1) break;
+      w[j] = split(xc[(size_t)m * P + p]);
+      int32_t arh = 0, arl = 0, aih = 0, ail = 0;
+#pragma unroll
+      for (int k = 0; k < T; ++k) {
+        const Digits& d = w[(j - k + T) % T];
+        arh = dot2(d.rh, ta[k], arh);
+        arl = dot2(d.rl, ta[k], arl);
+        aih = dot2(d.ih, tb[k], aih);
+        ail = dot2(d.il, tb[k], ail);
+      }
+      uint8_t f;
+      yc[(size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
+      if (fc) fc[(size_t)m * P + p] = f;
+    }
+  }
+}
+
+// Any T in [1, 64]: the window is re-read from memory (L1/L2) each beat.
+__global__ __launch_bounds__(256) void pfb_any_kernel(const uint32_t* __restrict__ x,
+                                                      const uint32_t* __restrict__ coeff, uint32_t P, uint32_t T,
+                                                      uint32_t n_beats, uint32_t chunk,
+                                                      const uint32_t* __restrict__ hist, uint32_t* __restrict__ y,
+                                                      uint8_t* __restrict__ flags) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const uint32_t c = blockIdx.y;
+  const uint32_t m0 = blockIdx.z * chunk;
+  const uint32_t m1 = min(m0 + chunk, n_beats);
+  const size_t span = (size_t)n_beats * P;
+  const uint32_t* xc = x + (size_t)c * span;
+  const uint32_t* hc = hist ? hist + (size_t)c * (T - 1) * P : nullptr;
+  for (uint32_t m = m0; m < m1; ++m) {
+    int32_t arh = 0, arl = 0, aih = 0, ail = 0;
+    for (uint32_t k = 0; k < T; ++k) {
+      const uint32_t h = coeff[(size_t)p * T + k];
+      const uint32_t hs = (h >> 16) | (h << 16);
+      const Digits d = split(pfb_sample(xc, hc, P, p, (int64_t)m - (int64_t)k));
+      arh = dot2(d.rh, h, arh);
+      arl = dot2(d.rl, h, arl);
+      aih = dot2(d.ih, hs, aih);
+      ail = dot2(d.il, hs, ail);
+    }
+    uint8_t f;
+    y[(size_t)c * span + (size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
+    if (flags) flags[(size_t)c * span + (size_t)m * P + p] = f;
+  }
+}
+
+// Final delay-line state after n_beats: hist[k-1] = x_p[n_beats - k], k = 1..T-1.
+// Runs after the filter kernel (stream order); k descends so an in-place read of
+// an older imported entry (index k-1-n_beats < k-1) happens before it is overwritten.
+__global__ void pfb_hist_kernel(const uint32_t* __restrict__ x, uint32_t P, uint32_t T, uint32_t n_beats,
+                                uint32_t* hist) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const uint32_t c = blockIdx.y;
+  const uint32_t* xc = x + (size_t)c * n_beats * P;
+  uint32_t* hc = hist + (size_t)c * (T - 1) * P;
+  for (uint32_t k = T - 1; k >= 1; --k) {
+    const int64_t m = (int64_t)n_beats - (int64_t)k;
+    hc[(size_t)(k - 1) * P + p] = pfb_sample(xc, hc, P, p, m);
+  }
+}
+
+hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, uint32_t P, uint32_t T,
+                      uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags) {
+  if (n_beats == 0 || n_chan == 0) return hipSuccess;
+  const uint32_t bx = (P + 255) / 256;
+  // Beat chunks: enough blocks to fill 256 CUs several times over, with each
+  // chunk long enough that the T-1 warm-up beats stay a small overhead.
+  uint32_t chunk = n_beats;
+  const uint64_t base_blocks = (uint64_t)bx * n_chan;
+  while (chunk > 4u * T && base_blocks * ((n_beats + chunk - 1) / chunk) < 2048) chunk = (chunk + 1) / 2;
+  const dim3 grid(bx, n_chan, (n_beats + chunk - 1) / chunk);
+  const dim3 block(P < 256 ? ((P + 63) / 64) * 64 : 256);
+  switch (T) {
+    case 4: pfb_reg_kernel<4><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
+    case 8: pfb_reg_kernel<8><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
+    case 16: pfb_reg_kernel<16><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
+    default: pfb_any_kernel<<<grid, block, 0, s>>>(x, coeff, P, T, n_beats, chunk, hist, y, flags); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (hist && T > 1) {
+    pfb_hist_kernel<<<dim3(bx, n_chan), block, 0, s>>>(x, P, T, n_beats, hist);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// ============================================================================
+// FFT
+// ----------------------------------------------------------------------------
+// One frame lives in LDS; each thread owns 4 samples of it per pass. A pass
+// applies two sub-stages (BF2I at even s, BF2II at s+1 with its trivial -j by
+// saturating negate) and, when s+2 < n_lane, the twiddle multiplier, all in
+// registers, so a 2^k-point lane needs ceil(k/2) LDS round trips. The last odd
+// sub-stage and the spc = 2 decimation-in-time merge form a final pass.
+// Every butterfly output is rounded by its schedule bit then saturated; every
+// twiddle product is rounded once at shift 15 (fft_ref.hpp:315-406).
+//
+// Lane q of an spc-lane transform is the subsequence x[spc*t + q]; the lanes sit
+// back to back in LDS, which leaves every block index parity and twiddle phase
+// of the second lane equal to the first's (block sizes divide the lane length).
+//
+// Twiddle products stay in 32 bits: every table entry has |w.re| + |w.im| <=
+// 65535 (checked by the tests), so |x.re*w.re -/+ x.im*w.im| <= 32768*65535 < 2^31.
+// ============================================================================
+
+struct Cv {
+  int32_t re, im;
+};
+
+__device__ __forceinline__ Cv ld_cv(uint32_t p) { return Cv{cre(p), cim(p)}; }
+
+struct StageFlags {
+  int32_t mx = -32768, mn = 32767;  // extremes of the rounded values
+  uint32_t extra = 0;               // flags from products / negates
+  __device__ __forceinline__ void see(int32_t r) {
+    mx = max(mx, r);
+    mn = min(mn, r);
+  }
+  __device__ __forceinline__ uint32_t word() const {
+    return extra | (mx > 32767 ? 2u : 0u) | (mn < -32768 ? 1u : 0u);
+  }
+};
+
+__device__ __forceinline__ int32_t q16(int32_t v, uint32_t sh, StageFlags& f) {
+  const int32_t r = rne_sh01(v, sh);
+  f.see(r);
+  return min(max(r, -32768), 32767);
+}
+
+// lo = q(a + b), hi = q(a - b)
+__device__ __forceinline__ void bfly(Cv& a, Cv& b, uint32_t sh, StageFlags& f) {
+  const int32_t sr = a.re + b.re, si = a.im + b.im, dr = a.re - b.re, di = a.im - b.im;
+  a.re = q16(sr, sh, f);
+  a.im = q16(si, sh, f);
+  b.re = q16(dr, sh, f);
+  b.im = q16(di, sh, f);
+}
+
+// -j * (re + j im) = im - j re, negate saturating (fft_ref.hpp:322-331)
+__device__ __forceinline__ Cv neg_j(Cv c, StageFlags& f) {
+  const int32_t n = -c.re;
+  f.extra |= (n > 32767 ? 2u : 0u);
+  return Cv{c.im, min(n, 32767)};
+}
+
+__device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t w, StageFlags& f) {
+  const int32_t wr = cre(w), wi = cim(w);
+  const int32_t re = x.re * wr - x.im * wi;
+  const int32_t im = x.re * wi + x.im * wr;
+  uint32_t fl = 0;
+  const int32_t yr = sat16(rne15_i32(re), fl);
+  const int32_t yi = sat16(rne15_i32(im), fl);
+  f.extra |= fl;
+  return Cv{yr, yi};
+}
+
+template <int SPC>
+__global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ in, uint32_t log2n, uint32_t sched,
+                                                   int reorder, uint32_t n_frames, uint32_t fpb,
+                                                   const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out,
+                                                   uint32_t* __restrict__ bins, uint32_t* __restrict__ stage_flags) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t n = 1u << log2n;
+  const uint32_t m = n / SPC;
+  const uint32_t nl = log2n - (SPC == 2 ? 1 : 0);
+  const uint32_t tpf = n / 4;
+  uint32_t* tw = smem;                       // [n]  W_n^e
+  uint32_t* sfl = smem + n;                  // [fpb] stage-flag words
+  const uint32_t lf = threadIdx.x / tpf;     // local frame
+  const uint32_t g = threadIdx.x % tpf;
+  uint32_t* v = smem + n + fpb + lf * n;     // this frame's samples
+  const uint32_t frame = blockIdx.x * fpb + lf;
+  const bool active = frame < n_frames;
+
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw4096[i << (12 - log2n)];
+  if (g == 0) sfl[lf] = 0;
+  if (active) {
+    const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * n + 4 * g);
+    const uint32_t w4[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t t = 4 * g + r;
+      v[SPC == 2 ? ((t & 1) * m + (t >> 1)) : t] = w4[r];
+    }
+  }
+  __syncthreads();
+
+  uint32_t fword = 0;
+  uint32_t s = 0;
+  for (; s + 1 < nl; s += 2) {
+    const uint32_t lq = nl - s - 2;  // log2(quarter)
+    const uint32_t quarter = 1u << lq;
+    const uint32_t i = g & (quarter - 1);
+    const uint32_t base = ((g >> lq) << (nl - s)) + i;
+    Cv e0 = ld_cv(v[base]), e1 = ld_cv(v[base + quarter]), e2 = ld_cv(v[base + 2 * quarter]),
+       e3 = ld_cv(v[base + 3 * quarter]);
+    StageFlags f0, f1;
+    const uint32_t sh0 = (sched >> s) & 1u, sh1 = (sched >> (s + 1)) & 1u;
+    bfly(e0, e2, sh0, f0);  // BF2I, half = l/2
+    bfly(e1, e3, sh0, f0);
+    e3 = neg_j(e3, f1);     // BF2II: odd sub-block takes -j
+    bfly(e0, e1, sh1, f1);
+    bfly(e2, e3, sh1, f1);
+    if (s + 2 < nl) {
+      const uint32_t l2l = nl - s, tsh = log2n - l2l;
+      // element c sits at quarter c of its block: exponent i * {0, 2, 1, 3}[c]
+      e1 = cmul_tw(e1, tw[(2 * i) << tsh], f1);
+      e2 = cmul_tw(e2, tw[i << tsh], f1);
+      e3 = cmul_tw(e3, tw[(3 * i) << tsh], f1);
+      e0 = cmul_tw(e0, tw[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
+    }
+    __syncthreads();
+    if (active) {
+      v[base] = cpack(e0.re, e0.im);
+      v[base + quarter] = cpack(e1.re, e1.im);
+      v[base + 2 * quarter] = cpack(e2.re, e2.im);
+      v[base + 3 * quarter] = cpack(e3.re, e3.im);
+    }
+    fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
+    __syncthreads();
+  }
+
+  // Final pass: the last (odd-count) sub-stage and/or the DIT merge.
+  const bool last_r2 = (nl & 1u) != 0;
+  if (SPC == 1) {
+    if (last_r2) {
+      Cv a = ld_cv(v[4 * g]), b = ld_cv(v[4 * g + 1]), c = ld_cv(v[4 * g + 2]), d = ld_cv(v[4 * g + 3]);
+      StageFlags f;
+      const uint32_t sh = (sched >> s) & 1u;
+      bfly(a, b, sh, f);
+      bfly(c, d, sh, f);
+      fword |= f.word() << (2 * s);
+      if (active) {
+        v[4 * g] = cpack(a.re, a.im);
+        v[4 * g + 1] = cpack(b.re, b.im);
+        v[4 * g + 2] = cpack(c.re, c.im);
+        v[4 * g + 3] = cpack(d.re, d.im);
+      }
+    }
+  } else {
+    // lane0[2g], lane0[2g+1], lane1[2g], lane1[2g+1]
+    Cv a0 = ld_cv(v[2 * g]), a1 = ld_cv(v[2 * g + 1]), b0 = ld_cv(v[m + 2 * g]), b1 = ld_cv(v[m + 2 * g + 1]);
+    if (last_r2) {
+      StageFlags f;
+      const uint32_t sh = (sched >> s) & 1u;
+      bfly(a0, a1, sh, f);
+      bfly(b0, b1, sh, f);
+      fword |= f.word() << (2 * s);
+    }
+    StageFlags fm;
+    const uint32_t shm = (sched >> nl) & 1u;
+    Cv t0 = cmul_tw(b0, tw[bitrev(2 * g, nl)], fm);
+    Cv t1 = cmul_tw(b1, tw[bitrev(2 * g + 1, nl)], fm);
+    bfly(a0, t0, shm, fm);
+    bfly(a1, t1, shm, fm);
+    fword |= fm.word() << (2 * nl);
+    __syncthreads();
+    if (active) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
+      v[4 * g] = cpack(a0.re, a0.im);
+      v[4 * g + 1] = cpack(t0.re, t0.im);
+      v[4 * g + 2] = cpack(a1.re, a1.im);
+      v[4 * g + 3] = cpack(t1.re, t1.im);
+    }
+  }
+  if (fword) atomicOr(&sfl[lf], fword);
+  __syncthreads();
+  if (!active) return;
+
+  // v now holds the beats in beat order. Natural bin k: spc = 1 -> beats[bitrev(k)];
+  // spc = 2 -> k = bitrev(j) + q*m reads beats[2j + q].
+  const size_t fo = (size_t)frame * n;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t k = g + r * tpf;
+    if (bins) {
+      const uint32_t src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k & (m - 1), nl) + (k >> nl);
+      bins[fo + k] = v[src];
+    }
+    if (out) {
+      uint32_t src = k;
+      if (reorder) src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k >> 1, nl) + (k & 1);
+      out[fo + k] = v[src];
+    }
+  }
+  if (stage_flags && g == 0) stage_flags[frame] = sfl[lf];
+}
+
+hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t n, uint32_t spc,
+                      uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
+                      uint32_t* stage_flags) {
+  if (n_frames == 0) return hipSuccess;
+  const uint32_t log2n = ilog2u(n);
+  const uint32_t tpf = n / 4;
+  const uint32_t fpb = tpf >= 256 ? 1 : 256 / tpf;
+  const uint32_t threads = tpf * fpb;
+  const size_t lds = (size_t)(n + fpb + fpb * n) * sizeof(uint32_t);
+  const dim3 grid((n_frames + fpb - 1) / fpb);
+  if (spc == 2)
+    fft_kernel<2><<<grid, threads, lds, s>>>(in, log2n, sched, reorder, n_frames, fpb, tw4096, out, bins,
+                                              stage_flags);
+  else
+    fft_kernel<1><<<grid, threads, lds, s>>>(in, log2n, sched, reorder, n_frames, fpb, tw4096, out, bins,
+                                              stage_flags);
+  return hipGetLastError();
+}
+
+// ============================================================================
+// Corner turn: X[ch][col] -> Xa[col][ch]. A block moves 64 columns of every
+// channel: reads are CH rows of 256 contiguous bytes, and the output tile
+// [64][CH] is one contiguous run.
+// ============================================================================
+__global__ __launch_bounds__(256) void align_kernel(const uint32_t* __restrict__ X, uint32_t n_chan, uint64_t cols,
+                                                    uint32_t* __restrict__ Xa) {
+  extern __shared__ uint32_t tile[];  // [n_chan][65]
+  const uint64_t c0 = (uint64_t)blockIdx.x * 64;
+  const uint32_t ncol = (uint32_t)min<uint64_t>(64, cols - c0);
+  for (uint32_t idx = threadIdx.x; idx < n_chan * 64; idx += blockDim.x) {
+    const uint32_t ch = idx / 64, j = idx % 64;
+    if (j < ncol) tile[ch * 65 + j] = X[(size_t)ch * cols + c0 + j];
+  }
+  __syncthreads();
+  for (uint32_t idx = threadIdx.x; idx < ncol * n_chan; idx += blockDim.x) {
+    const uint32_t j = idx / n_chan, ch = idx % n_chan;
+    Xa[(c0 + j) * n_chan + ch] = tile[ch * 65 + j];
+  }
+}
+
+hipError_t launch_align(hipStream_t s, const uint32_t* X, uint32_t n_chan, uint64_t cols, uint32_t* Xa) {
+  if (cols == 0) return hipSuccess;
+  const dim3 grid((uint32_t)((cols + 63) / 64));
+  align_kernel<<<grid, 256, (size_t)n_chan * 65 * sizeof(uint32_t), s>>>(X, n_chan, cols, Xa);
+  return hipGetLastError();
+}
+
+}  // namespace rd

@@ -1,0 +1,314 @@
+"""ctypes binding of librd_amd.so (the C ABI in include/rd_amd.h).
+
+Plumbing for tests, the smoke entry point and bench.py: device buffers are
+allocated and copied through the library's own rd_malloc / rd_h2d / rd_d2h, and
+every compute call goes to the HIP kernels. There is no CPU fallback: if the
+library is missing, :func:`load` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "librd_amd.so"
+
+RD_OK, RD_E_ARG, RD_E_GEOM, RD_E_HIP, RD_E_CAPACITY, RD_E_NODEV = 0, -1, -2, -3, -4, -5
+
+EXPORTS = [
+    "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
+    "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
+    "rd_bf_run", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue",
+]
+
+
+class CfarCfg(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in
+                ("enable", "mode", "out_mode", "guard_lead", "guard_lag", "ref_lead", "ref_lag", "alpha")]
+
+    @classmethod
+    def of(cls, words):
+        return cls(*[int(w) for w in words])
+
+
+class CpiDesc(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in
+                ("n_chan", "n_fft", "taps", "n_frames", "n_beams", "spc", "scale_sched", "max_guard", "max_ref")] + [
+        ("cfar", CfarCfg)]
+
+
+class CpiBufs(C.Structure):
+    _fields_ = [("iq", C.c_void_p), ("taps", C.c_void_p), ("weights", C.c_void_p), ("u", C.c_void_p),
+                ("pfb_flags", C.c_void_p), ("X", C.c_void_p), ("fft_flags", C.c_void_p), ("Xa", C.c_void_p),
+                ("y", C.c_void_p), ("yflags", C.c_void_p), ("power", C.c_void_p), ("R", C.c_void_p),
+                ("events", C.c_void_p), ("events_cap", C.c_uint64), ("row_count", C.c_void_p)]
+
+
+class RdError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed: {code} ({msg})")
+        self.code = code
+
+
+class Lib:
+    def __init__(self, path: Path = LIB_PATH):
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build() or make -C {HERE}")
+        self.lib = C.CDLL(str(path))
+        L = self.lib
+        vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+        L.rd_device_count.restype = i32
+        L.rd_status_str.restype = C.c_char_p
+        L.rd_status_str.argtypes = [i32]
+        L.rd_create.argtypes = [i32, C.POINTER(vp)]
+        L.rd_destroy.argtypes = [vp]
+        L.rd_sync.argtypes = [vp]
+        L.rd_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+        L.rd_free.argtypes = [vp, vp]
+        L.rd_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+        L.rd_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+        L.rd_memset.argtypes = [vp, vp, i32, C.c_size_t]
+        L.rd_stream.argtypes = [vp]
+        L.rd_stream.restype = vp
+        L.rd_twiddle_table.argtypes = [vp]
+        L.rd_pfb_run.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp]
+        L.rd_fft_run.argtypes = [vp, vp, u32, u32, u32, i32, u32, vp, vp, vp]
+        L.rd_align_run.argtypes = [vp, vp, u32, u32, u32, vp]
+        L.rd_bf_run.argtypes = [vp, vp, vp, u32, u32, u64, vp, vp, vp, vp]
+        L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
+        L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
+                                  C.POINTER(u64)]
+        L.rd_chain_run.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), C.POINTER(u64)]
+        L.rd_chain_enqueue.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
+        for name in EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("rd_device_count", "rd_status_str", "rd_stream"):
+                fn.restype = i32
+
+    def device_count(self) -> int:
+        return int(self.lib.rd_device_count())
+
+    def check(self, fn: str, code: int, ok=(RD_OK,)):
+        if code not in ok:
+            raise RdError(fn, code, self.lib.rd_status_str(code).decode())
+        return code
+
+    def twiddle_table(self) -> np.ndarray:
+        out = np.zeros(4096, np.uint32)
+        self.check("rd_twiddle_table", self.lib.rd_twiddle_table(out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def context(self, device: int = 0) -> "Ctx":
+        return Ctx(self, device)
+
+
+class DevBuf:
+    """A device allocation owned by a context."""
+
+    def __init__(self, ctx: "Ctx", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.lib.check("rd_malloc", ctx.L.rd_malloc(ctx.h, max(self.nbytes, 4), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> "DevBuf":
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx.lib.check("rd_h2d", self.ctx.L.rd_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return self
+
+    def download(self, dtype, count=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dt)
+        if n:
+            self.ctx.lib.check("rd_d2h", self.ctx.L.rd_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                                             out.nbytes))
+        return out
+
+    def zero(self) -> "DevBuf":
+        self.ctx.lib.check("rd_memset", self.ctx.L.rd_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.rd_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Ctx:
+    def __init__(self, lib: Lib, device: int = 0):
+        self.lib = lib
+        self.L = lib.lib
+        h = C.c_void_p()
+        lib.check("rd_create", self.L.rd_create(device, C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            self.L.rd_sync(self.h)
+            self.L.rd_destroy(self.h)
+            self.h = None
+
+    @property
+    def stream(self) -> int:
+        return int(self.L.rd_stream(self.h) or 0)
+
+    def sync(self):
+        self.lib.check("rd_sync", self.L.rd_sync(self.h))
+
+    def buf(self, nbytes: int) -> DevBuf:
+        return DevBuf(self, nbytes)
+
+    def put(self, arr: np.ndarray) -> DevBuf:
+        a = np.ascontiguousarray(arr)
+        return DevBuf(self, a.nbytes).upload(a)
+
+    # ---- per-stage host-array conveniences (upload, run, download) ----------
+    def pfb(self, x, coeff, P, T, n_chan, n_beats, hist=None):
+        S = n_chan * n_beats * P
+        dx, dc = self.put(np.asarray(x, np.uint32)), self.put(np.asarray(coeff, np.uint32))
+        dy, df = self.buf(4 * S), self.buf(S)
+        dh = self.put(np.asarray(hist, np.uint32)) if hist is not None else None
+        self.lib.check("rd_pfb_run", self.L.rd_pfb_run(self.h, dx.ptr, dc.ptr, P, T, n_chan, n_beats,
+                                                       dh.ptr if dh else None, dy.ptr, df.ptr))
+        y, f = dy.download(np.uint32, S), df.download(np.uint8, S)
+        if hist is not None:
+            hist[...] = dh.download(np.uint32, hist.size)
+        return y, f
+
+    def fft(self, x, n, spc=2, sched=0xFFFFFFFF, reorder=1):
+        x = np.asarray(x, np.uint32)
+        frames = x.size // n
+        dx = self.put(x)
+        do, db, dfl = self.buf(4 * x.size), self.buf(4 * x.size), self.buf(4 * max(frames, 1))
+        self.lib.check("rd_fft_run", self.L.rd_fft_run(self.h, dx.ptr, n, spc, sched, reorder, frames, do.ptr, db.ptr,
+                                                       dfl.ptr))
+        return do.download(np.uint32, x.size), db.download(np.uint32, x.size), dfl.download(np.uint32, frames)
+
+    def align(self, X, n_chan, n_frames, n_bins):
+        dX = self.put(np.asarray(X, np.uint32))
+        dA = self.buf(dX.nbytes)
+        self.lib.check("rd_align_run", self.L.rd_align_run(self.h, dX.ptr, n_chan, n_frames, n_bins, dA.ptr))
+        return dA.download(np.uint32, np.asarray(X).size)
+
+    def bf(self, Xa, W, CH, B, want_acc=True):
+        Xa = np.asarray(Xa, np.uint32)
+        cols = Xa.size // CH
+        dX, dW = self.put(Xa), self.put(np.asarray(W, np.uint32))
+        dy, df, dp = self.buf(4 * B * cols), self.buf(B * cols), self.buf(4 * B * cols)
+        da = self.buf(16 * B * cols) if want_acc else None
+        self.lib.check("rd_bf_run", self.L.rd_bf_run(self.h, dX.ptr, dW.ptr, CH, B, cols, dy.ptr, df.ptr, dp.ptr,
+                                                     da.ptr if da else None))
+        return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
+                da.download(np.int64, 2 * B * cols) if da else None)
+
+    def cov(self, Xa, CH, F, N):
+        dX = self.put(np.asarray(Xa, np.uint32))
+        npairs = CH * (CH + 1) // 2
+        dR = self.buf(16 * N * npairs)
+        self.lib.check("rd_cov_run", self.L.rd_cov_run(self.h, dX.ptr, CH, F, N, dR.ptr))
+        return dR.download(np.int64, 2 * N * npairs)
+
+    def cfar(self, cells, n, cfg8, max_guard, max_ref, frame_ids=None, frame_mod=0, cap=None):
+        cells = np.ascontiguousarray(cells)
+        assert cells.dtype in (np.uint32, np.uint64)
+        rows = cells.size // n
+        cap = rows * (n + 1) if cap is None else cap
+        dc = self.put(cells)
+        dfid = self.put(np.asarray(frame_ids, np.uint32)) if frame_ids is not None else None
+        dev, drc = self.buf(24 * max(cap, 1)), self.buf(4 * max(rows, 1))
+        tot = C.c_uint64()
+        cfg = CfarCfg.of(cfg8)
+        rc = self.L.rd_cfar_run(self.h, dc.ptr, cells.dtype.itemsize, n, rows, C.byref(cfg), max_guard, max_ref,
+                                dfid.ptr if dfid else None, frame_mod, dev.ptr, cap, drc.ptr, C.byref(tot))
+        self.lib.check("rd_cfar_run", rc, ok=(RD_OK, RD_E_CAPACITY))
+        t = min(tot.value, cap)
+        return dev.download(np.uint64, 3 * t).reshape(t, 3), drc.download(np.uint32, rows), tot.value, rc
+
+
+class Chain:
+    """Device buffers and descriptor for one CPI geometry (reused across runs)."""
+
+    def __init__(self, ctx: Ctx, CH, N, T, F, B, cfar_cfg8, max_guard=31, max_ref=63, spc=2, sched=0xFFFFFFFF,
+                 keep_y=True, keep_flags=True, keep_R=True, events_cap=None):
+        self.ctx = ctx
+        S = CH * N * F
+        self.shape = dict(CH=CH, N=N, T=T, F=F, B=B)
+        self.desc = CpiDesc(CH, N, T, F, B, spc, sched, max_guard, max_ref, CfarCfg.of(cfar_cfg8))
+        self.events_cap = B * F * (N + 1) if events_cap is None else events_cap
+        b = {}
+        b["iq"] = ctx.buf(4 * S)
+        b["taps"] = ctx.buf(4 * N * T)
+        b["weights"] = ctx.buf(4 * B * CH)
+        b["u"] = ctx.buf(4 * S)
+        b["pfb_flags"] = ctx.buf(S) if keep_flags else None
+        b["X"] = ctx.buf(4 * S)
+        b["fft_flags"] = ctx.buf(4 * CH * F) if keep_flags else None
+        b["Xa"] = ctx.buf(4 * S)
+        b["y"] = ctx.buf(4 * B * N * F) if keep_y else None
+        b["yflags"] = ctx.buf(B * N * F) if keep_flags else None
+        b["power"] = ctx.buf(4 * B * N * F)
+        b["R"] = ctx.buf(16 * N * CH * (CH + 1) // 2) if keep_R else None
+        b["events"] = ctx.buf(24 * max(self.events_cap, 1))
+        b["row_count"] = ctx.buf(4 * B * F)
+        self.total = ctx.buf(8)
+        self.b = b
+        self.bufs = CpiBufs(**{k: (v.ptr if v is not None else None) for k, v in b.items()},
+                            events_cap=self.events_cap)
+
+    def load(self, iq, taps, weights):
+        self.b["iq"].upload(np.asarray(iq, np.uint32))
+        self.b["taps"].upload(np.asarray(taps, np.uint32))
+        self.b["weights"].upload(np.asarray(weights, np.uint32))
+
+    def run(self) -> int:
+        tot = C.c_uint64()
+        rc = self.ctx.L.rd_chain_run(self.ctx.h, C.byref(self.desc), C.byref(self.bufs), C.byref(tot))
+        self.ctx.lib.check("rd_chain_run", rc)
+        return tot.value
+
+    def enqueue(self, iq_ptr=None, total_ptr=None):
+        bufs = self.bufs
+        if iq_ptr is not None:
+            bufs = CpiBufs.from_buffer_copy(self.bufs)
+            bufs.iq = iq_ptr
+        rc = self.ctx.L.rd_chain_enqueue(self.ctx.h, C.byref(self.desc), C.byref(bufs),
+                                         total_ptr if total_ptr is not None else self.total.ptr)
+        self.ctx.lib.check("rd_chain_enqueue", rc)
+
+    def fetch(self, names=("u", "pfb_flags", "X", "fft_flags", "Xa", "y", "yflags", "power", "R"), n_events=None):
+        CH, N, F, B = (self.shape[k] for k in ("CH", "N", "F", "B"))
+        S = CH * N * F
+        spec = dict(u=(np.uint32, S), pfb_flags=(np.uint8, S), X=(np.uint32, S), fft_flags=(np.uint32, CH * F),
+                    Xa=(np.uint32, S), y=(np.uint32, B * N * F), yflags=(np.uint8, B * N * F),
+                    power=(np.uint32, B * N * F), R=(np.int64, N * CH * (CH + 1)), row_count=(np.uint32, B * F))
+        out = {}
+        for k in names:
+            if self.b.get(k) is not None:
+                dt, n = spec[k]
+                out[k] = self.b[k].download(dt, n)
+        out["row_count"] = self.b["row_count"].download(np.uint32, B * F)
+        if n_events is not None:
+            t = min(n_events, self.events_cap)
+            out["events"] = self.b["events"].download(np.uint64, 3 * t).reshape(t, 3)
+        return out
+
+
+_lib = None
+
+
+def load() -> Lib:
+    global _lib
+    if _lib is None:
+        _lib = Lib()
+    return _lib

@@ -1,0 +1,192 @@
+/*
+ * rd_amd.h — C ABI of the MI355X (gfx950) bit-exact radar chain.
+ *
+ * The drop-in boundary for the reference's per-stage C++ model calls
+ * (model/cpp in fpga-professional-association/agilex-7-radar-detect). The
+ * reference has no FFI of its own: its callers bind the header-only classes in
+ * process. Each entry point below names the reference interface it replaces
+ * (file:line) and computes the same values, bit for bit, batched over
+ * channels / frames / rows.
+ *
+ * Conventions
+ *   - Complex samples are packed {im, re} in one uint32, real part in the low
+ *     half (model/cpp/fxp/fxp.hpp:288-297).
+ *   - Saturation flags are the reference's 2-bit word {sat_pos<<1 | sat_neg}
+ *     (fxp.hpp:146-153). Per-sample flag bytes hold f_re in bits [1:0] and
+ *     f_im in bits [3:2].
+ *   - Every data pointer is a DEVICE pointer owned by the caller (use
+ *     rd_malloc / rd_h2d / rd_d2h when the caller has no HIP headers).
+ *     Nullable outputs are skipped when NULL.
+ *   - Work is queued on the context's stream; nothing is synchronous unless
+ *     stated. Calls on one context must not overlap from several threads.
+ *   - No exceptions cross the ABI; every call returns an rd_status.
+ */
+#ifndef RD_AMD_H_
+#define RD_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  RD_OK = 0,
+  RD_E_ARG = -1,      /* null pointer or size out of range */
+  RD_E_GEOM = -2,     /* geometry the reference rejects (e.g. fft::Config::valid) */
+  RD_E_HIP = -3,      /* a HIP runtime call failed */
+  RD_E_CAPACITY = -4, /* event buffer too small (count still reported) */
+  RD_E_NODEV = -5     /* no such device */
+} rd_status;
+
+typedef struct rd_ctx rd_ctx;
+
+/* CFAR configuration as written to the register plane
+ * (model/cpp/cfar/cfar_model.hpp:112-128). mode: 0 CA, 1 GO.
+ * out_mode: 0 events (DETECT + SUMMARY), 1 dense (one event per bin + SUMMARY).
+ * alpha: UQ8.8 threshold multiplier. */
+typedef struct {
+  uint32_t enable, mode, out_mode, guard_lead, guard_lag, ref_lead, ref_lag, alpha;
+} rd_cfar_cfg;
+
+/* One coherent processing interval (CPI): geometry of the composed chain. */
+typedef struct {
+  uint32_t n_chan;      /* antennas / channels, 1..64                    */
+  uint32_t n_fft;       /* PFB branches = FFT length, 8..4096, power of 2 */
+  uint32_t taps;        /* PFB taps per branch, 1..64                    */
+  uint32_t n_frames;    /* PFB beats = FFT frames                        */
+  uint32_t n_beams;     /* beams, 1..64                                  */
+  uint32_t spc;         /* FFT samples per cycle: 1 or 2                 */
+  uint32_t scale_sched; /* FFT per-sub-stage shift bits (0xFFFFFFFF = all) */
+  uint32_t max_guard;   /* CFAR elaboration maxima (clamp)               */
+  uint32_t max_ref;
+  rd_cfar_cfg cfar;
+} rd_cpi_desc;
+
+/* Device buffers for one CPI (layouts: [a][b] is row-major, a slowest).
+ *   iq        [n_chan][n_frames*n_fft]      input
+ *   taps      [n_fft][taps]                 phase-major, tap-minor
+ *   weights   [n_beams][n_chan]             beam-major
+ *   u         [n_chan][n_frames][n_fft]     PFB output (required)
+ *   pfb_flags [n_chan][n_frames][n_fft]     nullable
+ *   X         [n_chan][n_frames][n_fft]     FFT bins, natural order (required)
+ *   fft_flags [n_chan][n_frames]            stage flags, sub-stage g at bits [2g+1:2g]; nullable
+ *   Xa        [n_frames][n_fft][n_chan]     aligned (bin-major, antenna-minor) (required)
+ *   y         [n_beams][n_frames][n_fft]    beam samples; nullable
+ *   yflags    [n_beams][n_frames][n_fft]    nullable
+ *   power     [n_beams][n_frames][n_fft]    |y|^2 as uint32 (exact, <= 2^31) (required)
+ *   R         [n_fft][n_chan*(n_chan+1)/2][2] int64 covariance re/im, pairs i<=j row-major; nullable
+ *   events    [events_cap][3] uint64        packed 176-bit events, rows (beam, frame) in order
+ *   row_count [n_beams*n_frames]            events per row (required)
+ */
+typedef struct {
+  const uint32_t* iq;
+  const uint32_t* taps;
+  const uint32_t* weights;
+  uint32_t* u;
+  uint8_t* pfb_flags;
+  uint32_t* X;
+  uint32_t* fft_flags;
+  uint32_t* Xa;
+  uint32_t* y;
+  uint8_t* yflags;
+  uint32_t* power;
+  int64_t* R;
+  uint64_t* events;
+  uint64_t events_cap;
+  uint32_t* row_count;
+} rd_cpi_bufs;
+
+/* ---- context and memory ------------------------------------------------ */
+int rd_device_count(void);
+rd_status rd_create(int device, rd_ctx** out);
+rd_status rd_destroy(rd_ctx* ctx);
+const char* rd_status_str(rd_status s);
+rd_status rd_sync(rd_ctx* ctx);
+rd_status rd_malloc(rd_ctx* ctx, size_t bytes, void** dptr);
+rd_status rd_free(rd_ctx* ctx, void* dptr);
+rd_status rd_h2d(rd_ctx* ctx, void* dst, const void* src, size_t bytes);
+rd_status rd_d2h(rd_ctx* ctx, void* dst, const void* src, size_t bytes);
+rd_status rd_memset(rd_ctx* ctx, void* dst, int value, size_t bytes);
+/* The context's HIP stream (a hipStream_t), for callers that time or order work. */
+void* rd_stream(rd_ctx* ctx);
+
+/* The 4096-entry Q1.15 twiddle table the FFT uses (host copy), generated by
+ * the rule of model/python/gen_fft_twiddles.py:97-159 with M = 4096; every 4th
+ * entry is the reference's committed 1024-entry table
+ * (model/cpp/fft/fft_twiddle_table.hpp:42). */
+rd_status rd_twiddle_table(uint32_t* host_out4096);
+
+/* ---- stages -------------------------------------------------------------- */
+
+/* Polyphase FIR bank. Replaces pfb::PfbModel(phases, taps, coeff) + reset() +
+ * step(beat) per beat (model/cpp/pfb/pfb_model.hpp:207-251), over n_chan
+ * independent channels: x [n_chan][n_beats*phases] -> y [n_chan][n_beats][phases].
+ * hist (nullable) [n_chan][taps-1][phases], most recent beat first: seeds the
+ * delay line and receives its final state, which carries the filter across calls
+ * like the model's persistent history (pfb_model.hpp:179-182, 225-233).
+ * NULL means the zero history of reset(). */
+rd_status rd_pfb_run(rd_ctx* ctx, const uint32_t* x, const uint32_t* coeff, uint32_t phases, uint32_t taps,
+                     uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags);
+
+/* Streaming R2^2 FFT with per-sub-stage rounding. Replaces
+ * fft::transform(Config{n_fft, spc, scale_sched, reorder}, in)
+ * (model/cpp/fft/fft_ref.hpp:409-412) for n_frames frames of n_fft samples.
+ * out: Result::out (beat order), bins: Result::bins (natural order),
+ * stage_flags: Result::stage_flags packed 2 bits per sub-stage (fft_vectors.hpp:27-29).
+ * Extends the reference's N <= 1024 to N <= 4096 with the same twiddle rule. */
+rd_status rd_fft_run(rd_ctx* ctx, const uint32_t* in, uint32_t n_fft, uint32_t spc, uint32_t scale_sched,
+                     int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins, uint32_t* stage_flags);
+
+/* Frequency-bin alignment (corner turn): X [n_chan][n_frames][n_bins] ->
+ * Xa [n_frames][n_bins][n_chan], the beamformer's bin-major, antenna-minor beat
+ * layout (model/cpp/align/align_model.hpp:172-186). Pure permutation. */
+rd_status rd_align_run(rd_ctx* ctx, const uint32_t* X, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                       uint32_t* Xa);
+
+/* Complex beamformer + power. Replaces bf::dot(x, w, n_ant) for every beam and
+ * column (model/cpp/beamformer/beamformer_model.hpp:162-179) and
+ * covar::power(y) (model/cpp/covariance/covar_model.hpp:99-102).
+ * Xa [n_cols][n_ant], W [n_beams][n_ant] -> y/flags/power [n_beams][n_cols],
+ * acc (nullable) [n_beams][n_cols][2] = DotResult::acc_re/acc_im.
+ * Computed as four int8 MFMA partial products with exact int64 recombination. */
+rd_status rd_bf_run(rd_ctx* ctx, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                    uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc);
+
+/* Covariance: R[bin][pair i<=j] = sum over frames of covar::cross_power(X_i, X_j)
+ * (covar_model.hpp:137-151), integrated by a block-mode covar::Integrator with
+ * acc_w = 64 and window_len = n_frames (covar_model.hpp:186-262); at that width
+ * no step can saturate, so the window result is the exact int64 sum.
+ * Xa [n_frames][n_bins][n_chan] -> R [n_bins][n_chan*(n_chan+1)/2][2]. */
+rd_status rd_cov_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                     int64_t* R);
+
+/* CA/GO-CFAR over rows of n_bins cells. Replaces cfar::run_frame(bins, cfg,
+ * max_guard, max_ref, frame_id) per row (model/cpp/cfar/cfar_model.hpp:295-340),
+ * with the events packed by the 176-bit layout of cfar_model.hpp:75-84 into three
+ * little-endian uint64 words (sof at bit 176, eof at bit 177).
+ * cells: uint32 (cell_bytes = 4) or uint64 (cell_bytes = 8) magnitudes.
+ * frame_id of row r = frame_ids[r] when frame_ids != NULL, else r % frame_mod
+ * (frame_mod = 0: r). row_count (device, n_rows) receives the events per row;
+ * *n_events_host (synchronous) the total. Returns RD_E_CAPACITY if it exceeds
+ * events_cap (no events are then written past the cap). */
+rd_status rd_cfar_run(rd_ctx* ctx, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
+                      const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
+                      uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
+                      uint64_t* n_events_host);
+
+/* The composed chain for one CPI: PFB -> FFT -> alignment -> beamformer+power ->
+ * covariance -> CFAR -> event packing, every stage as above, all on the
+ * context's stream. *n_events_host (synchronous) receives the event total. */
+rd_status rd_chain_run(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_host);
+
+/* Same as rd_chain_run without the final synchronisation: the event total is
+ * left in device memory at *n_events_dev (uint64). For timing loops. */
+rd_status rd_chain_enqueue(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RD_AMD_H_ */

@@ -35,10 +35,13 @@ def ref():
 
 @pytest.fixture(scope="session")
 def gpu():
-    """The HIP library, loaded through its C ABI; skips when no GPU is present."""
+    """A context on device 0 of the HIP library (loaded through its C ABI).
+    Raises if the library is not built; skips only when no GPU is visible."""
     import rd_amd
 
     lib = rd_amd.load()
     if lib.device_count() < 1:
         pytest.skip("no GPU visible")
-    return lib
+    ctx = lib.context(0)
+    yield ctx
+    ctx.close()

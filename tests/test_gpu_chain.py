@@ -1,0 +1,62 @@
+"""GPU parity of the composed chain (one CPI through rd_chain_run) against the
+oracle's composed chain: every stage output, every flag, the covariance and the
+packed event list, bit for bit, on the published workloads.
+
+The large workload (64 ch, 4096-pt, 512 frames) is checked at its full
+geometry with 16 frames here (the oracle finishes in seconds); the full frame
+count runs in bench.py with size-independent checks.
+"""
+import numpy as np
+import pytest
+
+import rd_amd
+import rd_scenario
+
+pytestmark = pytest.mark.gpu
+
+STAGES = ("u", "pfb_flags", "X", "fft_flags", "Xa", "y", "yflags", "power", "R")
+
+
+def _run_both(gpu, orc, sc, threads=16):
+    ch = rd_amd.Chain(gpu, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF)
+    iq, taps, W = sc.iq(), sc.taps(), sc.weights()
+    ch.load(iq, taps, W)
+    tot = ch.run()
+    got = ch.fetch(STAGES, n_events=tot)
+    exp = orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
+                    rd_scenario.MAX_REF, keep=STAGES + ("events",), threads=threads)
+    return got, exp, tot
+
+
+@pytest.mark.parametrize("name,frames", [("tiny", None), ("medium", None), ("large", 16)])
+def test_chain_bit_exact(gpu, orc, name, frames):
+    sc = rd_scenario.config(name)
+    if frames:
+        sc.F = frames
+    got, exp, tot = _run_both(gpu, orc, sc)
+    for k in STAGES + ("row_count",):
+        a, b = got[k], exp[k]
+        bad = np.flatnonzero(a != b)
+        assert bad.size == 0, f"{name}: stage {k}: {bad.size} mismatches (first {bad[:4]})"
+    assert tot == exp["n_events"]
+    assert np.array_equal(got["events"], exp["events"]), f"{name}: events differ"
+    assert tot > sc.B * sc.F  # at least one detection beyond the per-row summaries
+
+
+def test_chain_saturating_scene(gpu, orc):
+    """Full-scale input and max taps drive PFB, FFT and beamformer saturation."""
+    sc = rd_scenario.config("tiny")
+    rng = np.random.default_rng(3)
+    vals = np.array([-32768, 32767, -32767, 0])
+    iq = rd_scenario.pack(rng.choice(vals, sc.CH * sc.N * sc.F), rng.choice(vals, sc.CH * sc.N * sc.F))
+    taps = rd_scenario.pack(np.full(sc.N * sc.T, -32768), np.full(sc.N * sc.T, -32768))
+    W = rd_scenario.pack(np.full(sc.B * sc.CH, -32768), np.full(sc.B * sc.CH, -32768))
+    ch = rd_amd.Chain(gpu, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg())
+    ch.load(iq, taps, W)
+    tot = ch.run()
+    got = ch.fetch(STAGES, n_events=tot)
+    exp = orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), 31, 63, keep=STAGES + ("events",))
+    assert got["pfb_flags"].any() and got["yflags"].any()
+    for k in STAGES:
+        assert np.array_equal(got[k], exp[k]), k
+    assert np.array_equal(got["events"], exp["events"])

@@ -1,0 +1,199 @@
+"""GPU parity: every HIP stage, called through the C ABI, against the oracle.
+
+Bit-exact on every output (values, flags, exact accumulators, packed events),
+on the reference's golden vectors, seeded random data, corner values that drive
+every saturation path, and the edge geometries the reference tests.
+"""
+import numpy as np
+import pytest
+
+import stim
+import vecio
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    bad = np.flatnonzero(a != b)
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+
+
+# ---------------------------------------------------------------------------- PFB
+@pytest.mark.parametrize("set_name,geom", vecio.PFB_SETS)
+def test_pfb_golden(gpu, set_name, geom):
+    P, T, coeff, v = vecio.load_pfb(set_name, geom)
+    y, fl = gpu.pfb(v["x"], coeff, P, T, 1, v["beats"])
+    _eq(y, v["y"], "y")
+    _eq(fl & 3, v["f_re"], "f_re")
+    _eq(fl >> 2, v["f_im"], "f_im")
+
+
+@pytest.mark.parametrize("P,T,nb,ch", [(64, 8, 16, 4), (1024, 16, 40, 3), (4096, 16, 20, 2), (256, 4, 33, 2),
+                                       (128, 5, 17, 2), (64, 1, 9, 1), (1000, 16, 70, 2), (32, 32, 12, 2)])
+def test_pfb_random_and_extreme(gpu, orc, P, T, nb, ch):
+    rng = np.random.default_rng(P + T + nb)
+    for x, h in [(stim.rand_cx(rng, ch * nb * P), stim.rand_cx(rng, P * T)),
+                 (stim.rand_extreme(rng, ch * nb * P), stim.rand_extreme(rng, P * T))]:
+        yg, fg = gpu.pfb(x, h, P, T, ch, nb)
+        yo, fo = orc.pfb(x, h, P, T, ch, nb)
+        _eq(yg, yo, "y")
+        _eq(fg, fo, "flags")
+
+
+def test_pfb_long_run_chunks(gpu, orc):
+    """Many beats per channel so the launch splits beats into chunks with warm-up."""
+    rng = np.random.default_rng(5)
+    P, T, nb, ch = 256, 16, 600, 1
+    x, h = stim.rand_cx(rng, ch * nb * P), stim.rand_cx(rng, P * T)
+    _eq(gpu.pfb(x, h, P, T, ch, nb)[0], orc.pfb(x, h, P, T, ch, nb)[0], "y")
+
+
+@pytest.mark.parametrize("T", [16, 5])
+def test_pfb_history_carry(gpu, orc, T):
+    rng = np.random.default_rng(T)
+    P, nb, ch = 128, 30, 3
+    x, h = stim.rand_cx(rng, ch * nb * P), stim.rand_cx(rng, P * T)
+    y_all, f_all = orc.pfb(x, h, P, T, ch, nb)
+    xs = x.reshape(ch, nb * P)
+    hist = np.zeros(ch * (T - 1) * P, np.uint32)
+    outs = []
+    for lo, hi in [(0, 2), (2, 13), (13, 30)]:  # includes a call shorter than T-1 beats
+        seg = np.ascontiguousarray(xs[:, lo * P: hi * P]).reshape(-1)
+        y, _ = gpu.pfb(seg, h, P, T, ch, hi - lo, hist=hist)
+        outs.append(y.reshape(ch, -1))
+    _eq(np.concatenate(outs, axis=1).reshape(-1), y_all, "y across calls")
+
+
+# ---------------------------------------------------------------------------- FFT
+def test_fft_golden(gpu):
+    hdr, recs = vecio.load_fft()
+    for r in recs:
+        out, bins, sf = gpu.fft(r["inp"], hdr["n"], hdr["spc"], r["sched"], r["reorder"])
+        _eq(out, r["out"], r["id"])
+        assert int(sf[0]) == r["flags"], r["id"]
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("spc", [1, 2])
+def test_fft_random(gpu, orc, n, spc):
+    if n // spc < 4:
+        pytest.skip("geometry rejected by the reference")
+    rng = np.random.default_rng(n * 7 + spc)
+    frames = max(3, 4096 // n)
+    xs = [stim.rand_cx(rng, frames * n), stim.rand_extreme(rng, frames * n), stim.rand_cx(rng, frames * n, 2000)]
+    for x in xs:
+        for sched in (0xFFFFFFFF, 0, int(rng.integers(0, 2**32))):
+            for reorder in (1, 0):
+                g = gpu.fft(x, n, spc, sched, reorder)
+                o = orc.fft(x, n, spc, sched, reorder)
+                for a, b, w in zip(g, o, ("out", "bins", "stage_flags")):
+                    _eq(a, b, f"{w} n={n} spc={spc} sched={sched:#x} reorder={reorder}")
+
+
+def test_fft_rejects_invalid_geometry(gpu):
+    import rd_amd
+
+    for n, spc in [(4, 1), (8, 4), (6, 1), (8192, 2)]:
+        with pytest.raises(rd_amd.RdError) as e:
+            gpu.fft(np.zeros(max(n, 8) * 2, np.uint32), n, spc)
+        assert e.value.code == rd_amd.RD_E_GEOM
+
+
+# ---------------------------------------------------------------------------- align
+def test_align(gpu, orc):
+    rng = np.random.default_rng(1)
+    for CH, F, N in [(4, 16, 64), (16, 3, 1024), (7, 5, 100), (64, 2, 4096)]:
+        X = stim.rand_cx(rng, CH * F * N)
+        _eq(gpu.align(X, CH, F, N), orc.align(X, CH, F, N), f"align {CH}x{F}x{N}")
+
+
+# ---------------------------------------------------------------------------- beamformer
+@pytest.mark.parametrize("CH,B", [(4, 4), (16, 16), (64, 64), (7, 3), (33, 17), (64, 128), (1, 1), (48, 20)])
+def test_bf_random_and_extreme(gpu, orc, CH, B):
+    rng = np.random.default_rng(CH * 131 + B)
+    cols = 1000
+    for Xa, W in [(stim.rand_cx(rng, cols * CH), stim.rand_cx(rng, B * CH)),
+                  (stim.rand_extreme(rng, cols * CH), stim.rand_extreme(rng, B * CH)),
+                  (np.full(cols * CH, vecio.pack(-32768, -32768), np.uint32),
+                   np.full(B * CH, vecio.pack(-32768, -32768), np.uint32))]:
+        g = gpu.bf(Xa, W, CH, B)
+        o = orc.bf(Xa, W, CH, B)
+        for a, b, w in zip(g, o, ("y", "flags", "power", "acc")):
+            _eq(a, b, f"{w} CH={CH} B={B}")
+
+
+def test_bf_unit_weights_passthrough(gpu):
+    rng = np.random.default_rng(2)
+    CH, B, cols = 16, 16, 512
+    Xa = stim.rand_cx(rng, cols * CH)
+    W = np.zeros(B * CH, np.uint32)
+    for b in range(B):
+        W[b * CH + (b * 5) % CH] = vecio.pack(32767, 0)
+    y, fl, _, _ = gpu.bf(Xa, W, CH, B)
+    x = Xa.reshape(cols, CH)
+    for b in range(B):
+        col = x[:, (b * 5) % CH]
+        exp = [vecio.pack(stim.rne_int(vecio.unpack(v)[0] * 32767, 15), stim.rne_int(vecio.unpack(v)[1] * 32767, 15))
+               for v in col.tolist()]
+        _eq(y[b * cols:(b + 1) * cols], np.array(exp, np.uint32), f"beam {b}")
+    assert not fl.any()
+
+
+# ---------------------------------------------------------------------------- covariance
+@pytest.mark.parametrize("CH,F,N", [(4, 16, 64), (16, 40, 33), (64, 20, 5), (1, 7, 9), (20, 33, 8), (33, 17, 6),
+                                    (64, 512, 2), (16, 256, 12)])
+def test_cov_random_and_extreme(gpu, orc, CH, F, N):
+    rng = np.random.default_rng(CH * 7 + F + N)
+    for Xa in (stim.rand_cx(rng, F * N * CH), stim.rand_extreme(rng, F * N * CH),
+               np.full(F * N * CH, vecio.pack(-32768, -32768), np.uint32)):
+        _eq(gpu.cov(Xa, CH, F, N), orc.cov(Xa, CH, F, N)[0], f"R CH={CH} F={F} N={N}")
+
+
+# ---------------------------------------------------------------------------- CFAR
+CFAR_CFGS = [
+    [1, 0, 0, 2, 2, 8, 8, 0x800],
+    [1, 1, 0, 2, 3, 8, 4, 0x300],
+    [1, 0, 1, 1, 1, 4, 4, 0x200],
+    [1, 1, 1, 0, 0, 1, 0, 0x100],
+    [0, 0, 0, 2, 2, 8, 8, 0x800],
+    [1, 0, 0, 40, 2, 80, 8, 0xFFFF],
+    [1, 0, 0, 4, 4, 32, 32, 0x0800],
+    [1, 0, 0, 0, 0, 0, 0, 0x0800],
+    [1, 0, 1, 4, 4, 16, 16, 0x0800],
+]
+
+
+@pytest.mark.parametrize("cfg", CFAR_CFGS)
+@pytest.mark.parametrize("n", [64, 1024, 4096, 100])
+def test_cfar_u64(gpu, orc, cfg, n):
+    rng = np.random.default_rng(n + sum(cfg))
+    rows = 5
+    cells = rng.integers(0, 2**31, rows * n).astype(np.uint64)
+    cells[rng.integers(0, rows * n, 30)] = 2**39
+    cells[rng.integers(0, rows * n, 5)] = (1 << 63) + 77
+    fid = (np.arange(rows) * 13 + 2).astype(np.uint32)
+    eg, rcg, tot, _ = gpu.cfar(cells, n, cfg, 31, 63, frame_ids=fid)
+    eo, rco = orc.cfar(cells, n, cfg, 31, 63, frame_ids=fid)
+    _eq(rcg, rco, "row_count")
+    assert tot == eo.shape[0]
+    _eq(eg, eo, "events")
+
+
+def test_cfar_u32_frame_mod_and_capacity(gpu, orc):
+    import rd_amd
+
+    rng = np.random.default_rng(4)
+    n, rows, F = 1024, 12, 4
+    cells32 = rng.integers(0, 2**31, rows * n).astype(np.uint32)
+    cells32[rng.integers(0, rows * n, 50)] = 2**31
+    cfg = [1, 0, 0, 4, 4, 16, 16, 0x600]
+    eg, rcg, tot, rc = gpu.cfar(cells32, n, cfg, 31, 63, frame_mod=F)
+    eo, rco = orc.cfar(cells32.astype(np.uint64), n, cfg, 31, 63, frame_ids=np.arange(rows) % F)
+    _eq(eg, eo, "events")
+    assert rc == rd_amd.RD_OK
+    cap = tot - 3
+    eg2, _, tot2, rc2 = gpu.cfar(cells32, n, cfg, 31, 63, frame_mod=F, cap=cap)
+    assert rc2 == rd_amd.RD_E_CAPACITY and tot2 == tot
+    _eq(eg2, eo[:cap], "events under capacity")
