@@ -41,27 +41,36 @@ __host__ __device__ inline CfarGeom clamp_geom(const rd_cfar_cfg& c, uint32_t ma
   return g;
 }
 
-constexpr int kCfarThreads = 256;
-
-// Block-wide exclusive scan of one uint64 per thread; returns the block total.
-__device__ uint64_t block_exscan(uint64_t v, uint64_t* out_prefix, uint64_t* wsum /* [kCfarThreads/64] */) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// One wave per row; a block holds up to 4 rows (waves), each with its own
+// exclusive prefix P[0..n] in LDS. Lane l owns the contiguous bins
+// [l*per, (l+1)*per), per = ceil(n / 64).
+__device__ __forceinline__ uint64_t wave_exscan(uint64_t v, uint32_t lane, uint64_t* total) {
   uint64_t x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t y = __shfl_up(x, o);
     if (lane >= (uint32_t)o) x += y;
   }
-  if (lane == 63) wsum[wave] = x;
-  __syncthreads();
-  uint64_t before = 0, total = 0;
-  for (uint32_t w = 0; w < kCfarThreads / 64; ++w) {
-    if (w < wave) before += wsum[w];
-    total += wsum[w];
+  *total = __shfl(x, 63);
+  return x - v;
+}
+
+template <typename Cell>
+__device__ __forceinline__ void row_prefix(const Cell* row, uint32_t n, uint32_t per, uint32_t lane, uint64_t* P) {
+  const uint32_t b0 = lane * per;
+  uint64_t local = 0;
+  for (uint32_t q = 0; q < per; ++q)
+    if (b0 + q < n) local += (uint64_t)row[b0 + q];
+  uint64_t total;
+  uint64_t run = wave_exscan(local, lane, &total);
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t i = b0 + q;
+    if (i < n) {
+      P[i] = run;
+      run += (uint64_t)row[i];
+    }
   }
-  *out_prefix = before + x - v;
-  __syncthreads();
-  return total;
+  if (lane == 0) P[n] = total;
 }
 
 struct Decision {
@@ -94,48 +103,24 @@ __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32
   return d;
 }
 
-// Loads the row into LDS as an exclusive prefix P[0..n]; each thread owns a
-// contiguous chunk of `per` bins starting at t*per (read twice from global
-// memory: once for the chunk total, once to write the prefixes).
 template <typename Cell>
-__device__ void load_prefix(const Cell* row, uint32_t n, uint32_t per, uint64_t* P, uint64_t* wsum) {
-  const uint32_t b0 = threadIdx.x * per;
-  uint64_t local = 0;
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i < n) local += (uint64_t)row[i];
-  }
-  uint64_t before;
-  const uint64_t total = block_exscan(local, &before, wsum);
-  uint64_t run = before;
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i < n) {
-      P[i] = run;
-      run += (uint64_t)row[i];
-    }
-  }
-  if (threadIdx.x == 0) P[n] = total;
+__global__ __launch_bounds__(256) void cfar_count_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
+                                                         CfarGeom g, uint32_t* __restrict__ row_count) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
+  uint64_t* P = lds + (size_t)wave * (n + 1);
+  const uint32_t per = (n + 63) / 64;
+  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, per, lane, P);
   __syncthreads();
-}
-
-template <typename Cell>
-__global__ __launch_bounds__(kCfarThreads) void cfar_count_kernel(const Cell* __restrict__ cells, uint32_t n,
-                                                                  CfarGeom g, uint32_t* __restrict__ row_count) {
-  extern __shared__ uint64_t P[];  // [n + 1]
-  __shared__ uint64_t wsum[kCfarThreads / 64];
-  const uint32_t r = blockIdx.x;
-  const uint32_t per = (n + kCfarThreads - 1) / kCfarThreads;
-  load_prefix(cells + (size_t)r * n, n, per, P, wsum);
+  if (r >= n_rows) return;
   uint64_t det = 0;
-  const uint32_t b0 = threadIdx.x * per;
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i < n) det += decide(P, n, i, g).hit ? 1 : 0;
-  }
-  uint64_t before;
-  const uint64_t tot = block_exscan(det, &before, wsum);
-  if (threadIdx.x == 0) row_count[r] = (uint32_t)((g.dense ? n : tot) + 1);
+  const uint32_t b0 = lane * per;
+  for (uint32_t q = 0; q < per; ++q)
+    if (b0 + q < n) det += decide(P, n, b0 + q, g).hit ? 1 : 0;
+  uint64_t tot;
+  wave_exscan(det, lane, &tot);
+  if (lane == 0) row_count[r] = (uint32_t)((g.dense ? n : tot) + 1);
 }
 
 // Exclusive scan of row counts into 64-bit offsets; offs[n_rows] = total.
@@ -184,17 +169,20 @@ __device__ __forceinline__ void put_event(uint64_t* w, uint32_t kind, uint32_t b
 }
 
 template <typename Cell>
-__global__ __launch_bounds__(kCfarThreads) void cfar_emit_kernel(const Cell* __restrict__ cells, uint32_t n,
-                                                                 CfarGeom g, const uint32_t* __restrict__ frame_ids,
-                                                                 uint32_t frame_mod, const uint64_t* __restrict__ offs,
-                                                                 uint64_t* __restrict__ events, uint64_t cap) {
-  extern __shared__ uint64_t P[];
-  __shared__ uint64_t wsum[kCfarThreads / 64];
-  const uint32_t r = blockIdx.x;
-  const uint32_t per = (n + kCfarThreads - 1) / kCfarThreads;
-  load_prefix(cells + (size_t)r * n, n, per, P, wsum);
+__global__ __launch_bounds__(256) void cfar_emit_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
+                                                        CfarGeom g, const uint32_t* __restrict__ frame_ids,
+                                                        uint32_t frame_mod, const uint64_t* __restrict__ offs,
+                                                        uint64_t* __restrict__ events, uint64_t cap) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
+  uint64_t* P = lds + (size_t)wave * (n + 1);
+  const uint32_t per = (n + 63) / 64;
+  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, per, lane, P);
+  __syncthreads();
+  if (r >= n_rows) return;
   const uint32_t frame = frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
-  const uint32_t b0 = threadIdx.x * per;
+  const uint32_t b0 = lane * per;
   uint64_t nemit = 0, ndet = 0, nsup = 0;
   for (uint32_t q = 0; q < per; ++q) {
     const uint32_t i = b0 + q;
@@ -204,24 +192,25 @@ __global__ __launch_bounds__(kCfarThreads) void cfar_emit_kernel(const Cell* __r
     nsup += !d.eval;
     nemit += (d.hit || g.dense) ? 1 : 0;
   }
-  uint64_t before, bdet, bsup;
-  const uint64_t emitted = block_exscan(nemit, &before, wsum);
-  const uint64_t tdet = block_exscan(ndet, &bdet, wsum);
-  const uint64_t tsup = block_exscan(nsup, &bsup, wsum);
+  uint64_t emitted, tdet, tsup;
+  const uint64_t before = wave_exscan(nemit, lane, &emitted);
+  wave_exscan(ndet, lane, &tdet);
+  wave_exscan(nsup, lane, &tsup);
   const uint64_t base = offs[r];
   uint64_t k = before;
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i >= n) break;
-    const Decision d = decide(P, n, i, g);
-    if (!(d.hit || g.dense)) continue;
-    const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
-    if (base + k < cap)
-      put_event(events + 3 * (base + k), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
-                d.eval ? d.sum : 0ull, k == 0, false);
-    ++k;
-  }
-  if (threadIdx.x == 0 && base + emitted < cap)
+  if (nemit)
+    for (uint32_t q = 0; q < per; ++q) {
+      const uint32_t i = b0 + q;
+      if (i >= n) break;
+      const Decision d = decide(P, n, i, g);
+      if (!(d.hit || g.dense)) continue;
+      const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
+      if (base + k < cap)
+        put_event(events + 3 * (base + k), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
+                  d.eval ? d.sum : 0ull, k == 0, false);
+      ++k;
+    }
+  if (lane == 0 && base + emitted < cap)
     put_event(events + 3 * (base + emitted), 3u, n, frame, 0, g.alpha, (uint32_t)tdet, (uint32_t)tsup, 0, 0,
               emitted == 0, true);
 }
@@ -234,25 +223,29 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        uint64_t* total_dev) {
   if (n_rows == 0) return hipSuccess;
   const CfarGeom g = clamp_geom(cfg, max_guard, max_ref);
-  const size_t lds = ((size_t)n + 1) * sizeof(uint64_t);
+  const size_t row_lds = ((size_t)n + 1) * sizeof(uint64_t);
+  uint32_t rpb = (uint32_t)(49152 / row_lds);  // rows (waves) per block
+  rpb = rpb < 1 ? 1 : (rpb > 4 ? 4 : rpb);
+  const size_t lds = rpb * row_lds;
+  const dim3 grid((n_rows + rpb - 1) / rpb), block(64 * rpb);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   if (cell_bytes == 4)
-    cfar_count_kernel<uint32_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint32_t*>(cells), n, g,
-                                                                  row_count);
+    cfar_count_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
+                                                         row_count);
   else
-    cfar_count_kernel<uint64_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint64_t*>(cells), n, g,
-                                                                  row_count);
+    cfar_count_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
+                                                         row_count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   cfar_scan_kernel<<<1, 1024, 0, s>>>(row_count, n_rows, offs, total_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (cell_bytes == 4)
-    cfar_emit_kernel<uint32_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint32_t*>(cells), n, g,
-                                                                 frame_ids, frame_mod, offs, events, cap);
+    cfar_emit_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
+                                                        frame_ids, frame_mod, offs, events, cap);
   else
-    cfar_emit_kernel<uint64_t><<<n_rows, kCfarThreads, lds, s>>>(static_cast<const uint64_t*>(cells), n, g,
-                                                                 frame_ids, frame_mod, offs, events, cap);
+    cfar_emit_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
+                                                        frame_ids, frame_mod, offs, events, cap);
   return hipGetLastError();
 }
 

@@ -122,7 +122,9 @@ __global__ void bf_prep_kernel(const uint32_t* __restrict__ W, uint32_t CH, uint
 
 constexpr int kBfColTilesPerWave = 4;
 
-template <int STEPS>
+// VEC8: n_ant is a multiple of 8 and Xa is 32-byte aligned, so each lane's 8
+// channels of a column are two aligned 16-byte loads.
+template <int STEPS, bool VEC8>
 __global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ Xa, const uint32_t* __restrict__ prep,
                                                  uint32_t CH, uint32_t B, uint32_t bt_n, uint64_t cols,
                                                  uint32_t* __restrict__ y, uint8_t* __restrict__ flags,
@@ -154,11 +156,25 @@ __global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ Xa
       const uint32_t a0 = imag ? k0 - chp : k0;
       uint32_t p[8];
       uint64_t vmask = 0;
+      if (VEC8) {
+        const bool ok = col_ok && a0 < CH;
+        if (ok) {
+          const uint4* src = reinterpret_cast<const uint4*>(Xa + col * CH + a0);
+          const uint4 v0 = src[0], v1 = src[1];
+          p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w;
+          p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
+        } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const bool ok = col_ok && a0 + q < CH;
-        p[q] = ok ? Xa[col * CH + a0 + q] : 0u;
-        vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
+          for (int q = 0; q < 8; ++q) p[q] = 0u;
+        }
+        vmask = ok ? ~0ull : 0ull;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const bool ok = col_ok && a0 + q < CH;
+          p[q] = ok ? Xa[col * CH + a0 + q] : 0u;
+          vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
+        }
       }
       const Dig8 d = digits8(p, vmask);
       int32_t sr = 0, si = 0;
@@ -239,12 +255,16 @@ hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint3
   const uint64_t cols_per_block = 16ull * 4 * kBfColTilesPerWave;
   const dim3 grid((uint32_t)((n_cols + cols_per_block - 1) / cols_per_block));
   const size_t lds = bf_prep_bytes(n_ant, n_beams);
+  const bool vec8 = (n_ant % 8) == 0 && ((uintptr_t)Xa & 31u) == 0;
+#define RD_BF_LAUNCH(S, V) \
+  bf_kernel<S, V><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc)
   switch (steps) {
-    case 1: bf_kernel<1><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
-    case 2: bf_kernel<2><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
-    case 3: bf_kernel<3><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
-    default: bf_kernel<4><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc); break;
+    case 1: if (vec8) RD_BF_LAUNCH(1, true); else RD_BF_LAUNCH(1, false); break;
+    case 2: if (vec8) RD_BF_LAUNCH(2, true); else RD_BF_LAUNCH(2, false); break;
+    case 3: if (vec8) RD_BF_LAUNCH(3, true); else RD_BF_LAUNCH(3, false); break;
+    default: if (vec8) RD_BF_LAUNCH(4, true); else RD_BF_LAUNCH(4, false); break;
   }
+#undef RD_BF_LAUNCH
   return hipGetLastError();
 }
 
@@ -254,131 +274,136 @@ hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint3
 // per bin over K = 2F (frames, real then imaginary part):
 //   Re: A_i = [Xr_i | Xi_i],  B_j = [Xr_j | Xi_j]
 //   Im: A_i = [Xi_i | ~Xr_i], B_j = [Xr_j | Xi_j], plus sum_m xi_j
-// A block owns NB bins (NB * CHp = 64) and their upper-triangle 16x16 channel
-// tiles; each wave takes a few (bin, tile) jobs and keeps their six
-// accumulators in registers across all frames. Frames are staged through LDS
-// 16 at a time: one row [NB bins][CH] per frame is NB*CH*4 contiguous bytes.
+// A block owns NB bins (NB * CHp = 64, so one frame row of the block is
+// NB*CH*4 contiguous bytes of Xa) and their upper-triangle 16x16 channel tiles
+// ("jobs"). Each wave runs one job over one of FS frame splits; frames are
+// staged through LDS 16 at a time per split. Every wave turns its partial
+// sums into partial int64 results (all correction terms are linear in the
+// frames), and the splits of a job are added with LDS 64-bit atomics.
 // ============================================================================
-constexpr int kCovMaxJobsPerWave = 4;
-
-__global__ __launch_bounds__(256) void cov_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
-                                                  uint32_t N, uint32_t nb, uint32_t nt, int64_t* __restrict__ R) {
-  __shared__ uint32_t tile[16 * 64];  // [16 frames][nb * CH]
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t grp = lane >> 4, cl = lane & 15;
-  const uint32_t k_base = blockIdx.x * nb;
-  const uint32_t row_words = nb * CH;
+__global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
+                                                   uint32_t N, uint32_t nb, uint32_t nt, uint32_t fs_n,
+                                                   int64_t* __restrict__ R) {
+  extern __shared__ uint32_t smem[];
   const uint32_t ut = nt * (nt + 1) / 2;
   const uint32_t jobs = nb * ut;
-
-  // job -> (bin, ti, tj), upper triangle ti <= tj
-  uint32_t jb[kCovMaxJobsPerWave], jti[kCovMaxJobsPerWave], jtj[kCovMaxJobsPerWave];
-  bool jok[kCovMaxJobsPerWave];
-#pragma unroll
-  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
-    const uint32_t job = wave + 4 * q;
-    jok[q] = job < jobs && k_base + job / ut < N;
-    const uint32_t bin = job / ut;
-    uint32_t u = job % ut, ti = 0;
-    while (u >= nt - ti) {
-      u -= nt - ti;
-      ++ti;
-    }
-    jb[q] = bin;
-    jti[q] = ti;
-    jtj[q] = ti + u;
-  }
-
-  v4i acc[kCovMaxJobsPerWave][6];
-  int32_t sra[kCovMaxJobsPerWave], sia[kCovMaxJobsPerWave], srb[kCovMaxJobsPerWave], sib[kCovMaxJobsPerWave];
-#pragma unroll
-  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
-    for (int z = 0; z < 6; ++z) acc[q][z] = v4i{0, 0, 0, 0};
-    sra[q] = sia[q] = srb[q] = sib[q] = 0;
-  }
-
+  const uint32_t row_words = nb * CH;
+  uint32_t* tiles = smem;                                                      // [fs_n][16][row_words]
+  unsigned long long* red = reinterpret_cast<unsigned long long*>(smem + fs_n * 16 * 64);  // [jobs][64][8]
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+  const uint32_t job = wave % jobs, fs = wave / jobs;
+  const uint32_t k_base = blockIdx.x * nb;
   const uint32_t bins_here = min(nb, N - k_base);
-  for (uint32_t m0 = 0; m0 < F; m0 += 16) {
+
+  const uint32_t bin = job / ut;
+  uint32_t u = job % ut, ti = 0;
+  while (u >= nt - ti) {
+    u -= nt - ti;
+    ++ti;
+  }
+  const uint32_t tj = ti + u;
+  const bool live = bin < bins_here;
+  const uint32_t ci = 16 * ti + cl, cj = 16 * tj + cl;
+
+  if (fs_n > 1)
+    for (uint32_t i = threadIdx.x; i < jobs * 64 * 8; i += blockDim.x) red[i] = 0;
+
+  const uint32_t nsteps = (F + 15) / 16;
+  const uint32_t sps = (nsteps + fs_n - 1) / fs_n;  // steps per split
+  v4i acc[6];
+  for (int z = 0; z < 6; ++z) acc[z] = v4i{0, 0, 0, 0};
+  int32_t sra = 0, sia = 0, srb = 0, sib = 0, nfr = 0;
+
+  for (uint32_t st = 0; st < sps; ++st) {
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < 16 * row_words; idx += blockDim.x) {
-      const uint32_t f = idx / row_words, w = idx % row_words;
-      const uint32_t bin = w / CH;
-      tile[idx] = (m0 + f < F && bin < bins_here) ? Xa[((size_t)(m0 + f) * N + k_base) * CH + w] : 0u;
+    for (uint32_t idx = threadIdx.x; idx < fs_n * 16 * row_words; idx += blockDim.x) {
+      const uint32_t sp = idx / (16 * row_words), rem = idx % (16 * row_words);
+      const uint32_t f = rem / row_words, w = rem % row_words;
+      const uint32_t m = (sp * sps + st) * 16 + f;
+      tiles[idx] = (m < F && w / CH < bins_here) ? Xa[((size_t)m * N + k_base) * CH + w] : 0u;
     }
     __syncthreads();
-    // this lane's 8 frames: f = 8*(grp & 1) + t; part = grp >> 1
+    const uint32_t m0 = (fs * sps + st) * 16;
     const uint32_t f0 = 8 * (grp & 1);
     uint64_t fmask = 0;
 #pragma unroll
     for (int t = 0; t < 8; ++t) fmask |= (m0 + f0 + t < F) ? (0xFFull << (8 * t)) : 0ull;
+    if (m0 < F) nfr += (int32_t)min(16u, F - m0);
     const bool imag = (grp >> 1) != 0;
+    const uint32_t* tl = tiles + fs * 16 * row_words;
+    uint32_t pa[8], pb[8];
 #pragma unroll
-    for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
-      if (!jok[q]) continue;
-      const uint32_t ci = 16 * jti[q] + cl, cj = 16 * jtj[q] + cl;
-      uint32_t pa[8], pb[8];
+    for (int t = 0; t < 8; ++t) {
+      const uint32_t base = (f0 + t) * row_words + bin * CH;
+      pa[t] = (live && ci < CH) ? tl[base + ci] : 0u;
+      pb[t] = (live && cj < CH) ? tl[base + cj] : 0u;
+    }
+    const uint64_t amask = ci < CH ? fmask : 0ull;
+    const Dig8 da = digits8(pa, amask);
+    const Dig8 db = digits8(pb, cj < CH ? fmask : 0ull);
+    if (!imag) {
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        const uint32_t base = (f0 + t) * row_words + jb[q] * CH;
-        pa[t] = ci < CH ? tile[base + ci] : 0u;
-        pb[t] = cj < CH ? tile[base + cj] : 0u;
+        sra += cre(pa[t]);
+        sia += cim(pa[t]);
+        srb += cre(pb[t]);
+        sib += cim(pb[t]);
       }
-      const Dig8 da = digits8(pa, ci < CH ? fmask : 0ull);
-      const Dig8 db = digits8(pb, cj < CH ? fmask : 0ull);
-      if (!imag) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          sra[q] += cre(pa[t]);
-          sia[q] += cim(pa[t]);
-          srb[q] += cre(pb[t]);
-          sib[q] += cim(pb[t]);
-        }
-      }
-      const uint64_t amask = ci < CH ? fmask : 0ull;
-      // Re: A = [Xr | Xi], B = [Xr | Xi];  Im: A = [Xi | ~Xr]
-      const uint64_t reah = imag ? da.hi : da.hr, real_ = imag ? da.li : da.lr;
-      const uint64_t imah = imag ? (~da.hr & amask) : da.hi, imal = imag ? (~da.lr & amask) : da.li;
-      const uint64_t bh = imag ? db.hi : db.hr, bl = imag ? db.li : db.lr;
-      acc[q][0] = mfma8(reah, bh, acc[q][0]);
-      acc[q][1] = mfma8(reah, bl, acc[q][1]);
-      acc[q][1] = mfma8(real_, bh, acc[q][1]);
-      acc[q][2] = mfma8(real_, bl, acc[q][2]);
-      acc[q][3] = mfma8(imah, bh, acc[q][3]);
-      acc[q][4] = mfma8(imah, bl, acc[q][4]);
-      acc[q][4] = mfma8(imal, bh, acc[q][4]);
-      acc[q][5] = mfma8(imal, bl, acc[q][5]);
     }
+    // Re: A = [Xr | Xi], B = [Xr | Xi];  Im: A = [Xi | ~Xr]
+    const uint64_t reah = imag ? da.hi : da.hr, real_ = imag ? da.li : da.lr;
+    const uint64_t imah = imag ? (~da.hr & amask) : da.hi, imal = imag ? (~da.lr & amask) : da.li;
+    const uint64_t bh = imag ? db.hi : db.hr, bl = imag ? db.li : db.lr;
+    acc[0] = mfma8(reah, bh, acc[0]);
+    acc[1] = mfma8(reah, bl, acc[1]);
+    acc[1] = mfma8(real_, bh, acc[1]);
+    acc[2] = mfma8(real_, bl, acc[2]);
+    acc[3] = mfma8(imah, bh, acc[3]);
+    acc[4] = mfma8(imah, bl, acc[4]);
+    acc[4] = mfma8(imal, bh, acc[4]);
+    acc[5] = mfma8(imal, bl, acc[5]);
   }
 
+  // per-channel frame sums of this split: lanes of groups 0/1 hold the partials
+  int32_t a_r = sra + __shfl_xor(sra, 16), a_i = sia + __shfl_xor(sia, 16);
+  int32_t b_r = srb + __shfl_xor(srb, 16), b_i = sib + __shfl_xor(sib, 16);
+  a_r += __shfl_xor(a_r, 32);
+  a_i += __shfl_xor(a_i, 32);
+  b_r += __shfl_xor(b_r, 32);
+  b_i += __shfl_xor(b_i, 32);
+  const int64_t kconst = 16384LL * 2 * nfr;
+  const int64_t tjs = (int64_t)b_r + b_i;
+  int64_t part[8];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t il = 4 * grp + r;
+    const int32_t ir = __shfl(a_r, il), ii = __shfl(a_i, il);
+    const int64_t tis = (int64_t)ir + ii, uis = (int64_t)ii - ir - (int64_t)nfr;
+    part[2 * r] = combine(acc[0], acc[1], acc[2], r) + 128 * (tis + tjs) - kconst;
+    part[2 * r + 1] = combine(acc[3], acc[4], acc[5], r) + 128 * (uis + tjs) - kconst + b_i;
+  }
+  if (fs_n > 1) {
+    __syncthreads();
+    if (wave < jobs * fs_n)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) atomicAdd(&red[(job * 64 + lane) * 8 + q], (unsigned long long)part[q]);
+    __syncthreads();
+    if (fs != 0) return;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) part[q] = (int64_t)red[(job * 64 + lane) * 8 + q];
+  }
+  if (!live) return;
+  const uint32_t k = k_base + bin;
   const size_t npairs = (size_t)CH * (CH + 1) / 2;
-  const int64_t kconst = 16384LL * 2 * F;
 #pragma unroll
-  for (int q = 0; q < kCovMaxJobsPerWave; ++q) {
-    // per-channel frame sums: lanes of groups 0/1 hold the partials
-    int32_t a_r = sra[q] + __shfl_xor(sra[q], 16), a_i = sia[q] + __shfl_xor(sia[q], 16);
-    int32_t b_r = srb[q] + __shfl_xor(srb[q], 16), b_i = sib[q] + __shfl_xor(sib[q], 16);
-    a_r += __shfl_xor(a_r, 32);
-    a_i += __shfl_xor(a_i, 32);
-    b_r += __shfl_xor(b_r, 32);
-    b_i += __shfl_xor(b_i, 32);
-    if (!jok[q]) continue;
-    const uint32_t k = k_base + jb[q];
-    const uint32_t cj = 16 * jtj[q] + cl;
-    const int64_t tj = (int64_t)b_r + b_i;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const uint32_t il = 4 * grp + r;
-      const int32_t ir = __shfl(a_r, il), ii = __shfl(a_i, il);
-      const uint32_t ci = 16 * jti[q] + il;
-      if (ci >= CH || cj >= CH || ci > cj) continue;
-      const int64_t ti = (int64_t)ir + ii, ui = (int64_t)ii - ir - (int64_t)F;
-      const int64_t re = combine(acc[q][0], acc[q][1], acc[q][2], r) + 128 * (ti + tj) - kconst;
-      const int64_t im = combine(acc[q][3], acc[q][4], acc[q][5], r) + 128 * (ui + tj) - kconst + b_i;
-      const size_t pair = (size_t)ci * CH - (size_t)ci * (ci - 1) / 2 + (cj - ci);
-      int64_t* o = R + ((size_t)k * npairs + pair) * 2;
-      o[0] = re;
-      o[1] = im;
-    }
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t ii = 16 * ti + 4 * grp + r;
+    if (ii >= CH || cj >= CH || ii > cj) continue;
+    const size_t pair = (size_t)ii * CH - (size_t)ii * (ii - 1) / 2 + (cj - ii);
+    int64_t* o = R + ((size_t)k * npairs + pair) * 2;
+    o[0] = part[2 * r];
+    o[1] = part[2 * r + 1];
   }
 }
 
@@ -387,8 +412,14 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
   if (n_bins == 0) return hipSuccess;
   const uint32_t chp = (n_chan + 15) / 16 * 16, nt = chp / 16;
   const uint32_t nb = 64 / chp;
+  const uint32_t jobs = nb * nt * (nt + 1) / 2;
+  uint32_t fs_n = 16 / jobs;
+  const uint32_t nsteps = (n_frames + 15) / 16;
+  if (fs_n > nsteps) fs_n = nsteps;
+  if (fs_n < 1) fs_n = 1;
+  const size_t lds = (size_t)fs_n * 16 * 64 * 4 + (fs_n > 1 ? (size_t)jobs * 64 * 8 * 8 : 0);
   const dim3 grid((n_bins + nb - 1) / nb);
-  cov_kernel<<<grid, 256, 0, s>>>(Xa, n_chan, n_frames, n_bins, nb, nt, R);
+  cov_kernel<<<grid, 64 * jobs * fs_n, lds, s>>>(Xa, n_chan, n_frames, n_bins, nb, nt, fs_n, R);
   return hipGetLastError();
 }
 

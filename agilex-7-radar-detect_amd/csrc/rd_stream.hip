@@ -53,12 +53,20 @@ __device__ __forceinline__ Digits split(uint32_t v) {
   return Digits{pk16(xrh, -xih), pk16(xrh, xih), pk16(xrl, -xil), pk16(xrl, xil)};
 }
 
+// round_sat(256*ah + al, 15, 16) without 64-bit arithmetic: with
+// ah = 128*qh + rh and al = 2^15*ql + rl (floor division), the exact value is
+// 2^15*(qh + ql) + (256*rh + rl), and the last term lies in [0, 2^16).
+__device__ __forceinline__ int32_t rne15_split(int32_t ah, int32_t al) {
+  const int32_t s = ((ah & 127) << 8) + (al & 0x7FFF);
+  const int32_t q = (ah >> 7) + (al >> 15) + (s >> 15);
+  const int32_t rem = s & 0x7FFF;
+  return q + ((rem + (q & 1)) > 0x4000 ? 1 : 0);
+}
+
 __device__ __forceinline__ uint32_t pfb_finish(int32_t arh, int32_t arl, int32_t aih, int32_t ail, uint8_t* fl) {
-  const int64_t are = (int64_t)arh * 256 + arl;
-  const int64_t aim = (int64_t)aih * 256 + ail;
   uint32_t fr = 0, fi = 0;
-  const int32_t yr = sat16_i64(rne15_i64(are), fr);
-  const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+  const int32_t yr = sat16(rne15_split(arh, arl), fr);
+  const int32_t yi = sat16(rne15_split(aih, ail), fi);
   *fl = (uint8_t)(fr | (fi << 2));
   return cpack(yr, yi);
 }
@@ -172,11 +180,15 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
                       uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags) {
   if (n_beats == 0 || n_chan == 0) return hipSuccess;
   const uint32_t bx = (P + 255) / 256;
-  // Beat chunks: enough blocks to fill 256 CUs several times over, with each
-  // chunk long enough that the T-1 warm-up beats stay a small overhead.
-  uint32_t chunk = n_beats;
-  const uint64_t base_blocks = (uint64_t)bx * n_chan;
-  while (chunk > 4u * T && base_blocks * ((n_beats + chunk - 1) / chunk) < 2048) chunk = (chunk + 1) / 2;
+  // Beat chunks: about 4096 waves in flight (16 per CU at the register budget of
+  // the T = 16 kernel) so the dependent load/compute chains of different chunks
+  // overlap; a chunk never gets shorter than T beats, which bounds the warm-up
+  // re-reads (T - 1 beats per chunk, mostly L2 hits) to under 2x.
+  const uint64_t lanes = (uint64_t)P * n_chan;
+  const uint64_t want = (262144 + lanes - 1) / lanes;
+  uint32_t chunk = (uint32_t)((n_beats + want - 1) / want);
+  if (chunk < T) chunk = T;
+  if (chunk > n_beats) chunk = n_beats;
   const dim3 grid(bx, n_chan, (n_beats + chunk - 1) / chunk);
   const dim3 block(P < 256 ? ((P + 63) / 64) * 64 : 256);
   switch (T) {
@@ -319,7 +331,7 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
       e3 = cmul_tw(e3, tw[(3 * i) << tsh], f1);
       e0 = cmul_tw(e0, tw[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
     }
-    __syncthreads();
+    // each thread writes back exactly the four positions it read
     if (active) {
       v[base] = cpack(e0.re, e0.im);
       v[base + quarter] = cpack(e1.re, e1.im);

@@ -101,7 +101,8 @@ def time_stage(torch, stream, fn, reps: int) -> float:
 def cpu_baseline(sc, iq, taps, W, budget_s: float):
     import oracle
 
-    kind = "reference" if oracle.ref_available() else "port"
+    # the reference headers cap the FFT at N <= 1024 (fft_ref.hpp:216-219)
+    kind = "reference" if oracle.ref_available() and sc.N <= 1024 else "port"
     orc = oracle.load("ref" if kind == "reference" else "oracle")
     threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
@@ -129,6 +130,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=3.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -215,6 +218,15 @@ def main():
         "mismatches": mismatches,
         "events_per_cpi": n_ev,
     }
+    if args.calib_copy_mb:
+        nbytes = args.calib_copy_mb << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = nbytes
     print(json.dumps(out))
     ctx.close()
 
