@@ -259,7 +259,10 @@ rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_
   return tot > events_cap ? RD_E_CAPACITY : RD_OK;
 }
 
-rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+namespace {
+
+// Validates a CPI and sizes the context's scratch for it (no launches).
+rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
   rd_status st = enter(c);
   if (st != RD_OK) return st;
   st = check_cpi(d);
@@ -267,12 +270,15 @@ rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b
   if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
     return RD_E_ARG;
   if ((uintptr_t)b->u & 15u) return RD_E_ARG;
+  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
+  if (st != RD_OK) return st;
+  return grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_beams * d->n_frames));
+}
+
+// The chain's launches on the context stream (prepared CPI).
+rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
   const uint32_t CH = d->n_chan, N = d->n_fft, F = d->n_frames, B = d->n_beams;
   const uint64_t cols = (uint64_t)F * N;
-  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(CH, B));
-  if (st != RD_OK) return st;
-  st = grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(B * F));
-  if (st != RD_OK) return st;
   hipStream_t s = c->stream;
   RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
   RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
@@ -282,6 +288,65 @@ rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b
   RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
                          b->events ? b->events_cap : 0, b->row_count, c->scratch,
                          n_events_dev ? n_events_dev : c->total_dev));
+  return RD_OK;
+}
+
+}  // namespace
+
+struct rd_graph {
+  rd_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+  rd_status st = chain_prepare(c, d, b);
+  if (st != RD_OK) return st;
+  return chain_launch(c, d, b, n_events_dev);
+}
+
+rd_status rd_chain_graph(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev,
+                         rd_graph** out) {
+  if (!out) return RD_E_ARG;
+  *out = nullptr;
+  rd_status st = chain_prepare(c, d, b);
+  if (st != RD_OK) return st;
+  rd_graph* g = new (std::nothrow) rd_graph();
+  if (!g) return RD_E_ARG;
+  g->ctx = c;
+  RD_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  st = chain_launch(c, d, b, n_events_dev ? n_events_dev : c->total_dev);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (st != RD_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    delete g;
+    return st != RD_OK ? st : RD_E_HIP;
+  }
+  g->graph = graph;
+  if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    delete g;
+    return RD_E_HIP;
+  }
+  *out = g;
+  return RD_OK;
+}
+
+rd_status rd_graph_launch(rd_graph* g) {
+  if (!g) return RD_E_ARG;
+  rd_status st = enter(g->ctx);
+  if (st != RD_OK) return st;
+  return fin(hipGraphLaunch(g->exec, g->ctx->stream));
+}
+
+rd_status rd_graph_destroy(rd_graph* g) {
+  if (!g) return RD_E_ARG;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
   return RD_OK;
 }
 

@@ -20,7 +20,8 @@ RD_OK, RD_E_ARG, RD_E_GEOM, RD_E_HIP, RD_E_CAPACITY, RD_E_NODEV = 0, -1, -2, -3,
 EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
-    "rd_bf_run", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue",
+    "rd_bf_run", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
+    "rd_graph_launch", "rd_graph_destroy",
 ]
 
 
@@ -82,6 +83,9 @@ class Lib:
                                   C.POINTER(u64)]
         L.rd_chain_run.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), C.POINTER(u64)]
         L.rd_chain_enqueue.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
+        L.rd_chain_graph.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp, C.POINTER(vp)]
+        L.rd_graph_launch.argtypes = [vp]
+        L.rd_graph_destroy.argtypes = [vp]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("rd_device_count", "rd_status_str", "rd_stream"):
@@ -286,6 +290,17 @@ class Chain:
                                          total_ptr if total_ptr is not None else self.total.ptr)
         self.ctx.lib.check("rd_chain_enqueue", rc)
 
+    def graph(self, iq_ptr=None, total_ptr=None) -> "Graph":
+        """The whole chain for this CPI geometry captured as one HIP graph."""
+        bufs = CpiBufs.from_buffer_copy(self.bufs)
+        if iq_ptr is not None:
+            bufs.iq = iq_ptr
+        g = C.c_void_p()
+        rc = self.ctx.L.rd_chain_graph(self.ctx.h, C.byref(self.desc), C.byref(bufs),
+                                       total_ptr if total_ptr is not None else self.total.ptr, C.byref(g))
+        self.ctx.lib.check("rd_chain_graph", rc)
+        return Graph(self.ctx, g.value)
+
     def fetch(self, names=("u", "pfb_flags", "X", "fft_flags", "Xa", "y", "yflags", "power", "R"), n_events=None):
         CH, N, F, B = (self.shape[k] for k in ("CH", "N", "F", "B"))
         S = CH * N * F
@@ -302,6 +317,25 @@ class Chain:
             t = min(n_events, self.events_cap)
             out["events"] = self.b["events"].download(np.uint64, 3 * t).reshape(t, 3)
         return out
+
+
+class Graph:
+    def __init__(self, ctx: Ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self):
+        self.ctx.lib.check("rd_graph_launch", self.ctx.L.rd_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx.L.rd_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _lib = None

@@ -111,16 +111,26 @@ class Scene:
         return steering_weights(self.CH, self.beam_angles)
 
     def iq(self, seed: int | None = None) -> np.ndarray:
+        """Noise plus the targets' plane waves; the targets are separable in
+        (channel, time), so their sum is one (CH x K) @ (K x L) product."""
         rng = np.random.default_rng(self.seed if seed is None else seed)
         CH, L = self.CH, self.N * self.F
-        t = np.arange(L)
-        z = self.sigma * (rng.standard_normal((CH, L)) + 1j * rng.standard_normal((CH, L)))
-        a = np.arange(CH)[:, None]
-        for (k, ang, snr) in self.targets:
-            amp = self.sigma * 10 ** (snr / 20.0)
-            ph0 = rng.uniform(0, 2 * np.pi)
-            z = z + amp * np.exp(1j * (2 * np.pi * (k / self.N) * t[None, :] + np.pi * a * np.sin(np.deg2rad(ang)) + ph0))
-        return pack(q16(z.real), q16(z.imag)).reshape(-1)
+        re = rng.standard_normal((CH, L)) * self.sigma
+        im = rng.standard_normal((CH, L)) * self.sigma
+        if self.targets:
+            a = np.arange(CH)
+            t = np.arange(L)
+            steer = np.empty((CH, len(self.targets)), np.complex128)
+            tones = np.empty((len(self.targets), L), np.complex128)
+            for i, (k, ang, snr) in enumerate(self.targets):
+                amp = self.sigma * 10 ** (snr / 20.0)
+                ph0 = rng.uniform(0, 2 * np.pi)
+                steer[:, i] = amp * np.exp(1j * (np.pi * a * np.sin(np.deg2rad(ang)) + ph0))
+                tones[i] = np.exp(1j * (2 * np.pi * (k / self.N) * t))
+            sig = steer @ tones
+            re += sig.real
+            im += sig.imag
+        return pack(q16(re), q16(im)).reshape(-1)
 
 
 def _random_targets(rng, n, N, snr_lo=6.0, snr_hi=25.0, ang=60.0):

@@ -40,6 +40,10 @@ import rd_scenario  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
+def note(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def algo_chain_bytes(sc, n_events: int) -> int:
     """Compulsory traffic of the fused chain for one CPI: I/Q in, taps, weights,
     Hermitian covariance out (int64 re+im), 176-bit events (22 B)."""
@@ -148,14 +152,20 @@ def main():
     taps, W = sc.taps(), sc.weights()
     ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
                       keep_y=True, keep_flags=True, keep_R=True)
+    note(f"workload {sc.name}: generating {args.ring} input CPIs")
     ch.load(sc.iq(seed=0), taps, W)
-    ring = [ctx.put(sc.iq(seed=1000 + i)) for i in range(args.ring)]
+    ring = []
+    for i in range(args.ring):
+        ring.append(ctx.put(sc.iq(seed=1000 + i)))
+        if sc.samples > 50_000_000:
+            note(f"  input CPI {i + 1}/{args.ring} resident")
     totals = ctx.buf(8 * args.ring)
 
     # correctness of one CPI against the CPU run (also the timed CPU baseline)
     iq0 = sc.iq(seed=1000)
     cpu = None
     if not args.no_cpu:
+        note("CPU baseline")
         cpu, cpu_out = cpu_baseline(sc, iq0, taps, W, args.cpu_seconds)
     ch.enqueue(iq_ptr=ring[0].ptr)
     ctx.sync()
@@ -170,20 +180,44 @@ def main():
             mismatches += int(np.count_nonzero(got["events"] != cpu_out["events"]))
 
     stream = torch.cuda.ExternalStream(ctx.stream)
-    for i in range(args.warmup):
+    note(f"capturing {args.ring} chain graphs")
+    graphs = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i) for i in range(args.ring)]
+
+    def timed(launch_one, steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for i in range(steps):
+            launch_one(i)
+        t_host = time.perf_counter() - t0
+        e1.record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, t_host, e0.elapsed_time(e1) * 1e-3
+
+    def run_graph(i):
+        graphs[i % args.ring].launch()
+
+    def run_stream(i):
         ch.enqueue(iq_ptr=ring[i % args.ring].ptr, total_ptr=totals.ptr + 8 * (i % args.ring))
-    ctx.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ch.enqueue(iq_ptr=ring[i % args.ring].ptr, total_ptr=totals.ptr + 8 * (i % args.ring))
-    ctx.sync()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+
+    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay)")
+    timed(run_graph, args.warmup)
+    dt, host_dt, gpu_dt = timed(run_graph, args.steps)
     ms_per_step = dt * 1e3 / args.steps
     value = args.steps * sc.samples / dt / 1e6
+    note("stream-enqueue comparison")
+    timed(run_stream, min(args.warmup, 8))
+    s_dt, s_host, s_gpu = timed(run_stream, args.steps)
+    totals_ok = totals.download(np.uint64, args.ring)
+    if not np.all(totals_ok[:min(args.ring, args.steps)] > 0):
+        sys.exit("a timed CPI produced no events (summaries are always emitted)")
 
     # per-kernel launch durations (HIP events on the library's stream)
+    note("per-stage kernel timing")
     stages = stage_kernels(ctx, ch, sc)
     per = {}
     for name, (fn, nbytes) in stages.items():
@@ -193,7 +227,7 @@ def main():
     ach = per[dom]["gbs"]
     chain_bytes = algo_chain_bytes(sc, n_ev)
     out = {
-        "metric": "bit-exact Msamples/s PFB->CFAR (plain GPU, one CPI per launch sequence)",
+        "metric": "bit-exact Msamples/s PFB->CFAR (plain GPU, one CPI per graph launch)",
         "value": round(value, 2),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -207,12 +241,17 @@ def main():
         "data": "synthetic (seeded complex Gaussian noise + plane-wave targets, ring of distinct CPIs)",
         "config": {"workload": f"{sc.name}: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, "
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
-                   "ring": args.ring, "impl": "per-CPI kernel sequence (PFB, FFT, align, BF+power MFMA, cov MFMA, "
-                                              "CFAR x3)"},
+                   "ring": args.ring, "impl": "one hipGraph per CPI: PFB, FFT, align, BF+power MFMA, cov MFMA, "
+                                              "CFAR count/scan/emit"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
         "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                   "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "timing": {"graph": {"wall_ms": round(ms_per_step, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
+                             "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4)},
+                   "stream": {"wall_ms": round(s_dt * 1e3 / args.steps, 4), "host_ms": round(s_host * 1e3 / args.steps, 4),
+                              "gpu_ms": round(s_gpu * 1e3 / args.steps, 4),
+                              "msamples_s": round(args.steps * sc.samples / s_dt / 1e6, 2)}},
         "stages": per,
         "cpu_baseline": cpu,
         "mismatches": mismatches,

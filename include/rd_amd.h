@@ -185,6 +185,18 @@ rd_status rd_chain_run(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* 
  * left in device memory at *n_events_dev (uint64). For timing loops. */
 rd_status rd_chain_enqueue(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_dev);
 
+/* ---- graphs --------------------------------------------------------------- */
+typedef struct rd_graph rd_graph;
+
+/* Records the launches of rd_chain_enqueue(ctx, desc, bufs, n_events_dev) into an
+ * executable HIP graph (buffers and geometry are fixed at capture). Each
+ * rd_graph_launch then replays the whole chain with one launch on the context's
+ * stream. No graph node is a memset. */
+rd_status rd_chain_graph(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_dev,
+                         rd_graph** out);
+rd_status rd_graph_launch(rd_graph* graph);
+rd_status rd_graph_destroy(rd_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif
