@@ -21,6 +21,7 @@ struct rd_ctx {
   size_t scratch_bytes = 0;
   uint64_t* total_dev = nullptr;
   uint64_t* total_host = nullptr;  // pinned
+  int live_graphs = 0;             // graphs hold the context's stream
 };
 
 namespace {
@@ -136,7 +137,7 @@ rd_status rd_create(int device, rd_ctx** out) {
 }
 
 rd_status rd_destroy(rd_ctx* c) {
-  if (!c) return RD_E_ARG;
+  if (!c || c->live_graphs > 0) return RD_E_ARG;  // destroy the context's graphs first
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->tw) (void)hipFree(c->tw);
@@ -329,6 +330,7 @@ rd_status rd_chain_graph(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, 
     delete g;
     return RD_E_HIP;
   }
+  ++c->live_graphs;
   *out = g;
   return RD_OK;
 }
@@ -346,6 +348,7 @@ rd_status rd_graph_destroy(rd_graph* g) {
   (void)hipStreamSynchronize(g->ctx->stream);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  --g->ctx->live_graphs;
   delete g;
   return RD_OK;
 }

@@ -8,6 +8,7 @@ library is missing, :func:`load` raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -138,9 +139,9 @@ class DevBuf:
         return self
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx.L.rd_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -156,11 +157,16 @@ class Ctx:
         h = C.c_void_p()
         lib.check("rd_create", self.L.rd_create(device, C.byref(h)))
         self.h = h.value
+        self.graphs = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context's graphs, then the context (buffers still held by
+        DevBuf objects become inert)."""
         if self.h:
+            for g in list(self.graphs):
+                g.close()
             self.L.rd_sync(self.h)
-            self.L.rd_destroy(self.h)
+            self.lib.check("rd_destroy", self.L.rd_destroy(self.h))
             self.h = None
 
     @property
@@ -322,14 +328,15 @@ class Chain:
 class Graph:
     def __init__(self, ctx: Ctx, h):
         self.ctx, self.h = ctx, h
+        ctx.graphs.add(self)
 
     def launch(self):
         self.ctx.lib.check("rd_graph_launch", self.ctx.L.rd_graph_launch(self.h))
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.rd_graph_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

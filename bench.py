@@ -266,7 +266,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out["calib_copy_bytes"] = nbytes
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    for g in graphs:
+        g.close()
     ctx.close()
 
 

@@ -101,6 +101,7 @@ typedef struct {
 /* ---- context and memory ------------------------------------------------ */
 int rd_device_count(void);
 rd_status rd_create(int device, rd_ctx** out);
+/* Fails with RD_E_ARG (and destroys nothing) while graphs of the context live. */
 rd_status rd_destroy(rd_ctx* ctx);
 const char* rd_status_str(rd_status s);
 rd_status rd_sync(rd_ctx* ctx);
