@@ -42,35 +42,31 @@ __host__ __device__ inline CfarGeom clamp_geom(const rd_cfar_cfg& c, uint32_t ma
 }
 
 // One wave per row; a block holds up to 4 rows (waves), each with its own
-// exclusive prefix P[0..n] in LDS. Lane l owns the contiguous bins
-// [l*per, (l+1)*per), per = ceil(n / 64).
-__device__ __forceinline__ uint64_t wave_exscan(uint64_t v, uint32_t lane, uint64_t* total) {
+// exclusive prefix P[0..n] in LDS. Lane l owns the interleaved bins l + 64*j, so
+// global loads, LDS accesses and event stores are all consecutive across lanes;
+// the prefix is built one 64-bin block at a time with a wave scan and a carry,
+// and events are compacted in bin order with a ballot per block.
+__device__ __forceinline__ uint64_t wave_incscan(uint64_t v, uint32_t lane) {
   uint64_t x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t y = __shfl_up(x, o);
     if (lane >= (uint32_t)o) x += y;
   }
-  *total = __shfl(x, 63);
-  return x - v;
+  return x;
 }
 
 template <typename Cell>
-__device__ __forceinline__ void row_prefix(const Cell* row, uint32_t n, uint32_t per, uint32_t lane, uint64_t* P) {
-  const uint32_t b0 = lane * per;
-  uint64_t local = 0;
-  for (uint32_t q = 0; q < per; ++q)
-    if (b0 + q < n) local += (uint64_t)row[b0 + q];
-  uint64_t total;
-  uint64_t run = wave_exscan(local, lane, &total);
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i < n) {
-      P[i] = run;
-      run += (uint64_t)row[i];
-    }
+__device__ __forceinline__ void row_prefix(const Cell* row, uint32_t n, uint32_t lane, uint64_t* P) {
+  uint64_t carry = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+    const uint32_t i = b0 + lane;
+    const uint64_t c = i < n ? (uint64_t)row[i] : 0ull;
+    const uint64_t x = wave_incscan(c, lane);
+    if (i < n) P[i] = carry + x - c;
+    carry += __shfl(x, 63);
   }
-  if (lane == 0) P[n] = total;
+  if (lane == 0) P[n] = carry;
 }
 
 struct Decision {
@@ -110,17 +106,16 @@ __global__ __launch_bounds__(256) void cfar_count_kernel(const Cell* __restrict_
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
   uint64_t* P = lds + (size_t)wave * (n + 1);
-  const uint32_t per = (n + 63) / 64;
-  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, per, lane, P);
+  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, lane, P);
   __syncthreads();
   if (r >= n_rows) return;
-  uint64_t det = 0;
-  const uint32_t b0 = lane * per;
-  for (uint32_t q = 0; q < per; ++q)
-    if (b0 + q < n) det += decide(P, n, b0 + q, g).hit ? 1 : 0;
-  uint64_t tot;
-  wave_exscan(det, lane, &tot);
-  if (lane == 0) row_count[r] = (uint32_t)((g.dense ? n : tot) + 1);
+  uint32_t det = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+    const uint32_t i = b0 + lane;
+    const bool hit = i < n && decide(P, n, i, g).hit;
+    det += (uint32_t)__popcll(__ballot(hit));
+  }
+  if (lane == 0) row_count[r] = (g.dense ? n : det) + 1;
 }
 
 // Exclusive scan of row counts into 64-bit offsets; offs[n_rows] = total.
@@ -177,42 +172,34 @@ __global__ __launch_bounds__(256) void cfar_emit_kernel(const Cell* __restrict__
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
   uint64_t* P = lds + (size_t)wave * (n + 1);
-  const uint32_t per = (n + 63) / 64;
-  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, per, lane, P);
+  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, lane, P);
   __syncthreads();
   if (r >= n_rows) return;
   const uint32_t frame = frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
-  const uint32_t b0 = lane * per;
-  uint64_t nemit = 0, ndet = 0, nsup = 0;
-  for (uint32_t q = 0; q < per; ++q) {
-    const uint32_t i = b0 + q;
-    if (i >= n) break;
-    const Decision d = decide(P, n, i, g);
-    ndet += d.hit;
-    nsup += !d.eval;
-    nemit += (d.hit || g.dense) ? 1 : 0;
-  }
-  uint64_t emitted, tdet, tsup;
-  const uint64_t before = wave_exscan(nemit, lane, &emitted);
-  wave_exscan(ndet, lane, &tdet);
-  wave_exscan(nsup, lane, &tsup);
   const uint64_t base = offs[r];
-  uint64_t k = before;
-  if (nemit)
-    for (uint32_t q = 0; q < per; ++q) {
-      const uint32_t i = b0 + q;
-      if (i >= n) break;
-      const Decision d = decide(P, n, i, g);
-      if (!(d.hit || g.dense)) continue;
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
+  uint64_t k = 0;                                 // events emitted so far in the row
+  uint32_t det = 0, sup = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+    const uint32_t i = b0 + lane;
+    Decision d{};
+    const bool in = i < n;
+    if (in) d = decide(P, n, i, g);
+    const bool emit = in && (d.hit || g.dense);
+    const uint64_t em = __ballot(emit);
+    det += (uint32_t)__popcll(__ballot(in && d.hit));
+    sup += (uint32_t)__popcll(__ballot(in && !d.eval));
+    if (emit) {
+      const uint64_t pos = k + (uint64_t)__popcll(em & below);
       const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
-      if (base + k < cap)
-        put_event(events + 3 * (base + k), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
-                  d.eval ? d.sum : 0ull, k == 0, false);
-      ++k;
+      if (base + pos < cap)
+        put_event(events + 3 * (base + pos), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
+                  d.eval ? d.sum : 0ull, pos == 0, false);
     }
-  if (lane == 0 && base + emitted < cap)
-    put_event(events + 3 * (base + emitted), 3u, n, frame, 0, g.alpha, (uint32_t)tdet, (uint32_t)tsup, 0, 0,
-              emitted == 0, true);
+    k += (uint64_t)__popcll(em);
+  }
+  if (lane == 0 && base + k < cap)
+    put_event(events + 3 * (base + k), 3u, n, frame, 0, g.alpha, det, sup, 0, 0, k == 0, true);
 }
 
 size_t cfar_scratch_bytes(uint32_t n_rows) { return ((size_t)n_rows + 1) * sizeof(uint64_t); }

@@ -276,25 +276,31 @@ __device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t w, StageFlags& f) {
   return Cv{yr, yi};
 }
 
+__device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
+
 template <int SPC>
 __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ in, uint32_t log2n, uint32_t sched,
                                                    int reorder, uint32_t n_frames, uint32_t fpb,
                                                    const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out,
                                                    uint32_t* __restrict__ bins, uint32_t* __restrict__ stage_flags) {
   extern __shared__ uint32_t smem[];
+  // LDS positions are padded by one word per 32 (pad(p) = p + p/32), which makes
+  // the bit-reversed natural-order reads and the strided passes bank-conflict free
+  // or nearly so.
   const uint32_t n = 1u << log2n;
   const uint32_t m = n / SPC;
   const uint32_t nl = log2n - (SPC == 2 ? 1 : 0);
   const uint32_t tpf = n / 4;
-  uint32_t* tw = smem;                       // [n]  W_n^e
-  uint32_t* sfl = smem + n;                  // [fpb] stage-flag words
+  const uint32_t np = n + n / 32;           // padded frame / table length
+  uint32_t* tw = smem;                       // [np]  W_n^e at pad(e)
+  uint32_t* sfl = smem + np;                 // [fpb] stage-flag words
   const uint32_t lf = threadIdx.x / tpf;     // local frame
   const uint32_t g = threadIdx.x % tpf;
-  uint32_t* v = smem + n + fpb + lf * n;     // this frame's samples
+  uint32_t* v = smem + np + fpb + lf * np;   // this frame's samples at pad(p)
   const uint32_t frame = blockIdx.x * fpb + lf;
   const bool active = frame < n_frames;
 
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw4096[i << (12 - log2n)];
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - log2n)];
   if (g == 0) sfl[lf] = 0;
   if (active) {
     const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * n + 4 * g);
@@ -302,7 +308,7 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const uint32_t t = 4 * g + r;
-      v[SPC == 2 ? ((t & 1) * m + (t >> 1)) : t] = w4[r];
+      v[pad(SPC == 2 ? ((t & 1) * m + (t >> 1)) : t)] = w4[r];
     }
   }
   __syncthreads();
@@ -314,8 +320,8 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
     const uint32_t quarter = 1u << lq;
     const uint32_t i = g & (quarter - 1);
     const uint32_t base = ((g >> lq) << (nl - s)) + i;
-    Cv e0 = ld_cv(v[base]), e1 = ld_cv(v[base + quarter]), e2 = ld_cv(v[base + 2 * quarter]),
-       e3 = ld_cv(v[base + 3 * quarter]);
+    Cv e0 = ld_cv(v[pad(base)]), e1 = ld_cv(v[pad(base + quarter)]), e2 = ld_cv(v[pad(base + 2 * quarter)]),
+       e3 = ld_cv(v[pad(base + 3 * quarter)]);
     StageFlags f0, f1;
     const uint32_t sh0 = (sched >> s) & 1u, sh1 = (sched >> (s + 1)) & 1u;
     bfly(e0, e2, sh0, f0);  // BF2I, half = l/2
@@ -326,17 +332,17 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
     if (s + 2 < nl) {
       const uint32_t l2l = nl - s, tsh = log2n - l2l;
       // element c sits at quarter c of its block: exponent i * {0, 2, 1, 3}[c]
-      e1 = cmul_tw(e1, tw[(2 * i) << tsh], f1);
-      e2 = cmul_tw(e2, tw[i << tsh], f1);
-      e3 = cmul_tw(e3, tw[(3 * i) << tsh], f1);
+      e1 = cmul_tw(e1, tw[pad((2 * i) << tsh)], f1);
+      e2 = cmul_tw(e2, tw[pad(i << tsh)], f1);
+      e3 = cmul_tw(e3, tw[pad((3 * i) << tsh)], f1);
       e0 = cmul_tw(e0, tw[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
     }
     // each thread writes back exactly the four positions it read
     if (active) {
-      v[base] = cpack(e0.re, e0.im);
-      v[base + quarter] = cpack(e1.re, e1.im);
-      v[base + 2 * quarter] = cpack(e2.re, e2.im);
-      v[base + 3 * quarter] = cpack(e3.re, e3.im);
+      v[pad(base)] = cpack(e0.re, e0.im);
+      v[pad(base + quarter)] = cpack(e1.re, e1.im);
+      v[pad(base + 2 * quarter)] = cpack(e2.re, e2.im);
+      v[pad(base + 3 * quarter)] = cpack(e3.re, e3.im);
     }
     fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
     __syncthreads();
@@ -346,22 +352,24 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
   const bool last_r2 = (nl & 1u) != 0;
   if (SPC == 1) {
     if (last_r2) {
-      Cv a = ld_cv(v[4 * g]), b = ld_cv(v[4 * g + 1]), c = ld_cv(v[4 * g + 2]), d = ld_cv(v[4 * g + 3]);
+      Cv a = ld_cv(v[pad(4 * g)]), b = ld_cv(v[pad(4 * g + 1)]), c = ld_cv(v[pad(4 * g + 2)]),
+         d = ld_cv(v[pad(4 * g + 3)]);
       StageFlags f;
       const uint32_t sh = (sched >> s) & 1u;
       bfly(a, b, sh, f);
       bfly(c, d, sh, f);
       fword |= f.word() << (2 * s);
       if (active) {
-        v[4 * g] = cpack(a.re, a.im);
-        v[4 * g + 1] = cpack(b.re, b.im);
-        v[4 * g + 2] = cpack(c.re, c.im);
-        v[4 * g + 3] = cpack(d.re, d.im);
+        v[pad(4 * g)] = cpack(a.re, a.im);
+        v[pad(4 * g + 1)] = cpack(b.re, b.im);
+        v[pad(4 * g + 2)] = cpack(c.re, c.im);
+        v[pad(4 * g + 3)] = cpack(d.re, d.im);
       }
     }
   } else {
     // lane0[2g], lane0[2g+1], lane1[2g], lane1[2g+1]
-    Cv a0 = ld_cv(v[2 * g]), a1 = ld_cv(v[2 * g + 1]), b0 = ld_cv(v[m + 2 * g]), b1 = ld_cv(v[m + 2 * g + 1]);
+    Cv a0 = ld_cv(v[pad(2 * g)]), a1 = ld_cv(v[pad(2 * g + 1)]), b0 = ld_cv(v[pad(m + 2 * g)]),
+       b1 = ld_cv(v[pad(m + 2 * g + 1)]);
     if (last_r2) {
       StageFlags f;
       const uint32_t sh = (sched >> s) & 1u;
@@ -371,17 +379,17 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
     }
     StageFlags fm;
     const uint32_t shm = (sched >> nl) & 1u;
-    Cv t0 = cmul_tw(b0, tw[bitrev(2 * g, nl)], fm);
-    Cv t1 = cmul_tw(b1, tw[bitrev(2 * g + 1, nl)], fm);
+    Cv t0 = cmul_tw(b0, tw[pad(bitrev(2 * g, nl))], fm);
+    Cv t1 = cmul_tw(b1, tw[pad(bitrev(2 * g + 1, nl))], fm);
     bfly(a0, t0, shm, fm);
     bfly(a1, t1, shm, fm);
     fword |= fm.word() << (2 * nl);
     __syncthreads();
     if (active) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
-      v[4 * g] = cpack(a0.re, a0.im);
-      v[4 * g + 1] = cpack(t0.re, t0.im);
-      v[4 * g + 2] = cpack(a1.re, a1.im);
-      v[4 * g + 3] = cpack(t1.re, t1.im);
+      v[pad(4 * g)] = cpack(a0.re, a0.im);
+      v[pad(4 * g + 1)] = cpack(t0.re, t0.im);
+      v[pad(4 * g + 2)] = cpack(a1.re, a1.im);
+      v[pad(4 * g + 3)] = cpack(t1.re, t1.im);
     }
   }
   if (fword) atomicOr(&sfl[lf], fword);
@@ -396,12 +404,12 @@ __global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ 
     const uint32_t k = g + r * tpf;
     if (bins) {
       const uint32_t src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k & (m - 1), nl) + (k >> nl);
-      bins[fo + k] = v[src];
+      bins[fo + k] = v[pad(src)];
     }
     if (out) {
       uint32_t src = k;
       if (reorder) src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k >> 1, nl) + (k & 1);
-      out[fo + k] = v[src];
+      out[fo + k] = v[pad(src)];
     }
   }
   if (stage_flags && g == 0) stage_flags[frame] = sfl[lf];
@@ -415,7 +423,8 @@ hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in,
   const uint32_t tpf = n / 4;
   const uint32_t fpb = tpf >= 256 ? 1 : 256 / tpf;
   const uint32_t threads = tpf * fpb;
-  const size_t lds = (size_t)(n + fpb + fpb * n) * sizeof(uint32_t);
+  const size_t np = n + n / 32;
+  const size_t lds = (size_t)(np + fpb + fpb * np) * sizeof(uint32_t);
   const dim3 grid((n_frames + fpb - 1) / fpb);
   if (spc == 2)
     fft_kernel<2><<<grid, threads, lds, s>>>(in, log2n, sched, reorder, n_frames, fpb, tw4096, out, bins,

@@ -296,11 +296,17 @@ class Chain:
                                          total_ptr if total_ptr is not None else self.total.ptr)
         self.ctx.lib.check("rd_chain_enqueue", rc)
 
-    def graph(self, iq_ptr=None, total_ptr=None) -> "Graph":
-        """The whole chain for this CPI geometry captured as one HIP graph."""
+    def graph(self, iq_ptr=None, total_ptr=None, lean=False) -> "Graph":
+        """The whole chain for this CPI geometry captured as one HIP graph.
+        lean: the optional per-stage outputs (PFB/FFT/beamformer flags and the
+        beam samples y) are not written; every stage still runs and the
+        intermediates the next stage reads, the covariance and the events are
+        produced."""
         bufs = CpiBufs.from_buffer_copy(self.bufs)
         if iq_ptr is not None:
             bufs.iq = iq_ptr
+        if lean:
+            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = None
         g = C.c_void_p()
         rc = self.ctx.L.rd_chain_graph(self.ctx.h, C.byref(self.desc), C.byref(bufs),
                                        total_ptr if total_ptr is not None else self.total.ptr, C.byref(g))

@@ -4,7 +4,7 @@ covariance -> CFAR -> event-packing chain on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload medium]
 
-A step is one CPI of the workload through the whole chain (rd_chain_enqueue),
+A step is one CPI of the workload through the whole chain (one HIP graph launch),
 inputs already resident in HBM. Steps cycle through a ring of distinct input
 CPIs (default 32, 537 MB for the medium workload) so the input stream is not
 served from the 256 MB last-level cache. One JSON line is printed:
@@ -180,8 +180,9 @@ def main():
             mismatches += int(np.count_nonzero(got["events"] != cpu_out["events"]))
 
     stream = torch.cuda.ExternalStream(ctx.stream)
-    note(f"capturing {args.ring} chain graphs")
-    graphs = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i) for i in range(args.ring)]
+    note(f"capturing {args.ring} chain graphs (lean and full-output)")
+    graphs = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i, lean=True) for i in range(args.ring)]
+    graphs_full = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i) for i in range(args.ring)]
 
     def timed(launch_one, steps):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -201,6 +202,9 @@ def main():
     def run_graph(i):
         graphs[i % args.ring].launch()
 
+    def run_graph_full(i):
+        graphs_full[i % args.ring].launch()
+
     def run_stream(i):
         ch.enqueue(iq_ptr=ring[i % args.ring].ptr, total_ptr=totals.ptr + 8 * (i % args.ring))
 
@@ -209,6 +213,9 @@ def main():
     dt, host_dt, gpu_dt = timed(run_graph, args.steps)
     ms_per_step = dt * 1e3 / args.steps
     value = args.steps * sc.samples / dt / 1e6
+    note("full-output graphs")
+    timed(run_graph_full, min(args.warmup, 8))
+    f_dt, f_host, f_gpu = timed(run_graph_full, args.steps)
     note("stream-enqueue comparison")
     timed(run_stream, min(args.warmup, 8))
     s_dt, s_host, s_gpu = timed(run_stream, args.steps)
@@ -242,13 +249,18 @@ def main():
         "config": {"workload": f"{sc.name}: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, "
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
                    "ring": args.ring, "impl": "one hipGraph per CPI: PFB, FFT, align, BF+power MFMA, cov MFMA, "
-                                              "CFAR count/scan/emit"},
+                                              "CFAR count/scan/emit",
+                   "outputs": "events + covariance timed; every stage output verified bit-exact on a separate "
+                              "CPI each run (timing.graph_full_outputs also writes them all)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
         "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                   "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "timing": {"graph": {"wall_ms": round(ms_per_step, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
                              "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4)},
+                   "graph_full_outputs": {"wall_ms": round(f_dt * 1e3 / args.steps, 4),
+                                          "gpu_ms": round(f_gpu * 1e3 / args.steps, 4),
+                                          "msamples_s": round(args.steps * sc.samples / f_dt / 1e6, 2)},
                    "stream": {"wall_ms": round(s_dt * 1e3 / args.steps, 4), "host_ms": round(s_host * 1e3 / args.steps, 4),
                               "gpu_ms": round(s_gpu * 1e3 / args.steps, 4),
                               "msamples_s": round(args.steps * sc.samples / s_dt / 1e6, 2)}},
@@ -267,7 +279,7 @@ def main():
         torch.cuda.synchronize()
         out["calib_copy_bytes"] = nbytes
     print(json.dumps(out), flush=True)
-    for g in graphs:
+    for g in graphs + graphs_full:
         g.close()
     ctx.close()
 
