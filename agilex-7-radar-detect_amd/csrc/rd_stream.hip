@@ -230,189 +230,242 @@ struct Cv {
 };
 
 __device__ __forceinline__ Cv ld_cv(uint32_t p) { return Cv{cre(p), cim(p)}; }
+__device__ __forceinline__ uint32_t pk(Cv c) { return __builtin_amdgcn_perm((uint32_t)c.im, (uint32_t)c.re, 0x05040100u); }
 
+// Per sub-stage saturation evidence: the extremes of every value that is about
+// to be saturated to 16 bits; the stage's {sat_pos, sat_neg} word is then
+// (max > 32767, min < -32768) — the OR over the frame (fft_ref.hpp:315-319).
 struct StageFlags {
-  int32_t mx = -32768, mn = 32767;  // extremes of the rounded values
-  uint32_t extra = 0;               // flags from products / negates
-  __device__ __forceinline__ void see(int32_t r) {
-    mx = max(mx, r);
-    mn = min(mn, r);
+  int32_t mx = -32768, mn = 32767;
+  __device__ __forceinline__ void see(int32_t a, int32_t b) {
+    mx = max(mx, max(a, b));
+    mn = min(mn, min(a, b));
   }
-  __device__ __forceinline__ uint32_t word() const {
-    return extra | (mx > 32767 ? 2u : 0u) | (mn < -32768 ? 1u : 0u);
-  }
+  __device__ __forceinline__ uint32_t word() const { return (mx > 32767 ? 2u : 0u) | (mn < -32768 ? 1u : 0u); }
 };
 
-__device__ __forceinline__ int32_t q16(int32_t v, uint32_t sh, StageFlags& f) {
-  const int32_t r = rne_sh01(v, sh);
-  f.see(r);
-  return min(max(r, -32768), 32767);
+__device__ __forceinline__ int32_t sat(int32_t r) { return min(max(r, -32768), 32767); }
+
+// round-to-nearest-even by 1: (v + bit1(v)) >> 1 (the tie bit makes the quotient even)
+__device__ __forceinline__ int32_t rne1(int32_t v) { return (v + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 1, 1)) >> 1; }
+// round-to-nearest-even by 15 for |v| < 2^31 - 2^14: (v + 2^14 - 1 + bit15(v)) >> 15
+__device__ __forceinline__ int32_t rne15(int32_t v) {
+  return (v + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 15, 1)) >> 15;
 }
 
-// lo = q(a + b), hi = q(a - b)
-__device__ __forceinline__ void bfly(Cv& a, Cv& b, uint32_t sh, StageFlags& f) {
-  const int32_t sr = a.re + b.re, si = a.im + b.im, dr = a.re - b.re, di = a.im - b.im;
-  a.re = q16(sr, sh, f);
-  a.im = q16(si, sh, f);
-  b.re = q16(dr, sh, f);
-  b.im = q16(di, sh, f);
+// lo = q(a + b), hi = q(a - b), q = round by the schedule bit then saturate
+template <bool ALL1>
+__device__ __forceinline__ void bfly(Cv& a, Cv& b, bool sh, StageFlags& f) {
+  int32_t sr = a.re + b.re, si = a.im + b.im, dr = a.re - b.re, di = a.im - b.im;
+  if (ALL1 || sh) {
+    sr = rne1(sr);
+    si = rne1(si);
+    dr = rne1(dr);
+    di = rne1(di);
+  }
+  f.see(sr, si);
+  f.see(dr, di);
+  a = Cv{sat(sr), sat(si)};
+  b = Cv{sat(dr), sat(di)};
 }
 
-// -j * (re + j im) = im - j re, negate saturating (fft_ref.hpp:322-331)
+// -j * (re + j im) = im - j re with a saturating negate (fft_ref.hpp:322-331)
 __device__ __forceinline__ Cv neg_j(Cv c, StageFlags& f) {
   const int32_t n = -c.re;
-  f.extra |= (n > 32767 ? 2u : 0u);
+  f.mx = max(f.mx, n);
   return Cv{c.im, min(n, 32767)};
 }
 
+// x * w rounded once at shift 15 (fxp.hpp:314-318), as two packed int16 dot
+// products: im = x.re*w.im + x.im*w.re = dot2((re, im), (w.im, w.re)) and
+// re = x.re*w.re - x.im*w.im = dot2((re, ~im), (w.re, w.im)) + w.im, using
+// -im = ~im + 1 so no operand is ever negated. Every table entry has
+// |w.re| + |w.im| <= 65535, so the exact products fit in int32.
 __device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t w, StageFlags& f) {
-  const int32_t wr = cre(w), wi = cim(w);
-  const int32_t re = x.re * wr - x.im * wi;
-  const int32_t im = x.re * wi + x.im * wr;
-  uint32_t fl = 0;
-  const int32_t yr = sat16(rne15_i32(re), fl);
-  const int32_t yi = sat16(rne15_i32(im), fl);
-  f.extra |= fl;
-  return Cv{yr, yi};
+  const uint32_t xp = pk(x);
+  const uint32_t wswap = __builtin_amdgcn_alignbit(w, w, 16);
+  const int32_t re = dot2(xp ^ 0xFFFF0000u, w, cim(w));
+  const int32_t im = dot2(xp, wswap, 0);
+  const int32_t yr = rne15(re), yi = rne15(im);
+  f.see(yr, yi);
+  return Cv{sat(yr), sat(yi)};
 }
 
 __device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
 
-template <int SPC>
-__global__ __launch_bounds__(1024) void fft_kernel(const uint32_t* __restrict__ in, uint32_t log2n, uint32_t sched,
-                                                   int reorder, uint32_t n_frames, uint32_t fpb,
-                                                   const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out,
-                                                   uint32_t* __restrict__ bins, uint32_t* __restrict__ stage_flags) {
+// ----------------------------------------------------------------------------
+// One frame per LOG2N-specialised geometry; every index below is a compile-time
+// expression of the pass and the thread's position, so the passes unroll fully.
+// ALL1: every sub-stage scales by 1 (the chain's schedule); otherwise the
+// schedule bit of each sub-stage is read at run time.
+// ----------------------------------------------------------------------------
+template <int LOG2N, int SPC, bool ALL1>
+__global__ __launch_bounds__(LOG2N >= 12 ? 1024 : 256) void fft_kernel(
+    const uint32_t* __restrict__ in, uint32_t sched, int reorder, uint32_t n_frames,
+    const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out, uint32_t* __restrict__ bins,
+    uint32_t* __restrict__ stage_flags) {
+  constexpr uint32_t N = 1u << LOG2N;
+  constexpr uint32_t M = N / SPC;
+  constexpr uint32_t NL = LOG2N - (SPC == 2 ? 1 : 0);
+  constexpr uint32_t TPF = N / 4;
+  constexpr uint32_t FPB = TPF >= 256 ? 1 : 256 / TPF;
+  constexpr uint32_t NP = N + N / 32;
   extern __shared__ uint32_t smem[];
-  // LDS positions are padded by one word per 32 (pad(p) = p + p/32), which makes
-  // the bit-reversed natural-order reads and the strided passes bank-conflict free
-  // or nearly so.
-  const uint32_t n = 1u << log2n;
-  const uint32_t m = n / SPC;
-  const uint32_t nl = log2n - (SPC == 2 ? 1 : 0);
-  const uint32_t tpf = n / 4;
-  const uint32_t np = n + n / 32;           // padded frame / table length
-  uint32_t* tw = smem;                       // [np]  W_n^e at pad(e)
-  uint32_t* sfl = smem + np;                 // [fpb] stage-flag words
-  const uint32_t lf = threadIdx.x / tpf;     // local frame
-  const uint32_t g = threadIdx.x % tpf;
-  uint32_t* v = smem + np + fpb + lf * np;   // this frame's samples at pad(p)
-  const uint32_t frame = blockIdx.x * fpb + lf;
+  uint32_t* tw = smem;                        // [NP]  W_N^e at pad(e)
+  uint32_t* sfl = smem + NP;                  // [FPB] stage-flag words
+  const uint32_t lf = threadIdx.x / TPF;      // local frame
+  const uint32_t g = threadIdx.x % TPF;
+  uint32_t* v = smem + NP + FPB + lf * NP;    // this frame's samples at pad(p)
+  const uint32_t frame = blockIdx.x * FPB + lf;
   const bool active = frame < n_frames;
 
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - log2n)];
+  for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - LOG2N)];
   if (g == 0) sfl[lf] = 0;
   if (active) {
-    const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * n + 4 * g);
+    const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * g);
     const uint32_t w4[4] = {q4.x, q4.y, q4.z, q4.w};
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const uint32_t t = 4 * g + r;
-      v[pad(SPC == 2 ? ((t & 1) * m + (t >> 1)) : t)] = w4[r];
+      const uint32_t t = 4 * g + r;  // lane q of spc lanes is x[spc*t' + q]
+      v[pad(SPC == 2 ? ((t & 1) * M + (t >> 1)) : t)] = w4[r];
     }
   }
   __syncthreads();
 
   uint32_t fword = 0;
-  uint32_t s = 0;
-  for (; s + 1 < nl; s += 2) {
-    const uint32_t lq = nl - s - 2;  // log2(quarter)
-    const uint32_t quarter = 1u << lq;
-    const uint32_t i = g & (quarter - 1);
-    const uint32_t base = ((g >> lq) << (nl - s)) + i;
-    Cv e0 = ld_cv(v[pad(base)]), e1 = ld_cv(v[pad(base + quarter)]), e2 = ld_cv(v[pad(base + 2 * quarter)]),
-       e3 = ld_cv(v[pad(base + 3 * quarter)]);
+#pragma unroll
+  for (uint32_t s = 0; s + 1 < NL; s += 2) {
+    const uint32_t LQ = NL - s - 2;  // log2(quarter)
+    const uint32_t Q = 1u << LQ;
+    const uint32_t i = g & (Q - 1);
+    const uint32_t base = ((g >> LQ) << (NL - s)) + i;
+    uint32_t ad[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ad[c] = Q >= 32 ? pad(base) + c * (Q + Q / 32) : pad(base + c * Q);
+    Cv e0 = ld_cv(v[ad[0]]), e1 = ld_cv(v[ad[1]]), e2 = ld_cv(v[ad[2]]), e3 = ld_cv(v[ad[3]]);
     StageFlags f0, f1;
-    const uint32_t sh0 = (sched >> s) & 1u, sh1 = (sched >> (s + 1)) & 1u;
-    bfly(e0, e2, sh0, f0);  // BF2I, half = l/2
-    bfly(e1, e3, sh0, f0);
-    e3 = neg_j(e3, f1);     // BF2II: odd sub-block takes -j
-    bfly(e0, e1, sh1, f1);
-    bfly(e2, e3, sh1, f1);
-    if (s + 2 < nl) {
-      const uint32_t l2l = nl - s, tsh = log2n - l2l;
+    const bool sh0 = (sched >> s) & 1u, sh1 = (sched >> (s + 1)) & 1u;
+    bfly<ALL1>(e0, e2, sh0, f0);  // BF2I, half = l/2
+    bfly<ALL1>(e1, e3, sh0, f0);
+    e3 = neg_j(e3, f1);           // BF2II: the odd sub-block takes -j
+    bfly<ALL1>(e0, e1, sh1, f1);
+    bfly<ALL1>(e2, e3, sh1, f1);
+    if (s + 2 < NL) {
+      const uint32_t TSH = LOG2N - (NL - s);
       // element c sits at quarter c of its block: exponent i * {0, 2, 1, 3}[c]
-      e1 = cmul_tw(e1, tw[pad((2 * i) << tsh)], f1);
-      e2 = cmul_tw(e2, tw[pad(i << tsh)], f1);
-      e3 = cmul_tw(e3, tw[pad((3 * i) << tsh)], f1);
       e0 = cmul_tw(e0, tw[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
+      e1 = cmul_tw(e1, tw[pad((2 * i) << TSH)], f1);
+      e2 = cmul_tw(e2, tw[pad(i << TSH)], f1);
+      e3 = cmul_tw(e3, tw[pad((3 * i) << TSH)], f1);
     }
     // each thread writes back exactly the four positions it read
-    if (active) {
-      v[pad(base)] = cpack(e0.re, e0.im);
-      v[pad(base + quarter)] = cpack(e1.re, e1.im);
-      v[pad(base + 2 * quarter)] = cpack(e2.re, e2.im);
-      v[pad(base + 3 * quarter)] = cpack(e3.re, e3.im);
-    }
+    v[ad[0]] = pk(e0);
+    v[ad[1]] = pk(e1);
+    v[ad[2]] = pk(e2);
+    v[ad[3]] = pk(e3);
     fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
     __syncthreads();
   }
 
   // Final pass: the last (odd-count) sub-stage and/or the DIT merge.
-  const bool last_r2 = (nl & 1u) != 0;
+  constexpr uint32_t SL = NL - 1;  // index of the last lane sub-stage
+  constexpr bool LAST_R2 = (NL & 1u) != 0;
   if (SPC == 1) {
-    if (last_r2) {
+    if (LAST_R2) {
       Cv a = ld_cv(v[pad(4 * g)]), b = ld_cv(v[pad(4 * g + 1)]), c = ld_cv(v[pad(4 * g + 2)]),
          d = ld_cv(v[pad(4 * g + 3)]);
       StageFlags f;
-      const uint32_t sh = (sched >> s) & 1u;
-      bfly(a, b, sh, f);
-      bfly(c, d, sh, f);
-      fword |= f.word() << (2 * s);
-      if (active) {
-        v[pad(4 * g)] = cpack(a.re, a.im);
-        v[pad(4 * g + 1)] = cpack(b.re, b.im);
-        v[pad(4 * g + 2)] = cpack(c.re, c.im);
-        v[pad(4 * g + 3)] = cpack(d.re, d.im);
-      }
+      const bool sh = (sched >> SL) & 1u;
+      bfly<ALL1>(a, b, sh, f);
+      bfly<ALL1>(c, d, sh, f);
+      fword |= f.word() << (2 * SL);
+      v[pad(4 * g)] = pk(a);
+      v[pad(4 * g + 1)] = pk(b);
+      v[pad(4 * g + 2)] = pk(c);
+      v[pad(4 * g + 3)] = pk(d);
     }
   } else {
     // lane0[2g], lane0[2g+1], lane1[2g], lane1[2g+1]
-    Cv a0 = ld_cv(v[pad(2 * g)]), a1 = ld_cv(v[pad(2 * g + 1)]), b0 = ld_cv(v[pad(m + 2 * g)]),
-       b1 = ld_cv(v[pad(m + 2 * g + 1)]);
-    if (last_r2) {
+    Cv a0 = ld_cv(v[pad(2 * g)]), a1 = ld_cv(v[pad(2 * g + 1)]), b0 = ld_cv(v[pad(M + 2 * g)]),
+       b1 = ld_cv(v[pad(M + 2 * g + 1)]);
+    if (LAST_R2) {
       StageFlags f;
-      const uint32_t sh = (sched >> s) & 1u;
-      bfly(a0, a1, sh, f);
-      bfly(b0, b1, sh, f);
-      fword |= f.word() << (2 * s);
+      const bool sh = (sched >> SL) & 1u;
+      bfly<ALL1>(a0, a1, sh, f);
+      bfly<ALL1>(b0, b1, sh, f);
+      fword |= f.word() << (2 * SL);
     }
     StageFlags fm;
-    const uint32_t shm = (sched >> nl) & 1u;
-    Cv t0 = cmul_tw(b0, tw[pad(bitrev(2 * g, nl))], fm);
-    Cv t1 = cmul_tw(b1, tw[pad(bitrev(2 * g + 1, nl))], fm);
-    bfly(a0, t0, shm, fm);
-    bfly(a1, t1, shm, fm);
-    fword |= fm.word() << (2 * nl);
+    const bool shm = (sched >> NL) & 1u;
+    Cv t0 = cmul_tw(b0, tw[pad(bitrev(2 * g, NL))], fm);
+    Cv t1 = cmul_tw(b1, tw[pad(bitrev(2 * g + 1, NL))], fm);
+    bfly<ALL1>(a0, t0, shm, fm);
+    bfly<ALL1>(a1, t1, shm, fm);
+    fword |= fm.word() << (2 * NL);
     __syncthreads();
-    if (active) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
-      v[pad(4 * g)] = cpack(a0.re, a0.im);
-      v[pad(4 * g + 1)] = cpack(t0.re, t0.im);
-      v[pad(4 * g + 2)] = cpack(a1.re, a1.im);
-      v[pad(4 * g + 3)] = cpack(t1.re, t1.im);
-    }
+    // beat order: beats[2j] = lo, beats[2j+1] = hi
+    v[pad(4 * g)] = pk(a0);
+    v[pad(4 * g + 1)] = pk(t0);
+    v[pad(4 * g + 2)] = pk(a1);
+    v[pad(4 * g + 3)] = pk(t1);
   }
   if (fword) atomicOr(&sfl[lf], fword);
   __syncthreads();
   if (!active) return;
 
-  // v now holds the beats in beat order. Natural bin k: spc = 1 -> beats[bitrev(k)];
+  // v holds the beats in beat order. Natural bin k: spc = 1 -> beats[bitrev(k)];
   // spc = 2 -> k = bitrev(j) + q*m reads beats[2j + q].
-  const size_t fo = (size_t)frame * n;
+  const size_t fo = (size_t)frame * N;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const uint32_t k = g + r * tpf;
+    const uint32_t k = g + r * TPF;
     if (bins) {
-      const uint32_t src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k & (m - 1), nl) + (k >> nl);
+      const uint32_t src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
       bins[fo + k] = v[pad(src)];
     }
     if (out) {
       uint32_t src = k;
-      if (reorder) src = SPC == 1 ? bitrev(k, nl) : 2 * bitrev(k >> 1, nl) + (k & 1);
+      if (reorder) src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k >> 1, NL) + (k & 1);
       out[fo + k] = v[pad(src)];
     }
   }
   if (stage_flags && g == 0) stage_flags[frame] = sfl[lf];
+}
+
+template <int LOG2N, int SPC>
+static hipError_t launch_fft_n(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t sched,
+                               int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
+                               uint32_t* stage_flags) {
+  constexpr uint32_t N = 1u << LOG2N, TPF = N / 4, FPB = TPF >= 256 ? 1 : 256 / TPF, NP = N + N / 32;
+  const size_t lds = (size_t)(NP + FPB + FPB * NP) * sizeof(uint32_t);
+  const dim3 grid((n_frames + FPB - 1) / FPB), block(TPF * FPB);
+  const uint32_t used = (LOG2N >= 32) ? ~0u : ((1u << LOG2N) - 1u);
+  if ((sched & used) == used)
+    fft_kernel<LOG2N, SPC, true><<<grid, block, lds, s>>>(in, sched, reorder, n_frames, tw4096, out, bins,
+                                                           stage_flags);
+  else
+    fft_kernel<LOG2N, SPC, false><<<grid, block, lds, s>>>(in, sched, reorder, n_frames, tw4096, out, bins,
+                                                            stage_flags);
+  return hipGetLastError();
+}
+
+template <int SPC>
+static hipError_t launch_fft_spc(hipStream_t s, const uint32_t* tw, const uint32_t* in, uint32_t log2n,
+                                 uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
+                                 uint32_t* sf) {
+  switch (log2n) {
+    case 3: return launch_fft_n<3, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 4: return launch_fft_n<4, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 5: return launch_fft_n<5, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 6: return launch_fft_n<6, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 7: return launch_fft_n<7, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 8: return launch_fft_n<8, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 9: return launch_fft_n<9, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 10: return launch_fft_n<10, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 11: return launch_fft_n<11, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+    case 12: return launch_fft_n<12, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t n, uint32_t spc,
@@ -420,19 +473,12 @@ hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in,
                       uint32_t* stage_flags) {
   if (n_frames == 0) return hipSuccess;
   const uint32_t log2n = ilog2u(n);
-  const uint32_t tpf = n / 4;
-  const uint32_t fpb = tpf >= 256 ? 1 : 256 / tpf;
-  const uint32_t threads = tpf * fpb;
-  const size_t np = n + n / 32;
-  const size_t lds = (size_t)(np + fpb + fpb * np) * sizeof(uint32_t);
-  const dim3 grid((n_frames + fpb - 1) / fpb);
-  if (spc == 2)
-    fft_kernel<2><<<grid, threads, lds, s>>>(in, log2n, sched, reorder, n_frames, fpb, tw4096, out, bins,
-                                              stage_flags);
-  else
-    fft_kernel<1><<<grid, threads, lds, s>>>(in, log2n, sched, reorder, n_frames, fpb, tw4096, out, bins,
-                                              stage_flags);
-  return hipGetLastError();
+  if (spc == 2) {
+    if (log2n < 3) return hipErrorInvalidValue;
+    return launch_fft_spc<2>(s, tw4096, in, log2n, sched, reorder, n_frames, out, bins, stage_flags);
+  }
+  if (log2n < 3) return hipErrorInvalidValue;
+  return launch_fft_spc<1>(s, tw4096, in, log2n, sched, reorder, n_frames, out, bins, stage_flags);
 }
 
 // ============================================================================
