@@ -14,14 +14,19 @@
 // bin costs two differences whatever the reference length; u64 wrap-around of
 // the prefix differences equals the reference's u64 running sums modulo 2^64.
 //
-// Three launches: per-row counts, a scan of counts over rows into event
-// offsets, and the per-row emit with an in-block ordered compaction.
+// Three launches: per-row decisions with the events staged in a fixed slot per
+// row, a scan of the per-row counts into event offsets, and a gather that moves
+// each row's events to its offset (recomputing only rows that overflowed).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "rd_internal.h"
 
 namespace rd {
+
+// Events staged per row by the first pass (summary included); rows with more
+// are recomputed by the gather pass.
+constexpr uint32_t kCfarSlot = 256;
 
 struct CfarGeom {
   uint32_t enable, go, dense, gl, gg, rl, rg, alpha, refs_ok;
@@ -43,9 +48,10 @@ __host__ __device__ inline CfarGeom clamp_geom(const rd_cfar_cfg& c, uint32_t ma
 
 // One wave per row; a block holds up to 4 rows (waves), each with its own
 // exclusive prefix P[0..n] in LDS. Lane l owns the interleaved bins l + 64*j, so
-// global loads, LDS accesses and event stores are all consecutive across lanes;
-// the prefix is built one 64-bin block at a time with a wave scan and a carry,
-// and events are compacted in bin order with a ballot per block.
+// global loads, LDS accesses and event stores are consecutive across lanes.
+// The row is first copied into LDS with every load in flight, then scanned in
+// place one 64-bin block at a time (wave scan plus carry). Nothing below needs
+// a block-wide barrier: each wave only touches its own row.
 __device__ __forceinline__ uint64_t wave_incscan(uint64_t v, uint32_t lane) {
   uint64_t x = v;
 #pragma unroll
@@ -56,17 +62,27 @@ __device__ __forceinline__ uint64_t wave_incscan(uint64_t v, uint32_t lane) {
   return x;
 }
 
+// LDS writes of one lane become visible to the other lanes of the wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename Cell>
 __device__ __forceinline__ void row_prefix(const Cell* row, uint32_t n, uint32_t lane, uint64_t* P) {
+  for (uint32_t i = lane; i < n; i += 64) P[i + 1] = (uint64_t)row[i];
+  wave_lds_sync();
   uint64_t carry = 0;
   for (uint32_t b0 = 0; b0 < n; b0 += 64) {
     const uint32_t i = b0 + lane;
-    const uint64_t c = i < n ? (uint64_t)row[i] : 0ull;
+    const uint64_t c = i < n ? P[i + 1] : 0ull;
     const uint64_t x = wave_incscan(c, lane);
-    if (i < n) P[i] = carry + x - c;
+    if (i < n) P[i + 1] = carry + x;
     carry += __shfl(x, 63);
   }
-  if (lane == 0) P[n] = carry;
+  if (lane == 0) P[0] = 0;
+  wave_lds_sync();
 }
 
 struct Decision {
@@ -99,23 +115,69 @@ __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32
   return d;
 }
 
+// 176-bit event, LSB first: kind 2 | bin 16 | frame 16 | ref 7 | alpha 16 |
+// det 16 | sup 16 | cut 40 | sum 47 (cfar_model.hpp:75-84); sof bit 176, eof 177.
+__device__ __forceinline__ void put_event(uint64_t* w, uint32_t kind, uint32_t bin, uint32_t frame, uint32_t ref,
+                                          uint32_t alpha, uint32_t det, uint32_t sup, uint64_t cut, uint64_t sum,
+                                          bool sof, bool eof) {
+  const uint64_t det16 = det & 0xFFFFu;
+  const uint64_t cut40 = cut & ((1ull << 40) - 1);
+  const uint64_t sum47 = sum & ((1ull << 47) - 1);
+  w[0] = (uint64_t)(kind & 3u) | ((uint64_t)(bin & 0xFFFFu) << 2) | ((uint64_t)(frame & 0xFFFFu) << 18) |
+         ((uint64_t)(ref & 0x7Fu) << 34) | ((uint64_t)(alpha & 0xFFFFu) << 41) | (det16 << 57);
+  w[1] = (det16 >> 7) | ((uint64_t)(sup & 0xFFFFu) << 9) | (cut40 << 25);
+  w[2] = (cut40 >> 39) | (sum47 << 1) | ((uint64_t)sof << 48) | ((uint64_t)eof << 49);
+}
+
+// The events of one row, in bin order, summary last, written to dst[0 .. room);
+// returns the row's event count (summary included).
+__device__ uint64_t emit_row(const uint64_t* P, uint32_t n, const CfarGeom& g, uint32_t frame, uint32_t lane,
+                             uint64_t* dst, uint64_t room) {
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
+  uint64_t k = 0;                                 // events emitted so far in the row
+  uint32_t det = 0, sup = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+    const uint32_t i = b0 + lane;
+    Decision d{};
+    const bool in = i < n;
+    if (in) d = decide(P, n, i, g);
+    const bool emit = in && (d.hit || g.dense);
+    const uint64_t em = __ballot(emit);
+    det += (uint32_t)__popcll(__ballot(in && d.hit));
+    sup += (uint32_t)__popcll(__ballot(in && !d.eval));
+    if (emit) {
+      const uint64_t pos = k + (uint64_t)__popcll(em & below);
+      const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
+      if (pos < room)
+        put_event(dst + 3 * pos, kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut, d.eval ? d.sum : 0ull,
+                  pos == 0, false);
+    }
+    k += (uint64_t)__popcll(em);
+  }
+  if (lane == 0 && k < room) put_event(dst + 3 * k, 3u, n, frame, 0, g.alpha, det, sup, 0, 0, k == 0, true);
+  return k + 1;
+}
+
+__device__ __forceinline__ uint32_t frame_of(uint32_t r, const uint32_t* frame_ids, uint32_t frame_mod) {
+  return frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
+}
+
+// Pass 1: each row's events go to its staging slot (as many as fit);
+// row_count[r] receives the row's event count.
 template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_count_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
-                                                         CfarGeom g, uint32_t* __restrict__ row_count) {
+__global__ __launch_bounds__(256) void cfar_rows_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
+                                                        CfarGeom g, const uint32_t* __restrict__ frame_ids,
+                                                        uint32_t frame_mod, uint64_t* __restrict__ stage,
+                                                        uint32_t* __restrict__ row_count) {
   extern __shared__ uint64_t lds[];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
-  uint64_t* P = lds + (size_t)wave * (n + 1);
-  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, lane, P);
-  __syncthreads();
   if (r >= n_rows) return;
-  uint32_t det = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
-    const uint32_t i = b0 + lane;
-    const bool hit = i < n && decide(P, n, i, g).hit;
-    det += (uint32_t)__popcll(__ballot(hit));
-  }
-  if (lane == 0) row_count[r] = (g.dense ? n : det) + 1;
+  uint64_t* P = lds + (size_t)wave * (n + 1);
+  row_prefix(cells + (size_t)r * n, n, lane, P);
+  const uint64_t cnt =
+      emit_row(P, n, g, frame_of(r, frame_ids, frame_mod), lane, stage + (size_t)r * kCfarSlot * 3, kCfarSlot);
+  if (lane == 0) row_count[r] = (uint32_t)cnt;
 }
 
 // Exclusive scan of row counts into 64-bit offsets; offs[n_rows] = total.
@@ -149,60 +211,37 @@ __global__ __launch_bounds__(1024) void cfar_scan_kernel(const uint32_t* __restr
   }
 }
 
-// 176-bit event, LSB first: kind 2 | bin 16 | frame 16 | ref 7 | alpha 16 |
-// det 16 | sup 16 | cut 40 | sum 47 (cfar_model.hpp:75-84); sof bit 176, eof 177.
-__device__ __forceinline__ void put_event(uint64_t* w, uint32_t kind, uint32_t bin, uint32_t frame, uint32_t ref,
-                                          uint32_t alpha, uint32_t det, uint32_t sup, uint64_t cut, uint64_t sum,
-                                          bool sof, bool eof) {
-  const uint64_t det16 = det & 0xFFFFu;
-  const uint64_t cut40 = cut & ((1ull << 40) - 1);
-  const uint64_t sum47 = sum & ((1ull << 47) - 1);
-  w[0] = (uint64_t)(kind & 3u) | ((uint64_t)(bin & 0xFFFFu) << 2) | ((uint64_t)(frame & 0xFFFFu) << 18) |
-         ((uint64_t)(ref & 0x7Fu) << 34) | ((uint64_t)(alpha & 0xFFFFu) << 41) | (det16 << 57);
-  w[1] = (det16 >> 7) | ((uint64_t)(sup & 0xFFFFu) << 9) | (cut40 << 25);
-  w[2] = (cut40 >> 39) | (sum47 << 1) | ((uint64_t)sof << 48) | ((uint64_t)eof << 49);
-}
-
+// Pass 3: copy each row's staged events to its offset; a row that overflowed
+// its slot is recomputed and written directly.
 template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_emit_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
-                                                        CfarGeom g, const uint32_t* __restrict__ frame_ids,
-                                                        uint32_t frame_mod, const uint64_t* __restrict__ offs,
-                                                        uint64_t* __restrict__ events, uint64_t cap) {
+__global__ __launch_bounds__(256) void cfar_gather_kernel(const Cell* __restrict__ cells, uint32_t n,
+                                                          uint32_t n_rows, CfarGeom g,
+                                                          const uint32_t* __restrict__ frame_ids, uint32_t frame_mod,
+                                                          const uint64_t* __restrict__ stage,
+                                                          const uint32_t* __restrict__ row_count,
+                                                          const uint64_t* __restrict__ offs,
+                                                          uint64_t* __restrict__ events, uint64_t cap) {
   extern __shared__ uint64_t lds[];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
-  uint64_t* P = lds + (size_t)wave * (n + 1);
-  if (r < n_rows) row_prefix(cells + (size_t)r * n, n, lane, P);
-  __syncthreads();
   if (r >= n_rows) return;
-  const uint32_t frame = frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
   const uint64_t base = offs[r];
-  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
-  uint64_t k = 0;                                 // events emitted so far in the row
-  uint32_t det = 0, sup = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
-    const uint32_t i = b0 + lane;
-    Decision d{};
-    const bool in = i < n;
-    if (in) d = decide(P, n, i, g);
-    const bool emit = in && (d.hit || g.dense);
-    const uint64_t em = __ballot(emit);
-    det += (uint32_t)__popcll(__ballot(in && d.hit));
-    sup += (uint32_t)__popcll(__ballot(in && !d.eval));
-    if (emit) {
-      const uint64_t pos = k + (uint64_t)__popcll(em & below);
-      const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
-      if (base + pos < cap)
-        put_event(events + 3 * (base + pos), kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut,
-                  d.eval ? d.sum : 0ull, pos == 0, false);
-    }
-    k += (uint64_t)__popcll(em);
+  const uint32_t cnt = row_count[r];
+  const uint64_t room = base < cap ? cap - base : 0;
+  if (cnt <= kCfarSlot) {
+    const uint64_t words = 3ull * min((uint64_t)cnt, room);
+    const uint64_t* src = stage + (size_t)r * kCfarSlot * 3;
+    for (uint64_t w = lane; w < words; w += 64) events[3 * base + w] = src[w];
+    return;
   }
-  if (lane == 0 && base + k < cap)
-    put_event(events + 3 * (base + k), 3u, n, frame, 0, g.alpha, det, sup, 0, 0, k == 0, true);
+  uint64_t* P = lds + (size_t)wave * (n + 1);
+  row_prefix(cells + (size_t)r * n, n, lane, P);
+  emit_row(P, n, g, frame_of(r, frame_ids, frame_mod), lane, events + 3 * base, room);
 }
 
-size_t cfar_scratch_bytes(uint32_t n_rows) { return ((size_t)n_rows + 1) * sizeof(uint64_t); }
+size_t cfar_scratch_bytes(uint32_t n_rows) {
+  return ((size_t)n_rows + 1) * sizeof(uint64_t) + (size_t)n_rows * kCfarSlot * 3 * sizeof(uint64_t);
+}
 
 hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n, uint32_t n_rows,
                        const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
@@ -216,23 +255,24 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
   const size_t lds = rpb * row_lds;
   const dim3 grid((n_rows + rpb - 1) / rpb), block(64 * rpb);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
+  uint64_t* stage = offs + n_rows + 1;
   if (cell_bytes == 4)
-    cfar_count_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
-                                                         row_count);
+    cfar_rows_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
+                                                        frame_ids, frame_mod, stage, row_count);
   else
-    cfar_count_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
-                                                         row_count);
+    cfar_rows_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
+                                                        frame_ids, frame_mod, stage, row_count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   cfar_scan_kernel<<<1, 1024, 0, s>>>(row_count, n_rows, offs, total_dev);
   e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || cap == 0) return e;
   if (cell_bytes == 4)
-    cfar_emit_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
-                                                        frame_ids, frame_mod, offs, events, cap);
+    cfar_gather_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
+                                                          frame_ids, frame_mod, stage, row_count, offs, events, cap);
   else
-    cfar_emit_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
-                                                        frame_ids, frame_mod, offs, events, cap);
+    cfar_gather_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
+                                                          frame_ids, frame_mod, stage, row_count, offs, events, cap);
   return hipGetLastError();
 }
 

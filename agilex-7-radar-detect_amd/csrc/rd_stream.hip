@@ -300,7 +300,7 @@ __device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
 // schedule bit of each sub-stage is read at run time.
 // ----------------------------------------------------------------------------
 template <int LOG2N, int SPC, bool ALL1>
-__global__ __launch_bounds__(LOG2N >= 12 ? 1024 : 256) void fft_kernel(
+__global__ __launch_bounds__((1u << LOG2N) / 4 > 256 ? (1u << LOG2N) / 4 : 256) void fft_kernel(
     const uint32_t* __restrict__ in, uint32_t sched, int reorder, uint32_t n_frames,
     const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out, uint32_t* __restrict__ bins,
     uint32_t* __restrict__ stage_flags) {
