@@ -283,8 +283,8 @@ rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, ui
   hipStream_t s = c->stream;
   RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
   RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
-  RD_TRY(rd::launch_align(s, b->X, CH, cols, b->Xa));
-  RD_TRY(rd::launch_bf(s, b->Xa, b->weights, CH, B, cols, b->y, b->yflags, b->power, nullptr, c->prep));
+  // beamformer + power on the FFT layout; the same staged tiles give the aligned copy
+  RD_TRY(rd::launch_bf_chmajor(s, b->X, b->weights, CH, B, cols, b->y, b->yflags, b->power, b->Xa));
   if (b->R) RD_TRY(rd::launch_cov(s, b->Xa, CH, F, N, b->R));
   RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
                          b->events ? b->events_cap : 0, b->row_count, c->scratch,

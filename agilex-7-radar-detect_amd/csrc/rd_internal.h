@@ -19,6 +19,10 @@ hipError_t launch_align(hipStream_t s, const uint32_t* X, uint32_t n_chan, uint6
 size_t bf_prep_bytes(uint32_t n_ant, uint32_t n_beams);
 hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                      uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc, void* prep);
+// The beamformer reading the FFT's channel-major X[ch][col] and writing the
+// aligned copy Xa[col][ch] (nullable) from the same staged tiles.
+hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                             uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* xa_out);
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R);
 

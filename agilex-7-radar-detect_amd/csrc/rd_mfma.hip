@@ -71,42 +71,38 @@ __device__ __forceinline__ int64_t combine(v4i hh, v4i x, v4i ll, int r) {
 // channel k - CHp (CHp = CH rounded up to 16).
 // ============================================================================
 
-// Prepared weights: per beam tile, step, lane: {Are_h, Are_l, Aim_h, Aim_l};
-// then per beam: Sa[b] = sum_a (wr + wi), Wis[b] = sum_a wi.
-struct BfPrep {
-  static __host__ __device__ size_t frag_words(uint32_t bt, uint32_t steps) { return (size_t)bt * steps * 64 * 8; }
-};
+// Weight digits in LDS: per beam tile, step, lane: {Are_h, Are_l, Aim_h, Aim_l}
+// (8 words), then per beam: Sa[b] = sum_a (wr + wi), Wis[b] = sum_a wi.
+__host__ __device__ inline size_t bf_frag_words(uint32_t bt, uint32_t steps) { return (size_t)bt * steps * 64 * 8; }
 
-size_t bf_prep_bytes(uint32_t n_ant, uint32_t n_beams) {
-  const uint32_t chp = (n_ant + 15) / 16 * 16, steps = chp / 16, bt = (n_beams + 15) / 16;
-  return BfPrep::frag_words(bt, steps) * 4 + (size_t)bt * 16 * 2 * 4;
-}
+size_t bf_prep_bytes(uint32_t, uint32_t) { return 0; }
 
-__global__ void bf_prep_kernel(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
-                               uint32_t steps, uint32_t bt_n, uint32_t* __restrict__ prep) {
-  const uint32_t lane = threadIdx.x;  // 64
-  const uint32_t s = blockIdx.x, bt = blockIdx.y;
-  const uint32_t b = bt * 16 + lane % 16;
-  const uint32_t k0 = 32 * s + 8 * (lane / 16);
-  const bool imag = k0 >= chp;
-  const uint32_t a0 = imag ? k0 - chp : k0;
-  uint32_t p[8];
-  uint64_t vmask = 0;
-  for (int t = 0; t < 8; ++t) {
-    const bool ok = b < B && a0 + t < CH;
-    p[t] = ok ? W[(size_t)b * CH + a0 + t] : 0u;
-    if (ok) vmask |= 0xFFull << (8 * t);
+// Every block derives the weight digits itself (B x CH words, L2-resident).
+__device__ void bf_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
+                                  uint32_t steps, uint32_t bt_n, uint32_t* frag, int32_t* tail) {
+  for (uint32_t f = threadIdx.x; f < bt_n * steps * 64; f += blockDim.x) {
+    const uint32_t lane = f & 63, s = (f >> 6) % steps, bt = (f >> 6) / steps;
+    const uint32_t b = bt * 16 + (lane & 15);
+    const uint32_t k0 = 32 * s + 8 * (lane >> 4);
+    const bool imag = k0 >= chp;
+    const uint32_t a0 = imag ? k0 - chp : k0;
+    uint32_t p[8];
+    uint64_t vmask = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const bool ok = b < B && a0 + t < CH;
+      p[t] = ok ? W[(size_t)b * CH + a0 + t] : 0u;
+      vmask |= ok ? (0xFFull << (8 * t)) : 0ull;
+    }
+    const Dig8 d = digits8(p, vmask);
+    // Are = [Wr | Wi], Aim = [Wi | Wr]
+    uint64_t* o = reinterpret_cast<uint64_t*>(frag + (size_t)f * 8);
+    o[0] = imag ? d.hi : d.hr;
+    o[1] = imag ? d.li : d.lr;
+    o[2] = imag ? d.hr : d.hi;
+    o[3] = imag ? d.lr : d.li;
   }
-  const Dig8 d = digits8(p, vmask);
-  // Are = [Wr | Wi], Aim = [Wi | Wr]
-  const uint64_t re_h = imag ? d.hi : d.hr, re_l = imag ? d.li : d.lr;
-  const uint64_t im_h = imag ? d.hr : d.hi, im_l = imag ? d.lr : d.li;
-  uint32_t* o = prep + (((size_t)bt * steps + s) * 64 + lane) * 8;
-  o[0] = (uint32_t)re_h; o[1] = (uint32_t)(re_h >> 32);
-  o[2] = (uint32_t)re_l; o[3] = (uint32_t)(re_l >> 32);
-  o[4] = (uint32_t)im_h; o[5] = (uint32_t)(im_h >> 32);
-  o[6] = (uint32_t)im_l; o[7] = (uint32_t)(im_l >> 32);
-  if (s == 0 && lane < 16) {
+  for (uint32_t b = threadIdx.x; b < bt_n * 16; b += blockDim.x) {
     int32_t sa = 0, wis = 0;
     if (b < B)
       for (uint32_t a = 0; a < CH; ++a) {
@@ -114,158 +110,192 @@ __global__ void bf_prep_kernel(const uint32_t* __restrict__ W, uint32_t CH, uint
         sa += cre(w) + cim(w);
         wis += cim(w);
       }
-    int32_t* tail = reinterpret_cast<int32_t*>(prep + BfPrep::frag_words(bt_n, steps));
     tail[2 * b] = sa;
     tail[2 * b + 1] = wis;
   }
 }
 
-constexpr int kBfColTilesPerWave = 4;
+// Columns per block tile: 256 for up to 32 channels, 128 above.
+template <int STEPS>
+struct BfTile {
+  static constexpr uint32_t TC = STEPS <= 2 ? 256 : 128;
+};
 
-// VEC8: n_ant is a multiple of 8 and Xa is 32-byte aligned, so each lane's 8
-// channels of a column are two aligned 16-byte loads.
-template <int STEPS, bool VEC8>
-__global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ Xa, const uint32_t* __restrict__ prep,
+// A block stages a tile of TC columns x all channels in LDS as [TC][CH+1]
+// (odd row stride: conflict-free) and loops over tiles with a grid stride, so
+// the weight digits are derived once per block. CHMAJOR: the input is the FFT output layout
+// X[ch][col]; the block then also writes the aligned copy Xa[col][ch] (the
+// alignment stage) from the staged tile. Otherwise the input is Xa.
+template <int STEPS, bool CHMAJOR>
+__global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ X, const uint32_t* __restrict__ W,
                                                  uint32_t CH, uint32_t B, uint32_t bt_n, uint64_t cols,
                                                  uint32_t* __restrict__ y, uint8_t* __restrict__ flags,
-                                                 uint32_t* __restrict__ power, int64_t* __restrict__ acc) {
+                                                 uint32_t* __restrict__ power, int64_t* __restrict__ acc,
+                                                 uint32_t* __restrict__ xa_out) {
   constexpr uint32_t chp = STEPS * 16;
+  constexpr uint32_t TC = BfTile<STEPS>::TC;
+  constexpr uint32_t TILES_PER_WAVE = TC / 64;
   extern __shared__ uint32_t lds[];
-  const size_t fw = BfPrep::frag_words(bt_n, STEPS);
-  const size_t total = fw + (size_t)bt_n * 32;
-  for (size_t i = threadIdx.x; i < total; i += blockDim.x) lds[i] = prep[i];
-  __syncthreads();
-  const int32_t* tail = reinterpret_cast<const int32_t*>(lds + fw);
+  const size_t fw = bf_frag_words(bt_n, STEPS);
+  uint32_t* frag = lds;
+  int32_t* tail = reinterpret_cast<int32_t*>(lds + fw);
+  uint32_t* xt = lds + fw + (size_t)bt_n * 32;  // [TC][CH + 1]
+  const uint32_t rs = CH + 1;
+  bf_weights_to_lds(W, CH, B, chp, STEPS, bt_n, frag, tail);
 
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const int64_t kconst = 16384LL * 2 * CH;
+  const uint64_t n_tiles = (cols + TC - 1) / TC;
 
-  for (int t = 0; t < kBfColTilesPerWave; ++t) {
-    const uint64_t col0 = ((uint64_t)blockIdx.x * 4 * kBfColTilesPerWave + wave * kBfColTilesPerWave + t) * 16;
-    if (col0 >= cols) break;
-    const uint64_t col = col0 + cl;
-    const bool col_ok = col < cols;
-    // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
-    uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
-    int32_t ps_r = 0, ps_i = 0;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t c0 = tile * TC;
+    const uint32_t ncol = (uint32_t)min<uint64_t>(TC, cols - c0);
+    __syncthreads();  // previous tile fully consumed (and the weights are in place)
+    if (CHMAJOR) {  // X[ch][col]: each channel row of the tile is TC contiguous words
+      for (uint32_t idx = threadIdx.x; idx < CH * TC; idx += 256) {
+        const uint32_t a = idx / TC, j = idx % TC;
+        if (j < ncol) xt[j * rs + a] = X[(size_t)a * cols + c0 + j];
+      }
+    } else {  // Xa[col][ch]: the tile is one contiguous run
+      for (uint32_t idx = threadIdx.x; idx < ncol * CH; idx += 256) {
+        const uint32_t j = idx / CH, a = idx % CH;
+        xt[j * rs + a] = X[c0 * CH + idx];
+      }
+    }
+    __syncthreads();
+    if (CHMAJOR && xa_out) {
+      for (uint32_t w = threadIdx.x; w < ncol * CH; w += 256) xa_out[c0 * CH + w] = xt[(w / CH) * rs + w % CH];
+    }
+
+    for (uint32_t t = 0; t < TILES_PER_WAVE; ++t) {
+      const uint32_t jl = (wave * TILES_PER_WAVE + t) * 16;
+      if (jl >= ncol) break;
+      const uint32_t j = jl + cl;
+      const bool col_ok = j < ncol;
+      const uint64_t col = c0 + j;
+      // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
+      uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
+      int32_t ps_r = 0, ps_i = 0;
 #pragma unroll
-    for (int s = 0; s < STEPS; ++s) {
-      const uint32_t k0 = 32 * s + 8 * grp;
-      const bool imag = k0 >= chp;
-      const uint32_t a0 = imag ? k0 - chp : k0;
-      uint32_t p[8];
-      uint64_t vmask = 0;
-      if (VEC8) {
-        const bool ok = col_ok && a0 < CH;
-        if (ok) {
-          const uint4* src = reinterpret_cast<const uint4*>(Xa + col * CH + a0);
-          const uint4 v0 = src[0], v1 = src[1];
-          p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w;
-          p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) p[q] = 0u;
-        }
-        vmask = ok ? ~0ull : 0ull;
-      } else {
+      for (int s = 0; s < STEPS; ++s) {
+        const uint32_t k0 = 32 * s + 8 * grp;
+        const bool imag = k0 >= chp;
+        const uint32_t a0 = imag ? k0 - chp : k0;
+        uint32_t p[8];
+        uint64_t vmask = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const bool ok = col_ok && a0 + q < CH;
-          p[q] = ok ? Xa[col * CH + a0 + q] : 0u;
+          p[q] = ok ? xt[j * rs + a0 + q] : 0u;
           vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
         }
-      }
-      const Dig8 d = digits8(p, vmask);
-      int32_t sr = 0, si = 0;
+        const Dig8 d = digits8(p, vmask);
+        int32_t sr = 0, si = 0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        sr += cre(p[q]);
-        si += cim(p[q]);
+        for (int q = 0; q < 8; ++q) {
+          sr += cre(p[q]);
+          si += cim(p[q]);
+        }
+        if (imag) {
+          ps_i += si;
+          bih[s] = d.hi;
+          bil[s] = d.li;
+          brh[s] = ~d.hi & vmask;
+          brl[s] = ~d.li & vmask;
+        } else {
+          ps_r += sr;
+          bih[s] = d.hr;
+          bil[s] = d.lr;
+          brh[s] = d.hr;
+          brl[s] = d.lr;
+        }
       }
-      if (imag) {
-        ps_i += si;
-        bih[s] = d.hi;
-        bil[s] = d.li;
-        brh[s] = ~d.hi & vmask;
-        brl[s] = ~d.li & vmask;
-      } else {
-        ps_r += sr;
-        bih[s] = d.hr;
-        bil[s] = d.lr;
-        brh[s] = d.hr;
-        brl[s] = d.lr;
-      }
-    }
-    // column sums over all channels: the four lane groups of a column
-    ps_r += __shfl_xor(ps_r, 16);
-    ps_r += __shfl_xor(ps_r, 32);
-    ps_i += __shfl_xor(ps_i, 16);
-    ps_i += __shfl_xor(ps_i, 32);
-    const int64_t sb_im = (int64_t)ps_r + ps_i;
-    const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
+      // column sums over all channels: the four lane groups of a column
+      ps_r += __shfl_xor(ps_r, 16);
+      ps_r += __shfl_xor(ps_r, 32);
+      ps_i += __shfl_xor(ps_i, 16);
+      ps_i += __shfl_xor(ps_i, 32);
+      const int64_t sb_im = (int64_t)ps_r + ps_i;
+      const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
 
-    for (uint32_t bt = 0; bt < bt_n; ++bt) {
-      v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
-      v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
+      for (uint32_t bt = 0; bt < bt_n; ++bt) {
+        v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
+        v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
 #pragma unroll
-      for (int s = 0; s < STEPS; ++s) {
-        const uint64_t* a = reinterpret_cast<const uint64_t*>(lds + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
-        const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
-        rhh = mfma8(are_h, brh[s], rhh);
-        rx = mfma8(are_h, brl[s], rx);
-        rx = mfma8(are_l, brh[s], rx);
-        rll = mfma8(are_l, brl[s], rll);
-        ihh = mfma8(aim_h, bih[s], ihh);
-        ix = mfma8(aim_h, bil[s], ix);
-        ix = mfma8(aim_l, bih[s], ix);
-        ill = mfma8(aim_l, bil[s], ill);
-      }
-      if (!col_ok) continue;
+        for (int s = 0; s < STEPS; ++s) {
+          const uint64_t* a = reinterpret_cast<const uint64_t*>(frag + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
+          const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
+          rhh = mfma8(are_h, brh[s], rhh);
+          rx = mfma8(are_h, brl[s], rx);
+          rx = mfma8(are_l, brh[s], rx);
+          rll = mfma8(are_l, brl[s], rll);
+          ihh = mfma8(aim_h, bih[s], ihh);
+          ix = mfma8(aim_h, bil[s], ix);
+          ix = mfma8(aim_l, bih[s], ix);
+          ill = mfma8(aim_l, bil[s], ill);
+        }
+        if (!col_ok) continue;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t b = bt * 16 + 4 * grp + r;
-        if (b >= B) continue;
-        const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
-        const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
-        const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
-        uint32_t fr = 0, fi = 0;
-        const int32_t yr = sat16_i64(rne15_i64(are), fr);
-        const int32_t yi = sat16_i64(rne15_i64(aim), fi);
-        const size_t o = (size_t)b * cols + col;
-        if (y) y[o] = cpack(yr, yi);
-        if (flags) flags[o] = (uint8_t)(fr | (fi << 2));
-        if (power) power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
-        if (acc) {
-          acc[2 * o] = are;
-          acc[2 * o + 1] = aim;
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t b = bt * 16 + 4 * grp + r;
+          if (b >= B) continue;
+          const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
+          const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
+          const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
+          uint32_t fr = 0, fi = 0;
+          const int32_t yr = sat16_i64(rne15_i64(are), fr);
+          const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+          const size_t o = (size_t)b * cols + col;
+          if (y) y[o] = cpack(yr, yi);
+          if (flags) flags[o] = (uint8_t)(fr | (fi << 2));
+          if (power) power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+          if (acc) {
+            acc[2 * o] = are;
+            acc[2 * o + 1] = aim;
+          }
         }
       }
     }
   }
 }
 
-hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
-                     uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc, void* prep) {
-  const uint32_t chp = (n_ant + 15) / 16 * 16, steps = chp / 16, bt = (n_beams + 15) / 16;
-  uint32_t* pw = static_cast<uint32_t*>(prep);
-  bf_prep_kernel<<<dim3(steps, bt), 64, 0, s>>>(W, n_ant, n_beams, chp, steps, bt, pw);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || n_cols == 0) return e;
-  const uint64_t cols_per_block = 16ull * 4 * kBfColTilesPerWave;
-  const dim3 grid((uint32_t)((n_cols + cols_per_block - 1) / cols_per_block));
-  const size_t lds = bf_prep_bytes(n_ant, n_beams);
-  const bool vec8 = (n_ant % 8) == 0 && ((uintptr_t)Xa & 31u) == 0;
-#define RD_BF_LAUNCH(S, V) \
-  bf_kernel<S, V><<<grid, 256, lds, s>>>(Xa, pw, n_ant, n_beams, bt, n_cols, y, flags, power, acc)
-  switch (steps) {
-    case 1: if (vec8) RD_BF_LAUNCH(1, true); else RD_BF_LAUNCH(1, false); break;
-    case 2: if (vec8) RD_BF_LAUNCH(2, true); else RD_BF_LAUNCH(2, false); break;
-    case 3: if (vec8) RD_BF_LAUNCH(3, true); else RD_BF_LAUNCH(3, false); break;
-    default: if (vec8) RD_BF_LAUNCH(4, true); else RD_BF_LAUNCH(4, false); break;
-  }
-#undef RD_BF_LAUNCH
+template <int STEPS>
+static hipError_t launch_bf_s(hipStream_t s, const uint32_t* X, bool chmajor, const uint32_t* W, uint32_t n_ant,
+                              uint32_t n_beams, uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power,
+                              int64_t* acc, uint32_t* xa_out) {
+  const uint32_t bt = (n_beams + 15) / 16;
+  constexpr uint32_t TC = BfTile<STEPS>::TC;
+  const uint64_t tiles = (n_cols + TC - 1) / TC;
+  const dim3 grid((uint32_t)(tiles < 2048 ? tiles : 2048));
+  const size_t lds = (bf_frag_words(bt, STEPS) + (size_t)bt * 32 + (size_t)TC * (n_ant + 1)) * sizeof(uint32_t);
+  if (chmajor)
+    bf_kernel<STEPS, true><<<grid, 256, lds, s>>>(X, W, n_ant, n_beams, bt, n_cols, y, flags, power, acc, xa_out);
+  else
+    bf_kernel<STEPS, false><<<grid, 256, lds, s>>>(X, W, n_ant, n_beams, bt, n_cols, y, flags, power, acc, nullptr);
   return hipGetLastError();
+}
+
+static hipError_t launch_bf_any(hipStream_t s, const uint32_t* X, bool chmajor, const uint32_t* W, uint32_t n_ant,
+                                uint32_t n_beams, uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power,
+                                int64_t* acc, uint32_t* xa_out) {
+  if (n_cols == 0) return hipSuccess;
+  switch ((n_ant + 15) / 16) {
+    case 1: return launch_bf_s<1>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
+    case 2: return launch_bf_s<2>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
+    case 3: return launch_bf_s<3>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
+    default: return launch_bf_s<4>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
+  }
+}
+
+hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                     uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc, void*) {
+  return launch_bf_any(s, Xa, false, W, n_ant, n_beams, n_cols, y, flags, power, acc, nullptr);
+}
+
+hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                             uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* xa_out) {
+  return launch_bf_any(s, X, true, W, n_ant, n_beams, n_cols, y, flags, power, nullptr, xa_out);
 }
 
 // ============================================================================
@@ -281,6 +311,10 @@ hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint3
 // sums into partial int64 results (all correction terms are linear in the
 // frames), and the splits of a job are added with LDS 64-bit atomics.
 // ============================================================================
+// Per-thread share of one step's staged tiles: 16 frames x nb*CH words per
+// split over 64 threads per job and split = CH / (4 * upper tiles) <= 4 words.
+constexpr uint32_t kCovWords = 4;
+
 __global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
                                                    uint32_t N, uint32_t nb, uint32_t nt, uint32_t fs_n,
                                                    int64_t* __restrict__ R) {
@@ -315,15 +349,34 @@ __global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ 
   for (int z = 0; z < 6; ++z) acc[z] = v4i{0, 0, 0, 0};
   int32_t sra = 0, sia = 0, srb = 0, sib = 0, nfr = 0;
 
+  // The next step's tile words are loaded into registers while the current step
+  // computes (each thread owns at most kCovWords words of a step's tiles).
+  const uint32_t step_words = fs_n * 16 * row_words;
+  uint32_t nxt[kCovWords];
+  auto fetch = [&](uint32_t st) {
+#pragma unroll
+    for (uint32_t q = 0; q < kCovWords; ++q) {
+      const uint32_t idx = threadIdx.x + q * blockDim.x;
+      uint32_t v = 0;
+      if (idx < step_words) {
+        const uint32_t sp = idx / (16 * row_words), rem = idx % (16 * row_words);
+        const uint32_t f = rem / row_words, w = rem % row_words;
+        const uint32_t m = (sp * sps + st) * 16 + f;
+        if (m < F && w / CH < bins_here) v = Xa[((size_t)m * N + k_base) * CH + w];
+      }
+      nxt[q] = v;
+    }
+  };
+  fetch(0);
   for (uint32_t st = 0; st < sps; ++st) {
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < fs_n * 16 * row_words; idx += blockDim.x) {
-      const uint32_t sp = idx / (16 * row_words), rem = idx % (16 * row_words);
-      const uint32_t f = rem / row_words, w = rem % row_words;
-      const uint32_t m = (sp * sps + st) * 16 + f;
-      tiles[idx] = (m < F && w / CH < bins_here) ? Xa[((size_t)m * N + k_base) * CH + w] : 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < kCovWords; ++q) {
+      const uint32_t idx = threadIdx.x + q * blockDim.x;
+      if (idx < step_words) tiles[idx] = nxt[q];
     }
     __syncthreads();
+    if (st + 1 < sps) fetch(st + 1);
     const uint32_t m0 = (fs * sps + st) * 16;
     const uint32_t f0 = 8 * (grp & 1);
     uint64_t fmask = 0;
