@@ -232,6 +232,14 @@ rd_status rd_bf_run(rd_ctx* c, const uint32_t* Xa, const uint32_t* W, uint32_t n
   return fin(rd::launch_bf(c->stream, Xa, W, n_ant, n_beams, n_cols, y, flags, power, acc, c->prep));
 }
 
+rd_status rd_bf_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                      uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* Xa) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!X || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
+  return fin(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
+}
+
 rd_status rd_cov_run(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                      int64_t* R) {
   rd_status st = enter(c);

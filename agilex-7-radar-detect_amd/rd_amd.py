@@ -21,7 +21,7 @@ RD_OK, RD_E_ARG, RD_E_GEOM, RD_E_HIP, RD_E_CAPACITY, RD_E_NODEV = 0, -1, -2, -3,
 EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
-    "rd_bf_run", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
+    "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy",
 ]
 
@@ -79,6 +79,7 @@ class Lib:
         L.rd_fft_run.argtypes = [vp, vp, u32, u32, u32, i32, u32, vp, vp, vp]
         L.rd_align_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_bf_run.argtypes = [vp, vp, vp, u32, u32, u64, vp, vp, vp, vp]
+        L.rd_bf_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp]
         L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
                                   C.POINTER(u64)]
@@ -221,6 +222,17 @@ class Ctx:
                                                      da.ptr if da else None))
         return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
                 da.download(np.int64, 2 * B * cols) if da else None)
+
+    def bf_stage(self, X, W, CH, B, F, N):
+        """The chain's beamformer on the FFT layout; returns y, flags, power, Xa."""
+        X = np.asarray(X, np.uint32)
+        cols = F * N
+        dX, dW = self.put(X), self.put(np.asarray(W, np.uint32))
+        dy, df, dp, da = self.buf(4 * B * cols), self.buf(B * cols), self.buf(4 * B * cols), self.buf(4 * CH * cols)
+        self.lib.check("rd_bf_stage", self.L.rd_bf_stage(self.h, dX.ptr, dW.ptr, CH, B, F, N, dy.ptr, df.ptr, dp.ptr,
+                                                         da.ptr))
+        return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
+                da.download(np.uint32, CH * cols))
 
     def cov(self, Xa, CH, F, N):
         dX = self.put(np.asarray(Xa, np.uint32))

@@ -65,13 +65,10 @@ def stage_kernels(ctx, ch, sc):
         ctx.lib.check("fft", L.rd_fft_run(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, 1, sc.CH * sc.F, None, b["X"].ptr,
                                           b["fft_flags"].ptr if b["fft_flags"] else None))
 
-    def align():
-        ctx.lib.check("align", L.rd_align_run(h, b["X"].ptr, sc.CH, sc.F, sc.N, b["Xa"].ptr))
-
     def bf():
-        ctx.lib.check("bf", L.rd_bf_run(h, b["Xa"].ptr, b["weights"].ptr, sc.CH, sc.B, cols,
-                                        b["y"].ptr if b["y"] else None, b["yflags"].ptr if b["yflags"] else None,
-                                        b["power"].ptr, None))
+        # the chain's beamformer: FFT layout in, power + aligned copy out (lean outputs)
+        ctx.lib.check("bf", L.rd_bf_stage(h, b["X"].ptr, b["weights"].ptr, sc.CH, sc.B, sc.F, sc.N, None, None,
+                                          b["power"].ptr, b["Xa"].ptr))
 
     def cov():
         ctx.lib.check("cov", L.rd_cov_run(h, b["Xa"].ptr, sc.CH, sc.F, sc.N, b["R"].ptr))
@@ -81,8 +78,7 @@ def stage_kernels(ctx, ch, sc):
     stages = {
         "pfb": (pfb, S * (4 + 4 + flag_b) + 4 * sc.N * sc.T),
         "fft": (fft, S * 8 + 4 * sc.CH * sc.F),
-        "align": (align, S * 8),
-        "bf": (bf, S * 4 + 4 * sc.B * sc.CH + sc.B * cols * (4 + (4 if b["y"] else 0) + (1 if b["yflags"] else 0))),
+        "bf": (bf, S * 4 + 4 * sc.B * sc.CH + S * 4 + sc.B * cols * 4),
         "cov": (cov, S * 4 + 16 * sc.N * sc.CH * (sc.CH + 1) // 2),
     }
     return stages

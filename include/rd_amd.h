@@ -155,6 +155,13 @@ rd_status rd_align_run(rd_ctx* ctx, const uint32_t* X, uint32_t n_chan, uint32_t
 rd_status rd_bf_run(rd_ctx* ctx, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                     uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc);
 
+/* The chain's beamformer: the same values as rd_align_run followed by rd_bf_run,
+ * read straight from the FFT output X [n_ant][n_frames][n_bins] (channel-major).
+ * Writes y/flags/power [n_beams][n_frames*n_bins] (each nullable) and the aligned
+ * copy Xa [n_frames][n_bins][n_ant] (nullable) from the same staged tiles. */
+rd_status rd_bf_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                      uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* Xa);
+
 /* Covariance: R[bin][pair i<=j] = sum over frames of covar::cross_power(X_i, X_j)
  * (covar_model.hpp:137-151), integrated by a block-mode covar::Integrator with
  * acc_w = 64 and window_len = n_frames (covar_model.hpp:186-262); at that width

@@ -141,6 +141,21 @@ def test_bf_unit_weights_passthrough(gpu):
     assert not fl.any()
 
 
+@pytest.mark.parametrize("CH,B,F,N", [(16, 16, 5, 1024), (64, 64, 2, 4096), (4, 4, 16, 64), (7, 3, 3, 100),
+                                      (33, 20, 2, 512), (48, 17, 1, 256)])
+def test_bf_stage_matches_align_then_bf(gpu, orc, CH, B, F, N):
+    rng = np.random.default_rng(CH + B + F + N)
+    for X, W in [(stim.rand_cx(rng, CH * F * N), stim.rand_cx(rng, B * CH)),
+                 (stim.rand_extreme(rng, CH * F * N), stim.rand_extreme(rng, B * CH))]:
+        y, fl, pw, Xa = gpu.bf_stage(X, W, CH, B, F, N)
+        xa = orc.align(X, CH, F, N)
+        _eq(Xa, xa, "aligned copy")
+        oy, ofl, opw, _ = orc.bf(xa, W, CH, B, want_acc=False)
+        _eq(y, oy, "y")
+        _eq(fl, ofl, "flags")
+        _eq(pw, opw, "power")
+
+
 # ---------------------------------------------------------------------------- covariance
 @pytest.mark.parametrize("CH,F,N", [(4, 16, 64), (16, 40, 33), (64, 20, 5), (1, 7, 9), (20, 33, 8), (33, 17, 6),
                                     (64, 512, 2), (16, 256, 12)])
