@@ -11,19 +11,6 @@
 #include "rd_fxp.h"
 #include "rd_internal.h"
 
-struct rd_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint32_t* tw = nullptr;      // 4096-entry twiddle table (device)
-  void* prep = nullptr;        // beamformer weight digits
-  size_t prep_bytes = 0;
-  void* scratch = nullptr;     // CFAR event offsets
-  size_t scratch_bytes = 0;
-  uint64_t* total_dev = nullptr;
-  uint64_t* total_host = nullptr;  // pinned
-  int live_graphs = 0;             // graphs hold the context's stream
-};
-
 namespace {
 
 #define RD_TRY(expr)                              \
@@ -75,13 +62,19 @@ rd_status grow(void** buf, size_t* have, size_t need) {
 
 bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
-rd_status enter(rd_ctx* ctx) {
+rd_status fin(hipError_t e) { return e == hipSuccess ? RD_OK : RD_E_HIP; }
+
+rd_status enter(rd_ctx* ctx) { return rd::ctx_enter(ctx); }
+
+}  // namespace
+
+namespace rd {
+
+rd_status ctx_enter(rd_ctx* ctx) {
   if (!ctx) return RD_E_ARG;
   RD_TRY(hipSetDevice(ctx->device));
   return RD_OK;
 }
-
-rd_status fin(hipError_t e) { return e == hipSuccess ? RD_OK : RD_E_HIP; }
 
 rd_status check_cpi(const rd_cpi_desc* d) {
   if (!d) return RD_E_ARG;
@@ -92,7 +85,15 @@ rd_status check_cpi(const rd_cpi_desc* d) {
   return RD_OK;
 }
 
-}  // namespace
+rd_status check_cpi_bufs(const rd_cpi_bufs* b) {
+  if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
+    return RD_E_ARG;
+  if ((uintptr_t)b->u & 15u) return RD_E_ARG;
+  if (!b->events && b->events_cap) return RD_E_ARG;
+  return RD_OK;
+}
+
+}  // namespace rd
 
 extern "C" {
 
@@ -274,11 +275,10 @@ namespace {
 rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
   rd_status st = enter(c);
   if (st != RD_OK) return st;
-  st = check_cpi(d);
+  st = rd::check_cpi(d);
   if (st != RD_OK) return st;
-  if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
-    return RD_E_ARG;
-  if ((uintptr_t)b->u & 15u) return RD_E_ARG;
+  st = rd::check_cpi_bufs(b);
+  if (st != RD_OK) return st;
   st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
   if (st != RD_OK) return st;
   return grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_beams * d->n_frames));

@@ -17,6 +17,8 @@
 // Three launches: per-row decisions with the events staged in a fixed slot per
 // row, a scan of the per-row counts into event offsets, and a gather that moves
 // each row's events to its offset (recomputing only rows that overflowed).
+// Batched launches run the same row bodies over the rows of many CPIs and one
+// scan block per CPI.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -24,15 +26,11 @@
 
 namespace rd {
 
-// Events staged per row by the first pass (summary included); rows with more
-// are recomputed by the gather pass.
+// Events staged per row by the first pass of a single-CPI launch (summary
+// included); rows with more are recomputed by the gather pass.
 constexpr uint32_t kCfarSlot = 256;
 
-struct CfarGeom {
-  uint32_t enable, go, dense, gl, gg, rl, rg, alpha, refs_ok;
-};
-
-__host__ __device__ inline CfarGeom clamp_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
+CfarGeom cfar_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
   CfarGeom g;
   g.enable = c.enable != 0;
   g.go = c.mode == 1;
@@ -162,28 +160,43 @@ __device__ __forceinline__ uint32_t frame_of(uint32_t r, const uint32_t* frame_i
   return frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
 }
 
-// Pass 1: each row's events go to its staging slot (as many as fit);
-// row_count[r] receives the row's event count.
+// Pass 1, one wave per row: the row's events go to its staging slot (as many
+// as fit); row_count[r] receives the row's event count.
 template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_rows_kernel(const Cell* __restrict__ cells, uint32_t n, uint32_t n_rows,
-                                                        CfarGeom g, const uint32_t* __restrict__ frame_ids,
-                                                        uint32_t frame_mod, uint64_t* __restrict__ stage,
-                                                        uint32_t* __restrict__ row_count) {
-  extern __shared__ uint64_t lds[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
-  if (r >= n_rows) return;
-  uint64_t* P = lds + (size_t)wave * (n + 1);
-  row_prefix(cells + (size_t)r * n, n, lane, P);
-  const uint64_t cnt =
-      emit_row(P, n, g, frame_of(r, frame_ids, frame_mod), lane, stage + (size_t)r * kCfarSlot * 3, kCfarSlot);
-  if (lane == 0) row_count[r] = (uint32_t)cnt;
+__device__ __forceinline__ void cfar_row(const CfarJob& J, uint32_t r, uint32_t lane, uint64_t* P) {
+  if (r >= J.n_rows) return;
+  const uint32_t n = J.n;
+  row_prefix(static_cast<const Cell*>(J.cells) + (size_t)r * n, n, lane, P);
+  const uint64_t cnt = emit_row(P, n, J.g, frame_of(r, J.frame_ids, J.frame_mod), lane,
+                                J.stage + (size_t)r * J.slot * 3, J.slot);
+  if (lane == 0) J.row_count[r] = (uint32_t)cnt;
 }
 
-// Exclusive scan of row counts into 64-bit offsets; offs[n_rows] = total.
-__global__ __launch_bounds__(1024) void cfar_scan_kernel(const uint32_t* __restrict__ row_count, uint32_t n_rows,
-                                                         uint64_t* __restrict__ offs, uint64_t* total_out) {
+// Pass 3, one wave per row: copy the row's staged events to its offset; a row
+// that overflowed its slot is recomputed and written directly.
+template <typename Cell>
+__device__ __forceinline__ void cfar_gather_row(const CfarJob& J, uint32_t r, uint32_t lane, uint64_t* P) {
+  if (r >= J.n_rows) return;
+  const uint64_t base = J.offs[r];
+  const uint32_t cnt = J.row_count[r];
+  const uint64_t room = base < J.cap ? J.cap - base : 0;
+  if (cnt <= J.slot) {
+    const uint64_t words = 3ull * min((uint64_t)cnt, room);
+    const uint64_t* src = J.stage + (size_t)r * J.slot * 3;
+    for (uint64_t w = lane; w < words; w += 64) J.events[3 * base + w] = src[w];
+    return;
+  }
+  const uint32_t n = J.n;
+  row_prefix(static_cast<const Cell*>(J.cells) + (size_t)r * n, n, lane, P);
+  emit_row(P, n, J.g, frame_of(r, J.frame_ids, J.frame_mod), lane, J.events + 3 * base, room);
+}
+
+// Pass 2, one block: exclusive scan of the row counts into 64-bit offsets;
+// offs[n_rows] = total.
+__device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
   __shared__ uint64_t part[1024];
+  const uint32_t n_rows = J.n_rows;
+  const uint32_t* __restrict__ row_count = J.row_count;
   const uint32_t t = threadIdx.x;
   const uint32_t per = (n_rows + 1023) / 1024;
   const uint32_t r0 = t * per;
@@ -201,46 +214,66 @@ __global__ __launch_bounds__(1024) void cfar_scan_kernel(const uint32_t* __restr
   uint64_t run = part[t] - s;
   for (uint32_t q = 0; q < per; ++q) {
     if (r0 + q < n_rows) {
-      offs[r0 + q] = run;
+      J.offs[r0 + q] = run;
       run += row_count[r0 + q];
     }
   }
   if (t == 1023) {
-    offs[n_rows] = part[1023];
-    if (total_out) *total_out = part[1023];
+    J.offs[n_rows] = part[1023];
+    if (J.total) *J.total = part[1023];
   }
 }
 
-// Pass 3: copy each row's staged events to its offset; a row that overflowed
-// its slot is recomputed and written directly.
+// Single CPI kernels: block b holds rows [b*rpb, +rpb), one per wave.
 template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_gather_kernel(const Cell* __restrict__ cells, uint32_t n,
-                                                          uint32_t n_rows, CfarGeom g,
-                                                          const uint32_t* __restrict__ frame_ids, uint32_t frame_mod,
-                                                          const uint64_t* __restrict__ stage,
-                                                          const uint32_t* __restrict__ row_count,
-                                                          const uint64_t* __restrict__ offs,
-                                                          uint64_t* __restrict__ events, uint64_t cap) {
+__global__ __launch_bounds__(256) void cfar_rows_kernel(CfarJob J) {
   extern __shared__ uint64_t lds[];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + wave;
-  if (r >= n_rows) return;
-  const uint64_t base = offs[r];
-  const uint32_t cnt = row_count[r];
-  const uint64_t room = base < cap ? cap - base : 0;
-  if (cnt <= kCfarSlot) {
-    const uint64_t words = 3ull * min((uint64_t)cnt, room);
-    const uint64_t* src = stage + (size_t)r * kCfarSlot * 3;
-    for (uint64_t w = lane; w < words; w += 64) events[3 * base + w] = src[w];
-    return;
-  }
+  cfar_row<Cell>(J, blockIdx.x * (blockDim.x >> 6) + wave, lane, lds + (size_t)wave * (J.n + 1));
+}
+
+__global__ __launch_bounds__(1024) void cfar_scan_kernel(CfarJob J) { cfar_scan_block(J); }
+
+template <typename Cell>
+__global__ __launch_bounds__(256) void cfar_gather_kernel(CfarJob J) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  cfar_gather_row<Cell>(J, blockIdx.x * (blockDim.x >> 6) + wave, lane, lds + (size_t)wave * (J.n + 1));
+}
+
+// Batched: persistent grids over the row tiles of a class of CPIs (same row
+// length); every wave works on its own row, so no block barrier is needed.
+template <bool GATHER>
+__global__ __launch_bounds__(256) void cfar_batch_kernel(const CfarJob* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
+                                                         uint32_t n) {
+  extern __shared__ uint64_t lds[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, rpb = blockDim.x >> 6;
   uint64_t* P = lds + (size_t)wave * (n + 1);
-  row_prefix(cells + (size_t)r * n, n, lane, P);
-  emit_row(P, n, g, frame_of(r, frame_ids, frame_mod), lane, events + 3 * base, room);
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const CfarJob J = jobs[tile_job[t]];
+    const uint32_t r = (t - J.tile0) * rpb + wave;
+    if (GATHER)
+      cfar_gather_row<uint32_t>(J, r, lane, P);
+    else
+      cfar_row<uint32_t>(J, r, lane, P);
+  }
+}
+
+// Batched scan: one block per CPI.
+__global__ __launch_bounds__(1024) void cfar_scan_batch_kernel(const CfarJob* __restrict__ jobs) {
+  const CfarJob J = jobs[blockIdx.x];
+  cfar_scan_block(J);
 }
 
 size_t cfar_scratch_bytes(uint32_t n_rows) {
   return ((size_t)n_rows + 1) * sizeof(uint64_t) + (size_t)n_rows * kCfarSlot * 3 * sizeof(uint64_t);
+}
+
+// Rows per block (waves) so the prefix rows fit 48 KiB of LDS, at most 4.
+uint32_t cfar_rows_per_tile_max(uint32_t n) {
+  const uint32_t rpb = (uint32_t)(49152 / (((size_t)n + 1) * sizeof(uint64_t)));
+  return rpb < 1 ? 1 : (rpb > 4 ? 4 : rpb);
 }
 
 hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n, uint32_t n_rows,
@@ -248,31 +281,48 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        uint32_t frame_mod, uint64_t* events, uint64_t cap, uint32_t* row_count, void* scratch,
                        uint64_t* total_dev) {
   if (n_rows == 0) return hipSuccess;
-  const CfarGeom g = clamp_geom(cfg, max_guard, max_ref);
-  const size_t row_lds = ((size_t)n + 1) * sizeof(uint64_t);
-  uint32_t rpb = (uint32_t)(49152 / row_lds);  // rows (waves) per block
-  rpb = rpb < 1 ? 1 : (rpb > 4 ? 4 : rpb);
-  const size_t lds = rpb * row_lds;
+  const uint32_t rpb = cfar_rows_per_tile_max(n);
+  const size_t lds = rpb * ((size_t)n + 1) * sizeof(uint64_t);
   const dim3 grid((n_rows + rpb - 1) / rpb), block(64 * rpb);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   uint64_t* stage = offs + n_rows + 1;
+  const CfarJob J{cells, frame_ids, stage, row_count, offs, events, total_dev, cap,
+                  cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, kCfarSlot, 0};
   if (cell_bytes == 4)
-    cfar_rows_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
-                                                        frame_ids, frame_mod, stage, row_count);
+    cfar_rows_kernel<uint32_t><<<grid, block, lds, s>>>(J);
   else
-    cfar_rows_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
-                                                        frame_ids, frame_mod, stage, row_count);
+    cfar_rows_kernel<uint64_t><<<grid, block, lds, s>>>(J);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  cfar_scan_kernel<<<1, 1024, 0, s>>>(row_count, n_rows, offs, total_dev);
+  cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
   e = hipGetLastError();
   if (e != hipSuccess || cap == 0) return e;
   if (cell_bytes == 4)
-    cfar_gather_kernel<uint32_t><<<grid, block, lds, s>>>(static_cast<const uint32_t*>(cells), n, n_rows, g,
-                                                          frame_ids, frame_mod, stage, row_count, offs, events, cap);
+    cfar_gather_kernel<uint32_t><<<grid, block, lds, s>>>(J);
   else
-    cfar_gather_kernel<uint64_t><<<grid, block, lds, s>>>(static_cast<const uint64_t*>(cells), n, n_rows, g,
-                                                          frame_ids, frame_mod, stage, row_count, offs, events, cap);
+    cfar_gather_kernel<uint64_t><<<grid, block, lds, s>>>(J);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (rpb < 1 || rpb > cfar_rows_per_tile_max(n)) return hipErrorInvalidValue;
+  cfar_batch_kernel<false><<<L.grid, 64 * rpb, rpb * ((size_t)n + 1) * sizeof(uint64_t), s>>>(
+      static_cast<const CfarJob*>(L.jobs), L.tile_job, L.n_tiles, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs) {
+  if (n_jobs == 0) return hipSuccess;
+  cfar_scan_batch_kernel<<<n_jobs, 1024, 0, s>>>(static_cast<const CfarJob*>(jobs));
+  return hipGetLastError();
+}
+
+hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (rpb < 1 || rpb > cfar_rows_per_tile_max(n)) return hipErrorInvalidValue;
+  cfar_batch_kernel<true><<<L.grid, 64 * rpb, rpb * ((size_t)n + 1) * sizeof(uint64_t), s>>>(
+      static_cast<const CfarJob*>(L.jobs), L.tile_job, L.n_tiles, n);
   return hipGetLastError();
 }
 

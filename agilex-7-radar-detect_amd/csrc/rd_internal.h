@@ -1,12 +1,99 @@
 // rd_internal.h — launch interface between the C ABI and the kernel files.
+//
+// Every stage kernel is written as a per-tile body. A single-CPI launch maps
+// one block to one tile; a batched launch runs a persistent grid over the
+// tiles of many CPIs of one shape class, reading each tile's CPI from a device
+// job table (tile -> job index, job -> pointers and sizes).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/rd_amd.h"
 
+// The context behind rd_ctx* (rd_capi.hip).
+struct rd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t* tw = nullptr;      // 4096-entry twiddle table (device)
+  void* prep = nullptr;        // beamformer weight digits
+  size_t prep_bytes = 0;
+  void* scratch = nullptr;     // CFAR event offsets and staging
+  size_t scratch_bytes = 0;
+  uint64_t* total_dev = nullptr;
+  uint64_t* total_host = nullptr;  // pinned
+  int live_graphs = 0;             // graphs and batches hold the context's stream
+};
+
 namespace rd {
 
+// Shared C-ABI helpers (rd_capi.hip).
+rd_status ctx_enter(rd_ctx* ctx);                 // validates and makes the device current
+rd_status check_cpi(const rd_cpi_desc* d);        // chain geometry limits
+rd_status check_cpi_bufs(const rd_cpi_bufs* b);   // required chain buffers
+
+// ---------------------------------------------------------------- job records
+// PFB of one CPI: tiles = nbx * n_chan * nz (branch block fastest, then
+// channel, then beat chunk).
+struct PfbJob {
+  const uint32_t* x;
+  const uint32_t* coeff;
+  const uint32_t* hist;
+  uint32_t* y;
+  uint8_t* flags;
+  uint32_t P, T, n_chan, n_beats, chunk, nbx, nz, tile0;
+};
+
+// FFT of one CPI's frames: tiles = ceil(n_frames / frames per block).
+struct FftJob {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t* bins;
+  uint32_t* stage_flags;
+  uint32_t sched, reorder, n_frames, tile0;
+};
+
+// Beamformer of one CPI: tiles = ceil(cols / TC).
+struct BfJob {
+  const uint32_t* X;
+  const uint32_t* W;
+  uint32_t* y;
+  uint8_t* flags;
+  uint32_t* power;
+  int64_t* acc;
+  uint32_t* xa_out;
+  uint64_t cols;
+  uint32_t CH, B, bt_n, tile0;
+};
+
+// Covariance of one CPI: tiles = ceil(N / bins per block).
+struct CovJob {
+  const uint32_t* Xa;
+  int64_t* R;
+  uint32_t CH, F, N, tile0;
+};
+
+// CFAR geometry after clamping to the elaboration maxima.
+struct CfarGeom {
+  uint32_t enable, go, dense, gl, gg, rl, rg, alpha, refs_ok;
+};
+
+// CFAR of one CPI's rows: tiles = ceil(n_rows / rows per block).
+struct CfarJob {
+  const void* cells;
+  const uint32_t* frame_ids;
+  uint64_t* stage;   // [n_rows][slot][3] staged events
+  uint32_t* row_count;
+  uint64_t* offs;    // [n_rows + 1] event offsets
+  uint64_t* events;
+  uint64_t* total;   // event count out (nullable)
+  uint64_t cap;
+  CfarGeom g;
+  uint32_t n, n_rows, frame_mod, slot, tile0;
+};
+
+CfarGeom cfar_geom(const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref);
+
+// ---------------------------------------------------------------- single CPI
 // Streaming stages (rd_stream.hip)
 hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, uint32_t P, uint32_t T,
                       uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags);
@@ -32,5 +119,36 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                        uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
                        void* scratch, uint64_t* total_dev);
+
+// ---------------------------------------------------------------- batched
+// A class launch: `jobs` (device) holds n_jobs records of one shape class,
+// `tile_job` (device) maps each of n_tiles tiles to its job; `grid` blocks walk
+// the tiles with a grid stride.
+struct ClassLaunch {
+  const void* jobs;
+  const uint32_t* tile_job;
+  uint32_t n_tiles, grid;
+};
+
+// PFB, taps T (4, 8, 16 use the register-window kernel; others the generic one).
+hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, const ClassLaunch& L);
+// FFT of log2 length `log2n`, spc lanes; all1: every job's schedule has all
+// bits set for its stages. Frames per tile follow the single-CPI launch.
+uint32_t fft_frames_per_tile(uint32_t log2n);
+hipError_t launch_fft_batch(hipStream_t s, const uint32_t* tw4096, uint32_t log2n, uint32_t spc, bool all1,
+                            const ClassLaunch& L);
+// Beamformer over the FFT layout; steps = ceil(max CH / 16) of the class,
+// tc = columns per tile (128 or 256), max_ch / max_bt size the LDS.
+hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t max_ch, uint32_t max_bt,
+                           const ClassLaunch& L);
+// Covariance; chp = CH rounded up to 16 (shared by the class), fs_n frame splits.
+uint32_t cov_bins_per_tile(uint32_t chp);
+uint32_t cov_max_splits(uint32_t chp);
+hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const ClassLaunch& L);
+// CFAR over power rows (u32 cells) of n bins, rpb rows per tile.
+uint32_t cfar_rows_per_tile_max(uint32_t n);
+hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L);
+hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs);
+hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L);
 
 }  // namespace rd

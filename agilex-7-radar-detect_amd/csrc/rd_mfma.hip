@@ -115,187 +115,239 @@ __device__ void bf_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, u
   }
 }
 
-// Columns per block tile: 256 for up to 32 channels, 128 above.
-template <int STEPS>
-struct BfTile {
-  static constexpr uint32_t TC = STEPS <= 2 ? 256 : 128;
+// LDS of a beamformer block for at most bt beam tiles and ch channels: weight
+// digits, per-beam sums, and a column tile [TC][ch + 1] (odd row stride:
+// conflict-free).
+template <int STEPS, int TC>
+struct BfLds {
+  static size_t words(uint32_t bt, uint32_t ch) { return bf_frag_words(bt, STEPS) + (size_t)bt * 32 + (size_t)TC * (ch + 1); }
 };
 
-// A block stages a tile of TC columns x all channels in LDS as [TC][CH+1]
-// (odd row stride: conflict-free) and loops over tiles with a grid stride, so
-// the weight digits are derived once per block. CHMAJOR: the input is the FFT output layout
-// X[ch][col]; the block then also writes the aligned copy Xa[col][ch] (the
-// alignment stage) from the staged tile. Otherwise the input is Xa.
-template <int STEPS, bool CHMAJOR>
-__global__ __launch_bounds__(256) void bf_kernel(const uint32_t* __restrict__ X, const uint32_t* __restrict__ W,
-                                                 uint32_t CH, uint32_t B, uint32_t bt_n, uint64_t cols,
-                                                 uint32_t* __restrict__ y, uint8_t* __restrict__ flags,
-                                                 uint32_t* __restrict__ power, int64_t* __restrict__ acc,
-                                                 uint32_t* __restrict__ xa_out) {
+// One tile of TC columns of job J; the weights of J are in frag/tail (they
+// may have been written since the caller's last barrier: the barrier below
+// orders them). The caller issues a block barrier before each tile.
+// CHMAJOR: X is the FFT output layout X[ch][col] (each channel row of the tile
+// is TC contiguous words), and the tile is also written out as the aligned
+// copy Xa[col][ch] (the alignment stage) when J.xa_out is set. Otherwise X is
+// the aligned layout and the tile is one contiguous run.
+template <int STEPS, int TC, bool CHMAJOR>
+__device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uint32_t* frag, const int32_t* tail,
+                                        uint32_t* xt) {
   constexpr uint32_t chp = STEPS * 16;
-  constexpr uint32_t TC = BfTile<STEPS>::TC;
   constexpr uint32_t TILES_PER_WAVE = TC / 64;
-  extern __shared__ uint32_t lds[];
-  const size_t fw = bf_frag_words(bt_n, STEPS);
-  uint32_t* frag = lds;
-  int32_t* tail = reinterpret_cast<int32_t*>(lds + fw);
-  uint32_t* xt = lds + fw + (size_t)bt_n * 32;  // [TC][CH + 1]
+  const uint32_t CH = J.CH, B = J.B, bt_n = J.bt_n;
+  const uint64_t cols = J.cols;
+  const uint32_t* __restrict__ X = J.X;
   const uint32_t rs = CH + 1;
-  bf_weights_to_lds(W, CH, B, chp, STEPS, bt_n, frag, tail);
-
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const int64_t kconst = 16384LL * 2 * CH;
-  const uint64_t n_tiles = (cols + TC - 1) / TC;
+  const uint64_t c0 = tile * TC;
+  const uint32_t ncol = (uint32_t)min<uint64_t>(TC, cols - c0);
+  if (CHMAJOR) {
+    for (uint32_t idx = threadIdx.x; idx < CH * TC; idx += 256) {
+      const uint32_t a = idx / TC, j = idx % TC;
+      if (j < ncol) xt[j * rs + a] = X[(size_t)a * cols + c0 + j];
+    }
+  } else {
+    for (uint32_t idx = threadIdx.x; idx < ncol * CH; idx += 256) {
+      const uint32_t j = idx / CH, a = idx % CH;
+      xt[j * rs + a] = X[c0 * CH + idx];
+    }
+  }
+  __syncthreads();
+  if (CHMAJOR && J.xa_out) {
+    for (uint32_t w = threadIdx.x; w < ncol * CH; w += 256) J.xa_out[c0 * CH + w] = xt[(w / CH) * rs + w % CH];
+  }
 
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const uint64_t c0 = tile * TC;
-    const uint32_t ncol = (uint32_t)min<uint64_t>(TC, cols - c0);
-    __syncthreads();  // previous tile fully consumed (and the weights are in place)
-    if (CHMAJOR) {  // X[ch][col]: each channel row of the tile is TC contiguous words
-      for (uint32_t idx = threadIdx.x; idx < CH * TC; idx += 256) {
-        const uint32_t a = idx / TC, j = idx % TC;
-        if (j < ncol) xt[j * rs + a] = X[(size_t)a * cols + c0 + j];
+  for (uint32_t t = 0; t < TILES_PER_WAVE; ++t) {
+    const uint32_t jl = (wave * TILES_PER_WAVE + t) * 16;
+    if (jl >= ncol) break;
+    const uint32_t j = jl + cl;
+    const bool col_ok = j < ncol;
+    const uint64_t col = c0 + j;
+    // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
+    uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
+    int32_t ps_r = 0, ps_i = 0;
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const uint32_t k0 = 32 * s + 8 * grp;
+      const bool imag = k0 >= chp;
+      const uint32_t a0 = imag ? k0 - chp : k0;
+      uint32_t p[8];
+      uint64_t vmask = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool ok = col_ok && a0 + q < CH;
+        p[q] = ok ? xt[j * rs + a0 + q] : 0u;
+        vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
       }
-    } else {  // Xa[col][ch]: the tile is one contiguous run
-      for (uint32_t idx = threadIdx.x; idx < ncol * CH; idx += 256) {
-        const uint32_t j = idx / CH, a = idx % CH;
-        xt[j * rs + a] = X[c0 * CH + idx];
+      const Dig8 d = digits8(p, vmask);
+      int32_t sr = 0, si = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        sr += cre(p[q]);
+        si += cim(p[q]);
+      }
+      if (imag) {
+        ps_i += si;
+        bih[s] = d.hi;
+        bil[s] = d.li;
+        brh[s] = ~d.hi & vmask;
+        brl[s] = ~d.li & vmask;
+      } else {
+        ps_r += sr;
+        bih[s] = d.hr;
+        bil[s] = d.lr;
+        brh[s] = d.hr;
+        brl[s] = d.lr;
       }
     }
-    __syncthreads();
-    if (CHMAJOR && xa_out) {
-      for (uint32_t w = threadIdx.x; w < ncol * CH; w += 256) xa_out[c0 * CH + w] = xt[(w / CH) * rs + w % CH];
-    }
+    // column sums over all channels: the four lane groups of a column
+    ps_r += __shfl_xor(ps_r, 16);
+    ps_r += __shfl_xor(ps_r, 32);
+    ps_i += __shfl_xor(ps_i, 16);
+    ps_i += __shfl_xor(ps_i, 32);
+    const int64_t sb_im = (int64_t)ps_r + ps_i;
+    const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
 
-    for (uint32_t t = 0; t < TILES_PER_WAVE; ++t) {
-      const uint32_t jl = (wave * TILES_PER_WAVE + t) * 16;
-      if (jl >= ncol) break;
-      const uint32_t j = jl + cl;
-      const bool col_ok = j < ncol;
-      const uint64_t col = c0 + j;
-      // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
-      uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
-      int32_t ps_r = 0, ps_i = 0;
+    for (uint32_t bt = 0; bt < bt_n; ++bt) {
+      v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
+      v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
 #pragma unroll
       for (int s = 0; s < STEPS; ++s) {
-        const uint32_t k0 = 32 * s + 8 * grp;
-        const bool imag = k0 >= chp;
-        const uint32_t a0 = imag ? k0 - chp : k0;
-        uint32_t p[8];
-        uint64_t vmask = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const bool ok = col_ok && a0 + q < CH;
-          p[q] = ok ? xt[j * rs + a0 + q] : 0u;
-          vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
-        }
-        const Dig8 d = digits8(p, vmask);
-        int32_t sr = 0, si = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          sr += cre(p[q]);
-          si += cim(p[q]);
-        }
-        if (imag) {
-          ps_i += si;
-          bih[s] = d.hi;
-          bil[s] = d.li;
-          brh[s] = ~d.hi & vmask;
-          brl[s] = ~d.li & vmask;
-        } else {
-          ps_r += sr;
-          bih[s] = d.hr;
-          bil[s] = d.lr;
-          brh[s] = d.hr;
-          brl[s] = d.lr;
-        }
+        const uint64_t* a = reinterpret_cast<const uint64_t*>(frag + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
+        const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
+        rhh = mfma8(are_h, brh[s], rhh);
+        rx = mfma8(are_h, brl[s], rx);
+        rx = mfma8(are_l, brh[s], rx);
+        rll = mfma8(are_l, brl[s], rll);
+        ihh = mfma8(aim_h, bih[s], ihh);
+        ix = mfma8(aim_h, bil[s], ix);
+        ix = mfma8(aim_l, bih[s], ix);
+        ill = mfma8(aim_l, bil[s], ill);
       }
-      // column sums over all channels: the four lane groups of a column
-      ps_r += __shfl_xor(ps_r, 16);
-      ps_r += __shfl_xor(ps_r, 32);
-      ps_i += __shfl_xor(ps_i, 16);
-      ps_i += __shfl_xor(ps_i, 32);
-      const int64_t sb_im = (int64_t)ps_r + ps_i;
-      const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
-
-      for (uint32_t bt = 0; bt < bt_n; ++bt) {
-        v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
-        v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
+      if (!col_ok) continue;
 #pragma unroll
-        for (int s = 0; s < STEPS; ++s) {
-          const uint64_t* a = reinterpret_cast<const uint64_t*>(frag + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
-          const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
-          rhh = mfma8(are_h, brh[s], rhh);
-          rx = mfma8(are_h, brl[s], rx);
-          rx = mfma8(are_l, brh[s], rx);
-          rll = mfma8(are_l, brl[s], rll);
-          ihh = mfma8(aim_h, bih[s], ihh);
-          ix = mfma8(aim_h, bil[s], ix);
-          ix = mfma8(aim_l, bih[s], ix);
-          ill = mfma8(aim_l, bil[s], ill);
-        }
-        if (!col_ok) continue;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const uint32_t b = bt * 16 + 4 * grp + r;
-          if (b >= B) continue;
-          const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
-          const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
-          const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
-          uint32_t fr = 0, fi = 0;
-          const int32_t yr = sat16_i64(rne15_i64(are), fr);
-          const int32_t yi = sat16_i64(rne15_i64(aim), fi);
-          const size_t o = (size_t)b * cols + col;
-          if (y) y[o] = cpack(yr, yi);
-          if (flags) flags[o] = (uint8_t)(fr | (fi << 2));
-          if (power) power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
-          if (acc) {
-            acc[2 * o] = are;
-            acc[2 * o + 1] = aim;
-          }
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t b = bt * 16 + 4 * grp + r;
+        if (b >= B) continue;
+        const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
+        const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
+        const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
+        uint32_t fr = 0, fi = 0;
+        const int32_t yr = sat16_i64(rne15_i64(are), fr);
+        const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+        const size_t o = (size_t)b * cols + col;
+        if (J.y) J.y[o] = cpack(yr, yi);
+        if (J.flags) J.flags[o] = (uint8_t)(fr | (fi << 2));
+        if (J.power) J.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+        if (J.acc) {
+          J.acc[2 * o] = are;
+          J.acc[2 * o + 1] = aim;
         }
       }
     }
   }
 }
 
+// Single CPI: a grid stride over the column tiles, the weight digits derived
+// once per block.
+template <int STEPS, int TC, bool CHMAJOR>
+__global__ __launch_bounds__(256) void bf_kernel(BfJob J) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* frag = lds;
+  int32_t* tail = reinterpret_cast<int32_t*>(lds + bf_frag_words(J.bt_n, STEPS));
+  uint32_t* xt = lds + bf_frag_words(J.bt_n, STEPS) + (size_t)J.bt_n * 32;
+  bf_weights_to_lds(J.W, J.CH, J.B, STEPS * 16, STEPS, J.bt_n, frag, tail);
+  const uint64_t n_tiles = (J.cols + TC - 1) / TC;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    __syncthreads();  // previous tile fully consumed
+    bf_tile<STEPS, TC, CHMAJOR>(J, tile, frag, tail, xt);
+  }
+}
+
+// Batched: a persistent grid over the column tiles of a class of CPIs (same
+// step count); a block re-derives the weight digits when its CPI changes.
+template <int STEPS, int TC>
+__global__ __launch_bounds__(256) void bf_batch_kernel(const BfJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
+                                                       uint32_t max_bt) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* frag = lds;
+  int32_t* tail = reinterpret_cast<int32_t*>(lds + bf_frag_words(max_bt, STEPS));
+  uint32_t* xt = lds + bf_frag_words(max_bt, STEPS) + (size_t)max_bt * 32;
+  uint32_t cur = ~0u;
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint32_t jid = tile_job[t];
+    const BfJob J = jobs[jid];
+    __syncthreads();  // previous tile (and weights) fully consumed
+    if (jid != cur) {
+      bf_weights_to_lds(J.W, J.CH, J.B, STEPS * 16, STEPS, J.bt_n, frag, tail);
+      cur = jid;
+    }
+    bf_tile<STEPS, TC, true>(J, t - J.tile0, frag, tail, xt);
+  }
+}
+
 template <int STEPS>
-static hipError_t launch_bf_s(hipStream_t s, const uint32_t* X, bool chmajor, const uint32_t* W, uint32_t n_ant,
-                              uint32_t n_beams, uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power,
-                              int64_t* acc, uint32_t* xa_out) {
-  const uint32_t bt = (n_beams + 15) / 16;
-  constexpr uint32_t TC = BfTile<STEPS>::TC;
-  const uint64_t tiles = (n_cols + TC - 1) / TC;
+static hipError_t launch_bf_s(hipStream_t s, bool chmajor, const BfJob& J) {
+  constexpr uint32_t TC = STEPS <= 2 ? 256 : 128;  // columns per tile
+  const uint64_t tiles = (J.cols + TC - 1) / TC;
   const dim3 grid((uint32_t)(tiles < 2048 ? tiles : 2048));
-  const size_t lds = (bf_frag_words(bt, STEPS) + (size_t)bt * 32 + (size_t)TC * (n_ant + 1)) * sizeof(uint32_t);
+  const size_t lds = BfLds<STEPS, TC>::words(J.bt_n, J.CH) * sizeof(uint32_t);
   if (chmajor)
-    bf_kernel<STEPS, true><<<grid, 256, lds, s>>>(X, W, n_ant, n_beams, bt, n_cols, y, flags, power, acc, xa_out);
+    bf_kernel<STEPS, TC, true><<<grid, 256, lds, s>>>(J);
   else
-    bf_kernel<STEPS, false><<<grid, 256, lds, s>>>(X, W, n_ant, n_beams, bt, n_cols, y, flags, power, acc, nullptr);
+    bf_kernel<STEPS, TC, false><<<grid, 256, lds, s>>>(J);
   return hipGetLastError();
 }
 
-static hipError_t launch_bf_any(hipStream_t s, const uint32_t* X, bool chmajor, const uint32_t* W, uint32_t n_ant,
-                                uint32_t n_beams, uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power,
-                                int64_t* acc, uint32_t* xa_out) {
-  if (n_cols == 0) return hipSuccess;
-  switch ((n_ant + 15) / 16) {
-    case 1: return launch_bf_s<1>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
-    case 2: return launch_bf_s<2>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
-    case 3: return launch_bf_s<3>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
-    default: return launch_bf_s<4>(s, X, chmajor, W, n_ant, n_beams, n_cols, y, flags, power, acc, xa_out);
+static hipError_t launch_bf_any(hipStream_t s, bool chmajor, const BfJob& J) {
+  if (J.cols == 0) return hipSuccess;
+  switch ((J.CH + 15) / 16) {
+    case 1: return launch_bf_s<1>(s, chmajor, J);
+    case 2: return launch_bf_s<2>(s, chmajor, J);
+    case 3: return launch_bf_s<3>(s, chmajor, J);
+    default: return launch_bf_s<4>(s, chmajor, J);
   }
 }
 
 hipError_t launch_bf(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                      uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, int64_t* acc, void*) {
-  return launch_bf_any(s, Xa, false, W, n_ant, n_beams, n_cols, y, flags, power, acc, nullptr);
+  const BfJob J{Xa, W, y, flags, power, acc, nullptr, n_cols, n_ant, n_beams, (n_beams + 15) / 16, 0};
+  return launch_bf_any(s, false, J);
 }
 
 hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                              uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* xa_out) {
-  return launch_bf_any(s, X, true, W, n_ant, n_beams, n_cols, y, flags, power, nullptr, xa_out);
+  const BfJob J{X, W, y, flags, power, nullptr, xa_out, n_cols, n_ant, n_beams, (n_beams + 15) / 16, 0};
+  return launch_bf_any(s, true, J);
+}
+
+template <int STEPS, int TC>
+static hipError_t launch_bf_batch_s(hipStream_t s, uint32_t max_ch, uint32_t max_bt, const ClassLaunch& L) {
+  const size_t lds = BfLds<STEPS, TC>::words(max_bt, max_ch) * sizeof(uint32_t);
+  bf_batch_kernel<STEPS, TC><<<L.grid, 256, lds, s>>>(static_cast<const BfJob*>(L.jobs), L.tile_job, L.n_tiles,
+                                                      max_bt);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t max_ch, uint32_t max_bt,
+                           const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+#define RD_BF_TC(S)                                                                       \
+  return tc == 256 ? launch_bf_batch_s<S, 256>(s, max_ch, max_bt, L)                      \
+                   : (tc == 128 ? launch_bf_batch_s<S, 128>(s, max_ch, max_bt, L)         \
+                                : launch_bf_batch_s<S, 64>(s, max_ch, max_bt, L));
+  switch (steps) {
+    case 1: RD_BF_TC(1)
+    case 2: RD_BF_TC(2)
+    case 3: RD_BF_TC(3)
+    case 4: RD_BF_TC(4)
+  }
+#undef RD_BF_TC
+  return hipErrorInvalidValue;
 }
 
 // ============================================================================
@@ -315,10 +367,12 @@ hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W
 // split over 64 threads per job and split = CH / (4 * upper tiles) <= 4 words.
 constexpr uint32_t kCovWords = 4;
 
-__global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
-                                                   uint32_t N, uint32_t nb, uint32_t nt, uint32_t fs_n,
-                                                   int64_t* __restrict__ R) {
-  extern __shared__ uint32_t smem[];
+// One tile: bins [tile*nb, +nb) of job J. The caller issues a block barrier
+// before each tile (the LDS of the previous tile is reused).
+__device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_t nb, uint32_t nt, uint32_t fs_n,
+                                         uint32_t* smem) {
+  const uint32_t* __restrict__ Xa = J.Xa;
+  const uint32_t CH = J.CH, F = J.F, N = J.N;
   const uint32_t ut = nt * (nt + 1) / 2;
   const uint32_t jobs = nb * ut;
   const uint32_t row_words = nb * CH;
@@ -327,7 +381,7 @@ __global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ 
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const uint32_t job = wave % jobs, fs = wave / jobs;
-  const uint32_t k_base = blockIdx.x * nb;
+  const uint32_t k_base = tile * nb;
   const uint32_t bins_here = min(nb, N - k_base);
 
   const uint32_t bin = job / ut;
@@ -442,11 +496,10 @@ __global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 8; ++q) atomicAdd(&red[(job * 64 + lane) * 8 + q], (unsigned long long)part[q]);
     __syncthreads();
-    if (fs != 0) return;
 #pragma unroll
     for (int q = 0; q < 8; ++q) part[q] = (int64_t)red[(job * 64 + lane) * 8 + q];
   }
-  if (!live) return;
+  if (fs != 0 || !live) return;
   const uint32_t k = k_base + bin;
   const size_t npairs = (size_t)CH * (CH + 1) / 2;
 #pragma unroll
@@ -454,25 +507,60 @@ __global__ __launch_bounds__(1024) void cov_kernel(const uint32_t* __restrict__ 
     const uint32_t ii = 16 * ti + 4 * grp + r;
     if (ii >= CH || cj >= CH || ii > cj) continue;
     const size_t pair = (size_t)ii * CH - (size_t)ii * (ii - 1) / 2 + (cj - ii);
-    int64_t* o = R + ((size_t)k * npairs + pair) * 2;
+    int64_t* o = J.R + ((size_t)k * npairs + pair) * 2;
     o[0] = part[2 * r];
     o[1] = part[2 * r + 1];
   }
 }
 
+// Single CPI: one block per tile.
+__global__ __launch_bounds__(1024) void cov_kernel(CovJob J, uint32_t nb, uint32_t nt, uint32_t fs_n) {
+  extern __shared__ uint32_t smem[];
+  cov_tile(J, blockIdx.x, nb, nt, fs_n, smem);
+}
+
+// Batched: a persistent grid over the bin tiles of a class of CPIs (same
+// padded channel count).
+__global__ __launch_bounds__(1024) void cov_batch_kernel(const CovJob* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
+                                                         uint32_t nb, uint32_t nt, uint32_t fs_n) {
+  extern __shared__ uint32_t smem[];
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const CovJob J = jobs[tile_job[t]];
+    __syncthreads();
+    cov_tile(J, t - J.tile0, nb, nt, fs_n, smem);
+  }
+}
+
+struct CovGeo {
+  uint32_t nb, nt, jobs;
+  explicit CovGeo(uint32_t chp) : nb(64 / chp), nt(chp / 16), jobs((64 / chp) * (chp / 16) * (chp / 16 + 1) / 2) {}
+  size_t lds(uint32_t fs_n) const { return (size_t)fs_n * 16 * 64 * 4 + (fs_n > 1 ? (size_t)jobs * 64 * 8 * 8 : 0); }
+};
+
+uint32_t cov_bins_per_tile(uint32_t chp) { return CovGeo(chp).nb; }
+uint32_t cov_max_splits(uint32_t chp) { return 16 / CovGeo(chp).jobs; }
+
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R) {
   if (n_bins == 0) return hipSuccess;
-  const uint32_t chp = (n_chan + 15) / 16 * 16, nt = chp / 16;
-  const uint32_t nb = 64 / chp;
-  const uint32_t jobs = nb * nt * (nt + 1) / 2;
-  uint32_t fs_n = 16 / jobs;
+  const uint32_t chp = (n_chan + 15) / 16 * 16;
+  const CovGeo G(chp);
+  uint32_t fs_n = 16 / G.jobs;
   const uint32_t nsteps = (n_frames + 15) / 16;
   if (fs_n > nsteps) fs_n = nsteps;
   if (fs_n < 1) fs_n = 1;
-  const size_t lds = (size_t)fs_n * 16 * 64 * 4 + (fs_n > 1 ? (size_t)jobs * 64 * 8 * 8 : 0);
-  const dim3 grid((n_bins + nb - 1) / nb);
-  cov_kernel<<<grid, 64 * jobs * fs_n, lds, s>>>(Xa, n_chan, n_frames, n_bins, nb, nt, fs_n, R);
+  const CovJob J{Xa, R, n_chan, n_frames, n_bins, 0};
+  cov_kernel<<<(n_bins + G.nb - 1) / G.nb, 64 * G.jobs * fs_n, G.lds(fs_n), s>>>(J, G.nb, G.nt, fs_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  const CovGeo G(chp);
+  if (fs_n < 1 || G.jobs * fs_n > 16) return hipErrorInvalidValue;
+  cov_batch_kernel<<<L.grid, 64 * G.jobs * fs_n, G.lds(fs_n), s>>>(static_cast<const CovJob*>(L.jobs), L.tile_job,
+                                                                   L.n_tiles, G.nb, G.nt, fs_n);
   return hipGetLastError();
 }
 

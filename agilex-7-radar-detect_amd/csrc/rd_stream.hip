@@ -79,27 +79,24 @@ __device__ __forceinline__ uint32_t pfb_sample(const uint32_t* xc, const uint32_
   return hc ? hc[(size_t)(-m - 1) * P + p] : 0u;
 }
 
+// One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
 template <int T>
-__global__ __launch_bounds__(256) void pfb_reg_kernel(const uint32_t* __restrict__ x,
-                                                      const uint32_t* __restrict__ coeff, uint32_t P,
-                                                      uint32_t n_beats, uint32_t chunk,
-                                                      const uint32_t* __restrict__ hist, uint32_t* __restrict__ y,
-                                                      uint8_t* __restrict__ flags) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
+  const uint32_t P = J.P, n_beats = J.n_beats;
+  const uint32_t p = bx * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  const uint32_t c = blockIdx.y;
-  const uint32_t m0 = blockIdx.z * chunk;
-  const uint32_t m1 = min(m0 + chunk, n_beats);
+  const uint32_t m0 = z * J.chunk;
+  const uint32_t m1 = min(m0 + J.chunk, n_beats);
   const size_t span = (size_t)n_beats * P;
-  const uint32_t* xc = x + (size_t)c * span;
-  const uint32_t* hc = hist ? hist + (size_t)c * (T - 1) * P : nullptr;
-  uint32_t* yc = y + (size_t)c * span;
-  uint8_t* fc = flags ? flags + (size_t)c * span : nullptr;
+  const uint32_t* xc = J.x + (size_t)c * span;
+  const uint32_t* hc = J.hist ? J.hist + (size_t)c * (T - 1) * P : nullptr;
+  uint32_t* yc = J.y + (size_t)c * span;
+  uint8_t* fc = J.flags ? J.flags + (size_t)c * span : nullptr;
 
   uint32_t ta[T], tb[T];
 #pragma unroll
   for (int k = 0; k < T; ++k) {
-    ta[k] = coeff[(size_t)p * T + k];          // (hr, hi)
+    ta[k] = J.coeff[(size_t)p * T + k];        // (hr, hi)
     tb[k] = (ta[k] >> 16) | (ta[k] << 16);     // (hi, hr)
   }
   // Window slot (m - m0) mod T holds the digits of x_p[m].
@@ -130,23 +127,19 @@ __global__ __launch_bounds__(256) void pfb_reg_kernel(const uint32_t* __restrict
 }
 
 // Any T in [1, 64]: the window is re-read from memory (L1/L2) each beat.
-__global__ __launch_bounds__(256) void pfb_any_kernel(const uint32_t* __restrict__ x,
-                                                      const uint32_t* __restrict__ coeff, uint32_t P, uint32_t T,
-                                                      uint32_t n_beats, uint32_t chunk,
-                                                      const uint32_t* __restrict__ hist, uint32_t* __restrict__ y,
-                                                      uint8_t* __restrict__ flags) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pfb_any_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
+  const uint32_t P = J.P, T = J.T, n_beats = J.n_beats;
+  const uint32_t p = bx * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  const uint32_t c = blockIdx.y;
-  const uint32_t m0 = blockIdx.z * chunk;
-  const uint32_t m1 = min(m0 + chunk, n_beats);
+  const uint32_t m0 = z * J.chunk;
+  const uint32_t m1 = min(m0 + J.chunk, n_beats);
   const size_t span = (size_t)n_beats * P;
-  const uint32_t* xc = x + (size_t)c * span;
-  const uint32_t* hc = hist ? hist + (size_t)c * (T - 1) * P : nullptr;
+  const uint32_t* xc = J.x + (size_t)c * span;
+  const uint32_t* hc = J.hist ? J.hist + (size_t)c * (T - 1) * P : nullptr;
   for (uint32_t m = m0; m < m1; ++m) {
     int32_t arh = 0, arl = 0, aih = 0, ail = 0;
     for (uint32_t k = 0; k < T; ++k) {
-      const uint32_t h = coeff[(size_t)p * T + k];
+      const uint32_t h = J.coeff[(size_t)p * T + k];
       const uint32_t hs = (h >> 16) | (h << 16);
       const Digits d = split(pfb_sample(xc, hc, P, p, (int64_t)m - (int64_t)k));
       arh = dot2(d.rh, h, arh);
@@ -155,8 +148,34 @@ __global__ __launch_bounds__(256) void pfb_any_kernel(const uint32_t* __restrict
       ail = dot2(d.il, hs, ail);
     }
     uint8_t f;
-    y[(size_t)c * span + (size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
-    if (flags) flags[(size_t)c * span + (size_t)m * P + p] = f;
+    J.y[(size_t)c * span + (size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
+    if (J.flags) J.flags[(size_t)c * span + (size_t)m * P + p] = f;
+  }
+}
+
+template <int T>
+__device__ __forceinline__ void pfb_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
+  if constexpr (T > 0)
+    pfb_reg_tile<T>(J, bx, c, z);
+  else
+    pfb_any_tile(J, bx, c, z);
+}
+
+// Single CPI: grid (nbx, n_chan, nz). T = 0 selects the generic kernel.
+template <int T>
+__global__ __launch_bounds__(256) void pfb_kernel(PfbJob J) {
+  pfb_tile<T>(J, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Batched: a persistent grid over the tiles of a class of CPIs.
+template <int T>
+__global__ __launch_bounds__(256) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ tile_job, uint32_t n_tiles) {
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const PfbJob J = jobs[tile_job[t]];
+    const uint32_t local = t - J.tile0;
+    const uint32_t bx = local % J.nbx, rest = local / J.nbx;
+    pfb_tile<T>(J, bx, rest % J.n_chan, rest / J.n_chan);
   }
 }
 
@@ -189,13 +208,15 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
   uint32_t chunk = (uint32_t)((n_beats + want - 1) / want);
   if (chunk < T) chunk = T;
   if (chunk > n_beats) chunk = n_beats;
-  const dim3 grid(bx, n_chan, (n_beats + chunk - 1) / chunk);
+  const uint32_t nz = (n_beats + chunk - 1) / chunk;
+  const PfbJob J{x, coeff, hist, y, flags, P, T, n_chan, n_beats, chunk, bx, nz, 0};
+  const dim3 grid(bx, n_chan, nz);
   const dim3 block(P < 256 ? ((P + 63) / 64) * 64 : 256);
   switch (T) {
-    case 4: pfb_reg_kernel<4><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
-    case 8: pfb_reg_kernel<8><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
-    case 16: pfb_reg_kernel<16><<<grid, block, 0, s>>>(x, coeff, P, n_beats, chunk, hist, y, flags); break;
-    default: pfb_any_kernel<<<grid, block, 0, s>>>(x, coeff, P, T, n_beats, chunk, hist, y, flags); break;
+    case 4: pfb_kernel<4><<<grid, block, 0, s>>>(J); break;
+    case 8: pfb_kernel<8><<<grid, block, 0, s>>>(J); break;
+    case 16: pfb_kernel<16><<<grid, block, 0, s>>>(J); break;
+    default: pfb_kernel<0><<<grid, block, 0, s>>>(J); break;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -204,6 +225,18 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
     e = hipGetLastError();
   }
   return e;
+}
+
+hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  const PfbJob* jobs = static_cast<const PfbJob*>(L.jobs);
+  switch (T) {
+    case 4: pfb_batch_kernel<4><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    case 8: pfb_batch_kernel<8><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    case 16: pfb_batch_kernel<16><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    default: pfb_batch_kernel<0><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+  }
+  return hipGetLastError();
 }
 
 // ============================================================================
@@ -299,27 +332,44 @@ __device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
 // ALL1: every sub-stage scales by 1 (the chain's schedule); otherwise the
 // schedule bit of each sub-stage is read at run time.
 // ----------------------------------------------------------------------------
+// LDS layout of a block: [NP] twiddles W_N^e at pad(e), [FPB] stage-flag
+// words, then FPB frames of NP words each.
+template <int LOG2N>
+struct FftGeo {
+  static constexpr uint32_t N = 1u << LOG2N;
+  static constexpr uint32_t TPF = N / 4;                       // threads per frame
+  static constexpr uint32_t FPB = TPF >= 256 ? 1 : 256 / TPF;  // frames per block
+  static constexpr uint32_t NP = N + N / 32;
+  static constexpr size_t lds_bytes = (size_t)(NP + FPB + FPB * NP) * sizeof(uint32_t);
+};
+
+template <int LOG2N>
+__device__ __forceinline__ void fft_stage_twiddles(const uint32_t* __restrict__ tw4096, uint32_t* tw) {
+  for (uint32_t i = threadIdx.x; i < FftGeo<LOG2N>::N; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - LOG2N)];
+}
+
+// One tile: frames [grp*FPB, +FPB) of job J, twiddles already in LDS. Every
+// thread of the block calls it (it holds block barriers).
 template <int LOG2N, int SPC, bool ALL1>
-__global__ __launch_bounds__((1u << LOG2N) / 4 > 256 ? (1u << LOG2N) / 4 : 256) void fft_kernel(
-    const uint32_t* __restrict__ in, uint32_t sched, int reorder, uint32_t n_frames,
-    const uint32_t* __restrict__ tw4096, uint32_t* __restrict__ out, uint32_t* __restrict__ bins,
-    uint32_t* __restrict__ stage_flags) {
-  constexpr uint32_t N = 1u << LOG2N;
+__device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t* smem) {
+  using G = FftGeo<LOG2N>;
+  constexpr uint32_t N = G::N;
   constexpr uint32_t M = N / SPC;
   constexpr uint32_t NL = LOG2N - (SPC == 2 ? 1 : 0);
-  constexpr uint32_t TPF = N / 4;
-  constexpr uint32_t FPB = TPF >= 256 ? 1 : 256 / TPF;
-  constexpr uint32_t NP = N + N / 32;
-  extern __shared__ uint32_t smem[];
-  uint32_t* tw = smem;                        // [NP]  W_N^e at pad(e)
-  uint32_t* sfl = smem + NP;                  // [FPB] stage-flag words
+  constexpr uint32_t TPF = G::TPF, FPB = G::FPB, NP = G::NP;
+  const uint32_t* __restrict__ in = J.in;
+  uint32_t* __restrict__ out = J.out;
+  uint32_t* __restrict__ bins = J.bins;
+  const uint32_t sched = ALL1 ? ~0u : J.sched;
+  const uint32_t reorder = J.reorder;
+  uint32_t* tw = smem;
+  uint32_t* sfl = smem + NP;
   const uint32_t lf = threadIdx.x / TPF;      // local frame
   const uint32_t g = threadIdx.x % TPF;
   uint32_t* v = smem + NP + FPB + lf * NP;    // this frame's samples at pad(p)
-  const uint32_t frame = blockIdx.x * FPB + lf;
-  const bool active = frame < n_frames;
+  const uint32_t frame = grp * FPB + lf;
+  const bool active = frame < J.n_frames;
 
-  for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - LOG2N)];
   if (g == 0) sfl[lf] = 0;
   if (active) {
     const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * g);
@@ -429,43 +479,83 @@ __global__ __launch_bounds__((1u << LOG2N) / 4 > 256 ? (1u << LOG2N) / 4 : 256) 
       out[fo + k] = v[pad(src)];
     }
   }
-  if (stage_flags && g == 0) stage_flags[frame] = sfl[lf];
+  if (J.stage_flags && g == 0) J.stage_flags[frame] = sfl[lf];
+}
+
+// Single CPI: one block per tile.
+template <int LOG2N, int SPC, bool ALL1>
+__global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_kernel(
+    FftJob J, const uint32_t* __restrict__ tw4096) {
+  extern __shared__ uint32_t smem[];
+  fft_stage_twiddles<LOG2N>(tw4096, smem);
+  fft_tile<LOG2N, SPC, ALL1>(J, blockIdx.x, smem);
+}
+
+// Batched: a persistent grid over the frame groups of a class of CPIs (same
+// length and lane count); the twiddles are staged once per block.
+template <int LOG2N, int SPC, bool ALL1>
+__global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_batch_kernel(
+    const FftJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
+    const uint32_t* __restrict__ tw4096) {
+  extern __shared__ uint32_t smem[];
+  fft_stage_twiddles<LOG2N>(tw4096, smem);
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const FftJob J = jobs[tile_job[t]];
+    __syncthreads();  // the previous tile's frames have been read out
+    fft_tile<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
+  }
 }
 
 template <int LOG2N, int SPC>
-static hipError_t launch_fft_n(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t sched,
-                               int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
-                               uint32_t* stage_flags) {
-  constexpr uint32_t N = 1u << LOG2N, TPF = N / 4, FPB = TPF >= 256 ? 1 : 256 / TPF, NP = N + N / 32;
-  const size_t lds = (size_t)(NP + FPB + FPB * NP) * sizeof(uint32_t);
-  const dim3 grid((n_frames + FPB - 1) / FPB), block(TPF * FPB);
-  const uint32_t used = (LOG2N >= 32) ? ~0u : ((1u << LOG2N) - 1u);
-  if ((sched & used) == used)
-    fft_kernel<LOG2N, SPC, true><<<grid, block, lds, s>>>(in, sched, reorder, n_frames, tw4096, out, bins,
-                                                           stage_flags);
+static hipError_t launch_fft_n(hipStream_t s, const uint32_t* tw4096, const FftJob& J) {
+  using G = FftGeo<LOG2N>;
+  const dim3 grid((J.n_frames + G::FPB - 1) / G::FPB), block(G::TPF * G::FPB);
+  constexpr uint32_t used = (1u << LOG2N) - 1u;
+  if ((J.sched & used) == used)
+    fft_kernel<LOG2N, SPC, true><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
   else
-    fft_kernel<LOG2N, SPC, false><<<grid, block, lds, s>>>(in, sched, reorder, n_frames, tw4096, out, bins,
-                                                            stage_flags);
+    fft_kernel<LOG2N, SPC, false><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
   return hipGetLastError();
 }
 
-template <int SPC>
-static hipError_t launch_fft_spc(hipStream_t s, const uint32_t* tw, const uint32_t* in, uint32_t log2n,
-                                 uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
-                                 uint32_t* sf) {
-  switch (log2n) {
-    case 3: return launch_fft_n<3, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 4: return launch_fft_n<4, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 5: return launch_fft_n<5, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 6: return launch_fft_n<6, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 7: return launch_fft_n<7, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 8: return launch_fft_n<8, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 9: return launch_fft_n<9, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 10: return launch_fft_n<10, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 11: return launch_fft_n<11, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-    case 12: return launch_fft_n<12, SPC>(s, tw, in, sched, reorder, n_frames, out, bins, sf);
-  }
+template <int LOG2N, int SPC>
+static hipError_t launch_fft_batch_n(hipStream_t s, const uint32_t* tw4096, bool all1, const ClassLaunch& L) {
+  using G = FftGeo<LOG2N>;
+  const FftJob* jobs = static_cast<const FftJob*>(L.jobs);
+  if (all1)
+    fft_batch_kernel<LOG2N, SPC, true><<<L.grid, G::TPF * G::FPB, G::lds_bytes, s>>>(jobs, L.tile_job, L.n_tiles,
+                                                                                     tw4096);
+  else
+    fft_batch_kernel<LOG2N, SPC, false><<<L.grid, G::TPF * G::FPB, G::lds_bytes, s>>>(jobs, L.tile_job, L.n_tiles,
+                                                                                      tw4096);
+  return hipGetLastError();
+}
+
+// Dispatch of a run-time log2 length (3..12) to the compile-time kernels.
+#define RD_FFT_CASES(F, ...)                      \
+  switch (log2n) {                                \
+    case 3: return F<3, SPC>(__VA_ARGS__);        \
+    case 4: return F<4, SPC>(__VA_ARGS__);        \
+    case 5: return F<5, SPC>(__VA_ARGS__);        \
+    case 6: return F<6, SPC>(__VA_ARGS__);        \
+    case 7: return F<7, SPC>(__VA_ARGS__);        \
+    case 8: return F<8, SPC>(__VA_ARGS__);        \
+    case 9: return F<9, SPC>(__VA_ARGS__);        \
+    case 10: return F<10, SPC>(__VA_ARGS__);      \
+    case 11: return F<11, SPC>(__VA_ARGS__);      \
+    case 12: return F<12, SPC>(__VA_ARGS__);      \
+  }                                               \
   return hipErrorInvalidValue;
+
+template <int SPC>
+static hipError_t launch_fft_spc(hipStream_t s, const uint32_t* tw, uint32_t log2n, const FftJob& J) {
+  RD_FFT_CASES(launch_fft_n, s, tw, J)
+}
+
+template <int SPC>
+static hipError_t launch_fft_batch_spc(hipStream_t s, const uint32_t* tw, uint32_t log2n, bool all1,
+                                       const ClassLaunch& L) {
+  RD_FFT_CASES(launch_fft_batch_n, s, tw, all1, L)
 }
 
 hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t n, uint32_t spc,
@@ -473,12 +563,22 @@ hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in,
                       uint32_t* stage_flags) {
   if (n_frames == 0) return hipSuccess;
   const uint32_t log2n = ilog2u(n);
-  if (spc == 2) {
-    if (log2n < 3) return hipErrorInvalidValue;
-    return launch_fft_spc<2>(s, tw4096, in, log2n, sched, reorder, n_frames, out, bins, stage_flags);
-  }
-  if (log2n < 3) return hipErrorInvalidValue;
-  return launch_fft_spc<1>(s, tw4096, in, log2n, sched, reorder, n_frames, out, bins, stage_flags);
+  if (log2n < 3 || log2n > 12) return hipErrorInvalidValue;
+  const FftJob J{in, out, bins, stage_flags, sched, reorder ? 1u : 0u, n_frames, 0};
+  return spc == 2 ? launch_fft_spc<2>(s, tw4096, log2n, J) : launch_fft_spc<1>(s, tw4096, log2n, J);
+}
+
+uint32_t fft_frames_per_tile(uint32_t log2n) {
+  const uint32_t tpf = (1u << log2n) / 4;
+  return tpf >= 256 ? 1 : 256 / tpf;
+}
+
+hipError_t launch_fft_batch(hipStream_t s, const uint32_t* tw4096, uint32_t log2n, uint32_t spc, bool all1,
+                            const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (log2n < 3 || log2n > 12) return hipErrorInvalidValue;
+  return spc == 2 ? launch_fft_batch_spc<2>(s, tw4096, log2n, all1, L)
+                  : launch_fft_batch_spc<1>(s, tw4096, log2n, all1, L);
 }
 
 // ============================================================================
