@@ -138,7 +138,7 @@ uint32_t fft_frames_per_tile(uint32_t log2n);
 hipError_t launch_fft_batch(hipStream_t s, const uint32_t* tw4096, uint32_t log2n, uint32_t spc, bool all1,
                             const ClassLaunch& L);
 // Beamformer over the FFT layout; steps = ceil(max CH / 16) of the class,
-// tc = columns per tile (128 or 256), max_ch / max_bt size the LDS.
+// tc = columns per tile (64, 128 or 256), max_ch / max_bt size the LDS.
 hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t max_ch, uint32_t max_bt,
                            const ClassLaunch& L);
 // Covariance; chp = CH rounded up to 16 (shared by the class), fs_n frame splits.

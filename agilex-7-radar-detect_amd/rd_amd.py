@@ -22,8 +22,10 @@ EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
-    "rd_graph_launch", "rd_graph_destroy",
+    "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
+    "rd_batch_destroy",
 ]
+RD_BATCH_AUTOTUNE = 1
 
 
 class CfarCfg(C.Structure):
@@ -88,6 +90,11 @@ class Lib:
         L.rd_chain_graph.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp, C.POINTER(vp)]
         L.rd_graph_launch.argtypes = [vp]
         L.rd_graph_destroy.argtypes = [vp]
+        L.rd_batch_create.argtypes = [vp, u32, C.POINTER(CpiDesc), C.POINTER(CpiBufs), u32, C.POINTER(vp)]
+        L.rd_batch_launch.argtypes = [vp]
+        L.rd_batch_events.argtypes = [vp, vp]
+        L.rd_batch_plan.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.rd_batch_destroy.argtypes = [vp]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("rd_device_count", "rd_status_str", "rd_stream"):
@@ -354,6 +361,64 @@ class Graph:
     def close(self):
         if self.h and self.ctx.h:
             self.ctx.L.rd_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many CPIs through the chain with one graph launch (rd_batch_*):
+    `descs` / `bufs` are lists of :class:`CpiDesc` / :class:`CpiBufs`."""
+
+    def __init__(self, ctx: Ctx, descs, bufs, autotune=False, keep=None):
+        self.ctx = ctx
+        self.keep = keep  # objects owning the buffers
+        n = len(descs)
+        da = (CpiDesc * n)(*descs)
+        ba = (CpiBufs * n)(*bufs)
+        h = C.c_void_p()
+        rc = ctx.L.rd_batch_create(ctx.h, n, da, ba, RD_BATCH_AUTOTUNE if autotune else 0, C.byref(h))
+        ctx.lib.check("rd_batch_create", rc)
+        self.h = h.value
+        self.n = n
+        ctx.graphs.add(self)
+
+    @classmethod
+    def of_chains(cls, ctx: Ctx, chains, autotune=False, lean=False, iq_ptrs=None) -> "Batch":
+        """A batch over :class:`Chain` objects (each with its own buffers). lean:
+        the optional per-stage outputs are not written (as :meth:`Chain.graph`)."""
+        chains = list(chains)
+        bufs = []
+        for i, c in enumerate(chains):
+            b = CpiBufs.from_buffer_copy(c.bufs)
+            if iq_ptrs is not None:
+                b.iq = iq_ptrs[i]
+            if lean:
+                b.pfb_flags = b.fft_flags = b.y = b.yflags = None
+            bufs.append(b)
+        return cls(ctx, [c.desc for c in chains], bufs, autotune=autotune, keep=chains)
+
+    def launch(self):
+        self.ctx.lib.check("rd_batch_launch", self.ctx.L.rd_batch_launch(self.h))
+
+    def events(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint64)
+        rc = self.ctx.L.rd_batch_events(self.h, out.ctypes.data_as(C.c_void_p))
+        self.ctx.lib.check("rd_batch_events", rc, ok=(RD_OK, RD_E_CAPACITY))
+        return out
+
+    def plan(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        self.ctx.lib.check("rd_batch_plan", self.ctx.L.rd_batch_plan(self.h, buf, len(buf)))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.L.rd_batch_destroy(self.h)
         self.h = None
 
     def __del__(self):

@@ -56,7 +56,7 @@ typedef struct {
   uint32_t n_fft;       /* PFB branches = FFT length, 8..4096, power of 2 */
   uint32_t taps;        /* PFB taps per branch, 1..64                    */
   uint32_t n_frames;    /* PFB beats = FFT frames                        */
-  uint32_t n_beams;     /* beams, 1..64                                  */
+  uint32_t n_beams;     /* beams, 1..128                                 */
   uint32_t spc;         /* FFT samples per cycle: 1 or 2                 */
   uint32_t scale_sched; /* FFT per-sub-stage shift bits (0xFFFFFFFF = all) */
   uint32_t max_guard;   /* CFAR elaboration maxima (clamp)               */
@@ -204,6 +204,35 @@ rd_status rd_chain_graph(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs
                          rd_graph** out);
 rd_status rd_graph_launch(rd_graph* graph);
 rd_status rd_graph_destroy(rd_graph* graph);
+
+/* ---------------------------------------------------------------- batches
+ * No counterpart in the reference: its models run one stream of frames at a
+ * time (pfb::PfbModel::step, fft::transform per frame, cfar::run_frame per
+ * row). A batch runs many CPIs of uneven geometry (channels, FFT length, taps,
+ * frames, beams, CFAR settings) through the chain of rd_chain_enqueue with one
+ * graph launch: the CPIs are grouped into shape classes per stage, and each
+ * class runs as one persistent kernel over the tiles of all its CPIs. Every
+ * CPI's outputs are identical to rd_chain_enqueue on the same desc and bufs.
+ *
+ * rd_batch_create validates every CPI (as rd_chain_enqueue), builds the device
+ * job tables and the graph. With RD_BATCH_AUTOTUNE it first times candidate
+ * tile and grid shapes per class on the given buffers (running the chain's
+ * stages on them, so inputs must be loaded) and keeps the fastest; otherwise
+ * it uses fixed heuristics. The buffers of different CPIs must not overlap. */
+typedef struct rd_batch rd_batch;
+
+#define RD_BATCH_AUTOTUNE 1u
+
+rd_status rd_batch_create(rd_ctx* ctx, uint32_t n_cpi, const rd_cpi_desc* descs, const rd_cpi_bufs* bufs,
+                          uint32_t flags, rd_batch** out);
+/* Enqueues the whole batch (one graph launch) on the context's stream. */
+rd_status rd_batch_launch(rd_batch* batch);
+/* Waits for the context's stream and copies the per-CPI event totals
+ * (n_cpi values); RD_E_CAPACITY if any CPI's total exceeds its events_cap. */
+rd_status rd_batch_events(rd_batch* batch, uint64_t* n_events_host);
+/* The shape classes and the tile/grid shape chosen for each, as text. */
+rd_status rd_batch_plan(const rd_batch* batch, char* text, size_t cap);
+rd_status rd_batch_destroy(rd_batch* batch);
 
 #ifdef __cplusplus
 }
