@@ -10,8 +10,8 @@ import rd_amd
 ROOT = Path(__file__).resolve().parent.parent
 
 
-def declared():
-    text = (ROOT / "include" / "rd_amd.h").read_text()
+def declared(header="rd_amd.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(rd_[a-z0-9_]+)\s*\(", text)))
 
@@ -23,6 +23,17 @@ def test_every_declared_symbol_is_exported():
     missing = [n for n in names if not hasattr(lib.lib, n)]
     assert not missing, missing
     assert set(names) == set(rd_amd.EXPORTS)
+
+
+def test_every_shadow_symbol_is_exported():
+    import rd_shadow
+
+    lib = rd_amd.load()
+    names = declared("rd_shadow.h")
+    missing = [n for n in names if not hasattr(lib.lib, n)]
+    assert not missing, missing
+    assert set(names) == set(rd_shadow.SH_EXPORTS)
+    rd_shadow._lib()  # argtypes bind without a GPU
 
 
 def test_twiddle_table_matches_oracle(orc):
