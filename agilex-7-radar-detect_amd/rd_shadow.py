@@ -193,9 +193,10 @@ def soft_cfar_logits(p: torch.Tensor, n_fft: int, guard: int, ref: int, alpha: f
     return (P[..., lo:hi] / mean - alpha) / tau
 
 
-def loss_fn(z: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
-    """Binary cross-entropy of sigmoid(z) against the true target cells."""
-    return torch.nn.functional.binary_cross_entropy_with_logits(z, target)
+def loss_fn(z: torch.Tensor, target: torch.Tensor, weight: torch.Tensor | None = None) -> torch.Tensor:
+    """Binary cross-entropy of sigmoid(z) against the true target cells;
+    `weight` (optional, same shape) zeroes cells that count neither way."""
+    return torch.nn.functional.binary_cross_entropy_with_logits(z, target, weight=weight)
 
 
 def reference(x, W, gain, h):
