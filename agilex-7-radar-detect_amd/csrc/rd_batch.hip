@@ -8,9 +8,12 @@
 //   BF    MFMA K steps = ceil(channels / 16)
 //   COV   channels rounded up to 16
 //   CFAR  row length (bins)
-// A class is one persistent kernel launch over the tiles of all its CPIs: a
-// device job table holds each CPI's pointers and sizes, and a tile table maps
-// every tile to its job. Stages run in chain order; the covariance classes run
+// A class is one kernel launch over the tiles of all its CPIs: a device job
+// table holds each CPI's pointers and sizes, and a tile table maps every tile
+// (block) to its job. The beamformer's blocks walk their tiles with a grid
+// stride (persistent), so each block derives a CPI's weight digits once; the
+// other stages run one block per tile, which keeps their register budget at
+// that of the single-CPI kernels. Stages run in chain order; the covariance classes run
 // on a forked stream beside CFAR (they only share read-only inputs). The whole
 // sequence is captured once into a HIP graph.
 //
@@ -50,6 +53,7 @@ struct Shape {
   uint32_t a = 0;     // PFB: target lanes per CPI for beat chunking (0: one chunk);
                       // BF: columns per tile; COV: frame splits; CFAR: rows per tile
   uint32_t grid = 0;  // persistent grid size (0: one block per tile)
+  uint32_t b = 0;     // CFAR: waves per row
 };
 
 struct Class {
@@ -194,8 +198,8 @@ hipError_t launch_class(const rd_batch& B, const Class& c, hipStream_t s) {
     case S_FFT: return rd::launch_fft_batch(s, B.ctx->tw, c.key & 15u, (c.key >> 4) & 15u, (c.key >> 8) & 1u, c.L);
     case S_BF: return rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.L);
     case S_COV: return rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
-    case S_CFAR_ROWS: return rd::launch_cfar_rows_batch(s, c.key, c.shape.a, c.L);
-    case S_CFAR_GATHER: return rd::launch_cfar_gather_batch(s, c.key, c.shape.a, c.L);
+    case S_CFAR_ROWS: return rd::launch_cfar_rows_batch(s, c.key, c.shape.b, c.shape.a, c.L);
+    case S_CFAR_GATHER: return rd::launch_cfar_gather_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     default: return hipErrorInvalidValue;
   }
 }
@@ -247,7 +251,10 @@ Shape default_shape(const rd_batch& B, const Class& c) {
       break;
     }
     case S_CFAR_ROWS:
-    case S_CFAR_GATHER: s.a = rd::cfar_rows_per_tile_max(c.key); break;
+    case S_CFAR_GATHER:
+      s.b = rd::cfar_min_waves(c.key);
+      s.a = rd::cfar_rows_per_tile_max(c.key, s.b);
+      break;
     default: break;
   }
   return s;
@@ -258,11 +265,10 @@ std::vector<Shape> candidates(const rd_batch& B, const Class& c) {
   const uint32_t cu = B.n_cu;
   switch (c.stage) {
     case S_PFB:
-      for (uint32_t a : {0u, 65536u, 262144u})
-        for (uint32_t g : {0u, 8 * cu, 16 * cu, 32 * cu}) v.push_back(Shape{a, g});
+      for (uint32_t a : {0u, 32768u, 65536u, 131072u, 262144u}) v.push_back(Shape{a, 0});
       break;
     case S_FFT:
-      for (uint32_t g : {0u, 4 * cu, 8 * cu, 16 * cu}) v.push_back(Shape{0, g});
+      v.push_back(Shape{0, 0});
       break;
     case S_BF:
       for (uint32_t tc : {64u, 128u, 256u})
@@ -270,13 +276,12 @@ std::vector<Shape> candidates(const rd_batch& B, const Class& c) {
           for (uint32_t g : {0u, 4 * cu, 8 * cu}) v.push_back(Shape{tc, g});
       break;
     case S_COV:
-      for (uint32_t f = 1; f <= rd::cov_max_splits(c.key); f *= 2)
-        for (uint32_t g : {0u, 4 * cu, 8 * cu}) v.push_back(Shape{f, g});
+      for (uint32_t f = 1; f <= rd::cov_max_splits(c.key); f *= 2) v.push_back(Shape{f, 0});
       break;
     case S_CFAR_ROWS:
     case S_CFAR_GATHER:
-      for (uint32_t r = 1; r <= rd::cfar_rows_per_tile_max(c.key); r *= 2)
-        for (uint32_t g : {0u, 8 * cu, 16 * cu}) v.push_back(Shape{r, g});
+      for (uint32_t w = rd::cfar_min_waves(c.key), k = 0; w <= 8 && k < 3; w *= 2, ++k)
+        for (uint32_t r = 1; r <= rd::cfar_rows_per_tile_max(c.key, w); r *= 2) v.push_back(Shape{r, 0, w});
       break;
     default: break;
   }
@@ -353,6 +358,10 @@ std::string describe(const rd_batch& B) {
     snprintf(line, sizeof line, "%-11s %-22s cpis=%-4zu tiles=%-8u %s=%u grid=%u", kStageName[c.stage], key,
              c.cpis.size(), c.n_tiles, an, c.shape.a, c.L.grid);
     s += line;
+    if (c.stage == S_CFAR_ROWS || c.stage == S_CFAR_GATHER) {
+      snprintf(line, sizeof line, " waves_per_row=%u", c.shape.b);
+      s += line;
+    }
     if (c.ms >= 0) {
       snprintf(line, sizeof line, " tuned %.4f ms (%u candidates)", c.ms, c.tried);
       s += line;

@@ -44,12 +44,50 @@ CfarGeom cfar_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
   return g;
 }
 
-// One wave per row; a block holds up to 4 rows (waves), each with its own
-// exclusive prefix P[0..n] in LDS. Lane l owns the interleaved bins l + 64*j, so
-// global loads, LDS accesses and event stores are consecutive across lanes.
-// The row is first copied into LDS with every load in flight, then scanned in
-// place one 64-bin block at a time (wave scan plus carry). Nothing below needs
-// a block-wide barrier: each wave only touches its own row.
+// ----------------------------------------------------------------------------
+// Row geometry. A row of n bins is handled by W waves; a block holds R rows
+// (R*W <= 4 waves, or one row of 8 waves). Wave w of a row owns the bins
+// [w*64*S, (w+1)*64*S), S = 2^SLOG <= 16 a compile-time constant:
+//   prefix pass:   lane l owns the S contiguous bins starting at (w*64 + l)*S:
+//                  its cells go straight into registers, a sequential sum per
+//                  lane and ONE wave scan per row (plus a carry across the
+//                  row's waves) give the exclusive prefix P[e], e = 0..n;
+//   decision pass: the wave's S chunks of 64 bins, lane l bin l of each chunk,
+//                  so LDS reads and event stores are consecutive across lanes.
+// P sits in LDS at pidx(e) = e + (e >> SLOG) (SLOG >= 1): one padding word per
+// S entries keeps the strided segment writes of the prefix pass free of bank
+// conflicts, and puts a lane's S entries at fixed offsets from one base.
+// LDS per row (u64 words): P, then the per-wave statistics [W][4] = {prefix
+// total, events, detections, -}.
+// ----------------------------------------------------------------------------
+struct CfarRowGeo {
+  uint32_t W, slog, words;
+};
+
+// Fewest waves per row: a lane's prefix segment holds at most 16 bins.
+uint32_t cfar_min_waves(uint32_t n) {
+  uint32_t W = 1;
+  while (W < 8 && W * 1024u < n) W <<= 1;
+  return W;
+}
+
+// W: waves per row, a power of two from cfar_min_waves(n) to 8.
+__host__ __device__ inline CfarRowGeo cfar_row_geo(uint32_t n, uint32_t W) {
+  CfarRowGeo G;
+  G.W = W;
+  const uint32_t per = (n + 64 * W - 1) / (64 * W);
+  G.slog = 0;
+  while ((1u << G.slog) < per) ++G.slog;
+  const uint32_t pmax = n + (G.slog ? (n >> G.slog) : 0);
+  G.words = pmax + 1 + W * 4;
+  return G;
+}
+
+template <uint32_t SLOG>
+__device__ __forceinline__ uint32_t pidx(uint32_t e) {
+  return SLOG ? e + (e >> SLOG) : e;
+}
+
 __device__ __forceinline__ uint64_t wave_incscan(uint64_t v, uint32_t lane) {
   uint64_t x = v;
 #pragma unroll
@@ -67,20 +105,15 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename Cell>
-__device__ __forceinline__ void row_prefix(const Cell* row, uint32_t n, uint32_t lane, uint64_t* P) {
-  for (uint32_t i = lane; i < n; i += 64) P[i + 1] = (uint64_t)row[i];
-  wave_lds_sync();
-  uint64_t carry = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
-    const uint32_t i = b0 + lane;
-    const uint64_t c = i < n ? P[i + 1] : 0ull;
-    const uint64_t x = wave_incscan(c, lane);
-    if (i < n) P[i + 1] = carry + x;
-    carry += __shfl(x, 63);
-  }
-  if (lane == 0) P[0] = 0;
-  wave_lds_sync();
+// Synchronises the waves of a row: a block barrier when a row spans several
+// waves (MULTI; every thread of the block executes the same sequence), else a
+// wave barrier.
+template <bool MULTI>
+__device__ __forceinline__ void row_sync() {
+  if (MULTI)
+    __syncthreads();
+  else
+    wave_lds_sync();
 }
 
 struct Decision {
@@ -89,17 +122,22 @@ struct Decision {
   uint32_t n;
 };
 
+// Branch-free so the six prefix reads issue back to back: window indices are
+// clamped into [0, n] and the sums of windows that do not fit are masked.
+template <uint32_t SLOG>
 __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32_t i, const CfarGeom& g) {
-  Decision d;
-  d.cut = P[i + 1] - P[i];
+  const uint32_t a1 = min(i + g.gl + g.rl + 1, n), a0 = min(i + g.gl + 1, n);
+  const uint32_t b1 = i >= g.gg ? i - g.gg : 0u, b0 = i >= g.gg + g.rg ? i - g.gg - g.rg : 0u;
+  const uint64_t pc1 = P[pidx<SLOG>(i + 1)], pc0 = P[pidx<SLOG>(i)];
+  const uint64_t pa1 = P[pidx<SLOG>(a1)], pa0 = P[pidx<SLOG>(a0)];
+  const uint64_t pb1 = P[pidx<SLOG>(b1)], pb0 = P[pidx<SLOG>(b0)];
   const bool lead_ok = g.rl == 0 || (uint64_t)i + g.gl + g.rl < n;
   const bool lag_ok = g.rg == 0 || i >= g.gg + g.rg;
   const bool ok = lead_ok && lag_ok;
-  uint64_t slead = 0, slag = 0;
-  if (ok) {
-    if (g.rl) slead = P[i + g.gl + g.rl + 1] - P[i + g.gl + 1];
-    if (g.rg) slag = P[i - g.gg] - P[i - g.gg - g.rg];
-  }
+  const uint64_t slead = (ok && g.rl) ? pa1 - pa0 : 0ull;
+  const uint64_t slag = (ok && g.rg) ? pb1 - pb0 : 0ull;
+  Decision d;
+  d.cut = pc1 - pc0;
   if (g.go) {
     const bool lead = slead * g.rg >= slag * g.rl;
     d.sum = lead ? slead : slag;
@@ -111,6 +149,15 @@ __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32
   d.eval = ok && g.enable && g.refs_ok;
   d.hit = d.eval && ((d.cut * (uint64_t)d.n) << 8) > (uint64_t)g.alpha * d.sum;
   return d;
+}
+
+// Bins of a row of n that are not evaluated (window does not fit, or the
+// detector is off): a closed form of the reference's per-bin count.
+__device__ __forceinline__ uint32_t suppressed(uint32_t n, const CfarGeom& g) {
+  if (!(g.enable && g.refs_ok)) return n;
+  const uint32_t lo = g.rg ? g.gg + g.rg : 0u;
+  const uint32_t hi = g.rl ? (n > g.gl + g.rl ? n - g.gl - g.rl : 0u) : n;
+  return n - (hi > lo ? hi - lo : 0u);
 }
 
 // 176-bit event, LSB first: kind 2 | bin 16 | frame 16 | ref 7 | alpha 16 |
@@ -127,68 +174,162 @@ __device__ __forceinline__ void put_event(uint64_t* w, uint32_t kind, uint32_t b
   w[2] = (cut40 >> 39) | (sum47 << 1) | ((uint64_t)sof << 48) | ((uint64_t)eof << 49);
 }
 
-// The events of one row, in bin order, summary last, written to dst[0 .. room);
-// returns the row's event count (summary included).
-__device__ uint64_t emit_row(const uint64_t* P, uint32_t n, const CfarGeom& g, uint32_t frame, uint32_t lane,
-                             uint64_t* dst, uint64_t room) {
-  const uint64_t below = (1ull << lane) - 1ull;  // lanes before this one
-  uint64_t k = 0;                                 // events emitted so far in the row
-  uint32_t det = 0, sup = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 64) {
-    const uint32_t i = b0 + lane;
+__device__ __forceinline__ uint32_t frame_of(uint32_t r, const uint32_t* frame_ids, uint32_t frame_mod) {
+  return frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
+}
+
+// One row: prefix, decisions, and its events in bin order with the summary
+// last, written to dst[0 .. room). Every thread of the row's W waves calls it;
+// `work` false makes a thread skip all memory work but keep the barriers (so
+// rows of one block may differ in whether they run). wr = wave within the row.
+// Returns the row's event count (summary included).
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, uint32_t r, bool work, uint32_t wr,
+                                                uint32_t lane, uint64_t* lds, uint64_t* dst, uint64_t room) {
+  constexpr uint32_t S = 1u << SLOG;
+  const uint32_t n = J.n;
+  const CfarGeom g = J.g;
+  uint64_t* P = lds;
+  uint64_t* stats = lds + pidx<SLOG>(n) + 1;  // [W][4]
+  const uint32_t t = wr * 64 + lane;
+  const uint32_t b0 = t * S;
+  const bool full = (wr + 1) * 64 * S <= n;  // the wave's bins all inside the row
+
+  // the lane's segment straight into registers, local inclusive prefix
+  uint64_t loc[S];
+  {
+    Cell v[S];
+    const Cell* row = static_cast<const Cell*>(J.cells) + (size_t)r * n;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) v[j] = (work && (full || b0 + j < n)) ? row[b0 + j] : Cell(0);
+    uint64_t run = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) {
+      run += (uint64_t)v[j];
+      loc[j] = run;
+    }
+  }
+  const uint64_t incl = wave_incscan(loc[S - 1], lane);
+  uint64_t base = incl - loc[S - 1];
+  if (MULTI) {
+    const uint64_t wave_total = __shfl(incl, 63);
+    if (lane == 0) stats[wr * 4 + 0] = wave_total;
+    __syncthreads();
+    for (uint32_t w = 0; w < wr; ++w) base += stats[w * 4 + 0];
+  }
+  if (work) {
+    // P[b0 + j + 1] sits at a fixed offset from the lane's base
+    uint64_t* Pl = P + pidx<SLOG>(b0 + 1) - 1;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) {
+      const uint32_t off = j + 1 + ((SLOG && j + 1 == S) ? 1u : 0u);
+      if (full || b0 + j + 1 <= n) Pl[off] = base + loc[j];
+    }
+    if (t == 0) P[0] = 0;
+  }
+  row_sync<MULTI>();
+
+  // decisions, first pass: which bins emit (chunk masks) and the counts
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t c0 = wr * 64 * S;
+  uint64_t em[S];
+  uint32_t cnt = 0, det = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < S; ++q) {
+    const uint32_t i = c0 + q * 64 + lane;
+    const bool in = work && (full || i < n);
     Decision d{};
-    const bool in = i < n;
-    if (in) d = decide(P, n, i, g);
-    const bool emit = in && (d.hit || g.dense);
-    const uint64_t em = __ballot(emit);
-    det += (uint32_t)__popcll(__ballot(in && d.hit));
-    sup += (uint32_t)__popcll(__ballot(in && !d.eval));
-    if (emit) {
-      const uint64_t pos = k + (uint64_t)__popcll(em & below);
+    if (in) d = decide<SLOG>(P, n, i, g);
+    em[q] = __ballot(in && (d.hit || g.dense));
+    cnt += (uint32_t)__popcll(em[q]);
+    if (g.dense) det += (uint32_t)__popcll(__ballot(in && d.hit));
+  }
+  if (!g.dense) det = cnt;
+  uint64_t k = 0, K = cnt;
+  uint32_t DET = det;
+  if (MULTI) {
+    if (lane == 0) {
+      stats[wr * 4 + 1] = cnt;
+      stats[wr * 4 + 2] = det;
+    }
+    __syncthreads();
+    K = 0;
+    DET = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      const uint64_t c = stats[w * 4 + 1];
+      if (w < wr) k += c;
+      K += c;
+      DET += (uint32_t)stats[w * 4 + 2];
+    }
+  }
+
+  // second pass: the emitting bins write their events at their offsets
+  const uint32_t frame = frame_of(r, J.frame_ids, J.frame_mod);
+#pragma unroll
+  for (uint32_t q = 0; q < S; ++q) {
+    if (!em[q]) continue;
+    if ((em[q] >> lane) & 1ull) {
+      const uint32_t i = c0 + q * 64 + lane;
+      const Decision d = decide<SLOG>(P, n, i, g);
+      const uint64_t pos = k + (uint64_t)__popcll(em[q] & below);
       const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
       if (pos < room)
         put_event(dst + 3 * pos, kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut, d.eval ? d.sum : 0ull,
                   pos == 0, false);
     }
-    k += (uint64_t)__popcll(em);
+    k += (uint64_t)__popcll(em[q]);
   }
-  if (lane == 0 && k < room) put_event(dst + 3 * k, 3u, n, frame, 0, g.alpha, det, sup, 0, 0, k == 0, true);
-  return k + 1;
+  if (work && wr == 0 && lane == 0 && K < room)
+    put_event(dst + 3 * K, 3u, n, frame, 0, g.alpha, DET, suppressed(n, g), 0, 0, K == 0, true);
+  // the caller may reuse the row's LDS after this
+  row_sync<MULTI>();
+  return K + 1;
 }
 
-__device__ __forceinline__ uint32_t frame_of(uint32_t r, const uint32_t* frame_ids, uint32_t frame_mod) {
-  return frame_ids ? frame_ids[r] : (frame_mod ? r % frame_mod : r);
+// Pass 1 for the rows of one block (R rows x W waves): events to the staging
+// slots, row_count[r] = the row's event count.
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__device__ __forceinline__ void cfar_rows_tile(const CfarJob& J, const CfarRowGeo& G, uint32_t r0, uint64_t* lds) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t rl = wave / G.W, wr = wave % G.W;
+  const uint32_t r = r0 + rl;
+  const bool valid = r < J.n_rows;
+  const uint64_t cnt = cfar_row_run<Cell, MULTI, SLOG>(J, G.W, r, valid, wr, lane, lds + (size_t)rl * G.words,
+                                                       J.stage + (size_t)(valid ? r : 0) * J.slot * 3,
+                                                       valid ? J.slot : 0);
+  if (valid && wr == 0 && lane == 0) J.row_count[r] = (uint32_t)cnt;
 }
 
-// Pass 1, one wave per row: the row's events go to its staging slot (as many
-// as fit); row_count[r] receives the row's event count.
-template <typename Cell>
-__device__ __forceinline__ void cfar_row(const CfarJob& J, uint32_t r, uint32_t lane, uint64_t* P) {
-  if (r >= J.n_rows) return;
-  const uint32_t n = J.n;
-  row_prefix(static_cast<const Cell*>(J.cells) + (size_t)r * n, n, lane, P);
-  const uint64_t cnt = emit_row(P, n, J.g, frame_of(r, J.frame_ids, J.frame_mod), lane,
-                                J.stage + (size_t)r * J.slot * 3, J.slot);
-  if (lane == 0) J.row_count[r] = (uint32_t)cnt;
-}
-
-// Pass 3, one wave per row: copy the row's staged events to its offset; a row
-// that overflowed its slot is recomputed and written directly.
-template <typename Cell>
-__device__ __forceinline__ void cfar_gather_row(const CfarJob& J, uint32_t r, uint32_t lane, uint64_t* P) {
-  if (r >= J.n_rows) return;
-  const uint64_t base = J.offs[r];
-  const uint32_t cnt = J.row_count[r];
-  const uint64_t room = base < J.cap ? J.cap - base : 0;
-  if (cnt <= J.slot) {
+// Pass 3: each row's staged events to its offset; rows that overflowed their
+// slot are recomputed and written directly.
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__device__ __forceinline__ void cfar_gather_tile(const CfarJob& J, const CfarRowGeo& G, uint32_t r0,
+                                                 uint64_t* lds) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t rl = wave / G.W, wr = wave % G.W;
+  const uint32_t r = r0 + rl;
+  const bool valid = r < J.n_rows;
+  uint64_t base = 0, room = 0;
+  uint32_t cnt = 0;
+  if (valid) {
+    base = J.offs[r];
+    cnt = J.row_count[r];
+    room = base < J.cap ? J.cap - base : 0;
+  }
+  if (valid && cnt <= J.slot) {  // copy, the row's waves splitting the words
     const uint64_t words = 3ull * min((uint64_t)cnt, room);
     const uint64_t* src = J.stage + (size_t)r * J.slot * 3;
-    for (uint64_t w = lane; w < words; w += 64) J.events[3 * base + w] = src[w];
-    return;
+    for (uint64_t w = (uint64_t)wr * 64 + lane; w < words; w += 64ull * G.W) J.events[3 * base + w] = src[w];
   }
-  const uint32_t n = J.n;
-  row_prefix(static_cast<const Cell*>(J.cells) + (size_t)r * n, n, lane, P);
-  emit_row(P, n, J.g, frame_of(r, J.frame_ids, J.frame_mod), lane, J.events + 3 * base, room);
+  const bool redo = valid && cnt > J.slot;
+  if (!MULTI) {
+    if (redo)
+      cfar_row_run<Cell, false, SLOG>(J, 1, r, true, 0, lane, lds + (size_t)rl * G.words, J.events + 3 * base,
+                                      room);
+  } else if (__syncthreads_or(redo)) {
+    cfar_row_run<Cell, true, SLOG>(J, G.W, r, redo, wr, lane, lds + (size_t)rl * G.words, J.events + 3 * base,
+                                   redo ? room : 0);
+  }
 }
 
 // Pass 2, one block: exclusive scan of the row counts into 64-bit offsets;
@@ -224,40 +365,39 @@ __device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
   }
 }
 
-// Single CPI kernels: block b holds rows [b*rpb, +rpb), one per wave.
-template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_rows_kernel(CfarJob J) {
+// Single CPI kernels: block b holds rows [b*R, +R).
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__global__ __launch_bounds__(512) void cfar_rows_kernel(CfarJob J, CfarRowGeo G) {
   extern __shared__ uint64_t lds[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  cfar_row<Cell>(J, blockIdx.x * (blockDim.x >> 6) + wave, lane, lds + (size_t)wave * (J.n + 1));
+  const uint32_t R = blockDim.x / (64 * G.W);
+  cfar_rows_tile<Cell, MULTI, SLOG>(J, G, blockIdx.x * R, lds);
 }
 
 __global__ __launch_bounds__(1024) void cfar_scan_kernel(CfarJob J) { cfar_scan_block(J); }
 
-template <typename Cell>
-__global__ __launch_bounds__(256) void cfar_gather_kernel(CfarJob J) {
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__global__ __launch_bounds__(512) void cfar_gather_kernel(CfarJob J, CfarRowGeo G) {
   extern __shared__ uint64_t lds[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  cfar_gather_row<Cell>(J, blockIdx.x * (blockDim.x >> 6) + wave, lane, lds + (size_t)wave * (J.n + 1));
+  const uint32_t R = blockDim.x / (64 * G.W);
+  cfar_gather_tile<Cell, MULTI, SLOG>(J, G, blockIdx.x * R, lds);
 }
 
-// Batched: persistent grids over the row tiles of a class of CPIs (same row
-// length); every wave works on its own row, so no block barrier is needed.
-template <bool GATHER>
-__global__ __launch_bounds__(256) void cfar_batch_kernel(const CfarJob* __restrict__ jobs,
+// Batched: one block per row tile of a class of CPIs (same row length, so the
+// same row geometry), block -> job via the tile table.
+template <bool GATHER, bool MULTI, uint32_t SLOG>
+__global__ __launch_bounds__(512) void cfar_batch_kernel(const CfarJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
-                                                         uint32_t n) {
+                                                         CfarRowGeo G) {
   extern __shared__ uint64_t lds[];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, rpb = blockDim.x >> 6;
-  uint64_t* P = lds + (size_t)wave * (n + 1);
-  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const CfarJob J = jobs[tile_job[t]];
-    const uint32_t r = (t - J.tile0) * rpb + wave;
-    if (GATHER)
-      cfar_gather_row<uint32_t>(J, r, lane, P);
-    else
-      cfar_row<uint32_t>(J, r, lane, P);
-  }
+  const uint32_t R = blockDim.x / (64 * G.W);
+  const uint32_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const CfarJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
+  const uint32_t r0 = (t - J.tile0) * R;
+  if (GATHER)
+    cfar_gather_tile<uint32_t, MULTI, SLOG>(J, G, r0, lds);
+  else
+    cfar_rows_tile<uint32_t, MULTI, SLOG>(J, G, r0, lds);
 }
 
 // Batched scan: one block per CPI.
@@ -270,10 +410,56 @@ size_t cfar_scratch_bytes(uint32_t n_rows) {
   return ((size_t)n_rows + 1) * sizeof(uint64_t) + (size_t)n_rows * kCfarSlot * 3 * sizeof(uint64_t);
 }
 
-// Rows per block (waves) so the prefix rows fit 48 KiB of LDS, at most 4.
-uint32_t cfar_rows_per_tile_max(uint32_t n) {
-  const uint32_t rpb = (uint32_t)(49152 / (((size_t)n + 1) * sizeof(uint64_t)));
-  return rpb < 1 ? 1 : (rpb > 4 ? 4 : rpb);
+// Rows per block for W waves per row: 4 waves in all (one row of 8 waves).
+uint32_t cfar_rows_per_tile_max(uint32_t n, uint32_t W) {
+  (void)n;
+  return W >= 4 ? 1 : 4 / W;
+}
+
+static size_t cfar_lds_bytes(const CfarRowGeo& G, uint32_t R) { return (size_t)R * G.words * sizeof(uint64_t); }
+
+// Waves per row for a single-CPI launch: the fewest that fit the row, more
+// when there are too few rows to give every SIMD several waves.
+static uint32_t cfar_single_waves(uint32_t n, uint32_t n_rows) {
+  uint32_t W = cfar_min_waves(n);
+  while (W < 4 && (uint64_t)n_rows * W < 16384 && 128u * W <= n) W <<= 1;
+  return W;
+}
+
+// Run-time (W > 1, SLOG) to the compile-time kernels.
+#define RD_CFAR_DISPATCH(KERNEL, ...)                             \
+  switch (G.slog) {                                               \
+    case 0: KERNEL(0, __VA_ARGS__); break;                        \
+    case 1: KERNEL(1, __VA_ARGS__); break;                        \
+    case 2: KERNEL(2, __VA_ARGS__); break;                        \
+    case 3: KERNEL(3, __VA_ARGS__); break;                        \
+    default: KERNEL(4, __VA_ARGS__); break;                       \
+  }
+
+template <typename Cell>
+static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, const CfarRowGeo& G, uint32_t R) {
+  const size_t lds = cfar_lds_bytes(G, R);
+  const dim3 grid((J.n_rows + R - 1) / R), block(64 * G.W * R);
+#define RD_ROWS(SL, M) cfar_rows_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
+#define RD_GATHER(SL, M) cfar_gather_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
+  if (G.W > 1) {
+    RD_CFAR_DISPATCH(RD_ROWS, true)
+  } else {
+    RD_CFAR_DISPATCH(RD_ROWS, false)
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
+  e = hipGetLastError();
+  if (e != hipSuccess || J.cap == 0) return e;
+  if (G.W > 1) {
+    RD_CFAR_DISPATCH(RD_GATHER, true)
+  } else {
+    RD_CFAR_DISPATCH(RD_GATHER, false)
+  }
+#undef RD_ROWS
+#undef RD_GATHER
+  return hipGetLastError();
 }
 
 hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n, uint32_t n_rows,
@@ -281,35 +467,45 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        uint32_t frame_mod, uint64_t* events, uint64_t cap, uint32_t* row_count, void* scratch,
                        uint64_t* total_dev) {
   if (n_rows == 0) return hipSuccess;
-  const uint32_t rpb = cfar_rows_per_tile_max(n);
-  const size_t lds = rpb * ((size_t)n + 1) * sizeof(uint64_t);
-  const dim3 grid((n_rows + rpb - 1) / rpb), block(64 * rpb);
+  const uint32_t W = cfar_single_waves(n, n_rows);
+  const CfarRowGeo G = cfar_row_geo(n, W);
+  const uint32_t R = cfar_rows_per_tile_max(n, W);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   uint64_t* stage = offs + n_rows + 1;
   const CfarJob J{cells, frame_ids, stage, row_count, offs, events, total_dev, cap,
                   cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, kCfarSlot, 0};
-  if (cell_bytes == 4)
-    cfar_rows_kernel<uint32_t><<<grid, block, lds, s>>>(J);
-  else
-    cfar_rows_kernel<uint64_t><<<grid, block, lds, s>>>(J);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
-  e = hipGetLastError();
-  if (e != hipSuccess || cap == 0) return e;
-  if (cell_bytes == 4)
-    cfar_gather_kernel<uint32_t><<<grid, block, lds, s>>>(J);
-  else
-    cfar_gather_kernel<uint64_t><<<grid, block, lds, s>>>(J);
+  return cell_bytes == 4 ? launch_cfar_cell<uint32_t>(s, J, G, R) : launch_cfar_cell<uint64_t>(s, J, G, R);
+}
+
+static hipError_t launch_cfar_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L,
+                                    bool gather) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (W < cfar_min_waves(n) || W > 8 || (W & (W - 1)) || rpb < 1 || rpb > cfar_rows_per_tile_max(n, W))
+    return hipErrorInvalidValue;
+  const CfarRowGeo G = cfar_row_geo(n, W);
+  const size_t lds = cfar_lds_bytes(G, rpb);
+  const CfarJob* jobs = static_cast<const CfarJob*>(L.jobs);
+  const uint32_t block = 64 * W * rpb;
+#define RD_BATCH(SL, GA, M) cfar_batch_kernel<GA, M, SL><<<L.grid, block, lds, s>>>(jobs, L.tile_job, L.n_tiles, G)
+  if (gather) {
+    if (W > 1) {
+      RD_CFAR_DISPATCH(RD_BATCH, true, true)
+    } else {
+      RD_CFAR_DISPATCH(RD_BATCH, true, false)
+    }
+  } else {
+    if (W > 1) {
+      RD_CFAR_DISPATCH(RD_BATCH, false, true)
+    } else {
+      RD_CFAR_DISPATCH(RD_BATCH, false, false)
+    }
+  }
+#undef RD_BATCH
   return hipGetLastError();
 }
 
-hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L) {
-  if (L.n_tiles == 0) return hipSuccess;
-  if (rpb < 1 || rpb > cfar_rows_per_tile_max(n)) return hipErrorInvalidValue;
-  cfar_batch_kernel<false><<<L.grid, 64 * rpb, rpb * ((size_t)n + 1) * sizeof(uint64_t), s>>>(
-      static_cast<const CfarJob*>(L.jobs), L.tile_job, L.n_tiles, n);
-  return hipGetLastError();
+hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L) {
+  return launch_cfar_batch(s, n, W, rpb, L, false);
 }
 
 hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs) {
@@ -318,12 +514,8 @@ hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jo
   return hipGetLastError();
 }
 
-hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L) {
-  if (L.n_tiles == 0) return hipSuccess;
-  if (rpb < 1 || rpb > cfar_rows_per_tile_max(n)) return hipErrorInvalidValue;
-  cfar_batch_kernel<true><<<L.grid, 64 * rpb, rpb * ((size_t)n + 1) * sizeof(uint64_t), s>>>(
-      static_cast<const CfarJob*>(L.jobs), L.tile_job, L.n_tiles, n);
-  return hipGetLastError();
+hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L) {
+  return launch_cfar_batch(s, n, W, rpb, L, true);
 }
 
 }  // namespace rd

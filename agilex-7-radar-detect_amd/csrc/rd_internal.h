@@ -122,8 +122,9 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
 
 // ---------------------------------------------------------------- batched
 // A class launch: `jobs` (device) holds n_jobs records of one shape class,
-// `tile_job` (device) maps each of n_tiles tiles to its job; `grid` blocks walk
-// the tiles with a grid stride.
+// `tile_job` (device) maps each of n_tiles tiles to its job. The beamformer's
+// `grid` blocks walk the tiles with a grid stride; the other stages launch one
+// block per tile (grid = n_tiles).
 struct ClassLaunch {
   const void* jobs;
   const uint32_t* tile_job;
@@ -145,10 +146,12 @@ hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t 
 uint32_t cov_bins_per_tile(uint32_t chp);
 uint32_t cov_max_splits(uint32_t chp);
 hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const ClassLaunch& L);
-// CFAR over power rows (u32 cells) of n bins, rpb rows per tile.
-uint32_t cfar_rows_per_tile_max(uint32_t n);
-hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L);
+// CFAR over power rows (u32 cells) of n bins: W waves per row (a power of two
+// from cfar_min_waves(n) to 8), rpb rows per tile.
+uint32_t cfar_min_waves(uint32_t n);
+uint32_t cfar_rows_per_tile_max(uint32_t n, uint32_t W);
+hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L);
 hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs);
-hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t rpb, const ClassLaunch& L);
+hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L);
 
 }  // namespace rd

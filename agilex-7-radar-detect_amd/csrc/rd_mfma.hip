@@ -78,7 +78,7 @@ __host__ __device__ inline size_t bf_frag_words(uint32_t bt, uint32_t steps) { r
 size_t bf_prep_bytes(uint32_t, uint32_t) { return 0; }
 
 // Every block derives the weight digits itself (B x CH words, L2-resident).
-__device__ void bf_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
+__device__ __forceinline__ void bf_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
                                   uint32_t steps, uint32_t bt_n, uint32_t* frag, int32_t* tail) {
   for (uint32_t f = threadIdx.x; f < bt_n * steps * 64; f += blockDim.x) {
     const uint32_t lane = f & 63, s = (f >> 6) % steps, bt = (f >> 6) / steps;
@@ -144,15 +144,31 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
   const int64_t kconst = 16384LL * 2 * CH;
   const uint64_t c0 = tile * TC;
   const uint32_t ncol = (uint32_t)min<uint64_t>(TC, cols - c0);
+  // Staging: a thread's loads (at most chp*TC/256) are all issued before the
+  // first LDS write.
+  constexpr uint32_t ITER = chp * TC / 256;
+  uint32_t v[ITER];
   if (CHMAJOR) {
-    for (uint32_t idx = threadIdx.x; idx < CH * TC; idx += 256) {
-      const uint32_t a = idx / TC, j = idx % TC;
-      if (j < ncol) xt[j * rs + a] = X[(size_t)a * cols + c0 + j];
+#pragma unroll
+    for (uint32_t k = 0; k < ITER; ++k) {
+      const uint32_t idx = threadIdx.x + 256 * k, a = idx / TC, j = idx % TC;
+      v[k] = (a < CH && j < ncol) ? X[(size_t)a * cols + c0 + j] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < ITER; ++k) {
+      const uint32_t idx = threadIdx.x + 256 * k, a = idx / TC, j = idx % TC;
+      if (a < CH && j < ncol) xt[j * rs + a] = v[k];
     }
   } else {
-    for (uint32_t idx = threadIdx.x; idx < ncol * CH; idx += 256) {
-      const uint32_t j = idx / CH, a = idx % CH;
-      xt[j * rs + a] = X[c0 * CH + idx];
+#pragma unroll
+    for (uint32_t k = 0; k < ITER; ++k) {
+      const uint32_t idx = threadIdx.x + 256 * k;
+      v[k] = idx < ncol * CH ? X[c0 * CH + idx] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < ITER; ++k) {
+      const uint32_t idx = threadIdx.x + 256 * k;
+      if (idx < ncol * CH) xt[(idx / CH) * rs + idx % CH] = v[k];
     }
   }
   __syncthreads();
@@ -519,17 +535,16 @@ __global__ __launch_bounds__(1024) void cov_kernel(CovJob J, uint32_t nb, uint32
   cov_tile(J, blockIdx.x, nb, nt, fs_n, smem);
 }
 
-// Batched: a persistent grid over the bin tiles of a class of CPIs (same
-// padded channel count).
+// Batched: one block per bin tile of a class of CPIs (same padded channel
+// count), block -> job via the tile table.
 __global__ __launch_bounds__(1024) void cov_batch_kernel(const CovJob* __restrict__ jobs,
                                                          const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
                                                          uint32_t nb, uint32_t nt, uint32_t fs_n) {
   extern __shared__ uint32_t smem[];
-  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const CovJob J = jobs[tile_job[t]];
-    __syncthreads();
-    cov_tile(J, t - J.tile0, nb, nt, fs_n, smem);
-  }
+  const uint32_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const CovJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
+  cov_tile(J, t - J.tile0, nb, nt, fs_n, smem);
 }
 
 struct CovGeo {

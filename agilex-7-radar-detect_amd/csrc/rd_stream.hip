@@ -167,16 +167,16 @@ __global__ __launch_bounds__(256) void pfb_kernel(PfbJob J) {
   pfb_tile<T>(J, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Batched: a persistent grid over the tiles of a class of CPIs.
+// Batched: one block per tile of a class of CPIs (block -> job via the tile table).
 template <int T>
 __global__ __launch_bounds__(256) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
                                                         const uint32_t* __restrict__ tile_job, uint32_t n_tiles) {
-  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const PfbJob J = jobs[tile_job[t]];
-    const uint32_t local = t - J.tile0;
-    const uint32_t bx = local % J.nbx, rest = local / J.nbx;
-    pfb_tile<T>(J, bx, rest % J.n_chan, rest / J.n_chan);
-  }
+  const uint32_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const PfbJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
+  const uint32_t local = t - J.tile0;
+  const uint32_t bx = local % J.nbx, rest = local / J.nbx;
+  pfb_tile<T>(J, bx, rest % J.n_chan, rest / J.n_chan);
 }
 
 // Final delay-line state after n_beats: hist[k-1] = x_p[n_beats - k], k = 1..T-1.
@@ -491,19 +491,18 @@ __global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_k
   fft_tile<LOG2N, SPC, ALL1>(J, blockIdx.x, smem);
 }
 
-// Batched: a persistent grid over the frame groups of a class of CPIs (same
-// length and lane count); the twiddles are staged once per block.
+// Batched: one block per frame group of a class of CPIs (same length and lane
+// count), block -> job via the tile table.
 template <int LOG2N, int SPC, bool ALL1>
 __global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_batch_kernel(
     const FftJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
     const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
+  const uint32_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const FftJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
   fft_stage_twiddles<LOG2N>(tw4096, smem);
-  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const FftJob J = jobs[tile_job[t]];
-    __syncthreads();  // the previous tile's frames have been read out
-    fft_tile<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
-  }
+  fft_tile<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
 }
 
 template <int LOG2N, int SPC>

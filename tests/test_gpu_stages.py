@@ -181,7 +181,7 @@ CFAR_CFGS = [
 
 
 @pytest.mark.parametrize("cfg", CFAR_CFGS)
-@pytest.mark.parametrize("n", [64, 1024, 4096, 100])
+@pytest.mark.parametrize("n", [64, 1024, 4096, 100, 1, 7, 2049, 5000, 8191])
 def test_cfar_u64(gpu, orc, cfg, n):
     rng = np.random.default_rng(n + sum(cfg))
     rows = 5
