@@ -104,12 +104,23 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
 #pragma unroll
   for (int k = 1; k < T; ++k) w[T - k] = split(pfb_sample(xc, hc, P, p, (int64_t)m0 - k));
 
+  // Raw samples of the current block of T beats; the next block's loads are
+  // issued before the current block computes.
+  uint32_t cur[T];
+#pragma unroll
+  for (int j = 0; j < T; ++j) cur[j] = m0 + j < m1 ? xc[(size_t)(m0 + j) * P + p] : 0u;
   for (uint32_t mb = m0; mb < m1; mb += T) {
+    uint32_t nxt[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+      const uint32_t m = mb + T + j;
+      nxt[j] = m < m1 ? xc[(size_t)m * P + p] : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < T; ++j) {
       const uint32_t m = mb + j;
       if (m >= m1) break;
-      w[j] = split(xc[(size_t)m * P + p]);
+      w[j] = split(cur[j]);
       int32_t arh = 0, arl = 0, aih = 0, ail = 0;
 #pragma unroll
       for (int k = 0; k < T; ++k) {
@@ -123,6 +134,8 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
       yc[(size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
       if (fc) fc[(size_t)m * P + p] = f;
     }
+#pragma unroll
+    for (int j = 0; j < T; ++j) cur[j] = nxt[j];
   }
 }
 
@@ -286,7 +299,10 @@ __device__ __forceinline__ int32_t rne15(int32_t v) {
   return (v + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 15, 1)) >> 15;
 }
 
-// lo = q(a + b), hi = q(a - b), q = round by the schedule bit then saturate
+// lo = q(a + b), hi = q(a - b), q = round by the schedule bit then saturate.
+// A scaled sub-stage needs less: the rounded half-sum of two int16 values lies
+// in [-32768, 32767], and the rounded half-difference in [-32768, 32768], so
+// only the difference can saturate, and only upwards.
 template <bool ALL1>
 __device__ __forceinline__ void bfly(Cv& a, Cv& b, bool sh, StageFlags& f) {
   int32_t sr = a.re + b.re, si = a.im + b.im, dr = a.re - b.re, di = a.im - b.im;
@@ -295,11 +311,15 @@ __device__ __forceinline__ void bfly(Cv& a, Cv& b, bool sh, StageFlags& f) {
     si = rne1(si);
     dr = rne1(dr);
     di = rne1(di);
+    f.mx = max(f.mx, max(dr, di));
+    a = Cv{sr, si};
+    b = Cv{min(dr, 32767), min(di, 32767)};
+  } else {
+    f.see(sr, si);
+    f.see(dr, di);
+    a = Cv{sat(sr), sat(si)};
+    b = Cv{sat(dr), sat(di)};
   }
-  f.see(sr, si);
-  f.see(dr, di);
-  a = Cv{sat(sr), sat(si)};
-  b = Cv{sat(dr), sat(di)};
 }
 
 // -j * (re + j im) = im - j re with a saturating negate (fft_ref.hpp:322-331)
