@@ -61,6 +61,7 @@ struct Class {
   uint32_t key = 0;
   std::vector<uint32_t> cpis;
   uint32_t max_ch = 0, max_bt = 0;
+  int out_mask = 0;  // BF: outputs shared by every job of the class, -1 if they differ
   Shape shape;
   uint32_t n_tiles = 0;
   void* dev = nullptr;
@@ -196,7 +197,7 @@ hipError_t launch_class(const rd_batch& B, const Class& c, hipStream_t s) {
   switch (c.stage) {
     case S_PFB: return rd::launch_pfb_batch(s, c.key, c.L);
     case S_FFT: return rd::launch_fft_batch(s, B.ctx->tw, c.key & 15u, (c.key >> 4) & 15u, (c.key >> 8) & 1u, c.L);
-    case S_BF: return rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.L);
+    case S_BF: return rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.out_mask, c.L);
     case S_COV: return rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
     case S_CFAR_ROWS: return rd::launch_cfar_rows_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     case S_CFAR_GATHER: return rd::launch_cfar_gather_batch(s, c.key, c.shape.b, c.shape.a, c.L);
@@ -439,9 +440,14 @@ rd_status build(rd_batch& B, uint32_t flags) {
       c.stage = (Stage)st;
       c.key = kv.first;
       c.cpis = kv.second;
+      bool first = true;
       for (uint32_t i : c.cpis) {
         c.max_ch = std::max(c.max_ch, B.d[i].n_chan);
         c.max_bt = std::max(c.max_bt, (B.d[i].n_beams + 15) / 16);
+        const rd_cpi_bufs& bb = B.b[i];
+        const int m = (bb.y ? 1 : 0) | (bb.yflags ? 2 : 0) | 4;  // power is always written
+        c.out_mask = first ? m : (c.out_mask == m ? m : -1);
+        first = false;
       }
       B.cls.push_back(std::move(c));
     }

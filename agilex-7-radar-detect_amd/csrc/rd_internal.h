@@ -140,8 +140,10 @@ hipError_t launch_fft_batch(hipStream_t s, const uint32_t* tw4096, uint32_t log2
                             const ClassLaunch& L);
 // Beamformer over the FFT layout; steps = ceil(max CH / 16) of the class,
 // tc = columns per tile (64, 128 or 256), max_ch / max_bt size the LDS.
+// out_mask: the outputs every job of the class writes (1 y, 2 flags, 4 power,
+// 8 accumulators), or -1 when they differ between jobs.
 hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t max_ch, uint32_t max_bt,
-                           const ClassLaunch& L);
+                           int out_mask, const ClassLaunch& L);
 // Covariance; chp = CH rounded up to 16 (shared by the class), fs_n frame splits.
 uint32_t cov_bins_per_tile(uint32_t chp);
 uint32_t cov_max_splits(uint32_t chp);

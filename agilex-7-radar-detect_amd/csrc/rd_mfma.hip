@@ -61,6 +61,29 @@ __device__ __forceinline__ int64_t combine(v4i hh, v4i x, v4i ll, int r) {
   return (int64_t)hh[r] * 65536 + (int64_t)x[r] * 256 + (int64_t)ll[r];
 }
 
+// round_sat(v, 15, 16) of v = 65536*hh + 256*x + ll + 128*s + c in 32-bit
+// arithmetic. With 128*s = 2^15*(s >> 8) + 128*(s & 255):
+//   floor(v / 2^15) = 2*hh + (s >> 8) + floor(R / 2^15),  v mod 2^15 = R mod 2^15,
+//   R = 256*x + ll + 128*(s & 255) + c,
+// and R fits in int32 for the beamformer's ranges (|x| <= 2^22, |ll| <= 2^21,
+// |c| <= 2^22). Round half to even on the remainder, then saturate.
+__device__ __forceinline__ int32_t rne15_sat_parts(int32_t hh, int32_t x, int32_t ll, int32_t s, int32_t c,
+                                                   uint32_t& fl) {
+  const int32_t R = (x << 8) + ll + ((s & 255) << 7) + c;
+  int32_t q = 2 * hh + (s >> 8) + (R >> 15);
+  const int32_t rem = R & 0x7FFF;
+  q += (rem + (q & 1)) > 0x4000 ? 1 : 0;
+  return sat16(q, fl);
+}
+
+// Beamformer outputs as a compile-time set (bits: 1 y, 2 flags, 4 power,
+// 8 exact accumulators); -1 reads the set from the job's pointers.
+constexpr int kBfOutY = 1, kBfOutF = 2, kBfOutP = 4, kBfOutA = 8;
+
+__host__ __device__ inline int bf_out_mask(const BfJob& J) {
+  return (J.y ? kBfOutY : 0) | (J.flags ? kBfOutF : 0) | (J.power ? kBfOutP : 0) | (J.acc ? kBfOutA : 0);
+}
+
 // ============================================================================
 // Beamformer: y[b][col] = round_sat(sum_a x[col][a] * W[b][a], 15, 16), no
 // conjugate (beamformer_model.hpp:162-179), and p = yr^2 + yi^2
@@ -130,9 +153,10 @@ struct BfLds {
 // is TC contiguous words), and the tile is also written out as the aligned
 // copy Xa[col][ch] (the alignment stage) when J.xa_out is set. Otherwise X is
 // the aligned layout and the tile is one contiguous run.
-template <int STEPS, int TC, bool CHMAJOR>
+template <int STEPS, int TC, bool CHMAJOR, int OUT>
 __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uint32_t* frag, const int32_t* tail,
                                         uint32_t* xt) {
+  const int out = OUT >= 0 ? OUT : bf_out_mask(J);
   constexpr uint32_t chp = STEPS * 16;
   constexpr uint32_t TILES_PER_WAVE = TC / 64;
   const uint32_t CH = J.CH, B = J.B, bt_n = J.bt_n;
@@ -141,7 +165,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
   const uint32_t rs = CH + 1;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t grp = lane >> 4, cl = lane & 15;
-  const int64_t kconst = 16384LL * 2 * CH;
+  const int32_t kconst = 16384 * 2 * (int32_t)CH;
   const uint64_t c0 = tile * TC;
   const uint32_t ncol = (uint32_t)min<uint64_t>(TC, cols - c0);
   // Staging: a thread's loads (at most chp*TC/256) are all issued before the
@@ -224,8 +248,8 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
     ps_r += __shfl_xor(ps_r, 32);
     ps_i += __shfl_xor(ps_i, 16);
     ps_i += __shfl_xor(ps_i, 32);
-    const int64_t sb_im = (int64_t)ps_r + ps_i;
-    const int64_t sb_re = (int64_t)ps_r - ps_i - (int64_t)CH;
+    const int32_t sb_im = ps_r + ps_i;
+    const int32_t sb_re = ps_r - ps_i - (int32_t)CH;
 
     for (uint32_t bt = 0; bt < bt_n; ++bt) {
       v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
@@ -248,19 +272,17 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
       for (int r = 0; r < 4; ++r) {
         const uint32_t b = bt * 16 + 4 * grp + r;
         if (b >= B) continue;
-        const int64_t sa = tail[2 * b], wis = tail[2 * b + 1];
-        const int64_t are = combine(rhh, rx, rll, r) + 128 * (sa + sb_re) - kconst + wis;
-        const int64_t aim = combine(ihh, ix, ill, r) + 128 * (sa + sb_im) - kconst;
+        const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
         uint32_t fr = 0, fi = 0;
-        const int32_t yr = sat16_i64(rne15_i64(are), fr);
-        const int32_t yi = sat16_i64(rne15_i64(aim), fi);
+        const int32_t yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
+        const int32_t yi = rne15_sat_parts(ihh[r], ix[r], ill[r], sa + sb_im, -kconst, fi);
         const size_t o = (size_t)b * cols + col;
-        if (J.y) J.y[o] = cpack(yr, yi);
-        if (J.flags) J.flags[o] = (uint8_t)(fr | (fi << 2));
-        if (J.power) J.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
-        if (J.acc) {
-          J.acc[2 * o] = are;
-          J.acc[2 * o + 1] = aim;
+        if (out & kBfOutY) J.y[o] = cpack(yr, yi);
+        if (out & kBfOutF) J.flags[o] = (uint8_t)(fr | (fi << 2));
+        if (out & kBfOutP) J.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+        if (out & kBfOutA) {
+          J.acc[2 * o] = combine(rhh, rx, rll, r) + 128 * ((int64_t)sa + sb_re) - kconst + wis;
+          J.acc[2 * o + 1] = combine(ihh, ix, ill, r) + 128 * ((int64_t)sa + sb_im) - kconst;
         }
       }
     }
@@ -269,7 +291,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
 
 // Single CPI: a grid stride over the column tiles, the weight digits derived
 // once per block.
-template <int STEPS, int TC, bool CHMAJOR>
+template <int STEPS, int TC, bool CHMAJOR, int OUT>
 __global__ __launch_bounds__(256) void bf_kernel(BfJob J) {
   extern __shared__ uint32_t lds[];
   uint32_t* frag = lds;
@@ -279,13 +301,13 @@ __global__ __launch_bounds__(256) void bf_kernel(BfJob J) {
   const uint64_t n_tiles = (J.cols + TC - 1) / TC;
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     __syncthreads();  // previous tile fully consumed
-    bf_tile<STEPS, TC, CHMAJOR>(J, tile, frag, tail, xt);
+    bf_tile<STEPS, TC, CHMAJOR, OUT>(J, tile, frag, tail, xt);
   }
 }
 
 // Batched: a persistent grid over the column tiles of a class of CPIs (same
 // step count); a block re-derives the weight digits when its CPI changes.
-template <int STEPS, int TC>
+template <int STEPS, int TC, int OUT>
 __global__ __launch_bounds__(256) void bf_batch_kernel(const BfJob* __restrict__ jobs,
                                                        const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
                                                        uint32_t max_bt) {
@@ -302,7 +324,7 @@ __global__ __launch_bounds__(256) void bf_batch_kernel(const BfJob* __restrict__
       bf_weights_to_lds(J.W, J.CH, J.B, STEPS * 16, STEPS, J.bt_n, frag, tail);
       cur = jid;
     }
-    bf_tile<STEPS, TC, true>(J, t - J.tile0, frag, tail, xt);
+    bf_tile<STEPS, TC, true, OUT>(J, t - J.tile0, frag, tail, xt);
   }
 }
 
@@ -312,10 +334,17 @@ static hipError_t launch_bf_s(hipStream_t s, bool chmajor, const BfJob& J) {
   const uint64_t tiles = (J.cols + TC - 1) / TC;
   const dim3 grid((uint32_t)(tiles < 2048 ? tiles : 2048));
   const size_t lds = BfLds<STEPS, TC>::words(J.bt_n, J.CH) * sizeof(uint32_t);
-  if (chmajor)
-    bf_kernel<STEPS, TC, true><<<grid, 256, lds, s>>>(J);
-  else
-    bf_kernel<STEPS, TC, false><<<grid, 256, lds, s>>>(J);
+  const int out = bf_out_mask(J);
+  if (chmajor) {
+    if (out == kBfOutP)
+      bf_kernel<STEPS, TC, true, kBfOutP><<<grid, 256, lds, s>>>(J);
+    else if (out == (kBfOutY | kBfOutF | kBfOutP))
+      bf_kernel<STEPS, TC, true, kBfOutY | kBfOutF | kBfOutP><<<grid, 256, lds, s>>>(J);
+    else
+      bf_kernel<STEPS, TC, true, -1><<<grid, 256, lds, s>>>(J);
+  } else {
+    bf_kernel<STEPS, TC, false, -1><<<grid, 256, lds, s>>>(J);
+  }
   return hipGetLastError();
 }
 
@@ -342,20 +371,26 @@ hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W
 }
 
 template <int STEPS, int TC>
-static hipError_t launch_bf_batch_s(hipStream_t s, uint32_t max_ch, uint32_t max_bt, const ClassLaunch& L) {
+static hipError_t launch_bf_batch_s(hipStream_t s, uint32_t max_ch, uint32_t max_bt, int out, const ClassLaunch& L) {
   const size_t lds = BfLds<STEPS, TC>::words(max_bt, max_ch) * sizeof(uint32_t);
-  bf_batch_kernel<STEPS, TC><<<L.grid, 256, lds, s>>>(static_cast<const BfJob*>(L.jobs), L.tile_job, L.n_tiles,
-                                                      max_bt);
+  const BfJob* jobs = static_cast<const BfJob*>(L.jobs);
+  if (out == kBfOutP)
+    bf_batch_kernel<STEPS, TC, kBfOutP><<<L.grid, 256, lds, s>>>(jobs, L.tile_job, L.n_tiles, max_bt);
+  else if (out == (kBfOutY | kBfOutF | kBfOutP))
+    bf_batch_kernel<STEPS, TC, kBfOutY | kBfOutF | kBfOutP><<<L.grid, 256, lds, s>>>(jobs, L.tile_job, L.n_tiles,
+                                                                                    max_bt);
+  else
+    bf_batch_kernel<STEPS, TC, -1><<<L.grid, 256, lds, s>>>(jobs, L.tile_job, L.n_tiles, max_bt);
   return hipGetLastError();
 }
 
 hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t max_ch, uint32_t max_bt,
-                           const ClassLaunch& L) {
+                           int out_mask, const ClassLaunch& L) {
   if (L.n_tiles == 0) return hipSuccess;
-#define RD_BF_TC(S)                                                                       \
-  return tc == 256 ? launch_bf_batch_s<S, 256>(s, max_ch, max_bt, L)                      \
-                   : (tc == 128 ? launch_bf_batch_s<S, 128>(s, max_ch, max_bt, L)         \
-                                : launch_bf_batch_s<S, 64>(s, max_ch, max_bt, L));
+#define RD_BF_TC(S)                                                                            \
+  return tc == 256 ? launch_bf_batch_s<S, 256>(s, max_ch, max_bt, out_mask, L)                 \
+                   : (tc == 128 ? launch_bf_batch_s<S, 128>(s, max_ch, max_bt, out_mask, L)    \
+                                : launch_bf_batch_s<S, 64>(s, max_ch, max_bt, out_mask, L));
   switch (steps) {
     case 1: RD_BF_TC(1)
     case 2: RD_BF_TC(2)
