@@ -120,6 +120,12 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
                        void* scratch, uint64_t* total_dev);
 
+// Detection packets (rd_packet.hip): n_cap events, or min(n_cap, *n_dev) when
+// n_dev (device) is set.
+hipError_t launch_event_flits(hipStream_t s, const uint64_t* events, uint64_t n_cap, const uint64_t* n_dev,
+                              uint32_t packet_w, uint32_t dest, uint32_t src, uint32_t vc, uint32_t seq0,
+                              uint32_t* flits);
+
 // ---------------------------------------------------------------- batched
 // A class launch: `jobs` (device) holds n_jobs records of one shape class,
 // `tile_job` (device) maps each of n_tiles tiles to its job. The beamformer's

@@ -23,7 +23,7 @@ EXPORTS = [
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
-    "rd_batch_destroy",
+    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -95,9 +95,14 @@ class Lib:
         L.rd_batch_events.argtypes = [vp, vp]
         L.rd_batch_plan.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.rd_batch_destroy.argtypes = [vp]
+        L.rd_event_flit_count.argtypes = [u32]
+        L.rd_event_flit_count.restype = u32
+        L.rd_flit_words.argtypes = [u32]
+        L.rd_flit_words.restype = u32
+        L.rd_event_flits.argtypes = [vp, vp, u64, vp, u32, u32, u32, u32, u32, vp]
         for name in EXPORTS:
             fn = getattr(L, name)
-            if name not in ("rd_device_count", "rd_status_str", "rd_stream"):
+            if name not in ("rd_device_count", "rd_status_str", "rd_stream", "rd_event_flit_count", "rd_flit_words"):
                 fn.restype = i32
 
     def device_count(self) -> int:
@@ -263,6 +268,27 @@ class Ctx:
         self.lib.check("rd_cfar_run", rc, ok=(RD_OK, RD_E_CAPACITY))
         t = min(tot.value, cap)
         return dev.download(np.uint64, 3 * t).reshape(t, 3), drc.download(np.uint32, rows), tot.value, rc
+
+
+def _ctx_event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0, n_dev=None, cap=None):
+    """Detection packets of packed events [n][3] (host array or DevBuf with
+    cap events) -> flits [n][L][fw] u32."""
+    L = int(self.L.rd_event_flit_count(packet_w))
+    fw = int(self.L.rd_flit_words(packet_w))
+    if isinstance(events, DevBuf):
+        dev, n = events, int(cap)
+    else:
+        ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)
+        n = ev.shape[0]
+        dev = self.put(ev)
+    out = self.buf(4 * max(n, 1) * L * fw)
+    rc = self.L.rd_event_flits(self.h, dev.ptr, n, n_dev.ptr if n_dev is not None else None, packet_w, dest, src,
+                               vc, seq0, out.ptr)
+    self.lib.check("rd_event_flits", rc)
+    return out.download(np.uint32, n * L * fw).reshape(n, L, fw)
+
+
+Ctx.event_flits = _ctx_event_flits
 
 
 class Chain:

@@ -205,6 +205,24 @@ rd_status rd_chain_graph(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs
 rd_status rd_graph_launch(rd_graph* graph);
 rd_status rd_graph_destroy(rd_graph* graph);
 
+/* Detection packets for the packet fabric. Replaces, per event,
+ * packet_model::hdr_pack (model/cpp/packet/packet_model.hpp:150-160) and
+ * make_flit (:241-260) for a detection packet of detection_flits(packet_w)
+ * flits (:116-127): a header flit {dest 4, src 5, vc 2, type 0, length,
+ * seq = (seq0 + e) mod 2^16} then the 176-bit event LSB first in packet_w-bit
+ * slices; each flit has odd parity, vc, sof, eof in bits 0..4 and the data
+ * from bit 5. events [n][3] packed as rd_cfar_run writes them (bits 176-177,
+ * sof/eof of the event stream, are not payload). flits
+ * [n][rd_event_flit_count(packet_w)][rd_flit_words(packet_w)] u32, the
+ * reference's Flit::w words. n_events_dev (nullable, device): the count is
+ * min(n_events, *n_events_dev), e.g. the chain's event total, so the call can
+ * follow rd_chain_enqueue in the same stream (and graph) without a host round
+ * trip. packet_w in [36, 512]; dest < 16, src < 32, vc < 4. */
+uint32_t rd_event_flit_count(uint32_t packet_w);
+uint32_t rd_flit_words(uint32_t packet_w);
+rd_status rd_event_flits(rd_ctx* ctx, const uint64_t* events, uint64_t n_events, const uint64_t* n_events_dev,
+                         uint32_t packet_w, uint32_t dest, uint32_t src, uint32_t vc, uint32_t seq0, uint32_t* flits);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference: its models run one stream of frames at a
  * time (pfb::PfbModel::step, fft::transform per frame, cfar::run_frame per

@@ -168,6 +168,21 @@ class Oracle:
         self.lib.rdo_event_pack(_ptr(f), _ptr(w))
         return w
 
+    def event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0):
+        """Detection packets of packed events [n][3] -> flits [n][L][fw] u32."""
+        ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)
+        n = ev.shape[0]
+        L = 1 + (176 + packet_w - 1) // packet_w
+        fw = (packet_w + 5 + 31) // 32
+        out = np.zeros(max(n, 1) * L * fw, np.uint32)
+        f = self._f("event_flits")
+        f.restype = C.c_int
+        rc = f(_ptr(ev), C.c_uint64(n), C.c_uint(packet_w), C.c_uint(dest), C.c_uint(src), C.c_uint(vc),
+               C.c_uint(seq0), _ptr(out))
+        if rc < 0:
+            raise ValueError(f"event_flits failed ({rc})")
+        return out[: n * L * fw].reshape(n, L, fw)
+
     def chain(self, iq, taps, W, CH, N, T, F, B, cfar_cfg8, max_guard, max_ref, spc=2, sched=0xFFFFFFFF,
               keep=("u", "X", "y", "power", "R", "events"), threads=1):
         """Run the composed chain for one CPI; returns a dict of the kept outputs."""

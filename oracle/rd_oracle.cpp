@@ -703,6 +703,53 @@ void rdo_event_pack(const uint64_t* fields11, uint64_t* words) {
 }
 
 // ---------------------------------------------------------------------------
+// Detection packets (model/cpp/packet/packet_model.hpp: widths :74-100,
+// detection_flits :116-127, hdr_pack :150-160, make_flit :241-260; prose
+// ARCHITECTURE.md:1254-1330). One packet per 176-bit event: a header flit whose
+// data field holds the 36-bit header {dest 4, src 5, vc 2, type 3 (0 =
+// detection), length 6 (total flits), seq 16} LSB first, then
+// ceil(176 / packet_w) payload flits carrying the event LSB first. Flit bits:
+// 0 odd parity over the whole flit, 1-2 vc, 3 sof, 4 eof, data from bit 5.
+// A flit is stored as ceil((packet_w + 5) / 32) little-endian 32-bit words.
+// seq of event e = (seq0 + e) mod 2^16.
+// ---------------------------------------------------------------------------
+namespace {
+bool ev_bit(const uint64_t* ev, unsigned i) { return i < 176 && ((ev[i / 64] >> (i % 64)) & 1u); }
+}  // namespace
+
+extern "C" int rdo_event_flits(const uint64_t* events, uint64_t n_events, unsigned packet_w, unsigned dest,
+                               unsigned src, unsigned vc, unsigned seq0, uint32_t* out) {
+  if (packet_w < 36 || packet_w > 512) return -1;
+  const unsigned L = 1 + (176 + packet_w - 1) / packet_w;
+  const unsigned flit_w = packet_w + 5, fw = (flit_w + 31) / 32;
+  for (uint64_t e = 0; e < n_events; ++e) {
+    const uint64_t* ev = events + 3 * e;
+    const uint64_t seq = (seq0 + e) & 0xFFFFu;
+    const uint64_t hdr = (uint64_t)(dest & 15u) | ((uint64_t)(src & 31u) << 4) | ((uint64_t)(vc & 3u) << 9) |
+                         ((uint64_t)0 << 11) | ((uint64_t)(L & 63u) << 14) | (seq << 20);
+    for (unsigned f = 0; f < L; ++f) {
+      uint32_t* o = out + ((size_t)e * L + f) * fw;
+      for (unsigned k = 0; k < fw; ++k) o[k] = 0;
+      auto set = [&](unsigned i, bool v) {
+        if (v) o[i / 32] |= 1u << (i % 32);
+      };
+      set(1, vc & 1u);
+      set(2, (vc >> 1) & 1u);
+      set(3, f == 0);
+      set(4, f == L - 1);
+      for (unsigned j = 0; j < packet_w; ++j) {
+        const bool b = f == 0 ? (j < 36 && ((hdr >> j) & 1u)) : ev_bit(ev, (f - 1) * packet_w + j);
+        set(5 + j, b);
+      }
+      unsigned ones = 0;
+      for (unsigned k = 0; k < fw; ++k) ones += (unsigned)__builtin_popcount(o[k]);
+      set(0, (ones & 1u) == 0);
+    }
+  }
+  return (int)L;
+}
+
+// ---------------------------------------------------------------------------
 // The composed chain for one CPI (builder-defined glue; per-stage semantics as
 // above). Outputs that are null are not kept. Returns the number of events, or
 // a negative value on invalid geometry / event overflow.

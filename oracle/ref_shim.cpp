@@ -23,6 +23,7 @@
 #include "fxp/cmult.hpp"
 #include "fxp/fxp.hpp"
 #include "fxp/fxp_ops.hpp"
+#include "packet/packet_model.hpp"
 #include "pfb/pfb_model.hpp"
 
 namespace {
@@ -273,6 +274,48 @@ int64_t rdr_chain_run(const uint32_t* iq, const uint32_t* taps, const uint32_t* 
     total += (int64_t)per_row[r].size();
   }
   return total;
+}
+
+// Detection packets through packet_model::hdr_pack and make_flit: the payload
+// flit f carries event bits [(f-1)*packet_w, f*packet_w) in 64-bit slices.
+int rdr_event_flits(const uint64_t* events, uint64_t n_events, unsigned packet_w, unsigned dest, unsigned src,
+                    unsigned vc, unsigned seq0, uint32_t* out) {
+  namespace pm = packet_model;
+  if (packet_w < pm::kHdrW || packet_w > 512) return -1;
+  const unsigned L = pm::detection_flits(packet_w);
+  const unsigned fw = (packet_w + pm::kFlitCtrlW + 31) / 32;
+  auto ev_bits = [&](const uint64_t* ev, unsigned lsb, unsigned n) {
+    uint64_t v = 0;
+    for (unsigned i = 0; i < n; ++i) {
+      const unsigned b = lsb + i;
+      if (b < pm::kDetectionPayloadW && ((ev[b / 64] >> (b % 64)) & 1u)) v |= 1ull << i;
+    }
+    return v;
+  };
+  for (uint64_t e = 0; e < n_events; ++e) {
+    const uint64_t* ev = events + 3 * e;
+    pm::Header h;
+    h.dest = dest;
+    h.src = src;
+    h.vc = vc;
+    h.ptype = pm::kTypeDetection;
+    h.length = L;
+    h.seq = (unsigned)((seq0 + e) & 0xFFFFu);
+    for (unsigned f = 0; f < L; ++f) {
+      std::vector<uint64_t> data;
+      for (unsigned k = 0; k * 64 < packet_w; ++k) {
+        const unsigned n = (packet_w - k * 64 >= 64) ? 64 : packet_w - k * 64;
+        if (f == 0)
+          data.push_back(k == 0 ? pm::hdr_pack(h) : 0ull);
+        else
+          data.push_back(ev_bits(ev, (f - 1) * packet_w + k * 64, n));
+      }
+      const pm::Flit fl = pm::make_flit(packet_w, vc, f == 0, f == L - 1, data);
+      if (fl.w.size() != fw || !fl.parity_ok()) return -2;
+      for (unsigned k = 0; k < fw; ++k) out[((size_t)e * L + f) * fw + k] = fl.w[k];
+    }
+  }
+  return (int)L;
 }
 
 }  // extern "C"

@@ -266,3 +266,36 @@ def test_event_detect_recheck(orc):
         cut = (big >> 89) & ((1 << 40) - 1)
         s = (big >> 129) & ((1 << 47) - 1)
         assert (cut * refc) << 8 > alpha * s
+
+
+def _flit_bits(words, nbits):
+    return [(int(words[i // 32]) >> (i % 32)) & 1 for i in range(nbits)]
+
+
+@pytest.mark.parametrize("packet_w", [36, 64, 128, 256, 512])
+def test_event_flits_decode(orc, packet_w):
+    """Field-level check of the packet layout from the prose definition
+    (ARCHITECTURE.md:1254-1330): odd parity, sof/eof framing, header fields,
+    length = total flits, the payload reassembles the low 176 event bits."""
+    rng = np.random.default_rng(7)
+    ev = rng.integers(0, 2**63, size=(5, 3), dtype=np.uint64)
+    fl = orc.event_flits(ev, packet_w, dest=5, src=20, vc=2, seq0=65534)
+    L = 1 + -(-176 // packet_w)
+    assert fl.shape[1] == L
+    for e in range(5):
+        payload = []
+        for f in range(L):
+            bits = _flit_bits(fl[e, f], packet_w + 5)
+            assert sum(bits) % 2 == 1
+            assert bits[1] + 2 * bits[2] == 2
+            assert bits[3] == (f == 0) and bits[4] == (f == L - 1)
+            data = bits[5:]
+            if f == 0:
+                h = sum(b << i for i, b in enumerate(data[:36]))
+                assert not any(data[36:])
+                assert (h & 15, (h >> 4) & 31, (h >> 9) & 3, (h >> 11) & 7, (h >> 14) & 63, h >> 20) == \
+                    (5, 20, 2, 0, L, (65534 + e) & 0xFFFF)
+            else:
+                payload += data
+        want = [(int(ev[e, i // 64]) >> (i % 64)) & 1 for i in range(176)]
+        assert payload[:176] == want and not any(payload[176:])

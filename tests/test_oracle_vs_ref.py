@@ -127,3 +127,18 @@ def test_chain_vs_ref(orc, ref, name, frames):
     for k in keep + ("row_count",):
         assert np.array_equal(o[k], r[k]), k
     assert o["n_events"] == r["n_events"] >= sc.B * F
+
+
+@pytest.mark.parametrize("packet_w", [36, 64, 100, 128, 176, 177, 256, 512])
+def test_event_flits_vs_ref(orc, ref, packet_w):
+    """Detection packets: the restatement against packet_model::hdr_pack /
+    make_flit, on random events with the sof/eof bits above bit 176 set (they
+    are not payload) and a sequence number that wraps."""
+    rng = np.random.default_rng(packet_w)
+    ev = rng.integers(0, 2**63, size=(40, 3), dtype=np.uint64) | (
+        rng.integers(0, 2, size=(40, 3), dtype=np.uint64) << np.uint64(63))
+    ev[:, 2] |= np.uint64(3) << np.uint64(48)
+    for dest, src, vc, seq0 in [(0, 0, 0, 0), (15, 31, 3, 65530), (9, 17, 2, 1234)]:
+        a = orc.event_flits(ev, packet_w, dest, src, vc, seq0)
+        b = ref.event_flits(ev, packet_w, dest, src, vc, seq0)
+        assert np.array_equal(a, b), (packet_w, dest, src, vc, seq0)
