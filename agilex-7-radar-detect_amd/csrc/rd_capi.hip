@@ -146,6 +146,7 @@ rd_status rd_destroy(rd_ctx* c) {
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->total_dev) (void)hipFree(c->total_dev);
   if (c->total_host) (void)hipHostFree(c->total_host);
+  if (c->dop) (void)hipFree(c->dop);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return RD_OK;
@@ -239,6 +240,19 @@ rd_status rd_bf_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t 
   if (st != RD_OK) return st;
   if (!X || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
   return fin(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
+}
+
+rd_status rd_doppler_run(rd_ctx* c, const uint32_t* ring, uint32_t rows, uint32_t depth, uint32_t n_bins,
+                         uint32_t newest, uint32_t n_dop, uint32_t spc, uint32_t scale_sched, int shift, uint32_t* out,
+                         uint32_t* stage_flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!ring || !out || depth < 1 || newest >= depth || n_dop > depth) return RD_E_ARG;
+  if (!pow2(n_dop) || n_dop < 8 || n_dop > 4096 || (spc != 1 && spc != 2) || n_dop / spc < 4) return RD_E_GEOM;
+  st = grow(&c->dop, &c->dop_bytes, rd::doppler_scratch_bytes(rows, n_bins, n_dop));
+  if (st != RD_OK) return st;
+  return fin(rd::launch_doppler(c->stream, c->tw, ring, rows, depth, n_bins, newest, n_dop, spc, scale_sched, shift,
+                                out, stage_flags, c->dop));
 }
 
 rd_status rd_cov_run(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,

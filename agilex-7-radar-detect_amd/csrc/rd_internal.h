@@ -22,6 +22,8 @@ struct rd_ctx {
   uint64_t* total_dev = nullptr;
   uint64_t* total_host = nullptr;  // pinned
   int live_graphs = 0;             // graphs and batches hold the context's stream
+  void* dop = nullptr;             // slow-time transform columns
+  size_t dop_bytes = 0;
 };
 
 namespace rd {
@@ -101,6 +103,13 @@ hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in,
                       uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
                       uint32_t* stage_flags);
 hipError_t launch_align(hipStream_t s, const uint32_t* X, uint32_t n_chan, uint64_t cols, uint32_t* Xa);
+// Slow-time transform of the last H frames of a ring [rows][depth][n_bins]
+// (newest frame in slot `newest`) -> out [rows][H][n_bins], stage_flags
+// [rows][n_bins] (nullable). scratch holds doppler_scratch_bytes().
+size_t doppler_scratch_bytes(uint32_t rows, uint32_t n_bins, uint32_t H);
+hipError_t launch_doppler(hipStream_t s, const uint32_t* tw4096, const uint32_t* ring, uint32_t rows, uint32_t depth,
+                          uint32_t n_bins, uint32_t newest, uint32_t H, uint32_t spc, uint32_t sched, int shift,
+                          uint32_t* out, uint32_t* stage_flags, void* scratch);
 
 // MFMA contractions (rd_mfma.hip)
 size_t bf_prep_bytes(uint32_t n_ant, uint32_t n_beams);

@@ -628,4 +628,71 @@ hipError_t launch_align(hipStream_t s, const uint32_t* X, uint32_t n_chan, uint6
   return hipGetLastError();
 }
 
+// ============================================================================
+// Slow-time (Doppler) transform across frames. The frames of a row sit in a
+// ring of `depth` slots (frame f in slot f mod depth, as hist::slot_of,
+// model/cpp/history/history_model.hpp:127-129); the window is the last H
+// frames, the newest in slot `newest`. Each (row, bin) column of the window is
+// gathered into contiguous memory, transformed by the bit-exact FFT above
+// along the frame axis, and turned back to [row][doppler][bin], optionally
+// with the zero-Doppler bin moved to the centre (fftshift,
+// model/python/range_doppler.py:254-265).
+// ============================================================================
+// in [R][depth][N] -> out [R][N][H]: out[r][k][h] = in[r][(first + h) % depth][k]
+__global__ __launch_bounds__(256) void ring_to_cols_kernel(const uint32_t* __restrict__ in, uint32_t depth,
+                                                           uint32_t N, uint32_t H, uint32_t first,
+                                                           uint32_t* __restrict__ out) {
+  __shared__ uint32_t tile[64][65];
+  const uint32_t k0 = blockIdx.x * 64, h0 = blockIdx.y * 64, r = blockIdx.z;
+  for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const uint32_t hh = idx / 64, kk = idx % 64, h = h0 + hh, k = k0 + kk;
+    if (h < H && k < N) tile[hh][kk] = in[((size_t)r * depth + (first + h) % depth) * N + k];
+  }
+  __syncthreads();
+  for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const uint32_t kk = idx / 64, hh = idx % 64, h = h0 + hh, k = k0 + kk;
+    if (h < H && k < N) out[((size_t)r * N + k) * H + h] = tile[hh][kk];
+  }
+}
+
+// in [R][N][H] -> out [R][H][N], doppler row h' = shift ? (h + H/2) % H : h
+__global__ __launch_bounds__(256) void cols_to_rows_kernel(const uint32_t* __restrict__ in, uint32_t N, uint32_t H,
+                                                           int shift, uint32_t* __restrict__ out) {
+  __shared__ uint32_t tile[64][65];
+  const uint32_t k0 = blockIdx.x * 64, h0 = blockIdx.y * 64, r = blockIdx.z;
+  for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const uint32_t kk = idx / 64, hh = idx % 64, h = h0 + hh, k = k0 + kk;
+    if (h < H && k < N) tile[kk][hh] = in[((size_t)r * N + k) * H + h];
+  }
+  __syncthreads();
+  for (uint32_t idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const uint32_t hh = idx / 64, kk = idx % 64, h = h0 + hh, k = k0 + kk;
+    if (h < H && k < N) {
+      const uint32_t hd = shift ? (h + H / 2) % H : h;
+      out[((size_t)r * H + hd) * N + k] = tile[kk][hh];
+    }
+  }
+}
+
+size_t doppler_scratch_bytes(uint32_t rows, uint32_t n_bins, uint32_t H) {
+  return 2 * (size_t)rows * n_bins * H * sizeof(uint32_t) + 256;
+}
+
+hipError_t launch_doppler(hipStream_t s, const uint32_t* tw4096, const uint32_t* ring, uint32_t rows, uint32_t depth,
+                          uint32_t n_bins, uint32_t newest, uint32_t H, uint32_t spc, uint32_t sched, int shift,
+                          uint32_t* out, uint32_t* stage_flags, void* scratch) {
+  if (rows == 0 || n_bins == 0) return hipSuccess;
+  uint32_t* cols = static_cast<uint32_t*>(scratch);
+  uint32_t* spec = cols + (size_t)rows * n_bins * H;
+  const uint32_t first = (newest + depth - (H - 1) % depth) % depth;
+  const dim3 grid((n_bins + 63) / 64, (H + 63) / 64, rows);
+  ring_to_cols_kernel<<<grid, 256, 0, s>>>(ring, depth, n_bins, H, first, cols);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_fft(s, tw4096, cols, H, spc, sched, 1, rows * n_bins, nullptr, spec, stage_flags);
+  if (e != hipSuccess) return e;
+  cols_to_rows_kernel<<<grid, 256, 0, s>>>(spec, n_bins, H, shift, out);
+  return hipGetLastError();
+}
+
 }  // namespace rd

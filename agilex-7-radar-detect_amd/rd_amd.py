@@ -23,7 +23,7 @@ EXPORTS = [
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
-    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits",
+    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -100,6 +100,7 @@ class Lib:
         L.rd_flit_words.argtypes = [u32]
         L.rd_flit_words.restype = u32
         L.rd_event_flits.argtypes = [vp, vp, u64, vp, u32, u32, u32, u32, u32, vp]
+        L.rd_doppler_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("rd_device_count", "rd_status_str", "rd_stream", "rd_event_flit_count", "rd_flit_words"):
@@ -289,6 +290,21 @@ def _ctx_event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0, n_dev=
 
 
 Ctx.event_flits = _ctx_event_flits
+
+
+def _ctx_doppler(self, ring, rows, depth, n_bins, newest, n_dop, spc=2, sched=0xFFFFFFFF, shift=False):
+    """Slow-time transform of a ring [rows][depth][n_bins] -> (out
+    [rows][n_dop][n_bins], stage_flags [rows][n_bins])."""
+    dev = self.put(np.ascontiguousarray(ring, np.uint32))
+    out = self.buf(4 * rows * n_dop * n_bins)
+    sf = self.buf(4 * rows * n_bins)
+    rc = self.L.rd_doppler_run(self.h, dev.ptr, rows, depth, n_bins, newest, n_dop, spc, sched, 1 if shift else 0,
+                               out.ptr, sf.ptr)
+    self.lib.check("rd_doppler_run", rc)
+    return out.download(np.uint32, rows * n_dop * n_bins), sf.download(np.uint32, rows * n_bins)
+
+
+Ctx.doppler = _ctx_doppler
 
 
 class Chain:

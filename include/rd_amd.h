@@ -205,6 +205,22 @@ rd_status rd_chain_graph(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs
 rd_status rd_graph_launch(rd_graph* graph);
 rd_status rd_graph_destroy(rd_graph* graph);
 
+/* Slow-time (Doppler) transform: for every row (channel or beam) and range
+ * bin, the bit-exact FFT of fft::transform (model/cpp/fft/fft_ref.hpp:409-412,
+ * Config{n_dop, spc, scale_sched}, natural-order bins) across the last n_dop
+ * frames, the range-Doppler map of model/python/range_doppler.py:254-265 in
+ * the chain's fixed-point arithmetic. The frames sit in a ring
+ * ring [rows][depth][n_bins] (frame f in slot f mod depth, hist::slot_of,
+ * model/cpp/history/history_model.hpp:127-129), the newest in slot `newest`;
+ * a plain [rows][frames][n_bins] array is the ring with depth = frames,
+ * newest = frames - 1. out [rows][n_dop][n_bins]; with `shift` the Doppler
+ * axis is fftshifted (bin k at row (k + n_dop/2) mod n_dop, range_doppler.py:
+ * 262-265). stage_flags (nullable) [rows][n_bins], packed as rd_fft_run's.
+ * n_dop: power of two, 8..4096, n_dop/spc >= 4, n_dop <= depth. */
+rd_status rd_doppler_run(rd_ctx* ctx, const uint32_t* ring, uint32_t rows, uint32_t depth, uint32_t n_bins,
+                         uint32_t newest, uint32_t n_dop, uint32_t spc, uint32_t scale_sched, int shift, uint32_t* out,
+                         uint32_t* stage_flags);
+
 /* Detection packets for the packet fabric. Replaces, per event,
  * packet_model::hdr_pack (model/cpp/packet/packet_model.hpp:150-160) and
  * make_flit (:241-260) for a detection packet of detection_flits(packet_w)

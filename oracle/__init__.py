@@ -168,6 +168,21 @@ class Oracle:
         self.lib.rdo_event_pack(_ptr(f), _ptr(w))
         return w
 
+    def doppler(self, ring, rows, depth, n_bins, newest, n_dop, spc=2, sched=0xFFFFFFFF, shift=False):
+        """Slow-time transform: for every (row, bin) the FFT restatement across
+        the last n_dop frames of the ring (frame window oldest first, newest in
+        slot `newest`; fft_ref.hpp:409-412 semantics, natural-order bins), as
+        [rows][n_dop][n_bins], optionally fftshifted on the Doppler axis
+        (model/python/range_doppler.py:254-265). Also the stage flags [rows][n_bins]."""
+        r = np.asarray(ring, np.uint32).reshape(rows, depth, n_bins)
+        slots = (newest - (n_dop - 1) + np.arange(n_dop)) % depth
+        cols = np.ascontiguousarray(r[:, slots, :].transpose(0, 2, 1)).reshape(-1)  # [rows][bins][n_dop]
+        _, bins, sf = self.fft(cols, n_dop, spc, sched, 1)
+        spec = bins.reshape(rows, n_bins, n_dop).transpose(0, 2, 1)
+        if shift:
+            spec = np.roll(spec, n_dop // 2, axis=1)
+        return np.ascontiguousarray(spec).reshape(-1), sf.reshape(-1)
+
     def event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0):
         """Detection packets of packed events [n][3] -> flits [n][L][fw] u32."""
         ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)

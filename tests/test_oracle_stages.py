@@ -299,3 +299,48 @@ def test_event_flits_decode(orc, packet_w):
                 payload += data
         want = [(int(ev[e, i // 64]) >> (i % 64)) & 1 for i in range(176)]
         assert payload[:176] == want and not any(payload[176:])
+
+
+@pytest.mark.parametrize("H,spc,shift", [(16, 2, False), (64, 1, True), (256, 2, True)])
+def test_doppler_tone_lands_in_its_bin(orc, H, spc, shift):
+    """Closed form for the slow-time transform: a per-bin phase ramp across
+    frames exp(2 pi i k_d f / H) of amplitude A transforms (every stage scaled,
+    so DFT / H) to A in Doppler bin k_d and ~0 elsewhere, within the rounding
+    bound of the fixed-point stages; with the shift, bin k_d sits at row
+    (k_d + H/2) mod H (range_doppler.py:254-275). The ring's window starts
+    after `newest`, so the frames are taken oldest first across the wrap."""
+    rows, n_bins, depth = 2, 5, H + 3
+    newest = 4  # window wraps around the end of the ring
+    kd = np.array([[3, H // 2 + 1, 0, H - 1, 7 % H], [1, 2, H // 4, 5, H - 3]])
+    A = 12000.0
+    ring = np.zeros((rows, depth, n_bins), np.uint32)
+    for f in range(H):
+        slot = (newest - (H - 1) + f) % depth
+        z = A * np.exp(2j * np.pi * kd * f / H)
+        ring[:, slot, :] = rd_pack(np.rint(z.real), np.rint(z.imag))
+    out, _ = orc.doppler(ring, rows, depth, n_bins, newest, H, spc=spc, shift=shift)
+    spec = unpack_cx(out).reshape(rows, H, n_bins)
+    bound = np.log2(H) + 2  # LSB: one rounding per scaled stage plus the input quantisation
+    for r in range(rows):
+        for k in range(n_bins):
+            row = (kd[r, k] + H // 2) % H if shift else kd[r, k]
+            want = np.zeros(H, complex)
+            want[row] = A
+            assert np.abs(spec[r, :, k] - want).max() <= bound, (r, k)
+    # the same against the float range-Doppler picture, scaled by 1/H
+    flt = np.fft.fft(unpack_cx(ring[:, (newest - (H - 1) + np.arange(H)) % depth, :].reshape(-1)).reshape(
+        rows, H, n_bins), axis=1) / H
+    if shift:
+        flt = np.fft.fftshift(flt, axes=1)
+    assert np.abs(spec - flt).max() <= bound
+
+
+def rd_pack(re, im):
+    return ((np.asarray(im, np.int64) & 0xFFFF) << 16 | (np.asarray(re, np.int64) & 0xFFFF)).astype(np.uint32)
+
+
+def unpack_cx(w):
+    w = np.asarray(w, np.uint32).astype(np.int64)
+    re = (w & 0xFFFF) - ((w & 0x8000) << 1)
+    im = (w >> 16) - (((w >> 16) & 0x8000) << 1)
+    return re + 1j * im
