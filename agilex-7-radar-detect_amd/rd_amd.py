@@ -23,7 +23,7 @@ EXPORTS = [
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
-    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run",
+    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -101,6 +101,11 @@ class Lib:
         L.rd_flit_words.restype = u32
         L.rd_event_flits.argtypes = [vp, vp, u64, vp, u32, u32, u32, u32, u32, vp]
         L.rd_doppler_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]
+        sz, pu = C.c_size_t, C.POINTER(u32)
+        L.rd_read_coeff.argtypes = [C.c_char_p, vp, sz, pu, pu, C.c_char_p, sz]
+        L.rd_read_pfb_vec.argtypes = [C.c_char_p, vp, vp, vp, vp, sz, pu, pu, pu, C.c_char_p, sz]
+        L.rd_read_iq_ascii.argtypes = [C.c_char_p, vp, sz, pu, pu, pu, pu, C.c_char_p, sz]
+        L.rd_write_iq_ascii.argtypes = [C.c_char_p, vp, u32, u32, u32, u32, C.c_char_p]
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("rd_device_count", "rd_status_str", "rd_stream", "rd_event_flit_count", "rd_flit_words"):
@@ -121,6 +126,58 @@ class Lib:
 
     def context(self, device: int = 0) -> "Ctx":
         return Ctx(self, device)
+
+    # ---- reference file formats (host side)
+    def _sized(self, fn, name, path, make_bufs, dims):
+        """Two calls: sizes first (RD_E_CAPACITY), then the data."""
+        err = C.create_string_buffer(512)
+        d = [C.c_uint32() for _ in range(dims)]
+        rc = fn(str(path).encode(), *make_bufs(None), *[C.byref(x) for x in d], err, len(err))
+        if rc == RD_E_CAPACITY:
+            args, out = make_bufs([x.value for x in d])
+            rc = fn(str(path).encode(), *args, *[C.byref(x) for x in d], err, len(err))
+            if rc == RD_OK:
+                return out, [x.value for x in d]
+        raise RdError(name, rc, err.value.decode() or self.lib.rd_status_str(rc).decode())
+
+    def read_coeff(self, path):
+        """.coeff -> (packed phase-major taps, phases, taps)."""
+        def bufs(dims):
+            if dims is None:
+                return (None, 0)
+            a = np.zeros(dims[0] * dims[1], np.uint32)
+            return (a.ctypes.data_as(C.c_void_p), a.size), a
+        a, (P, T) = self._sized(self.lib.rd_read_coeff, "rd_read_coeff", path, bufs, 2)
+        return a, P, T
+
+    def read_pfb_vec(self, path):
+        """.vec -> dict(x, y, f_re, f_im [beats*phases], phases, taps, beats)."""
+        def bufs(dims):
+            if dims is None:
+                return (None, None, None, None, 0)
+            n = dims[0] * dims[2]
+            out = dict(x=np.zeros(n, np.uint32), y=np.zeros(n, np.uint32), f_re=np.zeros(n, np.uint8),
+                       f_im=np.zeros(n, np.uint8))
+            ptrs = tuple(out[k].ctypes.data_as(C.c_void_p) for k in ("x", "y", "f_re", "f_im")) + (dims[2],)
+            return ptrs, out
+        out, (P, T, nb) = self._sized(self.lib.rd_read_pfb_vec, "rd_read_pfb_vec", path, bufs, 3)
+        out.update(phases=P, taps=T, beats=nb)
+        return out
+
+    def read_iq_ascii(self, path):
+        """iq_ascii_v1 -> (iq [n_ant][frames*fft] packed, n_ant, fft_size, frames, spc)."""
+        def bufs(dims):
+            if dims is None:
+                return (None, 0)
+            a = np.zeros(dims[0] * dims[1] * dims[2], np.uint32)
+            return (a.ctypes.data_as(C.c_void_p), a.size), a
+        a, (A, N, H, P) = self._sized(self.lib.rd_read_iq_ascii, "rd_read_iq_ascii", path, bufs, 4)
+        return a, A, N, H, P
+
+    def write_iq_ascii(self, path, iq, n_ant, fft_size, frames, spc, scenario="scene"):
+        iq = np.ascontiguousarray(iq, np.uint32)
+        self.check("rd_write_iq_ascii", self.lib.rd_write_iq_ascii(str(path).encode(), iq.ctypes.data_as(C.c_void_p),
+                                                                   n_ant, fft_size, frames, spc, scenario.encode()))
 
 
 class DevBuf:

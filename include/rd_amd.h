@@ -239,6 +239,33 @@ uint32_t rd_flit_words(uint32_t packet_w);
 rd_status rd_event_flits(rd_ctx* ctx, const uint64_t* events, uint64_t n_events, const uint64_t* n_events_dev,
                          uint32_t packet_w, uint32_t dest, uint32_t src, uint32_t vc, uint32_t seq0, uint32_t* flits);
 
+/* ---- reference file formats (host side, no device work) -----------------
+ * The readers validate as the reference's loaders do and return RD_E_ARG with
+ * a diagnosis in err (nullable, err_cap bytes) on any mismatch. Called with a
+ * null or too small output they fill in the sizes and return RD_E_CAPACITY, so
+ * a caller can size its buffers first. */
+
+/* .coeff, pfb::load_coeff (model/cpp/pfb/pfb_model.hpp:357-442): schema 1,
+ * kind pfb_coeff, count = rows = phases*taps, rows in index order with
+ * index = phase*taps + tap. coeff_out [phases*taps] packed, phase-major (the
+ * layout rd_pfb_run takes). */
+rd_status rd_read_coeff(const char* path, uint32_t* coeff_out, size_t cap, uint32_t* phases, uint32_t* taps,
+                        char* err, size_t err_cap);
+/* .vec, pfb::load_io (pfb_model.hpp:444-544): per beat the stimulus x, the
+ * expected y and the flag words, [beats][phases] each (x/y packed). */
+rd_status rd_read_pfb_vec(const char* path, uint32_t* x, uint32_t* y, uint8_t* f_re, uint8_t* f_im,
+                          size_t cap_beats, uint32_t* phases, uint32_t* taps, uint32_t* beats, char* err,
+                          size_t err_cap);
+/* iq_ascii_v1 scenario files (writer model/python/gen_target_iq.py:520-617,
+ * reader sim/verilator/harness/iq_loader.h): one row per (frame, antenna, beat)
+ * "frame ant beat sof eof s0_re s0_im ..." with SAMPLES_PER_CYCLE samples per
+ * beat; every row present once, sof/eof on the first/last beat of a frame.
+ * iq [n_ant][frames*fft_size] packed, the chain's input layout. */
+rd_status rd_read_iq_ascii(const char* path, uint32_t* iq, size_t cap, uint32_t* n_ant, uint32_t* fft_size,
+                           uint32_t* frames, uint32_t* spc, char* err, size_t err_cap);
+rd_status rd_write_iq_ascii(const char* path, const uint32_t* iq, uint32_t n_ant, uint32_t fft_size, uint32_t frames,
+                            uint32_t spc, const char* scenario);
+
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference: its models run one stream of frames at a
  * time (pfb::PfbModel::step, fft::transform per frame, cfar::run_frame per

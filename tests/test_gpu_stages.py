@@ -212,3 +212,16 @@ def test_cfar_u32_frame_mod_and_capacity(gpu, orc):
     eg2, _, tot2, rc2 = gpu.cfar(cells32, n, cfg, 31, 63, frame_mod=F, cap=cap)
     assert rc2 == rd_amd.RD_E_CAPACITY and tot2 == tot
     _eq(eg2, eo[:cap], "events under capacity")
+
+
+@pytest.mark.parametrize("set_name,geom", vecio.PFB_SETS)
+def test_pfb_fed_from_reference_files(gpu, set_name, geom):
+    """The GPU path ingests the reference's .coeff/.vec files through the
+    library's own readers and reproduces the committed outputs."""
+    lib = gpu.lib
+    coeff, P, T = lib.read_coeff(vecio.VEC_DIR / f"pfb_{set_name}_{geom}.coeff")
+    v = lib.read_pfb_vec(vecio.VEC_DIR / f"pfb_{set_name}_{geom}.vec")
+    y, fl = gpu.pfb(v["x"], coeff, P, T, 1, v["beats"])
+    _eq(y, v["y"], "y")
+    _eq(fl & 3, v["f_re"], "f_re")
+    _eq(fl >> 2, v["f_im"], "f_im")
