@@ -263,6 +263,15 @@ rd_status rd_cov_run(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_
   return fin(rd::launch_cov(c->stream, Xa, n_chan, n_frames, n_bins, R));
 }
 
+rd_status rd_cov_integrate(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                           uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R, uint8_t* sat_flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !R || n_chan < 1 || n_chan > 64 || n_frames < 1 || mode > 1 || exp_k > 63 || acc_w < 2 || acc_w > 64)
+    return RD_E_ARG;
+  return fin(rd::launch_cov_integrate(c->stream, Xa, n_chan, n_frames, n_bins, mode, exp_k, acc_w, R, sat_flags));
+}
+
 rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
                       const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                       uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,

@@ -121,6 +121,11 @@ hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W
                              uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* xa_out);
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R);
+// The Integrator frame by frame: mode 0 block, 1 exponential (shift exp_k),
+// saturating to acc_w (2..64) with sticky flags (nullable) per [bin][pair].
+hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,
+                                uint32_t n_bins, uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R,
+                                uint8_t* sat_flags);
 
 // CFAR (rd_cfar.hip). scratch must hold cfar_scratch_bytes(n_rows).
 size_t cfar_scratch_bytes(uint32_t n_rows);

@@ -605,6 +605,62 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
   return hipGetLastError();
 }
 
+// ============================================================================
+// Covariance through the Integrator frame by frame (covar_model.hpp:196-214):
+// block mode adds the cross power, exponential mode moves the accumulator by
+// (x - acc) >> exp_k (floor); both saturate to acc_w every frame with sticky
+// flags. A recurrence, so no contraction: one thread per (bin, pair) walks the
+// frames; a wave's threads share a bin, so each frame's loads hit one row.
+// ============================================================================
+__device__ __forceinline__ int64_t sat_w(int64_t v, uint32_t acc_w, uint32_t& fl) {
+  if (acc_w >= 64) return v;
+  const int64_t hi = ((int64_t)1 << (acc_w - 1)) - 1, lo = -hi - 1;
+  fl |= (v > hi ? 2u : 0u) | (v < lo ? 1u : 0u);
+  return v > hi ? hi : (v < lo ? lo : v);
+}
+
+__global__ __launch_bounds__(256) void cov_integrate_kernel(const uint32_t* __restrict__ Xa, uint32_t CH, uint32_t F,
+                                                            uint32_t N, uint32_t mode, uint32_t exp_k,
+                                                            uint32_t acc_w, int64_t* __restrict__ R,
+                                                            uint8_t* __restrict__ sat_flags) {
+  const uint32_t npairs = CH * (CH + 1) / 2;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (uint64_t)N * npairs) return;
+  const uint32_t k = (uint32_t)(g / npairs);
+  uint32_t p = (uint32_t)(g % npairs), i = 0;
+  while (p >= CH - i) {  // row-major upper triangle: row i holds CH - i pairs
+    p -= CH - i;
+    ++i;
+  }
+  const uint32_t j = i + p;
+  int64_t are = 0, aim = 0;
+  uint32_t fre = 0, fim = 0;
+  for (uint32_t m = 0; m < F; ++m) {
+    const uint32_t* row = Xa + ((size_t)m * N + k) * CH;
+    const uint32_t x = row[i], y = row[j];
+    // x * conj(y), exact
+    const int64_t re = (int64_t)cre(x) * cre(y) + (int64_t)cim(x) * cim(y);
+    const int64_t im = (int64_t)cim(x) * cre(y) - (int64_t)cre(x) * cim(y);
+    const int64_t dre = mode ? (int64_t)((uint64_t)re - (uint64_t)are) >> exp_k : re;
+    const int64_t dim = mode ? (int64_t)((uint64_t)im - (uint64_t)aim) >> exp_k : im;
+    are = sat_w((int64_t)((uint64_t)are + (uint64_t)dre), acc_w, fre);
+    aim = sat_w((int64_t)((uint64_t)aim + (uint64_t)dim), acc_w, fim);
+  }
+  R[2 * g] = are;
+  R[2 * g + 1] = aim;
+  if (sat_flags) sat_flags[g] = (uint8_t)(fre | (fim << 2));
+}
+
+hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,
+                                uint32_t n_bins, uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R,
+                                uint8_t* sat_flags) {
+  const uint64_t total = (uint64_t)n_bins * n_chan * (n_chan + 1) / 2;
+  if (total == 0) return hipSuccess;
+  cov_integrate_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, s>>>(Xa, n_chan, n_frames, n_bins, mode, exp_k,
+                                                                       acc_w, R, sat_flags);
+  return hipGetLastError();
+}
+
 hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const ClassLaunch& L) {
   if (L.n_tiles == 0) return hipSuccess;
   const CovGeo G(chp);

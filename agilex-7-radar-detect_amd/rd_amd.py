@@ -23,7 +23,7 @@ EXPORTS = [
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
-    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii",
+    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -101,6 +101,7 @@ class Lib:
         L.rd_flit_words.restype = u32
         L.rd_event_flits.argtypes = [vp, vp, u64, vp, u32, u32, u32, u32, u32, vp]
         L.rd_doppler_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]
+        L.rd_cov_integrate.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp]
         sz, pu = C.c_size_t, C.POINTER(u32)
         L.rd_read_coeff.argtypes = [C.c_char_p, vp, sz, pu, pu, C.c_char_p, sz]
         L.rd_read_pfb_vec.argtypes = [C.c_char_p, vp, vp, vp, vp, sz, pu, pu, pu, C.c_char_p, sz]
@@ -362,6 +363,20 @@ def _ctx_doppler(self, ring, rows, depth, n_bins, newest, n_dop, spc=2, sched=0x
 
 
 Ctx.doppler = _ctx_doppler
+
+
+def _ctx_cov_integrate(self, Xa, CH, F, N, mode, exp_k, acc_w):
+    """Covariance through the Integrator -> (R int64 [N][pairs][2], sat flags [N][pairs])."""
+    dX = self.put(np.ascontiguousarray(Xa, np.uint32))
+    npairs = CH * (CH + 1) // 2
+    dR = self.buf(16 * N * npairs)
+    df = self.buf(N * npairs)
+    self.lib.check("rd_cov_integrate", self.L.rd_cov_integrate(self.h, dX.ptr, CH, F, N, mode, exp_k, acc_w, dR.ptr,
+                                                               df.ptr))
+    return dR.download(np.int64, 2 * N * npairs), df.download(np.uint8, N * npairs)
+
+
+Ctx.cov_integrate = _ctx_cov_integrate
 
 
 class Chain:

@@ -170,6 +170,18 @@ rd_status rd_bf_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_
 rd_status rd_cov_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                      int64_t* R);
 
+/* Covariance through covar::Integrator frame by frame, either mode
+ * (model/cpp/covariance/covar_model.hpp:196-214): mode 0 (block) adds the
+ * cross power, mode 1 (exponential) moves the accumulator by
+ * trunc((x - acc) >> exp_k) (floor division, fxp.hpp:185: the documented dead
+ * band); both saturate to acc_w bits every frame with sticky flags. For
+ * acc_w = 64 block mode equals rd_cov_run. R as rd_cov_run's; sat_flags
+ * (nullable) [n_bins][pairs], re in bits [1:0], im in [3:2]. The exponential
+ * accumulator is not cleared at a window close (covar_model.hpp:266-268), so R
+ * is its state after the n_frames frames. exp_k <= 63, acc_w 2..64. */
+rd_status rd_cov_integrate(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                           uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R, uint8_t* sat_flags);
+
 /* CA/GO-CFAR over rows of n_bins cells. Replaces cfar::run_frame(bins, cfg,
  * max_guard, max_ref, frame_id) per row (model/cpp/cfar/cfar_model.hpp:295-340),
  * with the events packed by the 176-bit layout of cfar_model.hpp:75-84 into three

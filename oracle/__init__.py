@@ -183,6 +183,17 @@ class Oracle:
             spec = np.roll(spec, n_dop // 2, axis=1)
         return np.ascontiguousarray(spec).reshape(-1), sf.reshape(-1)
 
+    def cov_integrate(self, Xa, CH, F, N, mode, exp_k, acc_w, threads=1):
+        """Covariance through the Integrator, block (0) or exponential (1)
+        mode; returns (R [N][pairs][2] int64, sticky sat flags [N][pairs])."""
+        Xa = np.ascontiguousarray(Xa, np.uint32)
+        npairs = CH * (CH + 1) // 2
+        R = np.zeros(N * npairs * 2, np.int64)
+        fl = np.zeros(N * npairs, np.uint8)
+        self._f("cov_integrate")(_ptr(Xa), C.c_uint(CH), C.c_uint(F), C.c_uint(N), C.c_uint(mode), C.c_uint(exp_k),
+                                 C.c_uint(acc_w), _ptr(R), _ptr(fl), C.c_uint(threads))
+        return R, fl
+
     def event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0):
         """Detection packets of packed events [n][3] -> flits [n][L][fw] u32."""
         ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)

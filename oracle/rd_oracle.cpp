@@ -631,6 +631,40 @@ void rdo_bf_run(const uint32_t* Xa, const uint32_t* W, unsigned CH, unsigned B, 
   });
 }
 
+// Covariance through the Integrator in either mode (covar_model.hpp:196-214):
+// block (mode 0) adds the cross power; exponential (mode 1) moves the
+// accumulator by trunc((x - acc) >> exp_k), floor division (fxp.hpp:185), the
+// dead band of DECISIONS.md decision 10. Both saturate to acc_w every frame
+// and keep sticky flags; the exponential accumulator is not cleared at a
+// window close, so the result after the n_frames frames is the last state.
+void rdo_cov_integrate(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned mode, unsigned exp_k,
+                       unsigned acc_w, int64_t* R, uint8_t* sat_flags, unsigned threads) {
+  const size_t npairs = (size_t)CH * (CH + 1) / 2;
+  parallel_for(N, threads, [&](size_t k) {
+    size_t pi = 0;
+    for (unsigned i = 0; i < CH; ++i) {
+      for (unsigned j = i; j < CH; ++j, ++pi) {
+        Integ ire{acc_w}, iim{acc_w};
+        for (unsigned m = 0; m < F; ++m) {
+          const size_t base = ((size_t)m * N + k) * CH;
+          i64 re, im;
+          cross_swap(unpack(Xa[base + i]), unpack(Xa[base + j]), &re, &im);
+          if (mode == 1) {
+            ire.add((i64)((u64)re - (u64)ire.acc) >> exp_k);
+            iim.add((i64)((u64)im - (u64)iim.acc) >> exp_k);
+          } else {
+            ire.add(re);
+            iim.add(im);
+          }
+        }
+        R[(k * npairs + pi) * 2] = ire.acc;
+        R[(k * npairs + pi) * 2 + 1] = iim.acc;
+        if (sat_flags) sat_flags[k * npairs + pi] = (uint8_t)(ire.sticky | (iim.sticky << 2));
+      }
+    }
+  });
+}
+
 // Covariance: R[bin][pair][2], pairs (i<=j) row-major over the upper triangle,
 // R_ij = sum_frames cross_power(X_i, X_j), each component integrated by a
 // block-mode Integrator of width acc_w over one window of n_frames.

@@ -178,12 +178,13 @@ void rdr_bf_run(const uint32_t* Xa, const uint32_t* W, unsigned CH, unsigned B, 
 
 // Covariance through the reference Integrator: block mode, window = F frames,
 // one accumulator per component; the result is what the window close emits.
-void rdr_cov_run(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w, int64_t* R,
-                 uint8_t* sat_flags, unsigned threads) {
+void rdr_cov_integrate(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned mode, unsigned exp_k,
+                       unsigned acc_w, int64_t* R, uint8_t* sat_flags, unsigned threads) {
   const size_t npairs = (size_t)CH * (CH + 1) / 2;
   covar::Config cfg;
   cfg.window_len = F;
-  cfg.mode = covar::Mode::kBlock;
+  cfg.mode = mode == 1 ? covar::Mode::kExp : covar::Mode::kBlock;
+  cfg.exp_k = exp_k;
   par(N, threads, [&](size_t k) {
     size_t pi = 0;
     for (unsigned i = 0; i < CH; ++i) {
@@ -208,6 +209,11 @@ void rdr_cov_run(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsign
       }
     }
   });
+}
+
+void rdr_cov_run(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w, int64_t* R,
+                 uint8_t* sat_flags, unsigned threads) {
+  rdr_cov_integrate(Xa, CH, F, N, 0, 0, acc_w, R, sat_flags, threads);
 }
 
 void rdr_cross_power(uint32_t x, uint32_t y, int64_t* sre, int64_t* sim, int64_t* dre, int64_t* dim) {

@@ -225,3 +225,16 @@ def test_pfb_fed_from_reference_files(gpu, set_name, geom):
     _eq(y, v["y"], "y")
     _eq(fl & 3, v["f_re"], "f_re")
     _eq(fl >> 2, v["f_im"], "f_im")
+
+
+@pytest.mark.parametrize("mode,exp_k,acc_w", [(1, 0, 64), (1, 4, 64), (1, 15, 40), (1, 2, 24), (0, 0, 36), (0, 0, 64)])
+@pytest.mark.parametrize("CH,F,N", [(4, 40, 16), (17, 9, 5), (64, 3, 2)])
+def test_cov_integrate(gpu, orc, mode, exp_k, acc_w, CH, F, N):
+    rng = np.random.default_rng(CH + F + N + exp_k + acc_w)
+    for Xa in (stim.rand_cx(rng, F * N * CH), stim.rand_extreme(rng, F * N * CH)):
+        R, fl = gpu.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
+        oR, ofl = orc.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
+        _eq(R, oR, "R")
+        _eq(fl, ofl, "sat flags")
+    if mode == 0 and acc_w == 64:
+        _eq(R, gpu.cov(Xa, CH, F, N), "block mode = the MFMA contraction")

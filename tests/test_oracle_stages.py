@@ -344,3 +344,23 @@ def unpack_cx(w):
     re = (w & 0xFFFF) - ((w & 0x8000) << 1)
     im = (w >> 16) - (((w >> 16) & 0x8000) << 1)
     return re + 1j * im
+
+
+def test_cov_exp_mode_closed_forms(orc):
+    """Exponential mode: k = 0 passes the last cross power through; for a
+    constant input the accumulator settles in (x - 2^k, x], approached from
+    below with the truncating dead band (covar_model.hpp:206-214)."""
+    rng = np.random.default_rng(3)
+    CH, N = 3, 4
+    F = 9
+    Xa = stim.rand_cx(rng, F * N * CH)
+    R, _ = orc.cov_integrate(Xa, CH, F, N, 1, 0, 64)
+    last, _ = orc.cov(Xa[(F - 1) * N * CH:], CH, 1, N)
+    assert np.array_equal(R, last)
+    for k in (1, 3, 4):
+        F = 2 ** k * 24
+        x = np.tile(stim.rand_cx(rng, N * CH), F)  # the same frame every time
+        R, _ = orc.cov_integrate(x, CH, F, N, 1, k, 64)
+        one, _ = orc.cov(x[: N * CH], CH, 1, N)
+        d = one - R
+        assert np.all((d >= 0) & (d < 2 ** k)), k

@@ -142,3 +142,14 @@ def test_event_flits_vs_ref(orc, ref, packet_w):
         a = orc.event_flits(ev, packet_w, dest, src, vc, seq0)
         b = ref.event_flits(ev, packet_w, dest, src, vc, seq0)
         assert np.array_equal(a, b), (packet_w, dest, src, vc, seq0)
+
+
+@pytest.mark.parametrize("mode,exp_k,acc_w", [(1, 0, 64), (1, 1, 64), (1, 5, 40), (1, 15, 64), (1, 3, 20),
+                                              (0, 0, 33), (0, 0, 64)])
+def test_cov_integrate_vs_ref(orc, ref, mode, exp_k, acc_w):
+    rng = np.random.default_rng(mode * 100 + exp_k + acc_w)
+    CH, F, N = 5, 23, 7
+    for Xa in (stim.rand_cx(rng, F * N * CH), stim.rand_extreme(rng, F * N * CH)):
+        a = orc.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
+        b = ref.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
