@@ -268,7 +268,7 @@ hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, const ClassLaunch& L) {
 // of the second lane equal to the first's (block sizes divide the lane length).
 //
 // Twiddle products stay in 32 bits: every table entry has |w.re| + |w.im| <=
-// 65535 (checked by the tests), so |x.re*w.re -/+ x.im*w.im| <= 32768*65535 < 2^31.
+// 65534 (checked by the tests), so |x.re*w.re -/+ x.im*w.im| + 32768 < 2^31.
 // ============================================================================
 
 struct Cv {
@@ -294,6 +294,14 @@ __device__ __forceinline__ int32_t sat(int32_t r) { return min(max(r, -32768), 3
 
 // round-to-nearest-even by 1: (v + bit1(v)) >> 1 (the tie bit makes the quotient even)
 __device__ __forceinline__ int32_t rne1(int32_t v) { return (v + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 1, 1)) >> 1; }
+// a + b as one v_add_u32 the optimiser cannot look through: with both operands
+// known to be sign-extended int16 it otherwise rewrites bit 1 of the sum as a
+// five-instruction 16-bit halving-add sequence.
+__device__ __forceinline__ int32_t add_v(int32_t a, int32_t b) {
+  int32_t r;
+  asm("v_add_u32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 // round-to-nearest-even by 15 for |v| < 2^31 - 2^14: (v + 2^14 - 1 + bit15(v)) >> 15
 __device__ __forceinline__ int32_t rne15(int32_t v) {
   return (v + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 15, 1)) >> 15;
@@ -305,7 +313,7 @@ __device__ __forceinline__ int32_t rne15(int32_t v) {
 // only the difference can saturate, and only upwards.
 template <bool ALL1>
 __device__ __forceinline__ void bfly(Cv& a, Cv& b, bool sh, StageFlags& f) {
-  int32_t sr = a.re + b.re, si = a.im + b.im, dr = a.re - b.re, di = a.im - b.im;
+  int32_t sr = add_v(a.re, b.re), si = add_v(a.im, b.im), dr = a.re - b.re, di = a.im - b.im;
   if (ALL1 || sh) {
     sr = rne1(sr);
     si = rne1(si);
@@ -330,15 +338,25 @@ __device__ __forceinline__ Cv neg_j(Cv c, StageFlags& f) {
 }
 
 // x * w rounded once at shift 15 (fxp.hpp:314-318), as two packed int16 dot
-// products: im = x.re*w.im + x.im*w.re = dot2((re, im), (w.im, w.re)) and
-// re = x.re*w.re - x.im*w.im = dot2((re, ~im), (w.re, w.im)) + w.im, using
-// -im = ~im + 1 so no operand is ever negated. Every table entry has
-// |w.re| + |w.im| <= 65535, so the exact products fit in int32.
-__device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t w, StageFlags& f) {
+// products against the pair the twiddle stage keeps per entry,
+// wc = (w.re, ~w.im) and ws = (w.im, w.re):
+//   re = x.re*w.re - x.im*w.im = dot2((re, im), wc) + x.im   (~w = -w - 1)
+//   im = x.re*w.im + x.im*w.re = dot2((re, im), ws)
+// so no operand is ever negated (w.im = -32768 occurs). Every table entry has
+// |w.re| + |w.im| <= 65534, so every partial sum fits in int32.
+__device__ __forceinline__ uint32_t tw_conj_compl(uint32_t w) { return w ^ 0xFFFF0000u; }
+__device__ __forceinline__ uint32_t tw_swap(uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16); }
+// dot2(a, b, 0) in the three-operand encoding (the builtin's two-operand
+// accumulate form costs a register clear per product).
+__device__ __forceinline__ int32_t dot2_0(uint32_t a, uint32_t b) {
+  int32_t r;
+  asm("v_dot2_i32_i16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t wc, uint32_t ws, StageFlags& f) {
   const uint32_t xp = pk(x);
-  const uint32_t wswap = __builtin_amdgcn_alignbit(w, w, 16);
-  const int32_t re = dot2(xp ^ 0xFFFF0000u, w, cim(w));
-  const int32_t im = dot2(xp, wswap, 0);
+  const int32_t re = dot2(xp, wc, x.im);
+  const int32_t im = dot2_0(xp, ws);
   const int32_t yr = rne15(re), yi = rne15(im);
   f.see(yr, yi);
   return Cv{sat(yr), sat(yi)};
@@ -347,25 +365,93 @@ __device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t w, StageFlags& f) {
 __device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
 
 // ----------------------------------------------------------------------------
+// The all-ones schedule (every sub-stage scales by 1, the chain's) runs its
+// butterflies on the packed words themselves, both components per instruction
+// (v_pk_* on the int16 halves), with x = a ^ b and s = x >> 1 (arithmetic):
+//   floor((a + b) / 2) = (a & b) + s
+//   floor((a - b) / 2) = (a | ~b) + s + 1        (the ceiling average of a, ~b)
+//   round to nearest even: + (x & floor & 1)     (odd sum with an odd floor)
+// The rounded half-sum always fits in int16. The rounded half-difference
+// reaches +32768 only for (32767, -32768): a saturating add clamps it, and the
+// sign flip of the wrapping add (bit 15 of wrapped & ~floor) is the sat_pos
+// evidence. These identities hold for every pair of int16 values (checked
+// exhaustively when they were written).
+// ----------------------------------------------------------------------------
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 S2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t U2(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Saturation evidence of one packed sub-stage: bits 15 / 31 of `ev` mark an
+// upward clamp in the low / high half; mx, mn bound the twiddle products
+// before saturation.
+struct PkFlags {
+  uint32_t ev = 0;
+  int32_t mx = -32768, mn = 32767;
+  __device__ __forceinline__ uint32_t word() const {
+    return ((ev & 0x80008000u) != 0u || mx > 32767 ? 2u : 0u) | (mn < -32768 ? 1u : 0u);
+  }
+};
+
+__device__ __forceinline__ void bfly_pk(uint32_t& a, uint32_t& b, PkFlags& f) {
+  const uint32_t x = a ^ b;
+  const s16x2 sx = S2(x) >> (s16x2){1, 1};
+  const uint32_t hs = U2(S2(a & b) + sx);
+  const uint32_t hd = U2(S2(a | ~b) + sx + (s16x2){1, 1});
+  const uint32_t ts = x & hs & 0x00010001u, td = x & hd & 0x00010001u;
+  f.ev |= U2(S2(hd) + S2(td)) & ~hd;
+  a = U2(S2(hs) + S2(ts));
+  b = U2(__builtin_elementwise_add_sat(S2(hd), S2(td)));
+}
+
+// -j * (re + j im) = im - j re: the halves swap and the new high half is
+// negated with saturation (fft_ref.hpp:322-331); the wrapping and the
+// saturating difference disagree (in bit 31) exactly when re = -32768.
+__device__ __forceinline__ uint32_t negj_pk(uint32_t c, PkFlags& f) {
+  const s16x2 lo = S2(c >> 16), hi = S2(c << 16);
+  const s16x2 r = __builtin_elementwise_sub_sat(lo, hi);
+  f.ev |= U2(lo - hi) ^ U2(r);
+  return U2(r);
+}
+
+// x * w on a packed x (cmul_tw's arithmetic), saturated into a packed word
+// by v_cvt_pk_i16_i32 (both halves clamped to int16).
+__device__ __forceinline__ uint32_t cmul_pk(uint32_t x, uint32_t wc, uint32_t ws, PkFlags& f) {
+  const int32_t re = dot2(x, wc, (int32_t)x >> 16);
+  const int32_t im = dot2_0(x, ws);
+  const int32_t yr = rne15(re), yi = rne15(im);
+  f.mx = max(f.mx, max(yr, yi));
+  f.mn = min(f.mn, min(yr, yi));
+  return U2(__builtin_amdgcn_cvt_pk_i16(yr, yi));
+}
+
+
+
+// ----------------------------------------------------------------------------
 // One frame per LOG2N-specialised geometry; every index below is a compile-time
 // expression of the pass and the thread's position, so the passes unroll fully.
 // ALL1: every sub-stage scales by 1 (the chain's schedule); otherwise the
 // schedule bit of each sub-stage is read at run time.
 // ----------------------------------------------------------------------------
-// LDS layout of a block: [NP] twiddles W_N^e at pad(e), [FPB] stage-flag
-// words, then FPB frames of NP words each.
+// LDS layout of a block: [NT] wc(W_N^e) at pad(e), [NT] ws(W_N^e) at NT +
+// pad(e) (NT a multiple of 64 words, so one ds_read2st64 fetches both),
+// [FPB] stage-flag words, then FPB frames of NP words each.
 template <int LOG2N>
 struct FftGeo {
   static constexpr uint32_t N = 1u << LOG2N;
   static constexpr uint32_t TPF = N / 4;                       // threads per frame
   static constexpr uint32_t FPB = TPF >= 256 ? 1 : 256 / TPF;  // frames per block
   static constexpr uint32_t NP = N + N / 32;
-  static constexpr size_t lds_bytes = (size_t)(NP + FPB + FPB * NP) * sizeof(uint32_t);
+  static constexpr uint32_t NT = (NP + 63) / 64 * 64;
+  static constexpr size_t lds_bytes = (size_t)(2 * NT + FPB + FPB * NP) * sizeof(uint32_t);
 };
 
 template <int LOG2N>
 __device__ __forceinline__ void fft_stage_twiddles(const uint32_t* __restrict__ tw4096, uint32_t* tw) {
-  for (uint32_t i = threadIdx.x; i < FftGeo<LOG2N>::N; i += blockDim.x) tw[pad(i)] = tw4096[i << (12 - LOG2N)];
+  for (uint32_t i = threadIdx.x; i < FftGeo<LOG2N>::N; i += blockDim.x) {
+    const uint32_t w = tw4096[i << (12 - LOG2N)];
+    tw[pad(i)] = tw_conj_compl(w);
+    tw[FftGeo<LOG2N>::NT + pad(i)] = tw_swap(w);
+  }
 }
 
 // One tile: frames [grp*FPB, +FPB) of job J, twiddles already in LDS. Every
@@ -382,11 +468,13 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
   uint32_t* __restrict__ bins = J.bins;
   const uint32_t sched = ALL1 ? ~0u : J.sched;
   const uint32_t reorder = J.reorder;
-  uint32_t* tw = smem;
-  uint32_t* sfl = smem + NP;
+  constexpr uint32_t NT = G::NT;
+  const uint32_t* twc = smem;       // wc of W_N^e at pad(e)
+  const uint32_t* tws = smem + NT;  // ws of W_N^e at pad(e)
+  uint32_t* sfl = smem + 2 * NT;
   const uint32_t lf = threadIdx.x / TPF;      // local frame
   const uint32_t g = threadIdx.x % TPF;
-  uint32_t* v = smem + NP + FPB + lf * NP;    // this frame's samples at pad(p)
+  uint32_t* v = smem + 2 * NT + FPB + lf * NP;  // this frame's samples at pad(p)
   const uint32_t frame = grp * FPB + lf;
   const bool active = frame < J.n_frames;
 
@@ -412,6 +500,30 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
     uint32_t ad[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) ad[c] = Q >= 32 ? pad(base) + c * (Q + Q / 32) : pad(base + c * Q);
+    if constexpr (ALL1) {
+      uint32_t p0 = v[ad[0]], p1 = v[ad[1]], p2 = v[ad[2]], p3 = v[ad[3]];
+      PkFlags f0, f1;
+      bfly_pk(p0, p2, f0);  // BF2I, half = l/2
+      bfly_pk(p1, p3, f0);
+      p3 = negj_pk(p3, f1);  // BF2II: the odd sub-block takes -j
+      bfly_pk(p0, p1, f1);
+      bfly_pk(p2, p3, f1);
+      if (s + 2 < NL) {
+        const uint32_t TSH = LOG2N - (NL - s);
+        const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
+        p0 = cmul_pk(p0, twc[0], tws[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
+        p1 = cmul_pk(p1, twc[t1], tws[t1], f1);
+        p2 = cmul_pk(p2, twc[t2], tws[t2], f1);
+        p3 = cmul_pk(p3, twc[t3], tws[t3], f1);
+      }
+      v[ad[0]] = p0;
+      v[ad[1]] = p1;
+      v[ad[2]] = p2;
+      v[ad[3]] = p3;
+      fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
+      __syncthreads();
+      continue;
+    }
     Cv e0 = ld_cv(v[ad[0]]), e1 = ld_cv(v[ad[1]]), e2 = ld_cv(v[ad[2]]), e3 = ld_cv(v[ad[3]]);
     StageFlags f0, f1;
     const bool sh0 = (sched >> s) & 1u, sh1 = (sched >> (s + 1)) & 1u;
@@ -423,10 +535,11 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
     if (s + 2 < NL) {
       const uint32_t TSH = LOG2N - (NL - s);
       // element c sits at quarter c of its block: exponent i * {0, 2, 1, 3}[c]
-      e0 = cmul_tw(e0, tw[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
-      e1 = cmul_tw(e1, tw[pad((2 * i) << TSH)], f1);
-      e2 = cmul_tw(e2, tw[pad(i << TSH)], f1);
-      e3 = cmul_tw(e3, tw[pad((3 * i) << TSH)], f1);
+      const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
+      e0 = cmul_tw(e0, twc[0], tws[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
+      e1 = cmul_tw(e1, twc[t1], tws[t1], f1);
+      e2 = cmul_tw(e2, twc[t2], tws[t2], f1);
+      e3 = cmul_tw(e3, twc[t3], tws[t3], f1);
     }
     // each thread writes back exactly the four positions it read
     v[ad[0]] = pk(e0);
@@ -440,7 +553,39 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
   // Final pass: the last (odd-count) sub-stage and/or the DIT merge.
   constexpr uint32_t SL = NL - 1;  // index of the last lane sub-stage
   constexpr bool LAST_R2 = (NL & 1u) != 0;
-  if (SPC == 1) {
+  if (ALL1 && SPC == 1) {
+    if (LAST_R2) {
+      uint32_t a = v[pad(4 * g)], b = v[pad(4 * g + 1)], c = v[pad(4 * g + 2)], d = v[pad(4 * g + 3)];
+      PkFlags f;
+      bfly_pk(a, b, f);
+      bfly_pk(c, d, f);
+      fword |= f.word() << (2 * SL);
+      v[pad(4 * g)] = a;
+      v[pad(4 * g + 1)] = b;
+      v[pad(4 * g + 2)] = c;
+      v[pad(4 * g + 3)] = d;
+    }
+  } else if (ALL1) {
+    uint32_t a0 = v[pad(2 * g)], a1 = v[pad(2 * g + 1)], b0 = v[pad(M + 2 * g)], b1 = v[pad(M + 2 * g + 1)];
+    if (LAST_R2) {
+      PkFlags f;
+      bfly_pk(a0, a1, f);
+      bfly_pk(b0, b1, f);
+      fword |= f.word() << (2 * SL);
+    }
+    PkFlags fm;
+    const uint32_t m0 = pad(bitrev(2 * g, NL)), m1 = pad(bitrev(2 * g + 1, NL));
+    uint32_t t0 = cmul_pk(b0, twc[m0], tws[m0], fm);
+    uint32_t t1 = cmul_pk(b1, twc[m1], tws[m1], fm);
+    bfly_pk(a0, t0, fm);
+    bfly_pk(a1, t1, fm);
+    fword |= fm.word() << (2 * NL);
+    __syncthreads();
+    v[pad(4 * g)] = a0;
+    v[pad(4 * g + 1)] = t0;
+    v[pad(4 * g + 2)] = a1;
+    v[pad(4 * g + 3)] = t1;
+  } else if (SPC == 1) {
     if (LAST_R2) {
       Cv a = ld_cv(v[pad(4 * g)]), b = ld_cv(v[pad(4 * g + 1)]), c = ld_cv(v[pad(4 * g + 2)]),
          d = ld_cv(v[pad(4 * g + 3)]);
@@ -467,8 +612,9 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
     }
     StageFlags fm;
     const bool shm = (sched >> NL) & 1u;
-    Cv t0 = cmul_tw(b0, tw[pad(bitrev(2 * g, NL))], fm);
-    Cv t1 = cmul_tw(b1, tw[pad(bitrev(2 * g + 1, NL))], fm);
+    const uint32_t m0 = pad(bitrev(2 * g, NL)), m1 = pad(bitrev(2 * g + 1, NL));
+    Cv t0 = cmul_tw(b0, twc[m0], tws[m0], fm);
+    Cv t1 = cmul_tw(b1, twc[m1], tws[m1], fm);
     bfly<ALL1>(a0, t0, shm, fm);
     bfly<ALL1>(a1, t1, shm, fm);
     fword |= fm.word() << (2 * NL);

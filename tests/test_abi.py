@@ -43,14 +43,14 @@ def test_twiddle_table_matches_oracle(orc):
 
 def test_twiddle_magnitudes_bound_the_fft_products():
     """The FFT kernel keeps twiddle products in 32 bits: with |x.re|, |x.im| <= 32768,
-    |x.re*w.re -/+ x.im*w.im| <= 32768*(|w.re| + |w.im|) < 2^31 needs
-    |w.re| + |w.im| <= 65535 for every entry (no entry is (-32768, -32768))."""
+    the partial sum x.re*w.re - x.im*w.im - x.im of the complemented form stays
+    within 32768*(|w.re| + |w.im| + 1) < 2^31, which needs |w.re| + |w.im| <= 65534."""
     t = rd_amd.load().twiddle_table()
     re = (t & 0xFFFF).astype(np.int64)
     im = (t >> 16).astype(np.int64)
     re = np.where(re >= 0x8000, re - 0x10000, re)
     im = np.where(im >= 0x8000, im - 0x10000, im)
-    assert np.max(np.abs(re) + np.abs(im)) <= 65535
+    assert np.max(np.abs(re) + np.abs(im)) <= 65534
     assert np.max(re * re + im * im) < 32769 * 32769
 
 

@@ -153,3 +153,18 @@ def test_cov_integrate_vs_ref(orc, ref, mode, exp_k, acc_w):
         a = orc.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
         b = ref.cov_integrate(Xa, CH, F, N, mode, exp_k, acc_w)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("n,spc", [(16, 1), (64, 2), (256, 1), (1024, 2)])
+def test_fft_twiddle_saturation_vs_ref(orc, ref, n, spc):
+    """Twiddle products that saturate under the all-ones schedule (quarters
+    0/2 at (-32768, -32768), 1/3 at (32767, 32767)): sat_neg of sub-stage 1
+    can only come from a twiddle product there."""
+    q = (np.arange(n) // (n // 4)) & 1
+    v = np.where(q == 0, -32768, 32767)
+    x = stim.pack(v, v)
+    o = orc.fft(x, n, spc, 0xFFFFFFFF, 1)
+    r = ref.fft(x, n, spc, 0xFFFFFFFF, 1)
+    assert (int(o[2][0]) >> 2) & 1
+    for a, b in zip(o, r):
+        assert np.array_equal(a, b)
