@@ -4,6 +4,7 @@
 // Plan. Each stage groups the CPIs into shape classes — the compile-time
 // parameters of its kernel:
 //   PFB   taps (4, 8, 16 register-window kernels; any other count generic)
+//         and whether the CPI keeps the PFB flags
 //   FFT   (log2 length, samples per cycle, all-ones schedule)
 //   BF    MFMA K steps = ceil(channels / 16)
 //   COV   channels rounded up to 16
@@ -195,7 +196,7 @@ rd_status upload(Class& c, const rd_batch& B, const Shape& sh) {
 
 hipError_t launch_class(const rd_batch& B, const Class& c, hipStream_t s) {
   switch (c.stage) {
-    case S_PFB: return rd::launch_pfb_batch(s, c.key, c.L);
+    case S_PFB: return rd::launch_pfb_batch(s, c.key & 0xFFu, (c.key >> 8) & 1u, c.L);
     case S_FFT: return rd::launch_fft_batch(s, B.ctx->tw, c.key & 15u, (c.key >> 4) & 15u, (c.key >> 8) & 1u, c.L);
     case S_BF: return rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.out_mask, c.L);
     case S_COV: return rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
@@ -347,7 +348,8 @@ std::string describe(const rd_batch& B) {
     if (c.stage == S_FFT)
       snprintf(key, sizeof key, "n=%u spc=%u all1=%u", 1u << (c.key & 15u), (c.key >> 4) & 15u, (c.key >> 8) & 1u);
     else if (c.stage == S_PFB)
-      snprintf(key, sizeof key, "taps=%s", c.key ? std::to_string(c.key).c_str() : "any");
+      snprintf(key, sizeof key, "taps=%s flags=%u", (c.key & 0xFFu) ? std::to_string(c.key & 0xFFu).c_str() : "any",
+               (c.key >> 8) & 1u);
     else if (c.stage == S_BF)
       snprintf(key, sizeof key, "steps=%u", c.key);
     else if (c.stage == S_COV)
@@ -426,7 +428,7 @@ rd_status build(rd_batch& B, uint32_t flags) {
   for (uint32_t i = 0; i < n; ++i) {
     const rd_cpi_desc& d = B.d[i];
     const uint32_t T = d.taps;
-    by[S_PFB][(T == 4 || T == 8 || T == 16) ? T : 0].push_back(i);
+    by[S_PFB][((T == 4 || T == 8 || T == 16) ? T : 0) | (B.b[i].pfb_flags ? 0x100u : 0u)].push_back(i);
     const uint32_t log2n = rd::ilog2u(d.n_fft), used = (1u << log2n) - 1u;
     by[S_FFT][log2n | (d.spc << 4) | (((d.scale_sched & used) == used ? 1u : 0u) << 8)].push_back(i);
     by[S_BF][(d.n_chan + 15) / 16].push_back(i);

@@ -152,7 +152,7 @@ struct ClassLaunch {
 };
 
 // PFB, taps T (4, 8, 16 use the register-window kernel; others the generic one).
-hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, const ClassLaunch& L);
+hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, bool flags, const ClassLaunch& L);
 // FFT of log2 length `log2n`, spc lanes; all1: every job's schedule has all
 // bits set for its stages. Frames per tile follow the single-CPI launch.
 uint32_t fft_frames_per_tile(uint32_t log2n);

@@ -94,8 +94,10 @@ __host__ __device__ inline int bf_out_mask(const BfJob& J) {
 // channel k - CHp (CHp = CH rounded up to 16).
 // ============================================================================
 
-// Weight digits in LDS: per beam tile, step, lane: {Are_h, Are_l, Aim_h, Aim_l}
-// (8 words), then per beam: Sa[b] = sum_a (wr + wi), Wis[b] = sum_a wi.
+// Weight digits in LDS: per beam tile and step, the four operands
+// {Are_h, Are_l, Aim_h, Aim_l}, each as 64 lane-consecutive 8-byte words (so a
+// wave's fragment read is one contiguous, conflict-free 512-byte run), then
+// per beam: Sa[b] = sum_a (wr + wi), Wis[b] = sum_a wi.
 __host__ __device__ inline size_t bf_frag_words(uint32_t bt, uint32_t steps) { return (size_t)bt * steps * 64 * 8; }
 
 size_t bf_prep_bytes(uint32_t, uint32_t) { return 0; }
@@ -119,22 +121,35 @@ __device__ __forceinline__ void bf_weights_to_lds(const uint32_t* __restrict__ W
     }
     const Dig8 d = digits8(p, vmask);
     // Are = [Wr | Wi], Aim = [Wi | Wr]
-    uint64_t* o = reinterpret_cast<uint64_t*>(frag + (size_t)f * 8);
+    uint64_t* o = reinterpret_cast<uint64_t*>(frag) + (size_t)(f >> 6) * 256 + lane;
     o[0] = imag ? d.hi : d.hr;
-    o[1] = imag ? d.li : d.lr;
-    o[2] = imag ? d.hr : d.hi;
-    o[3] = imag ? d.lr : d.li;
+    o[64] = imag ? d.li : d.lr;
+    o[128] = imag ? d.hr : d.hi;
+    o[192] = imag ? d.lr : d.li;
   }
-  for (uint32_t b = threadIdx.x; b < bt_n * 16; b += blockDim.x) {
+  // per-beam sums: four threads per beam, 8 independent loads in flight each
+  for (uint32_t t = threadIdx.x; t < bt_n * 64; t += blockDim.x) {
+    const uint32_t b = t >> 2, part = t & 3;
     int32_t sa = 0, wis = 0;
     if (b < B)
-      for (uint32_t a = 0; a < CH; ++a) {
-        const uint32_t w = W[(size_t)b * CH + a];
-        sa += cre(w) + cim(w);
-        wis += cim(w);
+      for (uint32_t a0 = 8 * part; a0 < CH; a0 += 32) {
+        uint32_t w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = a0 + q < CH ? W[(size_t)b * CH + a0 + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          sa += cre(w[q]) + cim(w[q]);
+          wis += cim(w[q]);
+        }
       }
-    tail[2 * b] = sa;
-    tail[2 * b + 1] = wis;
+    sa += __shfl_xor(sa, 1);
+    sa += __shfl_xor(sa, 2);
+    wis += __shfl_xor(wis, 1);
+    wis += __shfl_xor(wis, 2);
+    if (part == 0) {
+      tail[2 * b] = sa;
+      tail[2 * b + 1] = wis;
+    }
   }
 }
 
@@ -256,8 +271,8 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
       v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
 #pragma unroll
       for (int s = 0; s < STEPS; ++s) {
-        const uint64_t* a = reinterpret_cast<const uint64_t*>(frag + (((size_t)bt * STEPS + s) * 64 + lane) * 8);
-        const uint64_t are_h = a[0], are_l = a[1], aim_h = a[2], aim_l = a[3];
+        const uint64_t* a = reinterpret_cast<const uint64_t*>(frag) + ((size_t)bt * STEPS + s) * 256 + lane;
+        const uint64_t are_h = a[0], are_l = a[64], aim_h = a[128], aim_l = a[192];
         rhh = mfma8(are_h, brh[s], rhh);
         rx = mfma8(are_h, brl[s], rx);
         rx = mfma8(are_l, brh[s], rx);

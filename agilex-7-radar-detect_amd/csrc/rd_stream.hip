@@ -19,6 +19,22 @@ typedef short v2s __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int32_t dot2(uint32_t a, uint32_t b, int32_t c) {
   return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, a), __builtin_bit_cast(v2s, b), c, false);
 }
+// dot2(a, b, 0) in the three-operand encoding (the builtin's two-operand
+// accumulate form costs a register clear per product).
+__device__ __forceinline__ int32_t dot2_0(uint32_t a, uint32_t b) {
+  int32_t r;
+  asm("v_dot2_i32_i16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// round-to-nearest-even by 15 of the exact value v = 256*ah + al
+// (|v| < 2^38), as 64-bit adds: (v + 2^14 - 1 + bit15(v)) >> 15
+// (fxp.hpp:198-208); the quotient (|q| < 2^23) fits in 32 bits.
+__device__ __forceinline__ int32_t rne15_wide(int32_t ah, int32_t al) {
+  const int64_t v = ((int64_t)ah << 8) + (int64_t)al;
+  const int64_t t = v + 0x3FFF + ((v >> 15) & 1);
+  return (int32_t)(t >> 15);
+}
 __device__ __forceinline__ uint32_t pk16(int32_t lo, int32_t hi) {
   return ((uint32_t)hi << 16) | ((uint32_t)lo & 0xFFFFu);
 }
@@ -37,10 +53,10 @@ __device__ __forceinline__ uint32_t pk16(int32_t lo, int32_t hi) {
 // Exact products without 64-bit multiplies: each sample component is split
 // x = 256*xh + xl (xh in [-128,127], xl in [0,255]), and
 //   re = 256*sum(xrh*hr - xih*hi) + sum(xrl*hr - xil*hi)
-//   im = 256*sum(xrh*hi + xih*hr) + sum(xrl*hi + xil*hr)
-// Each bracket is one packed int16 dot product (v_dot2c_i32_i16) per tap into
-// an int32 partial that cannot overflow for T <= 64 (|partial| < 2^30); the two
-// partials are recombined exactly in int64 before the single rounding.
+// (and im alike). Each bracket is one packed int16 dot product
+// (v_dot2_i32_i16) per tap into an int32 partial that cannot overflow for
+// T <= 64 (|partial| < 2^30); the two partials are recombined exactly before
+// the single rounding.
 // ============================================================================
 
 struct Digits {
@@ -79,8 +95,60 @@ __device__ __forceinline__ uint32_t pfb_sample(const uint32_t* xc, const uint32_
   return hc ? hc[(size_t)(-m - 1) * P + p] : 0u;
 }
 
-// One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
+// Register-window form for a compile-time T. A sample enters the window as
+// two digit words, hi = (xr >> 8, xi >> 8) (one v_pk_ashrrev_i16) and
+// lo = (xr & 255, xi & 255) (one AND), shared by both outputs: with the taps
+// held as a = (hr, ~hi) and b = (hi, hr) (~hi = -hi - 1, so hi = -32768 needs
+// no special case),
+//   re = 256*sum dot2(hi, a) + sum dot2(lo, a) + sum xi
+//   im = 256*sum dot2(hi, b) + sum dot2(lo, b)
+// where sum xi, the sum of the imaginary parts over the window, is kept as a
+// running sum (one add and one subtract per beat).
+__device__ __forceinline__ uint32_t digits_hi(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2s, v) >> (v2s){8, 8});
+}
+__device__ __forceinline__ uint32_t digits_lo(uint32_t v) { return v & 0x00FF00FFu; }
+
 template <int T>
+struct PfbWindow {
+  uint32_t hi[T], lo[T];
+  int32_t sum_xi = 0;
+  // slot j takes sample v; the sample T beats older leaves it, its imaginary
+  // part read back from its digits (256 * (xi >> 8) + (xi & 255))
+  __device__ __forceinline__ void put(int j, uint32_t v) {
+    const int32_t old = (((int32_t)hi[j] >> 16) << 8) + (int32_t)(lo[j] >> 16);
+    sum_xi += cim(v) - old;
+    hi[j] = digits_hi(v);
+    lo[j] = digits_lo(v);
+  }
+  // output of the beat whose sample sits in slot j
+  template <bool FLAGS>
+  __device__ __forceinline__ uint32_t out(int j, const uint32_t (&ta)[T], const uint32_t (&tb)[T], uint8_t* f) const {
+    int32_t arh = dot2_0(hi[j], ta[0]), arl = dot2_0(lo[j], ta[0]);
+    int32_t aih = dot2_0(hi[j], tb[0]), ail = dot2_0(lo[j], tb[0]);
+#pragma unroll
+    for (int k = 1; k < T; ++k) {
+      const int d = (j - k + T) % T;
+      arh = dot2(hi[d], ta[k], arh);
+      arl = dot2(lo[d], ta[k], arl);
+      aih = dot2(hi[d], tb[k], aih);
+      ail = dot2(lo[d], tb[k], ail);
+    }
+    const int32_t qr = rne15_wide(arh, arl + sum_xi), qi = rne15_wide(aih, ail);
+    if constexpr (FLAGS) {
+      uint32_t fr = 0, fi = 0;
+      const int32_t yr = sat16(qr, fr), yi = sat16(qi, fi);
+      *f = (uint8_t)(fr | (fi << 2));
+      return cpack(yr, yi);
+    } else {
+      return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pk_i16(qr, qi));  // both halves clamped
+    }
+  }
+};
+
+// One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
+// FLAGS: the job has a flag buffer (otherwise the flags are not formed).
+template <int T, bool FLAGS>
 __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
   const uint32_t P = J.P, n_beats = J.n_beats;
   const uint32_t p = bx * blockDim.x + threadIdx.x;
@@ -88,54 +156,54 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
   const uint32_t m0 = z * J.chunk;
   const uint32_t m1 = min(m0 + J.chunk, n_beats);
   const size_t span = (size_t)n_beats * P;
-  const uint32_t* xc = J.x + (size_t)c * span;
+  const uint32_t* xc = J.x + (size_t)c * span + p;
   const uint32_t* hc = J.hist ? J.hist + (size_t)c * (T - 1) * P : nullptr;
-  uint32_t* yc = J.y + (size_t)c * span;
-  uint8_t* fc = J.flags ? J.flags + (size_t)c * span : nullptr;
+  uint32_t* yc = J.y + (size_t)c * span + p;
+  uint8_t* fc = J.flags ? J.flags + (size_t)c * span + p : nullptr;
 
   uint32_t ta[T], tb[T];
 #pragma unroll
   for (int k = 0; k < T; ++k) {
-    ta[k] = J.coeff[(size_t)p * T + k];        // (hr, hi)
-    tb[k] = (ta[k] >> 16) | (ta[k] << 16);     // (hi, hr)
+    const uint32_t h = J.coeff[(size_t)p * T + k];  // (hr, hi)
+    ta[k] = h ^ 0xFFFF0000u;                        // (hr, ~hi)
+    tb[k] = (h >> 16) | (h << 16);                  // (hi, hr)
   }
-  // Window slot (m - m0) mod T holds the digits of x_p[m].
-  Digits w[T];
+  // Slot (m - m0) mod T holds x_p[m]; the warm-up fills slots 1..T-1 with
+  // x_p[m0-T+1 .. m0-1], and slot 0 starts empty (its first sample replaces
+  // nothing).
+  PfbWindow<T> w;
 #pragma unroll
-  for (int k = 1; k < T; ++k) w[T - k] = split(pfb_sample(xc, hc, P, p, (int64_t)m0 - k));
+  for (int k = 0; k < T; ++k) w.hi[k] = w.lo[k] = 0u;
+#pragma unroll
+  for (int k = 1; k < T; ++k) w.put(T - k, pfb_sample(xc - p, hc, P, p, (int64_t)m0 - k));
 
-  // Raw samples of the current block of T beats; the next block's loads are
-  // issued before the current block computes.
+  // Raw samples of the current block of T beats; as soon as sample j enters
+  // the window its register is reloaded with sample j of the next block, so
+  // the load has the rest of the block to land (indices clamped into the
+  // chunk, so every load is in bounds; values past m1 are never used).
   uint32_t cur[T];
 #pragma unroll
-  for (int j = 0; j < T; ++j) cur[j] = m0 + j < m1 ? xc[(size_t)(m0 + j) * P + p] : 0u;
-  for (uint32_t mb = m0; mb < m1; mb += T) {
-    uint32_t nxt[T];
+  for (int j = 0; j < T; ++j) cur[j] = xc[(size_t)min(m0 + j, m1 - 1) * P];
+  uint32_t mb = m0;
+  for (; mb + T <= m1; mb += T) {  // full blocks: no per-beat bounds
 #pragma unroll
     for (int j = 0; j < T; ++j) {
-      const uint32_t m = mb + T + j;
-      nxt[j] = m < m1 ? xc[(size_t)m * P + p] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-      const uint32_t m = mb + j;
-      if (m >= m1) break;
-      w[j] = split(cur[j]);
-      int32_t arh = 0, arl = 0, aih = 0, ail = 0;
-#pragma unroll
-      for (int k = 0; k < T; ++k) {
-        const Digits& d = w[(j - k + T) % T];
-        arh = dot2(d.rh, ta[k], arh);
-        arl = dot2(d.rl, ta[k], arl);
-        aih = dot2(d.ih, tb[k], aih);
-        ail = dot2(d.il, tb[k], ail);
-      }
+      const size_t o = (size_t)(mb + j) * P;
+      w.put(j, cur[j]);
+      cur[j] = xc[(size_t)min(mb + T + j, m1 - 1) * P];
       uint8_t f;
-      yc[(size_t)m * P + p] = pfb_finish(arh, arl, aih, ail, &f);
-      if (fc) fc[(size_t)m * P + p] = f;
+      yc[o] = w.template out<FLAGS>(j, ta, tb, &f);
+      if (FLAGS) fc[o] = f;
     }
+  }
 #pragma unroll
-    for (int j = 0; j < T; ++j) cur[j] = nxt[j];
+  for (int j = 0; j < T; ++j) {  // the last, partial block
+    if (mb + j >= m1) break;
+    const size_t o = (size_t)(mb + j) * P;
+    w.put(j, cur[j]);
+    uint8_t f;
+    yc[o] = w.template out<FLAGS>(j, ta, tb, &f);
+    if (FLAGS) fc[o] = f;
   }
 }
 
@@ -166,30 +234,32 @@ __device__ __forceinline__ void pfb_any_tile(const PfbJob& J, uint32_t bx, uint3
   }
 }
 
-template <int T>
+template <int T, bool FLAGS>
 __device__ __forceinline__ void pfb_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
   if constexpr (T > 0)
-    pfb_reg_tile<T>(J, bx, c, z);
+    pfb_reg_tile<T, FLAGS>(J, bx, c, z);
   else
     pfb_any_tile(J, bx, c, z);
 }
 
 // Single CPI: grid (nbx, n_chan, nz). T = 0 selects the generic kernel.
-template <int T>
-__global__ __launch_bounds__(256) void pfb_kernel(PfbJob J) {
-  pfb_tile<T>(J, blockIdx.x, blockIdx.y, blockIdx.z);
+template <int T, bool FLAGS>
+__global__ __launch_bounds__(256, FLAGS ? 1 : 4) void pfb_kernel(PfbJob J) {
+  pfb_tile<T, FLAGS>(J, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Batched: one block per tile of a class of CPIs (block -> job via the tile table).
-template <int T>
-__global__ __launch_bounds__(256) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
-                                                        const uint32_t* __restrict__ tile_job, uint32_t n_tiles) {
+// Batched: one block per tile of a class of CPIs (block -> job via the tile
+// table); a class holds CPIs that all keep, or all drop, the PFB flags.
+template <int T, bool FLAGS>
+__global__ __launch_bounds__(256, FLAGS ? 1 : 4) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
+                                                                       const uint32_t* __restrict__ tile_job,
+                                                                       uint32_t n_tiles) {
   const uint32_t t = blockIdx.x;
   if (t >= n_tiles) return;
   const PfbJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
   const uint32_t local = t - J.tile0;
   const uint32_t bx = local % J.nbx, rest = local / J.nbx;
-  pfb_tile<T>(J, bx, rest % J.n_chan, rest / J.n_chan);
+  pfb_tile<T, FLAGS>(J, bx, rest % J.n_chan, rest / J.n_chan);
 }
 
 // Final delay-line state after n_beats: hist[k-1] = x_p[n_beats - k], k = 1..T-1.
@@ -212,12 +282,13 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
                       uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags) {
   if (n_beats == 0 || n_chan == 0) return hipSuccess;
   const uint32_t bx = (P + 255) / 256;
-  // Beat chunks: about 4096 waves in flight (16 per CU at the register budget of
-  // the T = 16 kernel) so the dependent load/compute chains of different chunks
-  // overlap; a chunk never gets shorter than T beats, which bounds the warm-up
-  // re-reads (T - 1 beats per chunk, mostly L2 hits) to under 2x.
+  // Beat chunks: about 2048 waves (8 per CU), so a chunk is usually a few
+  // blocks of T beats and the reload of the next block overlaps the current
+  // one (measured best between 1024 and 4096 waves on the medium workload); a
+  // chunk never gets shorter than T beats, which bounds the warm-up re-reads
+  // (T - 1 beats per chunk, mostly L2 hits) to under 2x.
   const uint64_t lanes = (uint64_t)P * n_chan;
-  const uint64_t want = (262144 + lanes - 1) / lanes;
+  const uint64_t want = (131072 + lanes - 1) / lanes;
   uint32_t chunk = (uint32_t)((n_beats + want - 1) / want);
   if (chunk < T) chunk = T;
   if (chunk > n_beats) chunk = n_beats;
@@ -225,11 +296,20 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
   const PfbJob J{x, coeff, hist, y, flags, P, T, n_chan, n_beats, chunk, bx, nz, 0};
   const dim3 grid(bx, n_chan, nz);
   const dim3 block(P < 256 ? ((P + 63) / 64) * 64 : 256);
-  switch (T) {
-    case 4: pfb_kernel<4><<<grid, block, 0, s>>>(J); break;
-    case 8: pfb_kernel<8><<<grid, block, 0, s>>>(J); break;
-    case 16: pfb_kernel<16><<<grid, block, 0, s>>>(J); break;
-    default: pfb_kernel<0><<<grid, block, 0, s>>>(J); break;
+  if (flags) {
+    switch (T) {
+      case 4: pfb_kernel<4, true><<<grid, block, 0, s>>>(J); break;
+      case 8: pfb_kernel<8, true><<<grid, block, 0, s>>>(J); break;
+      case 16: pfb_kernel<16, true><<<grid, block, 0, s>>>(J); break;
+      default: pfb_kernel<0, true><<<grid, block, 0, s>>>(J); break;
+    }
+  } else {
+    switch (T) {
+      case 4: pfb_kernel<4, false><<<grid, block, 0, s>>>(J); break;
+      case 8: pfb_kernel<8, false><<<grid, block, 0, s>>>(J); break;
+      case 16: pfb_kernel<16, false><<<grid, block, 0, s>>>(J); break;
+      default: pfb_kernel<0, false><<<grid, block, 0, s>>>(J); break;
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -240,15 +320,23 @@ hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, u
   return e;
 }
 
-hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, const ClassLaunch& L) {
-  if (L.n_tiles == 0) return hipSuccess;
+template <bool FLAGS>
+static void launch_pfb_batch_f(hipStream_t s, uint32_t T, const ClassLaunch& L) {
   const PfbJob* jobs = static_cast<const PfbJob*>(L.jobs);
   switch (T) {
-    case 4: pfb_batch_kernel<4><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
-    case 8: pfb_batch_kernel<8><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
-    case 16: pfb_batch_kernel<16><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
-    default: pfb_batch_kernel<0><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    case 4: pfb_batch_kernel<4, FLAGS><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    case 8: pfb_batch_kernel<8, FLAGS><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    case 16: pfb_batch_kernel<16, FLAGS><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
+    default: pfb_batch_kernel<0, FLAGS><<<L.grid, 256, 0, s>>>(jobs, L.tile_job, L.n_tiles); break;
   }
+}
+
+hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, bool flags, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (flags)
+    launch_pfb_batch_f<true>(s, T, L);
+  else
+    launch_pfb_batch_f<false>(s, T, L);
   return hipGetLastError();
 }
 
@@ -346,13 +434,6 @@ __device__ __forceinline__ Cv neg_j(Cv c, StageFlags& f) {
 // |w.re| + |w.im| <= 65534, so every partial sum fits in int32.
 __device__ __forceinline__ uint32_t tw_conj_compl(uint32_t w) { return w ^ 0xFFFF0000u; }
 __device__ __forceinline__ uint32_t tw_swap(uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16); }
-// dot2(a, b, 0) in the three-operand encoding (the builtin's two-operand
-// accumulate form costs a register clear per product).
-__device__ __forceinline__ int32_t dot2_0(uint32_t a, uint32_t b) {
-  int32_t r;
-  asm("v_dot2_i32_i16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
-  return r;
-}
 __device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t wc, uint32_t ws, StageFlags& f) {
   const uint32_t xp = pk(x);
   const int32_t re = dot2(xp, wc, x.im);

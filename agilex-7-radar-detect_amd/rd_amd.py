@@ -256,14 +256,15 @@ class Ctx:
         return DevBuf(self, a.nbytes).upload(a)
 
     # ---- per-stage host-array conveniences (upload, run, download) ----------
-    def pfb(self, x, coeff, P, T, n_chan, n_beats, hist=None):
+    def pfb(self, x, coeff, P, T, n_chan, n_beats, hist=None, flags=True):
+        """PFB through rd_pfb_run -> (y, flag bytes or None when flags=False)."""
         S = n_chan * n_beats * P
         dx, dc = self.put(np.asarray(x, np.uint32)), self.put(np.asarray(coeff, np.uint32))
-        dy, df = self.buf(4 * S), self.buf(S)
+        dy, df = self.buf(4 * S), (self.buf(S) if flags else None)
         dh = self.put(np.asarray(hist, np.uint32)) if hist is not None else None
         self.lib.check("rd_pfb_run", self.L.rd_pfb_run(self.h, dx.ptr, dc.ptr, P, T, n_chan, n_beats,
-                                                       dh.ptr if dh else None, dy.ptr, df.ptr))
-        y, f = dy.download(np.uint32, S), df.download(np.uint8, S)
+                                                       dh.ptr if dh else None, dy.ptr, df.ptr if df else None))
+        y, f = dy.download(np.uint32, S), (df.download(np.uint8, S) if df else None)
         if hist is not None:
             hist[...] = dh.download(np.uint32, hist.size)
         return y, f

@@ -28,6 +28,7 @@ def test_pfb_golden(gpu, set_name, geom):
     _eq(y, v["y"], "y")
     _eq(fl & 3, v["f_re"], "f_re")
     _eq(fl >> 2, v["f_im"], "f_im")
+    _eq(gpu.pfb(v["x"], coeff, P, T, 1, v["beats"], flags=False)[0], v["y"], "y without the flag buffer")
 
 
 @pytest.mark.parametrize("P,T,nb,ch", [(64, 8, 16, 4), (1024, 16, 40, 3), (4096, 16, 20, 2), (256, 4, 33, 2),
@@ -40,6 +41,7 @@ def test_pfb_random_and_extreme(gpu, orc, P, T, nb, ch):
         yo, fo = orc.pfb(x, h, P, T, ch, nb)
         _eq(yg, yo, "y")
         _eq(fg, fo, "flags")
+        _eq(gpu.pfb(x, h, P, T, ch, nb, flags=False)[0], yo, "y without the flag buffer")
 
 
 def test_pfb_long_run_chunks(gpu, orc):
