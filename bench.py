@@ -38,6 +38,8 @@ import rd_amd  # noqa: E402
 import rd_scenario  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024  # 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9
 
 
 def note(msg: str) -> None:
@@ -118,6 +120,101 @@ def cpu_baseline(sc, iq, taps, W, budget_s: float):
     return dict(value=round(n * sc.samples / dt / 1e6, 3), unit="Msamples/s", cores=threads, kind=kind,
                 sample=f"{n} CPI(s) of the {sc.name} workload ({sc.samples} samples each), full chain, "
                        f"{threads} threads, {dt:.1f} s"), out
+
+
+# ----------------------------------------------------------------------------- large leg
+def profiled(stage: str, workload: str):
+    """HBM bytes and VALU wave-instructions per dispatch of a stage kernel, from
+    the committed rocprofv3 counter summary of that workload (None when absent)."""
+    path = ROOT / "profiles" / f"chain_{workload}_hbm_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 8 and cells[0] == stage:
+            try:
+                return dict(hbm_bytes=int(cells[5]), valu_insts=int(cells[7]))
+            except ValueError:
+                return None
+    return None
+
+
+def valu_frac(valu_insts: int, ms: float) -> float:
+    """Fraction of the VALU issue rate a launch used: wave instructions x 4
+    cycles over 1024 SIMDs x 2.4 GHz x the launch time."""
+    return round(valu_insts * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * ms * 1e-3), 4)
+
+
+def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
+    """The large workload (64 ch x 4096-pt x 16 taps x 512 frames, 64 beams):
+    graph replay over two device-generated CPIs, per-stage times with the
+    beamformer and covariance MFMA rates, and one CPI checked against the
+    oracle on the host's cores."""
+    sc = rd_scenario.config("large")
+    taps, W = sc.taps(), sc.weights()
+    ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
+                      keep_y=False, keep_flags=False, keep_R=True)
+    note("large: generating 2 CPIs on the device")
+    ring = [device_iq(torch, sc, 7001 + i) for i in range(2)]
+    iq0 = ring[0].cpu().numpy().view(np.uint32)
+    ch.load(iq0, taps, W)
+    totals = ctx.buf(16)
+    graphs = [ch.graph(iq_ptr=ring[i].data_ptr(), total_ptr=totals.ptr + 8 * i, lean=True) for i in range(2)]
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for i in range(2):
+        graphs[i].launch()
+    ctx.sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(steps):
+        graphs[i % 2].launch()
+    e1.record(stream)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / steps
+    note("large: per-stage kernel timing")
+    per = {}
+    for name, (fn, nbytes) in stage_kernels(ctx, ch, sc).items():
+        t = time_stage(torch, stream, fn, stage_reps)
+        per[name] = dict(ms=round(t, 4), algo_bytes=nbytes, gbs=round(nbytes / (t * 1e-3) / 1e9, 1))
+    pairs = sc.CH * (sc.CH + 1) // 2
+    cols = sc.N * sc.F
+    # int8 MFMA work as issued: 4 digit products x (re, im) x 2 ops per MAC
+    mfma = {"bf": 32 * sc.B * sc.CH * cols, "cov": 32 * pairs * sc.F * sc.N}
+    for k, ops in mfma.items():
+        per[k]["mfma_int8_tops"] = round(ops / (per[k]["ms"] * 1e-3) / 1e12, 1)
+        per[k]["mfma_frac_of_5pops"] = round(ops / (per[k]["ms"] * 1e-3) / 5e15, 4)
+    for g in graphs:
+        g.close()
+    res = dict(workload=f"large: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, {sc.B} beams, "
+                        f"CA-CFAR {sc.ref}/{sc.guard}",
+               cpi_samples=sc.samples, steps=steps, ms_per_cpi=round(ms, 4),
+               msamples_s=round(sc.samples / (ms * 1e-3) / 1e6, 1), stages=per, mismatches=None)
+    if verify:
+        import oracle
+
+        note("large: checking one CPI against the oracle on the host")
+        ch.enqueue()
+        ctx.sync()
+        n_ev = int(ch.total.download(np.uint64, 1)[0])
+        got = ch.fetch(("power", "R"), n_events=n_ev)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        ref = oracle.load("oracle").chain(iq0, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(),
+                                          rd_scenario.MAX_GUARD, rd_scenario.MAX_REF, keep=("power", "R", "events"),
+                                          threads=threads)
+        cpu_s = time.perf_counter() - t0
+        mism = int(np.count_nonzero(got["power"] != ref["power"]) + np.count_nonzero(got["R"] != ref["R"]))
+        if n_ev != ref["n_events"]:
+            mism += abs(n_ev - ref["n_events"]) + 1
+        else:
+            mism += int(np.count_nonzero(got["events"] != ref["events"]))
+        res.update(mismatches=mism, events_per_cpi=n_ev,
+                   oracle_cpu=dict(seconds=round(cpu_s, 2), threads=threads,
+                                   msamples_s=round(sc.samples / cpu_s / 1e6, 2), kind="port"))
+    del ring
+    return res
 
 
 # ----------------------------------------------------------------------------- batched leg
@@ -531,6 +628,8 @@ def main():
     ap.add_argument("--shadow-cpis", type=int, default=64, help="CPIs of the FP32 shadow leg (0: skip it)")
     ap.add_argument("--shadow-steps", type=int, default=20)
     ap.add_argument("--shadow-only", action="store_true", help="run only the FP32 shadow leg")
+    ap.add_argument("--large-steps", type=int, default=10, help="graph replays of the large-workload leg (0: skip it)")
+    ap.add_argument("--large-verify", type=int, default=1, help="check one large CPI against the oracle (0: skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
@@ -649,6 +748,7 @@ def main():
         per[name] = dict(ms=round(ms, 5), algo_bytes=nbytes, gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
     dom = max(per, key=lambda k: per[k]["ms"])
     ach = per[dom]["gbs"]
+    prof = profiled(dom, sc.name)
     chain_bytes = algo_chain_bytes(sc, n_ev)
     out = {
         "metric": "bit-exact Msamples/s PFB->CFAR (plain GPU, one CPI per graph launch)",
@@ -665,12 +765,13 @@ def main():
         "data": "synthetic (seeded complex Gaussian noise + plane-wave targets, ring of distinct CPIs)",
         "config": {"workload": f"{sc.name}: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, "
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
-                   "ring": args.ring, "impl": "one hipGraph per CPI: PFB, FFT, align, BF+power MFMA, cov MFMA, "
-                                              "CFAR count/scan/emit",
+                   "ring": args.ring, "impl": "one hipGraph per CPI: PFB, FFT, BF+power MFMA (writes the aligned "
+                                              "copy), cov MFMA, CFAR rows/scan/gather",
                    "outputs": "events + covariance timed; every stage output verified bit-exact on a separate "
                               "CPI each run (timing.graph_full_outputs also writes them all)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": prof["hbm_bytes"] if prof else None,
+                     "valu_frac": valu_frac(prof["valu_insts"], per[dom]["ms"]) if prof else None},
         "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                   "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "timing": {"graph": {"wall_ms": round(ms_per_step, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
@@ -704,6 +805,9 @@ def main():
     if args.shadow_cpis:
         torch.cuda.empty_cache()
         out["shadow"] = shadow_leg(torch, ctx, args.shadow_cpis, args.shadow_steps, 2)
+    if args.large_steps and sc.name != "large":
+        torch.cuda.empty_cache()
+        out["large"] = large_leg(torch, ctx, args.large_steps, bool(args.large_verify), max(2, args.stage_reps // 10))
     print(json.dumps(out), flush=True)
     for g in graphs + graphs_full:
         g.close()
