@@ -248,7 +248,7 @@ Shape default_shape(const rd_batch& B, const Class& c) {
       break;
     case S_COV: {
       uint32_t min_steps = ~0u;
-      for (uint32_t i : c.cpis) min_steps = std::min(min_steps, (B.d[i].n_frames + 15) / 16);
+      for (uint32_t i : c.cpis) min_steps = std::min(min_steps, (B.d[i].n_frames + 31) / 32);  // 32-frame steps
       s.a = std::max(1u, std::min(rd::cov_max_splits(c.key), min_steps));
       break;
     }

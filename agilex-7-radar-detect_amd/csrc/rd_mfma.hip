@@ -422,16 +422,23 @@ hipError_t launch_bf_batch(hipStream_t s, uint32_t steps, uint32_t tc, uint32_t 
 // per bin over K = 2F (frames, real then imaginary part):
 //   Re: A_i = [Xr_i | Xi_i],  B_j = [Xr_j | Xi_j]
 //   Im: A_i = [Xi_i | ~Xr_i], B_j = [Xr_j | Xi_j], plus sum_m xi_j
-// A block owns NB bins (NB * CHp = 64, so one frame row of the block is
-// NB*CH*4 contiguous bytes of Xa) and their upper-triangle 16x16 channel tiles
-// ("jobs"). Each wave runs one job over one of FS frame splits; frames are
-// staged through LDS 16 at a time per split. Every wave turns its partial
-// sums into partial int64 results (all correction terms are linear in the
-// frames), and the splits of a job are added with LDS 64-bit atomics.
+// A block owns NB bins (NB * CHp = 64 channel rows) and their upper-triangle
+// 16x16 channel tiles ("jobs"); each wave runs one job over one of FS frame
+// splits. A split walks its frames 32 at a time: the block's threads load the
+// 64 rows x 32 frames of a step two steps ahead (coalesced along the rows,
+// two register sets in flight) and write each
+// sample's int8 digits once into six LDS digit planes, [plane][row][frame]:
+//   hr, lr, hi, li (digits of the real and imaginary part, low digits offset),
+//   ~hr, ~lr       (the complemented real digits of the Im GEMM's A operand),
+// so every MFMA operand is one 8-byte LDS read of 8 consecutive frames.
+// Padded frames and channel rows hold zero bytes. Every wave turns its
+// partial sums into partial int64 results (all correction terms are linear in
+// the frames), and the splits of a job are added with LDS 64-bit atomics.
 // ============================================================================
-// Per-thread share of one step's staged tiles: 16 frames x nb*CH words per
-// split over 64 threads per job and split = CH / (4 * upper tiles) <= 4 words.
-constexpr uint32_t kCovWords = 4;
+constexpr uint32_t kCovFS = 32;                // frames per step
+constexpr uint32_t kCovRow = kCovFS + 8;       // bytes per plane row (padding spreads the banks)
+constexpr uint32_t kCovPlane = 64 * kCovRow;   // bytes per plane
+constexpr uint32_t kCovSplitBytes = 6 * kCovPlane + 64 * 2 * 4;  // planes + per-row frame sums
 
 // One tile: bins [tile*nb, +nb) of job J. The caller issues a block barrier
 // before each tile (the LDS of the previous tile is reused).
@@ -439,16 +446,18 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
                                          uint32_t* smem) {
   const uint32_t* __restrict__ Xa = J.Xa;
   const uint32_t CH = J.CH, F = J.F, N = J.N;
+  const uint32_t chp = 64 / nb;
   const uint32_t ut = nt * (nt + 1) / 2;
   const uint32_t jobs = nb * ut;
-  const uint32_t row_words = nb * CH;
-  uint32_t* tiles = smem;                                                      // [fs_n][16][row_words]
-  unsigned long long* red = reinterpret_cast<unsigned long long*>(smem + fs_n * 16 * 64);  // [jobs][64][8]
+  unsigned char* lds = reinterpret_cast<unsigned char*>(smem);
+  unsigned long long* red = reinterpret_cast<unsigned long long*>(lds + (size_t)fs_n * kCovSplitBytes);  // [jobs][64][8]
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const uint32_t job = wave % jobs, fs = wave / jobs;
   const uint32_t k_base = tile * nb;
   const uint32_t bins_here = min(nb, N - k_base);
+  unsigned char* planes = lds + (size_t)fs * kCovSplitBytes;
+  int32_t* sums = reinterpret_cast<int32_t*>(planes + 6 * kCovPlane);  // [64 rows][2]: sum xr, sum xi
 
   const uint32_t bin = job / ut;
   uint32_t u = job % ut, ti = 0;
@@ -458,106 +467,126 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
   }
   const uint32_t tj = ti + u;
   const bool live = bin < bins_here;
-  const uint32_t ci = 16 * ti + cl, cj = 16 * tj + cl;
+  const uint32_t ra = bin * chp + 16 * ti + cl, rb = bin * chp + 16 * tj + cl;  // operand rows of this lane
 
   if (fs_n > 1)
     for (uint32_t i = threadIdx.x; i < jobs * 64 * 8; i += blockDim.x) red[i] = 0;
+  for (uint32_t i = threadIdx.x; i < fs_n * 128; i += blockDim.x)
+    reinterpret_cast<int32_t*>(lds + (size_t)(i >> 7) * kCovSplitBytes + 6 * kCovPlane)[i & 127] = 0;
 
-  const uint32_t nsteps = (F + 15) / 16;
+  const uint32_t nsteps = (F + kCovFS - 1) / kCovFS;
   const uint32_t sps = (nsteps + fs_n - 1) / fs_n;  // steps per split
   v4i acc[6];
   for (int z = 0; z < 6; ++z) acc[z] = v4i{0, 0, 0, 0};
-  int32_t sra = 0, sia = 0, srb = 0, sib = 0, nfr = 0;
+  int32_t nfr = 0;
 
-  // The next step's tile words are loaded into registers while the current step
-  // computes (each thread owns at most kCovWords words of a step's tiles).
-  const uint32_t step_words = fs_n * 16 * row_words;
-  uint32_t nxt[kCovWords];
-  auto fetch = [&](uint32_t st) {
+  // Conversion tasks of a step: (row = lane, group g of 4 frames), g = 0..7,
+  // spread over the split's waves; at most two per thread (jobs >= 4). Two
+  // register sets hold the next two steps' words.
+  constexpr uint32_t kTasks = 64 * (kCovFS / 4), kMaxT = 2;
+  const uint32_t tsplit = (wave % jobs) * 64 + lane;  // thread index within the split
+  const uint32_t stride = 64 * jobs;
+  const uint32_t ch = lane % chp, rbin = lane / chp;
+  const bool row_ok = ch < CH && rbin < bins_here;
+  int32_t srow_r = 0, srow_i = 0;  // this thread's frame sums of its row (its tasks)
+  uint32_t buf0[kMaxT][4], buf1[kMaxT][4];
+  auto fetch = [&](uint32_t st, uint32_t (&nxt)[kMaxT][4]) {
 #pragma unroll
-    for (uint32_t q = 0; q < kCovWords; ++q) {
-      const uint32_t idx = threadIdx.x + q * blockDim.x;
-      uint32_t v = 0;
-      if (idx < step_words) {
-        const uint32_t sp = idx / (16 * row_words), rem = idx % (16 * row_words);
-        const uint32_t f = rem / row_words, w = rem % row_words;
-        const uint32_t m = (sp * sps + st) * 16 + f;
-        if (m < F && w / CH < bins_here) v = Xa[((size_t)m * N + k_base) * CH + w];
+    for (uint32_t q = 0; q < kMaxT; ++q) {
+      const uint32_t task = tsplit + q * stride, g = task >> 6;
+#pragma unroll
+      for (uint32_t t = 0; t < 4; ++t) {
+        const uint32_t m = (fs * sps + st) * kCovFS + 4 * g + t;
+        nxt[q][t] = (task < kTasks && row_ok && st < sps && m < F) ? Xa[((size_t)m * N + k_base + rbin) * CH + ch]
+                                                                     : 0u;
       }
-      nxt[q] = v;
     }
   };
-  fetch(0);
-  for (uint32_t st = 0; st < sps; ++st) {
-    __syncthreads();
+  fetch(0, buf0);
+  fetch(1, buf1);
+  // one step from the register set `cur` (even steps buf0, odd steps buf1, so
+  // the sets stay in fixed registers)
+  auto step = [&](uint32_t st, uint32_t (&cur)[kMaxT][4]) {
+    const uint32_t m0 = (fs * sps + st) * kCovFS;
+    __syncthreads();  // the previous step's planes are consumed
 #pragma unroll
-    for (uint32_t q = 0; q < kCovWords; ++q) {
-      const uint32_t idx = threadIdx.x + q * blockDim.x;
-      if (idx < step_words) tiles[idx] = nxt[q];
-    }
-    __syncthreads();
-    if (st + 1 < sps) fetch(st + 1);
-    const uint32_t m0 = (fs * sps + st) * 16;
-    const uint32_t f0 = 8 * (grp & 1);
-    uint64_t fmask = 0;
+    for (uint32_t q = 0; q < kMaxT; ++q) {
+      const uint32_t task = tsplit + q * stride, g = task >> 6;
+      if (task < kTasks) {
+        const uint32_t* w = cur[q];
+        uint32_t bm = 0;  // byte mask of the valid frames among the four
 #pragma unroll
-    for (int t = 0; t < 8; ++t) fmask |= (m0 + f0 + t < F) ? (0xFFull << (8 * t)) : 0ull;
-    if (m0 < F) nfr += (int32_t)min(16u, F - m0);
-    const bool imag = (grp >> 1) != 0;
-    const uint32_t* tl = tiles + fs * 16 * row_words;
-    uint32_t pa[8], pb[8];
+        for (uint32_t t = 0; t < 4; ++t) bm |= (row_ok && m0 + 4 * g + t < F) ? (0xFFu << (8 * t)) : 0u;
+        const uint32_t hr = gather4(w[0], w[1], w[2], w[3], 1) & bm;
+        const uint32_t lr = (gather4(w[0], w[1], w[2], w[3], 0) ^ 0x80808080u) & bm;
+        const uint32_t hi = gather4(w[0], w[1], w[2], w[3], 3) & bm;
+        const uint32_t li = (gather4(w[0], w[1], w[2], w[3], 2) ^ 0x80808080u) & bm;
+        unsigned char* o = planes + lane * kCovRow + 4 * g;
+        *reinterpret_cast<uint32_t*>(o) = hr;
+        *reinterpret_cast<uint32_t*>(o + kCovPlane) = lr;
+        *reinterpret_cast<uint32_t*>(o + 2 * kCovPlane) = hi;
+        *reinterpret_cast<uint32_t*>(o + 3 * kCovPlane) = li;
+        *reinterpret_cast<uint32_t*>(o + 4 * kCovPlane) = ~hr & bm;
+        *reinterpret_cast<uint32_t*>(o + 5 * kCovPlane) = ~lr & bm;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const uint32_t base = (f0 + t) * row_words + bin * CH;
-      pa[t] = (live && ci < CH) ? tl[base + ci] : 0u;
-      pb[t] = (live && cj < CH) ? tl[base + cj] : 0u;
-    }
-    const uint64_t amask = ci < CH ? fmask : 0ull;
-    const Dig8 da = digits8(pa, amask);
-    const Dig8 db = digits8(pb, cj < CH ? fmask : 0ull);
-    if (!imag) {
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        sra += cre(pa[t]);
-        sia += cim(pa[t]);
-        srb += cre(pb[t]);
-        sib += cim(pb[t]);
+        for (uint32_t t = 0; t < 4; ++t) {  // padded words are zero
+          srow_r += cre(w[t]);
+          srow_i += cim(w[t]);
+        }
       }
     }
-    // Re: A = [Xr | Xi], B = [Xr | Xi];  Im: A = [Xi | ~Xr]
-    const uint64_t reah = imag ? da.hi : da.hr, real_ = imag ? da.li : da.lr;
-    const uint64_t imah = imag ? (~da.hr & amask) : da.hi, imal = imag ? (~da.lr & amask) : da.li;
-    const uint64_t bh = imag ? db.hi : db.hr, bl = imag ? db.li : db.lr;
-    acc[0] = mfma8(reah, bh, acc[0]);
-    acc[1] = mfma8(reah, bl, acc[1]);
-    acc[1] = mfma8(real_, bh, acc[1]);
-    acc[2] = mfma8(real_, bl, acc[2]);
-    acc[3] = mfma8(imah, bh, acc[3]);
-    acc[4] = mfma8(imah, bl, acc[4]);
-    acc[4] = mfma8(imal, bh, acc[4]);
-    acc[5] = mfma8(imal, bl, acc[5]);
+    __syncthreads();
+    fetch(st + 2, cur);  // the set just converted takes the step after next
+    if (m0 < F) nfr += (int32_t)min(kCovFS, F - m0);
+    // K-steps of 32 (16 frames, real half in lane groups 0-1, imaginary in 2-3)
+#pragma unroll
+    for (uint32_t ks = 0; ks < kCovFS / 16; ++ks) {
+      const uint32_t fb = 16 * ks + 8 * (grp & 1);
+      const bool imag = (grp >> 1) != 0;
+      const unsigned char* pa = planes + ra * kCovRow + fb;
+      const unsigned char* pb = planes + rb * kCovRow + fb;
+      // Re: A = [Xr | Xi], B = [Xr | Xi];  Im: A = [Xi | ~Xr]
+      const uint32_t pre = imag ? 2 : 0, pim = imag ? 4 : 2;
+      const uint64_t reah = *reinterpret_cast<const uint64_t*>(pa + pre * kCovPlane);
+      const uint64_t real_ = *reinterpret_cast<const uint64_t*>(pa + (pre + 1) * kCovPlane);
+      const uint64_t imah = *reinterpret_cast<const uint64_t*>(pa + pim * kCovPlane);
+      const uint64_t imal = *reinterpret_cast<const uint64_t*>(pa + (pim + 1) * kCovPlane);
+      const uint64_t bh = *reinterpret_cast<const uint64_t*>(pb + pre * kCovPlane);
+      const uint64_t bl = *reinterpret_cast<const uint64_t*>(pb + (pre + 1) * kCovPlane);
+      acc[0] = mfma8(reah, bh, acc[0]);
+      acc[1] = mfma8(reah, bl, acc[1]);
+      acc[1] = mfma8(real_, bh, acc[1]);
+      acc[2] = mfma8(real_, bl, acc[2]);
+      acc[3] = mfma8(imah, bh, acc[3]);
+      acc[4] = mfma8(imah, bl, acc[4]);
+      acc[4] = mfma8(imal, bh, acc[4]);
+      acc[5] = mfma8(imal, bl, acc[5]);
+    }
+  };
+  for (uint32_t st = 0; st < sps; st += 2) {
+    step(st, buf0);
+    if (st + 1 < sps) step(st + 1, buf1);
   }
 
-  // per-channel frame sums of this split: lanes of groups 0/1 hold the partials
-  int32_t a_r = sra + __shfl_xor(sra, 16), a_i = sia + __shfl_xor(sia, 16);
-  int32_t b_r = srb + __shfl_xor(srb, 16), b_i = sib + __shfl_xor(sib, 16);
-  a_r += __shfl_xor(a_r, 32);
-  a_i += __shfl_xor(a_i, 32);
-  b_r += __shfl_xor(b_r, 32);
-  b_i += __shfl_xor(b_i, 32);
+  // per-row frame sums of this split (the split's threads sharing a row add up)
+  if (srow_r | srow_i) {
+    atomicAdd(&sums[2 * lane], srow_r);
+    atomicAdd(&sums[2 * lane + 1], srow_i);
+  }
+  __syncthreads();
+  const int32_t b_r = sums[2 * rb], b_i = sums[2 * rb + 1];
   const int64_t kconst = 16384LL * 2 * nfr;
   const int64_t tjs = (int64_t)b_r + b_i;
   int64_t part[8];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const uint32_t il = 4 * grp + r;
-    const int32_t ir = __shfl(a_r, il), ii = __shfl(a_i, il);
+    const uint32_t il = bin * chp + 16 * ti + 4 * grp + r;
+    const int32_t ir = sums[2 * il], ii = sums[2 * il + 1];
     const int64_t tis = (int64_t)ir + ii, uis = (int64_t)ii - ir - (int64_t)nfr;
     part[2 * r] = combine(acc[0], acc[1], acc[2], r) + 128 * (tis + tjs) - kconst;
     part[2 * r + 1] = combine(acc[3], acc[4], acc[5], r) + 128 * (uis + tjs) - kconst + b_i;
   }
   if (fs_n > 1) {
-    __syncthreads();
     if (wave < jobs * fs_n)
 #pragma unroll
       for (int q = 0; q < 8; ++q) atomicAdd(&red[(job * 64 + lane) * 8 + q], (unsigned long long)part[q]);
@@ -567,6 +596,7 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
   }
   if (fs != 0 || !live) return;
   const uint32_t k = k_base + bin;
+  const uint32_t cj = 16 * tj + cl;
   const size_t npairs = (size_t)CH * (CH + 1) / 2;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -600,7 +630,9 @@ __global__ __launch_bounds__(1024) void cov_batch_kernel(const CovJob* __restric
 struct CovGeo {
   uint32_t nb, nt, jobs;
   explicit CovGeo(uint32_t chp) : nb(64 / chp), nt(chp / 16), jobs((64 / chp) * (chp / 16) * (chp / 16 + 1) / 2) {}
-  size_t lds(uint32_t fs_n) const { return (size_t)fs_n * 16 * 64 * 4 + (fs_n > 1 ? (size_t)jobs * 64 * 8 * 8 : 0); }
+  size_t lds(uint32_t fs_n) const {
+    return (size_t)fs_n * kCovSplitBytes + (fs_n > 1 ? (size_t)jobs * 64 * 8 * 8 : 0);
+  }
 };
 
 uint32_t cov_bins_per_tile(uint32_t chp) { return CovGeo(chp).nb; }
@@ -612,7 +644,7 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
   const uint32_t chp = (n_chan + 15) / 16 * 16;
   const CovGeo G(chp);
   uint32_t fs_n = 16 / G.jobs;
-  const uint32_t nsteps = (n_frames + 15) / 16;
+  const uint32_t nsteps = (n_frames + kCovFS - 1) / kCovFS;
   if (fs_n > nsteps) fs_n = nsteps;
   if (fs_n < 1) fs_n = 1;
   const CovJob J{Xa, R, n_chan, n_frames, n_bins, 0};
