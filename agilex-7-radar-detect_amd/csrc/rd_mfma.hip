@@ -307,7 +307,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
 // Single CPI: a grid stride over the column tiles, the weight digits derived
 // once per block.
 template <int STEPS, int TC, bool CHMAJOR, int OUT>
-__global__ __launch_bounds__(256) void bf_kernel(BfJob J) {
+__global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_kernel(BfJob J) {
   extern __shared__ uint32_t lds[];
   uint32_t* frag = lds;
   int32_t* tail = reinterpret_cast<int32_t*>(lds + bf_frag_words(J.bt_n, STEPS));
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void bf_kernel(BfJob J) {
 // Batched: a persistent grid over the column tiles of a class of CPIs (same
 // step count); a block re-derives the weight digits when its CPI changes.
 template <int STEPS, int TC, int OUT>
-__global__ __launch_bounds__(256) void bf_batch_kernel(const BfJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_batch_kernel(const BfJob* __restrict__ jobs,
                                                        const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
                                                        uint32_t max_bt) {
   extern __shared__ uint32_t lds[];
