@@ -58,7 +58,8 @@ CfarGeom cfar_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
 // S entries keeps the strided segment writes of the prefix pass free of bank
 // conflicts, and puts a lane's S entries at fixed offsets from one base.
 // LDS per row (u64 words): P, then the per-wave statistics [W][4] = {prefix
-// total, events, detections, -}.
+// total, events, detections, -}, then per wave a list of its emitting bins
+// (64 * S u32 entries).
 // ----------------------------------------------------------------------------
 struct CfarRowGeo {
   uint32_t W, slog, words;
@@ -79,7 +80,7 @@ __host__ __device__ inline CfarRowGeo cfar_row_geo(uint32_t n, uint32_t W) {
   G.slog = 0;
   while ((1u << G.slog) < per) ++G.slog;
   const uint32_t pmax = n + (G.slog ? (n >> G.slog) : 0);
-  G.words = pmax + 1 + W * 4;
+  G.words = pmax + 1 + W * 4 + (W * (64u << G.slog) + 1) / 2;
   return G;
 }
 
@@ -149,6 +150,16 @@ __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32
   d.eval = ok && g.enable && g.refs_ok;
   d.hit = d.eval && ((d.cut * (uint64_t)d.n) << 8) > (uint64_t)g.alpha * d.sum;
   return d;
+}
+
+// CA decision of a bin whose whole window lies inside the row (the chunk
+// test below guarantees it): no clamping or masking, only the hit.
+template <uint32_t SLOG>
+__device__ __forceinline__ bool ca_hit_inner(const uint64_t* P, uint32_t i, const CfarGeom& g, uint32_t n_ref) {
+  const uint64_t cut = P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)];
+  const uint64_t sum = (P[pidx<SLOG>(i + g.gl + g.rl + 1)] - P[pidx<SLOG>(i + g.gl + 1)]) +
+                       (P[pidx<SLOG>(i - g.gg)] - P[pidx<SLOG>(i - g.gg - g.rg)]);
+  return ((cut * (uint64_t)n_ref) << 8) > (uint64_t)g.alpha * sum;
 }
 
 // Bins of a row of n that are not evaluated (window does not fit, or the
@@ -229,20 +240,30 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
   }
   row_sync<MULTI>();
 
-  // decisions, first pass: which bins emit (chunk masks) and the counts
+  // decisions, first pass: which bins emit (chunk masks) and the counts. A
+  // CA chunk whose 64 bins all have their whole window inside the row takes
+  // the unclamped form.
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t c0 = wr * 64 * S;
+  const bool ca = g.enable && g.refs_ok && !g.go;
+  const uint32_t n_ref = g.rl + g.rg;
   uint64_t em[S];
   uint32_t cnt = 0, det = 0;
 #pragma unroll
   for (uint32_t q = 0; q < S; ++q) {
-    const uint32_t i = c0 + q * 64 + lane;
+    const uint32_t cq = c0 + q * 64, i = cq + lane;
     const bool in = work && (full || i < n);
-    Decision d{};
-    if (in) d = decide<SLOG>(P, n, i, g);
-    em[q] = __ballot(in && (d.hit || g.dense));
+    bool hit;
+    if (work && ca && cq >= g.gg + g.rg && (uint64_t)cq + 64 + g.gl + g.rl <= n) {
+      hit = ca_hit_inner<SLOG>(P, i, g, n_ref);
+    } else {
+      Decision d{};
+      if (in) d = decide<SLOG>(P, n, i, g);
+      hit = d.hit;
+    }
+    em[q] = __ballot(in && (hit || g.dense));
     cnt += (uint32_t)__popcll(em[q]);
-    if (g.dense) det += (uint32_t)__popcll(__ballot(in && d.hit));
+    if (g.dense) det += (uint32_t)__popcll(__ballot(in && hit));
   }
   if (!g.dense) det = cnt;
   uint64_t k = 0, K = cnt;
@@ -263,22 +284,32 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
     }
   }
 
-  // second pass: the emitting bins write their events at their offsets
+  // second pass: the wave's emitting bins, compacted in bin order into its
+  // LDS list, are decided again 64 at a time and written at their offsets
   const uint32_t frame = frame_of(r, J.frame_ids, J.frame_mod);
+  uint32_t* lst = reinterpret_cast<uint32_t*>(stats + W * 4) + wr * 64 * S;
+  {
+    uint32_t kq = 0;
 #pragma unroll
-  for (uint32_t q = 0; q < S; ++q) {
-    if (!em[q]) continue;
-    if ((em[q] >> lane) & 1ull) {
-      const uint32_t i = c0 + q * 64 + lane;
+    for (uint32_t q = 0; q < S; ++q) {
+      if ((em[q] >> lane) & 1ull) lst[kq + (uint32_t)__popcll(em[q] & below)] = c0 + q * 64 + lane;
+      kq += (uint32_t)__popcll(em[q]);
+    }
+  }
+  wave_lds_sync();
+  for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    if (e < cnt) {
+      const uint32_t i = lst[e];
       const Decision d = decide<SLOG>(P, n, i, g);
-      const uint64_t pos = k + (uint64_t)__popcll(em[q] & below);
+      const uint64_t pos = k + e;
       const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
       if (pos < room)
         put_event(dst + 3 * pos, kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut, d.eval ? d.sum : 0ull,
                   pos == 0, false);
     }
-    k += (uint64_t)__popcll(em[q]);
   }
+  k += cnt;
   if (work && wr == 0 && lane == 0 && K < room)
     put_event(dst + 3 * K, 3u, n, frame, 0, g.alpha, DET, suppressed(n, g), 0, 0, K == 0, true);
   // the caller may reuse the row's LDS after this
