@@ -363,27 +363,25 @@ __device__ __forceinline__ void cfar_gather_tile(const CfarJob& J, const CfarRow
   }
 }
 
-// Pass 2, one block: exclusive scan of the row counts into 64-bit offsets;
-// offs[n_rows] = total.
+// Pass 2, one block of 1024: exclusive scan of the row counts into 64-bit
+// offsets; offs[n_rows] = total. Each thread sums a contiguous run of rows,
+// the waves scan with shuffles and one barrier combines the 16 wave totals.
 __device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
-  __shared__ uint64_t part[1024];
+  __shared__ uint64_t wsum[16];
   const uint32_t n_rows = J.n_rows;
   const uint32_t* __restrict__ row_count = J.row_count;
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t per = (n_rows + 1023) / 1024;
   const uint32_t r0 = t * per;
   uint64_t s = 0;
   for (uint32_t q = 0; q < per; ++q)
     if (r0 + q < n_rows) s += row_count[r0 + q];
-  part[t] = s;
+  const uint64_t x = wave_incscan(s, lane);
+  if (lane == 63) wsum[wave] = x;
   __syncthreads();
-  for (uint32_t o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
-    const uint64_t v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint64_t run = part[t] - s;
+  uint64_t wb = 0;
+  for (uint32_t w = 0; w < wave; ++w) wb += wsum[w];
+  uint64_t run = wb + x - s;
   for (uint32_t q = 0; q < per; ++q) {
     if (r0 + q < n_rows) {
       J.offs[r0 + q] = run;
@@ -391,8 +389,8 @@ __device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
     }
   }
   if (t == 1023) {
-    J.offs[n_rows] = part[1023];
-    if (J.total) *J.total = part[1023];
+    J.offs[n_rows] = wb + x;
+    if (J.total) *J.total = wb + x;
   }
 }
 

@@ -212,7 +212,21 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
   }
   __syncthreads();
   if (CHMAJOR && J.xa_out) {
-    for (uint32_t w = threadIdx.x; w < ncol * CH; w += 256) J.xa_out[c0 * CH + w] = xt[(w / CH) * rs + w % CH];
+    if (CH == chp && ncol == TC && (reinterpret_cast<uintptr_t>(J.xa_out) & 15u) == 0) {
+      // a full tile of a CPI whose channel count fills the steps (and a
+      // 16-byte aligned output): four consecutive words of one column per
+      // thread and 16-byte stores, the divisions by the compile-time count
+      constexpr uint32_t PER = TC * chp / 1024;  // quads per thread (TC * chp is a multiple of 1024)
+      uint4* dst = reinterpret_cast<uint4*>(J.xa_out + c0 * chp);
+#pragma unroll
+      for (uint32_t it = 0; it < PER; ++it) {
+        const uint32_t q = threadIdx.x + 256 * it, w = 4 * q, col = w / chp, a = w % chp;
+        const uint32_t* src = xt + col * rs + a;
+        dst[q] = uint4{src[0], src[1], src[2], src[3]};
+      }
+    } else {
+      for (uint32_t w = threadIdx.x; w < ncol * CH; w += 256) J.xa_out[c0 * CH + w] = xt[(w / CH) * rs + w % CH];
+    }
   }
 
   for (uint32_t t = 0; t < TILES_PER_WAVE; ++t) {
