@@ -494,6 +494,18 @@ __device__ __forceinline__ uint32_t negj_pk(uint32_t c, PkFlags& f) {
   return U2(r);
 }
 
+// x * W^0 = x * 0x7FFF rounded at 15 (the product is formed, never bypassed:
+// 0x7FFF is not 1): per component x - 1 for x > 16384, x + 1 for x < -16384,
+// x otherwise (-32768 gives -32767). Never saturates. The two masks come
+// from saturating int16 arithmetic; checked against the rounded product for
+// all 65536 values.
+__device__ __forceinline__ uint32_t mul_w0_pk(uint32_t x) {
+  const s16x2 v = S2(x), k = {16384, 16384}, sh = {15, 15};
+  const s16x2 gt = __builtin_elementwise_sub_sat(k, v) >> sh;  // -1 where x > 16384
+  const s16x2 lt = __builtin_elementwise_add_sat(v, k) >> sh;  // -1 where x < -16384
+  return U2(v + gt - lt);
+}
+
 // x * w on a packed x (cmul_tw's arithmetic), saturated into a packed word
 // by v_cvt_pk_i16_i32 (both halves clamped to int16).
 __device__ __forceinline__ uint32_t cmul_pk(uint32_t x, uint32_t wc, uint32_t ws, PkFlags& f) {
@@ -592,7 +604,7 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
       if (s + 2 < NL) {
         const uint32_t TSH = LOG2N - (NL - s);
         const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
-        p0 = cmul_pk(p0, twc[0], tws[0], f1);  // W^0 = 0x7FFF is multiplied, never bypassed
+        p0 = mul_w0_pk(p0);  // element 0 of every group takes W^0
         p1 = cmul_pk(p1, twc[t1], tws[t1], f1);
         p2 = cmul_pk(p2, twc[t2], tws[t2], f1);
         p3 = cmul_pk(p3, twc[t3], tws[t3], f1);
