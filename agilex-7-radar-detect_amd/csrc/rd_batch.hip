@@ -136,7 +136,8 @@ uint32_t build_tables(const rd_batch& B, const Class& c, const Shape& sh, std::v
         break;
       }
       case S_FFT: {
-        const uint32_t fpb = rd::fft_frames_per_tile(rd::ilog2u(N));
+        const uint32_t log2n = rd::ilog2u(N), used = (1u << log2n) - 1u;
+        const uint32_t fpb = rd::fft_frames_per_tile(log2n, (d.scale_sched & used) == used);
         tiles = (CH * F + fpb - 1) / fpb;
         put_job(jobs, rd::FftJob{b.u, nullptr, b.X, b.fft_flags, d.scale_sched, 1u, CH * F, tile0});
         break;

@@ -155,7 +155,7 @@ struct ClassLaunch {
 hipError_t launch_pfb_batch(hipStream_t s, uint32_t T, bool flags, const ClassLaunch& L);
 // FFT of log2 length `log2n`, spc lanes; all1: every job's schedule has all
 // bits set for its stages. Frames per tile follow the single-CPI launch.
-uint32_t fft_frames_per_tile(uint32_t log2n);
+uint32_t fft_frames_per_tile(uint32_t log2n, bool all1);
 hipError_t launch_fft_batch(hipStream_t s, const uint32_t* tw4096, uint32_t log2n, uint32_t spc, bool all1,
                             const ClassLaunch& L);
 // Beamformer over the FFT layout; steps = ceil(max CH / 16) of the class,

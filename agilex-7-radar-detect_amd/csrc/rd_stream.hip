@@ -528,10 +528,10 @@ __device__ __forceinline__ uint32_t cmul_pk(uint32_t x, uint32_t wc, uint32_t ws
 // LDS layout of a block: [NT] wc(W_N^e) at pad(e), [NT] ws(W_N^e) at NT +
 // pad(e) (NT a multiple of 64 words, so one ds_read2st64 fetches both),
 // [FPB] stage-flag words, then FPB frames of NP words each.
-template <int LOG2N>
+template <int LOG2N, int PPT = 4>
 struct FftGeo {
   static constexpr uint32_t N = 1u << LOG2N;
-  static constexpr uint32_t TPF = N / 4;                       // threads per frame
+  static constexpr uint32_t TPF = N / PPT;                     // threads per frame (PPT points each)
   static constexpr uint32_t FPB = TPF >= 256 ? 1 : 256 / TPF;  // frames per block
   static constexpr uint32_t NP = N + N / 32;
   static constexpr uint32_t NT = (NP + 63) / 64 * 64;
@@ -741,19 +741,253 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
   if (J.stage_flags && g == 0) J.stage_flags[frame] = sfl[lf];
 }
 
+// Points per thread of a transform: 16 (radix-16 passes) for the all-ones
+// schedule from 64 to 1024 points, else 4 (at 2048 and 4096 points the
+// 16-point form's register count, about 140, costs more occupancy than its
+// ~8% fewer instructions save; measured).
+constexpr int fft_ppt(int log2n, bool all1) { return (all1 && log2n >= 6 && log2n <= 10) ? 16 : 4; }
+
+// One tile under the all-ones schedule with 16 points per thread: each LDS
+// round trip runs four sub-stages (two radix-2^2 stages with their twiddles)
+// on the thread's 16 points, element (q1, q2) of a block of length L at
+// q1*L/4 + q2*L/16 + r. The first stage combines over q1 (twiddle exponent
+// (q2*L/16 + r) * {0, 2, 1, 3}[q1]), the second over q2 inside each quarter
+// (exponent r * {0, 2, 1, 3}[q2]); the arithmetic of every sub-stage is the
+// radix-4 pass's. Leftover sub-stages run as one radix-4 pass over the
+// thread's four groups, then the final pass as in fft_tile.
+template <int LOG2N, int SPC>
+__device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32_t* smem) {
+  using G = FftGeo<LOG2N, 16>;
+  constexpr uint32_t N = G::N;
+  constexpr uint32_t M = N / SPC;
+  constexpr uint32_t NL = LOG2N - (SPC == 2 ? 1 : 0);
+  constexpr uint32_t TPF = G::TPF, FPB = G::FPB, NP = G::NP, NT = G::NT;
+  const uint32_t* __restrict__ in = J.in;
+  uint32_t* __restrict__ out = J.out;
+  uint32_t* __restrict__ bins = J.bins;
+  const uint32_t reorder = J.reorder;
+  const uint32_t* twc = smem;
+  const uint32_t* tws = smem + NT;
+  uint32_t* sfl = smem + 2 * NT;
+  const uint32_t lf = threadIdx.x / TPF;
+  const uint32_t g = threadIdx.x % TPF;
+  uint32_t* v = smem + 2 * NT + FPB + lf * NP;
+  const uint32_t frame = grp * FPB + lf;
+  const bool active = frame < J.n_frames;
+
+  if (g == 0) sfl[lf] = 0;
+  if (active) {
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+      const uint32_t c = g + t * TPF;  // four consecutive samples 4c .. 4c+3
+      const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * c);
+      const uint32_t w4[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+      for (uint32_t r = 0; r < 4; ++r) {
+        const uint32_t x = 4 * c + r;  // lane q of spc lanes is x[spc*t' + q]
+        v[pad(SPC == 2 ? ((x & 1) * M + (x >> 1)) : x)] = w4[r];
+      }
+    }
+  }
+  __syncthreads();
+
+  uint32_t fword = 0;
+  constexpr uint32_t SR = (NL / 4) * 4;  // sub-stages of the radix-16 passes
+#pragma unroll
+  for (uint32_t s = 0; s + 3 < NL; s += 4) {
+    const uint32_t LL = NL - s;             // log2 of the block length
+    const uint32_t L4 = 1u << (LL - 2), L16 = 1u << (LL - 4);
+    const uint32_t r = g & (L16 - 1);
+    const uint32_t base = ((g >> (LL - 4)) << LL) + r;
+    uint32_t ad[16], p[16];
+#pragma unroll
+    for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e >> 2) * L4 + (e & 3) * L16);
+#pragma unroll
+    for (uint32_t e = 0; e < 16; ++e) p[e] = v[ad[e]];
+    PkFlags fa, fb, fc, fd;
+    // sub-stages s, s+1 over q1 (index e = 4*q1 + q2)
+#pragma unroll
+    for (uint32_t q2 = 0; q2 < 4; ++q2) {
+      bfly_pk(p[q2], p[8 + q2], fa);
+      bfly_pk(p[4 + q2], p[12 + q2], fa);
+      p[12 + q2] = negj_pk(p[12 + q2], fb);
+      bfly_pk(p[q2], p[4 + q2], fb);
+      bfly_pk(p[8 + q2], p[12 + q2], fb);
+    }
+    {
+      const uint32_t TSH = LOG2N - LL;
+#pragma unroll
+      for (uint32_t q2 = 0; q2 < 4; ++q2) {
+        const uint32_t i = q2 * L16 + r;
+        const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
+        p[q2] = mul_w0_pk(p[q2]);
+        p[4 + q2] = cmul_pk(p[4 + q2], twc[t1], tws[t1], fb);
+        p[8 + q2] = cmul_pk(p[8 + q2], twc[t2], tws[t2], fb);
+        p[12 + q2] = cmul_pk(p[12 + q2], twc[t3], tws[t3], fb);
+      }
+    }
+    // sub-stages s+2, s+3 over q2 inside each quarter
+#pragma unroll
+    for (uint32_t q1 = 0; q1 < 4; ++q1) {
+      uint32_t* b = p + 4 * q1;
+      bfly_pk(b[0], b[2], fc);
+      bfly_pk(b[1], b[3], fc);
+      b[3] = negj_pk(b[3], fd);
+      bfly_pk(b[0], b[1], fd);
+      bfly_pk(b[2], b[3], fd);
+    }
+    if (s + 4 < NL) {
+      const uint32_t TSH = LOG2N - LL + 2;
+      const uint32_t t1 = pad((2 * r) << TSH), t2 = pad(r << TSH), t3 = pad((3 * r) << TSH);
+      const uint32_t c1 = twc[t1], s1 = tws[t1], c2 = twc[t2], s2 = tws[t2], c3 = twc[t3], s3 = tws[t3];
+#pragma unroll
+      for (uint32_t q1 = 0; q1 < 4; ++q1) {
+        uint32_t* b = p + 4 * q1;
+        b[0] = mul_w0_pk(b[0]);
+        b[1] = cmul_pk(b[1], c1, s1, fd);
+        b[2] = cmul_pk(b[2], c2, s2, fd);
+        b[3] = cmul_pk(b[3], c3, s3, fd);
+      }
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 16; ++e) v[ad[e]] = p[e];
+    fword |= (fa.word() << (2 * s)) | (fb.word() << (2 * (s + 1))) | (fc.word() << (2 * (s + 2))) |
+             (fd.word() << (2 * (s + 3)));
+    __syncthreads();
+  }
+
+  if constexpr (SR + 1 < NL) {  // one radix-4 pass over the thread's four groups
+    constexpr uint32_t s = SR;
+    constexpr uint32_t LQ = NL - s - 2, Q = 1u << LQ;
+    PkFlags f0, f1;
+    uint32_t ad[4][4], p[4][4];
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+      const uint32_t gi = g + t * TPF;
+      const uint32_t i = gi & (Q - 1);
+      const uint32_t base = ((gi >> LQ) << (NL - s)) + i;
+#pragma unroll
+      for (uint32_t c = 0; c < 4; ++c) ad[t][c] = pad(base + c * Q);
+#pragma unroll
+      for (uint32_t c = 0; c < 4; ++c) p[t][c] = v[ad[t][c]];
+      bfly_pk(p[t][0], p[t][2], f0);
+      bfly_pk(p[t][1], p[t][3], f0);
+      p[t][3] = negj_pk(p[t][3], f1);
+      bfly_pk(p[t][0], p[t][1], f1);
+      bfly_pk(p[t][2], p[t][3], f1);
+      if (s + 2 < NL) {
+        constexpr uint32_t TSH = LOG2N - (NL - s);
+        const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
+        p[t][0] = mul_w0_pk(p[t][0]);
+        p[t][1] = cmul_pk(p[t][1], twc[t1], tws[t1], f1);
+        p[t][2] = cmul_pk(p[t][2], twc[t2], tws[t2], f1);
+        p[t][3] = cmul_pk(p[t][3], twc[t3], tws[t3], f1);
+      }
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t)
+#pragma unroll
+      for (uint32_t c = 0; c < 4; ++c) v[ad[t][c]] = p[t][c];
+    fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
+    __syncthreads();
+  }
+
+  // final pass: the last (odd-count) sub-stage and/or the DIT merge, four
+  // groups per thread
+  constexpr uint32_t SL = NL - 1;
+  constexpr bool LAST_R2 = (NL & 1u) != 0;
+  if (SPC == 1) {
+    if (LAST_R2) {
+      PkFlags f;
+#pragma unroll
+      for (uint32_t t = 0; t < 4; ++t) {
+        const uint32_t c = 4 * (g + t * TPF);
+        uint32_t a = v[pad(c)], b = v[pad(c + 1)], cc = v[pad(c + 2)], d = v[pad(c + 3)];
+        bfly_pk(a, b, f);
+        bfly_pk(cc, d, f);
+        v[pad(c)] = a;
+        v[pad(c + 1)] = b;
+        v[pad(c + 2)] = cc;
+        v[pad(c + 3)] = d;
+      }
+      fword |= f.word() << (2 * SL);
+    }
+  } else {
+    uint32_t a0[4], a1[4], t0[4], t1[4];
+    PkFlags f, fm;
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+      const uint32_t j = g + t * TPF;  // lane0[2j], lane0[2j+1], lane1[2j], lane1[2j+1]
+      a0[t] = v[pad(2 * j)];
+      a1[t] = v[pad(2 * j + 1)];
+      uint32_t b0 = v[pad(M + 2 * j)], b1 = v[pad(M + 2 * j + 1)];
+      if (LAST_R2) {
+        bfly_pk(a0[t], a1[t], f);
+        bfly_pk(b0, b1, f);
+      }
+      const uint32_t m0 = pad(bitrev(2 * j, NL)), m1 = pad(bitrev(2 * j + 1, NL));
+      t0[t] = cmul_pk(b0, twc[m0], tws[m0], fm);
+      t1[t] = cmul_pk(b1, twc[m1], tws[m1], fm);
+      bfly_pk(a0[t], t0[t], fm);
+      bfly_pk(a1[t], t1[t], fm);
+    }
+    if (LAST_R2) fword |= f.word() << (2 * SL);
+    fword |= fm.word() << (2 * NL);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
+      const uint32_t j = g + t * TPF;
+      v[pad(4 * j)] = a0[t];
+      v[pad(4 * j + 1)] = t0[t];
+      v[pad(4 * j + 2)] = a1[t];
+      v[pad(4 * j + 3)] = t1[t];
+    }
+  }
+  if (fword) atomicOr(&sfl[lf], fword);
+  __syncthreads();
+  if (!active) return;
+
+  const size_t fo = (size_t)frame * N;
+#pragma unroll
+  for (int rr = 0; rr < 16; ++rr) {
+    const uint32_t k = g + rr * TPF;
+    if (bins) {
+      const uint32_t src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
+      bins[fo + k] = v[pad(src)];
+    }
+    if (out) {
+      uint32_t src = k;
+      if (reorder) src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k >> 1, NL) + (k & 1);
+      out[fo + k] = v[pad(src)];
+    }
+  }
+  if (J.stage_flags && g == 0) J.stage_flags[frame] = sfl[lf];
+}
+
+template <int LOG2N, bool ALL1>
+using FftGeoOf = FftGeo<LOG2N, fft_ppt(LOG2N, ALL1)>;
+
+template <int LOG2N, int SPC, bool ALL1>
+__device__ __forceinline__ void fft_tile_any(const FftJob& J, uint32_t grp, uint32_t* smem) {
+  if constexpr (fft_ppt(LOG2N, ALL1) == 16)
+    fft_tile16<LOG2N, SPC>(J, grp, smem);
+  else
+    fft_tile<LOG2N, SPC, ALL1>(J, grp, smem);
+}
+
 // Single CPI: one block per tile.
 template <int LOG2N, int SPC, bool ALL1>
-__global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_kernel(
+__global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB)) void fft_kernel(
     FftJob J, const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
   fft_stage_twiddles<LOG2N>(tw4096, smem);
-  fft_tile<LOG2N, SPC, ALL1>(J, blockIdx.x, smem);
+  fft_tile_any<LOG2N, SPC, ALL1>(J, blockIdx.x, smem);
 }
 
 // Batched: one block per frame group of a class of CPIs (same length and lane
 // count), block -> job via the tile table.
 template <int LOG2N, int SPC, bool ALL1>
-__global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_batch_kernel(
+__global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB)) void fft_batch_kernel(
     const FftJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job, uint32_t n_tiles,
     const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
@@ -761,31 +995,39 @@ __global__ __launch_bounds__(FftGeo<LOG2N>::TPF * FftGeo<LOG2N>::FPB) void fft_b
   if (t >= n_tiles) return;
   const FftJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
   fft_stage_twiddles<LOG2N>(tw4096, smem);
-  fft_tile<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
+  fft_tile_any<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
+}
+
+template <int LOG2N, int SPC, bool ALL1>
+static void launch_fft_g(hipStream_t s, const uint32_t* tw4096, const FftJob& J) {
+  using G = FftGeoOf<LOG2N, ALL1>;
+  const dim3 grid((J.n_frames + G::FPB - 1) / G::FPB), block(G::TPF * G::FPB);
+  fft_kernel<LOG2N, SPC, ALL1><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
 }
 
 template <int LOG2N, int SPC>
 static hipError_t launch_fft_n(hipStream_t s, const uint32_t* tw4096, const FftJob& J) {
-  using G = FftGeo<LOG2N>;
-  const dim3 grid((J.n_frames + G::FPB - 1) / G::FPB), block(G::TPF * G::FPB);
   constexpr uint32_t used = (1u << LOG2N) - 1u;
   if ((J.sched & used) == used)
-    fft_kernel<LOG2N, SPC, true><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
+    launch_fft_g<LOG2N, SPC, true>(s, tw4096, J);
   else
-    fft_kernel<LOG2N, SPC, false><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
+    launch_fft_g<LOG2N, SPC, false>(s, tw4096, J);
   return hipGetLastError();
+}
+
+template <int LOG2N, int SPC, bool ALL1>
+static void launch_fft_batch_g(hipStream_t s, const uint32_t* tw4096, const ClassLaunch& L) {
+  using G = FftGeoOf<LOG2N, ALL1>;
+  fft_batch_kernel<LOG2N, SPC, ALL1><<<L.grid, G::TPF * G::FPB, G::lds_bytes, s>>>(
+      static_cast<const FftJob*>(L.jobs), L.tile_job, L.n_tiles, tw4096);
 }
 
 template <int LOG2N, int SPC>
 static hipError_t launch_fft_batch_n(hipStream_t s, const uint32_t* tw4096, bool all1, const ClassLaunch& L) {
-  using G = FftGeo<LOG2N>;
-  const FftJob* jobs = static_cast<const FftJob*>(L.jobs);
   if (all1)
-    fft_batch_kernel<LOG2N, SPC, true><<<L.grid, G::TPF * G::FPB, G::lds_bytes, s>>>(jobs, L.tile_job, L.n_tiles,
-                                                                                     tw4096);
+    launch_fft_batch_g<LOG2N, SPC, true>(s, tw4096, L);
   else
-    fft_batch_kernel<LOG2N, SPC, false><<<L.grid, G::TPF * G::FPB, G::lds_bytes, s>>>(jobs, L.tile_job, L.n_tiles,
-                                                                                      tw4096);
+    launch_fft_batch_g<LOG2N, SPC, false>(s, tw4096, L);
   return hipGetLastError();
 }
 
@@ -826,8 +1068,8 @@ hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in,
   return spc == 2 ? launch_fft_spc<2>(s, tw4096, log2n, J) : launch_fft_spc<1>(s, tw4096, log2n, J);
 }
 
-uint32_t fft_frames_per_tile(uint32_t log2n) {
-  const uint32_t tpf = (1u << log2n) / 4;
+uint32_t fft_frames_per_tile(uint32_t log2n, bool all1) {
+  const uint32_t tpf = (1u << log2n) / (uint32_t)fft_ppt((int)log2n, all1);
   return tpf >= 256 ? 1 : 256 / tpf;
 }
 
