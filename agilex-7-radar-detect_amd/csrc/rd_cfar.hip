@@ -451,7 +451,7 @@ static size_t cfar_lds_bytes(const CfarRowGeo& G, uint32_t R) { return (size_t)R
 // when there are too few rows to give every SIMD several waves.
 static uint32_t cfar_single_waves(uint32_t n, uint32_t n_rows) {
   uint32_t W = cfar_min_waves(n);
-  while (W < 4 && (uint64_t)n_rows * W < 16384 && 128u * W <= n) W <<= 1;
+  while (W < 4 && (uint64_t)n_rows * W < 8192 && 128u * W <= n) W <<= 1;
   return W;
 }
 
@@ -465,29 +465,42 @@ static uint32_t cfar_single_waves(uint32_t n, uint32_t n_rows) {
     default: KERNEL(4, __VA_ARGS__); break;                       \
   }
 
+// Rows pass with W waves per row; the gather (mostly a copy) with the fewest
+// waves per row and as many rows per block as fit.
 template <typename Cell>
-static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, const CfarRowGeo& G, uint32_t R) {
-  const size_t lds = cfar_lds_bytes(G, R);
-  const dim3 grid((J.n_rows + R - 1) / R), block(64 * G.W * R);
+static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, uint32_t W) {
+  {
+    const CfarRowGeo G = cfar_row_geo(J.n, W);
+    const uint32_t R = cfar_rows_per_tile_max(J.n, W);
+    const size_t lds = cfar_lds_bytes(G, R);
+    const dim3 grid((J.n_rows + R - 1) / R), block(64 * G.W * R);
 #define RD_ROWS(SL, M) cfar_rows_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
-#define RD_GATHER(SL, M) cfar_gather_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
-  if (G.W > 1) {
-    RD_CFAR_DISPATCH(RD_ROWS, true)
-  } else {
-    RD_CFAR_DISPATCH(RD_ROWS, false)
+    if (G.W > 1) {
+      RD_CFAR_DISPATCH(RD_ROWS, true)
+    } else {
+      RD_CFAR_DISPATCH(RD_ROWS, false)
+    }
+#undef RD_ROWS
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
   e = hipGetLastError();
   if (e != hipSuccess || J.cap == 0) return e;
-  if (G.W > 1) {
-    RD_CFAR_DISPATCH(RD_GATHER, true)
-  } else {
-    RD_CFAR_DISPATCH(RD_GATHER, false)
-  }
-#undef RD_ROWS
+  {
+    const uint32_t Wg = cfar_min_waves(J.n);
+    const CfarRowGeo G = cfar_row_geo(J.n, Wg);
+    const uint32_t R = cfar_rows_per_tile_max(J.n, Wg);
+    const size_t lds = cfar_lds_bytes(G, R);
+    const dim3 grid((J.n_rows + R - 1) / R), block(64 * G.W * R);
+#define RD_GATHER(SL, M) cfar_gather_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
+    if (G.W > 1) {
+      RD_CFAR_DISPATCH(RD_GATHER, true)
+    } else {
+      RD_CFAR_DISPATCH(RD_GATHER, false)
+    }
 #undef RD_GATHER
+  }
   return hipGetLastError();
 }
 
@@ -497,13 +510,11 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        uint64_t* total_dev) {
   if (n_rows == 0) return hipSuccess;
   const uint32_t W = cfar_single_waves(n, n_rows);
-  const CfarRowGeo G = cfar_row_geo(n, W);
-  const uint32_t R = cfar_rows_per_tile_max(n, W);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   uint64_t* stage = offs + n_rows + 1;
   const CfarJob J{cells, frame_ids, stage, row_count, offs, events, total_dev, cap,
                   cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, kCfarSlot, 0};
-  return cell_bytes == 4 ? launch_cfar_cell<uint32_t>(s, J, G, R) : launch_cfar_cell<uint64_t>(s, J, G, R);
+  return cell_bytes == 4 ? launch_cfar_cell<uint32_t>(s, J, W) : launch_cfar_cell<uint64_t>(s, J, W);
 }
 
 static hipError_t launch_cfar_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L,
