@@ -168,13 +168,15 @@ struct BfLds {
 // is TC contiguous words), and the tile is also written out as the aligned
 // copy Xa[col][ch] (the alignment stage) when J.xa_out is set. Otherwise X is
 // the aligned layout and the tile is one contiguous run.
-template <int STEPS, int TC, bool CHMAJOR, int OUT>
+// FULL: a whole tile of a CPI whose channels fill the steps and whose beams
+// fill their tiles (no bounds to test).
+template <int STEPS, int TC, bool CHMAJOR, int OUT, bool FULL>
 __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uint32_t* frag, const int32_t* tail,
                                         uint32_t* xt) {
   const int out = OUT >= 0 ? OUT : bf_out_mask(J);
   constexpr uint32_t chp = STEPS * 16;
   constexpr uint32_t TILES_PER_WAVE = TC / 64;
-  const uint32_t CH = J.CH, B = J.B, bt_n = J.bt_n;
+  const uint32_t CH = FULL ? STEPS * 16 : J.CH, B = J.B, bt_n = J.bt_n;
   const uint64_t cols = J.cols;
   const uint32_t* __restrict__ X = J.X;
   const uint32_t rs = CH + 1;
@@ -191,12 +193,12 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
 #pragma unroll
     for (uint32_t k = 0; k < ITER; ++k) {
       const uint32_t idx = threadIdx.x + 256 * k, a = idx / TC, j = idx % TC;
-      v[k] = (a < CH && j < ncol) ? X[(size_t)a * cols + c0 + j] : 0u;
+      v[k] = (FULL || (a < CH && j < ncol)) ? X[(size_t)a * cols + c0 + j] : 0u;
     }
 #pragma unroll
     for (uint32_t k = 0; k < ITER; ++k) {
       const uint32_t idx = threadIdx.x + 256 * k, a = idx / TC, j = idx % TC;
-      if (a < CH && j < ncol) xt[j * rs + a] = v[k];
+      if (FULL || (a < CH && j < ncol)) xt[j * rs + a] = v[k];
     }
   } else {
 #pragma unroll
@@ -233,7 +235,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
     const uint32_t jl = (wave * TILES_PER_WAVE + t) * 16;
     if (jl >= ncol) break;
     const uint32_t j = jl + cl;
-    const bool col_ok = j < ncol;
+    const bool col_ok = FULL || j < ncol;
     const uint64_t col = c0 + j;
     // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits.
     uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
@@ -247,7 +249,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
       uint64_t vmask = 0;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const bool ok = col_ok && a0 + q < CH;
+        const bool ok = FULL || (col_ok && a0 + q < CH);
         p[q] = ok ? xt[j * rs + a0 + q] : 0u;
         vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
       }
@@ -300,7 +302,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const uint32_t b = bt * 16 + 4 * grp + r;
-        if (b >= B) continue;
+        if (!FULL && b >= B) continue;
         const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
         uint32_t fr = 0, fi = 0;
         const int32_t yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
@@ -328,9 +330,13 @@ __global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_kernel(BfJob J) {
   uint32_t* xt = lds + bf_frag_words(J.bt_n, STEPS) + (size_t)J.bt_n * 32;
   bf_weights_to_lds(J.W, J.CH, J.B, STEPS * 16, STEPS, J.bt_n, frag, tail);
   const uint64_t n_tiles = (J.cols + TC - 1) / TC;
+  const bool full = J.CH == STEPS * 16 && J.B % 16 == 0;
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     __syncthreads();  // previous tile fully consumed
-    bf_tile<STEPS, TC, CHMAJOR, OUT>(J, tile, frag, tail, xt);
+    if (STEPS == 1 && full && (tile + 1) * TC <= J.cols)  // (two bodies fit the registers at one step)
+      bf_tile<STEPS, TC, CHMAJOR, OUT, STEPS == 1>(J, tile, frag, tail, xt);
+    else
+      bf_tile<STEPS, TC, CHMAJOR, OUT, false>(J, tile, frag, tail, xt);
   }
 }
 
@@ -353,7 +359,11 @@ __global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_batch_kernel(const
       bf_weights_to_lds(J.W, J.CH, J.B, STEPS * 16, STEPS, J.bt_n, frag, tail);
       cur = jid;
     }
-    bf_tile<STEPS, TC, true, OUT>(J, t - J.tile0, frag, tail, xt);
+    const uint32_t lt = t - J.tile0;
+    if (STEPS <= 2 && J.CH == STEPS * 16 && J.B % 16 == 0 && ((uint64_t)lt + 1) * TC <= J.cols)
+      bf_tile<STEPS, TC, true, OUT, STEPS <= 2>(J, lt, frag, tail, xt);
+    else
+      bf_tile<STEPS, TC, true, OUT, false>(J, lt, frag, tail, xt);
   }
 }
 

@@ -117,9 +117,26 @@ def cpu_baseline(sc, iq, taps, W, budget_s: float):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
+    # the same chain on one thread, one CPI
+    t1 = time.perf_counter()
+    orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
+              keep=("power", "R", "events"), threads=1)
+    dt1 = time.perf_counter() - t1
     return dict(value=round(n * sc.samples / dt / 1e6, 3), unit="Msamples/s", cores=threads, kind=kind,
                 sample=f"{n} CPI(s) of the {sc.name} workload ({sc.samples} samples each), full chain, "
-                       f"{threads} threads, {dt:.1f} s"), out
+                       f"{threads} threads, {dt:.1f} s",
+                value_1thread=round(sc.samples / dt1 / 1e6, 3), hardware_concurrency=os.cpu_count(),
+                cpu_model=cpu_model()), out
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 # ----------------------------------------------------------------------------- large leg
