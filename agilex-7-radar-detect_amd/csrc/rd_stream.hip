@@ -799,9 +799,20 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     const uint32_t L4 = 1u << (LL - 2), L16 = 1u << (LL - 4);
     const uint32_t r = g & (L16 - 1);
     const uint32_t base = ((g >> (LL - 4)) << LL) + r;
+    // padded addresses: a stride that is a multiple of 32 words adds a
+    // constant (so the LDS accesses take immediate offsets from one base)
     uint32_t ad[16], p[16];
+    if (L16 >= 32) {
+      const uint32_t pb = pad(base);
 #pragma unroll
-    for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e >> 2) * L4 + (e & 3) * L16);
+      for (uint32_t e = 0; e < 16; ++e) ad[e] = pb + (e >> 2) * (L4 + L4 / 32) + (e & 3) * (L16 + L16 / 32);
+    } else if (L4 >= 32) {
+#pragma unroll
+      for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e & 3) * L16) + (e >> 2) * (L4 + L4 / 32);
+    } else {
+#pragma unroll
+      for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e >> 2) * L4 + (e & 3) * L16);
+    }
 #pragma unroll
     for (uint32_t e = 0; e < 16; ++e) p[e] = v[ad[e]];
     PkFlags fa, fb, fc, fd;
