@@ -1,7 +1,7 @@
 """The CPU restatement against the reference's own header models (oracle/_ref).
 
-Skipped where the reference tree was absent at build time (e.g. on a GPU box
-that received no prebuilt oracle/_ref).  Covers every stage on seeded random
+Skipped where the reference tree was absent at build time (a machine without
+a prebuilt oracle/_ref).  Covers every stage on seeded random
 and corner-value inputs inside the reference's domain (FFT N <= 1024), and the
 composed chain on the tiny workload and a frame-reduced medium workload.
 """
