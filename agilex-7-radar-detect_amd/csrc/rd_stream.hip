@@ -742,10 +742,8 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
 }
 
 // Points per thread of a transform: 16 (radix-16 passes) for the all-ones
-// schedule from 64 to 1024 points, else 4 (at 2048 and 4096 points the
-// 16-point form's register count, about 140, costs more occupancy than its
-// ~8% fewer instructions save; measured).
-constexpr int fft_ppt(int log2n, bool all1) { return (all1 && log2n >= 6 && log2n <= 10) ? 16 : 4; }
+// schedule from 64 points up, else 4.
+constexpr int fft_ppt(int log2n, bool all1) { return (all1 && log2n >= 6) ? 16 : 4; }
 
 // One tile under the all-ones schedule with 16 points per thread: each LDS
 // round trip runs four sub-stages (two radix-2^2 stages with their twiddles)
@@ -815,12 +813,20 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     }
 #pragma unroll
     for (uint32_t e = 0; e < 16; ++e) p[e] = v[ad[e]];
-    PkFlags fa, fb, fc, fd;
-    // sub-stages s, s+1 over q1 (index e = 4*q1 + q2)
+    // sub-stages s, s+1 over q1 (index e = 4*q1 + q2); each sub-stage's flag
+    // word is folded in as soon as the sub-stage is complete
+    {
+      PkFlags fa;
+#pragma unroll
+      for (uint32_t q2 = 0; q2 < 4; ++q2) {
+        bfly_pk(p[q2], p[8 + q2], fa);
+        bfly_pk(p[4 + q2], p[12 + q2], fa);
+      }
+      fword |= fa.word() << (2 * s);
+    }
+    PkFlags fb;
 #pragma unroll
     for (uint32_t q2 = 0; q2 < 4; ++q2) {
-      bfly_pk(p[q2], p[8 + q2], fa);
-      bfly_pk(p[4 + q2], p[12 + q2], fa);
       p[12 + q2] = negj_pk(p[12 + q2], fb);
       bfly_pk(p[q2], p[4 + q2], fb);
       bfly_pk(p[8 + q2], p[12 + q2], fb);
@@ -837,12 +843,21 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
         p[12 + q2] = cmul_pk(p[12 + q2], twc[t3], tws[t3], fb);
       }
     }
+    fword |= fb.word() << (2 * (s + 1));
     // sub-stages s+2, s+3 over q2 inside each quarter
+    {
+      PkFlags fc;
+#pragma unroll
+      for (uint32_t q1 = 0; q1 < 4; ++q1) {
+        bfly_pk(p[4 * q1], p[4 * q1 + 2], fc);
+        bfly_pk(p[4 * q1 + 1], p[4 * q1 + 3], fc);
+      }
+      fword |= fc.word() << (2 * (s + 2));
+    }
+    PkFlags fd;
 #pragma unroll
     for (uint32_t q1 = 0; q1 < 4; ++q1) {
       uint32_t* b = p + 4 * q1;
-      bfly_pk(b[0], b[2], fc);
-      bfly_pk(b[1], b[3], fc);
       b[3] = negj_pk(b[3], fd);
       bfly_pk(b[0], b[1], fd);
       bfly_pk(b[2], b[3], fd);
@@ -862,8 +877,7 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     }
 #pragma unroll
     for (uint32_t e = 0; e < 16; ++e) v[ad[e]] = p[e];
-    fword |= (fa.word() << (2 * s)) | (fb.word() << (2 * (s + 1))) | (fc.word() << (2 * (s + 2))) |
-             (fd.word() << (2 * (s + 3)));
+    fword |= fd.word() << (2 * (s + 3));
     __syncthreads();
   }
 
