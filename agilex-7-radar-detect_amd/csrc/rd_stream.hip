@@ -772,6 +772,17 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
   uint32_t* v = smem + 2 * NT + FPB + lf * NP;
   const uint32_t frame = grp * FPB + lf;
   const bool active = frame < J.n_frames;
+  // the threads of a frame share one wave when TPF <= 64: their LDS exchanges
+  // then need only a wave-level sync, and the block's waves run independently
+  auto frame_sync = [] {
+    if constexpr (TPF <= 64) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  };
 
   if (g == 0) sfl[lf] = 0;
   if (active) {
@@ -878,7 +889,7 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
 #pragma unroll
     for (uint32_t e = 0; e < 16; ++e) v[ad[e]] = p[e];
     fword |= fd.word() << (2 * (s + 3));
-    __syncthreads();
+    frame_sync();
   }
 
   if constexpr (SR + 1 < NL) {  // one radix-4 pass over the thread's four groups
@@ -914,7 +925,7 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
 #pragma unroll
       for (uint32_t c = 0; c < 4; ++c) v[ad[t][c]] = p[t][c];
     fword |= (f0.word() << (2 * s)) | (f1.word() << (2 * (s + 1)));
-    __syncthreads();
+    frame_sync();
   }
 
   // final pass: the last (odd-count) sub-stage and/or the DIT merge, four
@@ -958,7 +969,7 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     }
     if (LAST_R2) fword |= f.word() << (2 * SL);
     fword |= fm.word() << (2 * NL);
-    __syncthreads();
+    frame_sync();
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
       const uint32_t j = g + t * TPF;
@@ -969,7 +980,7 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     }
   }
   if (fword) atomicOr(&sfl[lf], fword);
-  __syncthreads();
+  frame_sync();
   if (!active) return;
 
   const size_t fo = (size_t)frame * N;
