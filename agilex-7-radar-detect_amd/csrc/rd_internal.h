@@ -139,6 +139,11 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
 hipError_t launch_event_flits(hipStream_t s, const uint64_t* events, uint64_t n_cap, const uint64_t* n_dev,
                               uint32_t packet_w, uint32_t dest, uint32_t src, uint32_t vc, uint32_t seq0,
                               uint32_t* flits);
+// Detection writeback: n_cap packets (or min(n_cap, *n_dev)) of flits into
+// 64-byte ring words, checks counted into stats[RD_WB_NSTATS].
+hipError_t launch_event_writeback(hipStream_t s, const uint32_t* flits, uint64_t n_cap, const uint64_t* n_dev,
+                                  uint32_t packet_w, uint32_t port, uint32_t seq0, uint32_t* ring,
+                                  uint64_t ring_words, uint64_t addr0, uint64_t* stats);
 
 // ---------------------------------------------------------------- batched
 // A class launch: `jobs` (device) holds n_jobs records of one shape class,

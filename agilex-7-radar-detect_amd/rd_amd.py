@@ -23,7 +23,7 @@ EXPORTS = [
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
-    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
+    "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -100,6 +100,7 @@ class Lib:
         L.rd_flit_words.argtypes = [u32]
         L.rd_flit_words.restype = u32
         L.rd_event_flits.argtypes = [vp, vp, u64, vp, u32, u32, u32, u32, u32, vp]
+        L.rd_event_writeback.argtypes = [vp, vp, u64, vp, u32, u32, u32, vp, u64, u64, vp]
         L.rd_doppler_run.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]
         L.rd_cov_integrate.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp]
         sz, pu = C.c_size_t, C.POINTER(u32)
@@ -349,6 +350,35 @@ def _ctx_event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0, n_dev=
 
 
 Ctx.event_flits = _ctx_event_flits
+
+RD_WB_NSTATS = 5
+
+
+def _ctx_event_writeback(self, flits, packet_w, port=0, seq0=0, ring_words=None, addr0=0, ring=None, stats=None,
+                         n_dev=None, cap=None):
+    """Detection writeback of flits [n][L][fw] (host array, or DevBuf with cap
+    packets) into a ring of 64-byte words. ring / stats: DevBufs to keep across
+    calls (created, zeroed, when None). Returns (ring [ring_words][16] u32,
+    stats [5] u64) as host arrays."""
+    if isinstance(flits, DevBuf):
+        dev, n = flits, int(cap)
+    else:
+        fl = np.ascontiguousarray(flits, np.uint32)
+        n = fl.shape[0]
+        dev = self.put(fl)
+    if ring_words is None:
+        ring_words = max(n, 1)
+    if ring is None:
+        ring = self.buf(64 * ring_words).zero()
+    if stats is None:
+        stats = self.buf(8 * RD_WB_NSTATS).zero()
+    rc = self.L.rd_event_writeback(self.h, dev.ptr, n, n_dev.ptr if n_dev is not None else None, packet_w, port,
+                                   seq0, ring.ptr, ring_words, addr0, stats.ptr)
+    self.lib.check("rd_event_writeback", rc)
+    return ring.download(np.uint32, 16 * ring_words).reshape(ring_words, 16), stats.download(np.uint64, RD_WB_NSTATS)
+
+
+Ctx.event_writeback = _ctx_event_writeback
 
 
 def _ctx_doppler(self, ring, rows, depth, n_bins, newest, n_dop, spc=2, sched=0xFFFFFFFF, shift=False):

@@ -251,6 +251,33 @@ uint32_t rd_flit_words(uint32_t packet_w);
 rd_status rd_event_flits(rd_ctx* ctx, const uint64_t* events, uint64_t n_events, const uint64_t* n_events_dev,
                          uint32_t packet_w, uint32_t dest, uint32_t src, uint32_t vc, uint32_t seq0, uint32_t* flits);
 
+/* Detection writeback: the receive end of the detection packets, as the
+ * pipeline's DMA tap does it (deserializer and memory writer,
+ * rtl/top/benchmark_sim_top.sv:1994-2134; the C++ model has none). Packet k of
+ * n (flits [n][rd_event_flit_count(packet_w)][rd_flit_words(packet_w)] in
+ * rd_event_flits' layout; n = n_packets, or min(n_packets, *n_packets_dev)
+ * when that is set) is reassembled and its 176-bit event written zero-extended
+ * into one 64-byte word of a ring of ring_words words: word
+ * (addr0 + k) mod ring_words, ring 16-byte aligned (with more packets than
+ * ring words the ring ends as in-order writes leave it, holding the last
+ * ring_words). Every packet lands, as the tap's writer checks nothing; the
+ * receive checks of packet_model::Scoreboard::observe
+ * (model/cpp/packet/packet_model.hpp:360-429), taken per packet on its own L
+ * flits, are added to stats
+ * [RD_WB_NSTATS] (device u64; added to, so zero them once):
+ *   [0] packets written       [1] flits failing odd parity
+ *   [2] framing violations    [3] misrouted (header dest != port)
+ *   [4] sequence breaks (header seq != (seq0 + k) mod 2^16).
+ * Framing violations: header flit without sof or with eof, header length !=
+ * L, type != detection, header vc != flit vc; a payload flit with sof, with a
+ * vc other than the header flit's, or whose eof is not set exactly on the
+ * last flit (one each). packet_w in [36, 512], port < 16, ring_words in
+ * [1, 2^48], n_packets <= 2^40. */
+#define RD_WB_NSTATS 5
+rd_status rd_event_writeback(rd_ctx* ctx, const uint32_t* flits, uint64_t n_packets, const uint64_t* n_packets_dev,
+                             uint32_t packet_w, uint32_t port, uint32_t seq0, uint32_t* ring, uint64_t ring_words,
+                             uint64_t addr0, uint64_t* stats);
+
 /* ---- reference file formats (host side, no device work) -----------------
  * The readers validate as the reference's loaders do and return RD_E_ARG with
  * a diagnosis in err (nullable, err_cap bytes) on any mismatch. Called with a

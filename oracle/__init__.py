@@ -209,6 +209,38 @@ class Oracle:
             raise ValueError(f"event_flits failed ({rc})")
         return out[: n * L * fw].reshape(n, L, fw)
 
+    def event_writeback(self, flits, packet_w, port=0, seq0=0, ring_words=None, addr0=0, ring=None):
+        """Detection writeback of flits [n][L][fw] -> (ring [ring_words][16] u32,
+        stats [5] u64). ring (optional) is updated in place."""
+        fl = np.ascontiguousarray(flits, np.uint32)
+        n = fl.shape[0]
+        if ring is None:
+            ring = np.zeros((max(n, 1) if ring_words is None else ring_words, 16), np.uint32)
+        stats = np.zeros(5, np.uint64)
+        f = self._f("event_writeback")
+        f.restype = C.c_int
+        rc = f(_ptr(fl), C.c_uint64(n), C.c_uint(packet_w), C.c_uint(port), C.c_uint(seq0), _ptr(ring),
+               C.c_uint64(ring.shape[0]), C.c_uint64(addr0), _ptr(stats))
+        if rc < 0:
+            raise ValueError(f"event_writeback failed ({rc})")
+        return ring, stats
+
+    def scoreboard(self, flits, packet_w, port, events, dest=0, src=0, vc=0, seq0=0):
+        """packet_model::Scoreboard over flits [n][L][fw] against packets built
+        from events [m][3] (reference build only) -> dict of its counts."""
+        fl = np.ascontiguousarray(flits, np.uint32)
+        ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)
+        out = np.zeros(10, np.uint64)
+        f = self.lib.rdr_scoreboard
+        f.restype = C.c_int
+        rc = f(_ptr(fl), C.c_uint64(fl.shape[0]), C.c_uint(packet_w), C.c_uint(port), _ptr(ev),
+               C.c_uint64(ev.shape[0]), C.c_uint(dest), C.c_uint(src), C.c_uint(vc), C.c_uint(seq0), _ptr(out))
+        if rc < 0:
+            raise ValueError(f"scoreboard failed ({rc})")
+        names = ("delivered", "lost", "duplicated", "misrouted", "corrupted", "reordered", "framing", "parity",
+                 "unexpected", "injected")
+        return {k: int(v) for k, v in zip(names, out)}
+
     def chain(self, iq, taps, W, CH, N, T, F, B, cfar_cfg8, max_guard, max_ref, spc=2, sched=0xFFFFFFFF,
               keep=("u", "X", "y", "power", "R", "events"), threads=1):
         """Run the composed chain for one CPI; returns a dict of the kept outputs."""

@@ -784,6 +784,60 @@ extern "C" int rdo_event_flits(const uint64_t* events, uint64_t n_events, unsign
 }
 
 // ---------------------------------------------------------------------------
+// Detection writeback (rtl/top/benchmark_sim_top.sv:1994-2134: deserializer
+// and memory writer; no C++ model). Packet k of n (L flits in the layout
+// above) becomes one 64-byte ring word (16 u32) at (addr0 + k) mod ring_words:
+// payload bit q is flit 1 + q / packet_w, flit bit 5 + q % packet_w; bits
+// 176..511 of the word are zero. Every packet is written. The receive checks
+// of packet_model::Scoreboard::observe (packet_model.hpp:360-429), taken per
+// packet on its own L flits, are added to stats[5]: packets written, flits
+// with even parity, framing violations, misrouted packets (dest != port),
+// sequence breaks (seq != (seq0 + k) mod 2^16). Framing, one count each: the
+// header flit lacks sof or carries eof; header length != L; header type != 0;
+// header vc != the header flit's vc; a payload flit carries sof, a vc other
+// than the header flit's, or an eof other than (f == L - 1).
+// ---------------------------------------------------------------------------
+extern "C" int rdo_event_writeback(const uint32_t* flits, uint64_t n, unsigned packet_w, unsigned port, unsigned seq0,
+                                   uint32_t* ring, uint64_t ring_words, uint64_t addr0, uint64_t* stats) {
+  if (packet_w < 36 || packet_w > 512 || ring_words == 0) return -1;
+  const unsigned L = 1 + (176 + packet_w - 1) / packet_w;
+  const unsigned flit_w = packet_w + 5, fw = (flit_w + 31) / 32;
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint32_t* pkt = flits + (size_t)k * L * fw;
+    auto bit = [&](unsigned f, unsigned i) { return (pkt[(size_t)f * fw + i / 32] >> (i % 32)) & 1u; };
+    unsigned vc0 = 0;
+    for (unsigned f = 0; f < L; ++f) {
+      unsigned ones = 0;
+      for (unsigned i = 0; i < flit_w; ++i) ones += bit(f, i);
+      if ((ones & 1u) == 0) ++stats[1];
+      const unsigned vc = bit(f, 1) | (bit(f, 2) << 1), sof = bit(f, 3), eof = bit(f, 4);
+      if (f == 0) {
+        vc0 = vc;
+        if (!sof) ++stats[2];
+        if (eof) ++stats[2];
+      } else {
+        if (sof) ++stats[2];
+        if (vc != vc0) ++stats[2];
+        if (eof != (f == L - 1 ? 1u : 0u)) ++stats[2];
+      }
+    }
+    uint64_t hdr = 0;
+    for (unsigned i = 0; i < 36; ++i) hdr |= (uint64_t)bit(0, 5 + i) << i;
+    if (((hdr >> 14) & 63u) != L) ++stats[2];
+    if (((hdr >> 11) & 7u) != 0) ++stats[2];
+    if (((hdr >> 9) & 3u) != vc0) ++stats[2];
+    if ((hdr & 15u) != port) ++stats[3];
+    if (((hdr >> 20) & 0xFFFFu) != ((seq0 + k) & 0xFFFFu)) ++stats[4];
+    uint32_t* o = ring + (size_t)((addr0 + k) % ring_words) * 16;
+    for (unsigned j = 0; j < 16; ++j) o[j] = 0;
+    for (unsigned q = 0; q < 176; ++q)
+      if (bit(1 + q / packet_w, 5 + q % packet_w)) o[q / 32] |= 1u << (q % 32);
+    ++stats[0];
+  }
+  return (int)L;
+}
+
+// ---------------------------------------------------------------------------
 // The composed chain for one CPI (builder-defined glue; per-stage semantics as
 // above). Outputs that are null are not kept. Returns the number of events, or
 // a negative value on invalid geometry / event overflow.

@@ -324,4 +324,64 @@ int rdr_event_flits(const uint64_t* events, uint64_t n_events, unsigned packet_w
   return (int)L;
 }
 
+// The reference's receive side: packets built from `events` as rdr_event_flits
+// builds them are injected into a packet_model::Scoreboard (a 32-port,
+// one-stage fabric, so every src < 32 to dest < 16 has a route), then every
+// flit of `flits` [n_packets][L][fw] is observed at egress `port` and the run
+// finished. counts: delivered, lost, duplicated, misrouted, corrupted,
+// reordered, framing, parity, unexpected, injected.
+int rdr_scoreboard(const uint32_t* flits, uint64_t n_packets, unsigned packet_w, unsigned port,
+                   const uint64_t* events, uint64_t n_events, unsigned dest, unsigned src, unsigned vc, unsigned seq0,
+                   uint64_t* counts) {
+  namespace pm = packet_model;
+  if (packet_w < pm::kHdrW || packet_w > 512) return -1;
+  const unsigned L = pm::detection_flits(packet_w);
+  const unsigned fw = (packet_w + pm::kFlitCtrlW + 31) / 32;
+  pm::Scoreboard sb(packet_w, 32, 1);
+  for (uint64_t e = 0; e < n_events; ++e) {
+    const uint64_t* ev = events + 3 * e;
+    pm::Packet p;
+    p.hdr.dest = dest;
+    p.hdr.src = src;
+    p.hdr.vc = vc;
+    p.hdr.ptype = pm::kTypeDetection;
+    p.hdr.length = L;
+    p.hdr.seq = (unsigned)((seq0 + e) & 0xFFFFu);
+    for (unsigned f = 1; f < L; ++f) {
+      std::vector<uint64_t> words;
+      for (unsigned k = 0; k * 64 < packet_w; ++k) {
+        const unsigned n = (packet_w - k * 64 >= 64) ? 64 : packet_w - k * 64;
+        uint64_t v = 0;
+        for (unsigned i = 0; i < n; ++i) {
+          const unsigned b = (f - 1) * packet_w + k * 64 + i;
+          if (b < pm::kDetectionPayloadW && ((ev[b / 64] >> (b % 64)) & 1u)) v |= 1ull << i;
+        }
+        words.push_back(v);
+      }
+      p.payload.push_back(words);
+    }
+    sb.inject(p);
+  }
+  for (uint64_t k = 0; k < n_packets * L; ++k) {
+    pm::Flit fl;
+    fl.resize_for(packet_w);
+    if (fl.w.size() != fw) return -2;
+    for (unsigned j = 0; j < fw; ++j) fl.w[j] = flits[k * fw + j];
+    sb.observe(port, fl);
+  }
+  sb.finish();
+  const pm::Counts& c = sb.counts();
+  counts[0] = c.delivered;
+  counts[1] = c.lost;
+  counts[2] = c.duplicated;
+  counts[3] = c.misrouted;
+  counts[4] = c.corrupted;
+  counts[5] = c.reordered;
+  counts[6] = c.framing;
+  counts[7] = c.parity;
+  counts[8] = c.unexpected;
+  counts[9] = sb.injected();
+  return (int)L;
+}
+
 }  // extern "C"

@@ -9,6 +9,7 @@ flat-spectrum and edge-suppression cases.
 import numpy as np
 import pytest
 
+import flitfault
 import stim
 import vecio
 
@@ -299,6 +300,46 @@ def test_event_flits_decode(orc, packet_w):
                 payload += data
         want = [(int(ev[e, i // 64]) >> (i % 64)) & 1 for i in range(176)]
         assert payload[:176] == want and not any(payload[176:])
+
+
+@pytest.mark.parametrize("packet_w", [36, 64, 100, 176, 512])
+def test_event_writeback_round_trip(orc, packet_w):
+    """Flits -> writeback is the identity on the low 176 event bits: every
+    packet lands zero-extended in its 64-byte word, the counters show only the
+    packets written; a ring shorter than the stream wraps (later packets
+    overwrite earlier ones)."""
+    rng = np.random.default_rng(packet_w + 5)
+    ev = rng.integers(0, 2**63, size=(20, 3), dtype=np.uint64)
+    ev[:, 2] |= np.uint64(3) << np.uint64(48)  # the stream's sof/eof bits are not payload
+    fl = orc.event_flits(ev, packet_w, dest=4, src=9, vc=3, seq0=65530)
+    ring, st = orc.event_writeback(fl, packet_w, port=4, seq0=65530)
+    assert np.array_equal(ring, flitfault.ring_of_events(ev))
+    assert st.tolist() == [20, 0, 0, 0, 0]
+    ring, st = orc.event_writeback(fl, packet_w, port=4, seq0=65530, ring_words=7, addr0=5)
+    assert np.array_equal(ring, flitfault.ring_of_events(ev, 7, 5))
+
+
+# (parity, framing, misrouted, sequence) each planted fault must add: one
+# packet-local violation apiece, derived from the checks' definitions.
+_FAULT_COUNTS = {"data": (1, 0, 0, 0), "parity": (1, 0, 0, 0), "type": (0, 1, 0, 0), "hvc": (0, 1, 0, 0),
+                 "dest": (0, 0, 1, 0), "seq": (0, 0, 0, 1), "len": (0, 1, 0, 0), "sof_body": (0, 1, 0, 0),
+                 "eof_clear": (0, 1, 0, 0), "flit_vc": (0, 1, 0, 0), "hdr_eof": (0, 1, 0, 0), "hdr_sof": (0, 1, 0, 0)}
+
+
+@pytest.mark.parametrize("kind", flitfault.ALL)
+def test_event_writeback_counts_planted_faults(orc, kind):
+    """Each planted fault adds exactly its one count, and only a payload fault
+    changes what lands in the ring (the one flipped bit)."""
+    pw = 64
+    rng = np.random.default_rng(3)
+    ev = rng.integers(0, 2**63, size=(6, 3), dtype=np.uint64)
+    fl = flitfault.plant(orc.event_flits(ev, pw, dest=2, src=1, vc=1, seq0=10), 4, kind, pw)
+    ring, st = orc.event_writeback(fl, pw, port=2, seq0=10)
+    assert tuple(int(x) for x in st[1:]) == _FAULT_COUNTS[kind] and st[0] == 6
+    want = flitfault.ring_of_events(ev)
+    if kind == "data":
+        want[4, 0] ^= np.uint32(1 << 3)
+    assert np.array_equal(ring, want)
 
 
 @pytest.mark.parametrize("H,spc,shift", [(16, 2, False), (64, 1, True), (256, 2, True)])

@@ -144,6 +144,37 @@ def test_event_flits_vs_ref(orc, ref, packet_w):
         assert np.array_equal(a, b), (packet_w, dest, src, vc, seq0)
 
 
+@pytest.mark.parametrize("packet_w", [36, 64, 177, 512])
+def test_event_writeback_vs_scoreboard(orc, ref, packet_w):
+    """The writeback's receive checks against packet_model::Scoreboard
+    (packet_model.hpp:360-429) at egress `port`: a clean stream is delivered
+    whole with no error of any kind, and for each single fault whose count
+    the Scoreboard takes per packet (a payload bit, the parity bit, the
+    header's type, vc, dest, length or seq) the parity, framing and misrouted
+    counts agree, and a broken seq is the Scoreboard's unknown-seq count.
+    Faults in sof/eof/vc bits make the Scoreboard re-frame the stream, which
+    fixed packet slots do not, so they are pinned by the closed-form counts
+    instead."""
+    import flitfault
+
+    rng = np.random.default_rng(packet_w + 11)
+    n, port, src, vc, seq0 = 12, 6, 19, 2, 65533
+    ev = rng.integers(0, 2**63, size=(n, 3), dtype=np.uint64)
+    fl = orc.event_flits(ev, packet_w, dest=port, src=src, vc=vc, seq0=seq0)
+    sb = ref.scoreboard(fl, packet_w, port, ev, dest=port, src=src, vc=vc, seq0=seq0)
+    ring, st = orc.event_writeback(fl, packet_w, port=port, seq0=seq0)
+    assert sb["delivered"] == sb["injected"] == n and sum(sb.values()) == 2 * n
+    assert st.tolist() == [n, 0, 0, 0, 0]
+    assert np.array_equal(ring, flitfault.ring_of_events(ev))
+    for kind in flitfault.AGREEING:
+        bad = flitfault.plant(fl, 5, kind, packet_w)
+        sb = ref.scoreboard(bad, packet_w, port, ev, dest=port, src=src, vc=vc, seq0=seq0)
+        _, st = orc.event_writeback(bad, packet_w, port=port, seq0=seq0)
+        got = (int(st[1]), int(st[2]), int(st[3]), int(st[4]))
+        assert got == (sb["parity"], sb["framing"], sb["misrouted"], sb["duplicated"]), (kind, sb)
+        assert sum(got) == 1, kind
+
+
 @pytest.mark.parametrize("mode,exp_k,acc_w", [(1, 0, 64), (1, 1, 64), (1, 5, 40), (1, 15, 64), (1, 3, 20),
                                               (0, 0, 33), (0, 0, 64)])
 def test_cov_integrate_vs_ref(orc, ref, mode, exp_k, acc_w):
