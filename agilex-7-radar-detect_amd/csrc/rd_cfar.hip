@@ -22,9 +22,23 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "rd_internal.h"
 
 namespace rd {
+
+// f(integral_constant<0>), ..., f(integral_constant<N-1>): an unrolled loop
+// whose index is a compile-time constant in the body.
+template <typename F, uint32_t... I>
+__device__ __forceinline__ void static_for_seq(F&& f, std::integer_sequence<uint32_t, I...>) {
+  (f(std::integral_constant<uint32_t, I>{}), ...);
+}
+template <uint32_t N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<uint32_t, N>{});
+}
 
 // Events staged per row by the first pass of a single-CPI launch (summary
 // included); rows with more are recomputed by the gather pass.
@@ -152,14 +166,37 @@ __device__ __forceinline__ Decision decide(const uint64_t* P, uint32_t n, uint32
   return d;
 }
 
-// CA decision of a bin whose whole window lies inside the row (the chunk
-// test below guarantees it): no clamping or masking, only the hit.
+// The interior CA form reads the prefix at six fixed displacements d from the
+// bin. For bin i = c + lane, c a multiple of 64 (so of S),
+// pidx(i + d) = pidx(c) + pidx(lane + d) (arithmetic shift, d may be
+// negative): the lane term of each displacement is computed once per wave and
+// the chunk's term is uniform, a compile-time step per chunk of the wave.
 template <uint32_t SLOG>
-__device__ __forceinline__ bool ca_hit_inner(const uint64_t* P, uint32_t i, const CfarGeom& g, uint32_t n_ref) {
-  const uint64_t cut = P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)];
-  const uint64_t sum = (P[pidx<SLOG>(i + g.gl + g.rl + 1)] - P[pidx<SLOG>(i + g.gl + 1)]) +
-                       (P[pidx<SLOG>(i - g.gg)] - P[pidx<SLOG>(i - g.gg - g.rg)]);
-  return ((cut * (uint64_t)n_ref) << 8) > (uint64_t)g.alpha * sum;
+__device__ __forceinline__ uint32_t lane_run(uint32_t lane, int d) {
+  const int e = (int)lane + d;
+  return (uint32_t)(SLOG ? e + (e >> SLOG) : e);
+}
+
+// LDS addresses of the six runs for one lane at the wave's first chunk.
+struct CaRuns {
+  const uint64_t *c1, *c0, *a1, *a0, *b1, *b0;  // i+1, i, lead end/start, lag end/start
+};
+
+// CA decision of a bin whose whole window lies inside the row (the chunk test
+// in the caller guarantees it): no clamping or masking, only the hit. OFF is
+// the chunk's step from the wave's first chunk; n8 = (ref_lead + ref_lag) << 8.
+template <typename Cell, uint32_t OFF>
+__device__ __forceinline__ bool ca_hit_inner(const CaRuns& r, uint32_t alpha, uint32_t n8) {
+  uint64_t lhs;
+  if constexpr (sizeof(Cell) == 4) {
+    // one u32 cell: the low words of its two prefix entries differ by it exactly
+    const uint32_t cut = (uint32_t)r.c1[OFF] - (uint32_t)r.c0[OFF];
+    lhs = (uint64_t)cut * n8;
+  } else {
+    lhs = (r.c1[OFF] - r.c0[OFF]) * (uint64_t)n8;
+  }
+  const uint64_t sum = (r.a1[OFF] - r.a0[OFF]) + (r.b1[OFF] - r.b0[OFF]);
+  return lhs > (uint64_t)alpha * sum;
 }
 
 // Bins of a row of n that are not evaluated (window does not fit, or the
@@ -245,17 +282,26 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
   // the unclamped form.
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t c0 = wr * 64 * S;
-  const bool ca = g.enable && g.refs_ok && !g.go;
   const uint32_t n_ref = g.rl + g.rg;
+  const bool ca = g.enable && g.refs_ok && !g.go && n_ref < (1u << 24);
+  const uint64_t* Q = P + pidx<SLOG>(c0);
+  constexpr uint32_t kStep = 64 + (64 >> SLOG);  // pidx advance per chunk of 64 bins
+  CaRuns runs;
+  runs.c1 = Q + lane_run<SLOG>(lane, 1);
+  runs.c0 = Q + lane_run<SLOG>(lane, 0);
+  runs.a1 = Q + lane_run<SLOG>(lane, (int)(g.gl + g.rl + 1));
+  runs.a0 = Q + lane_run<SLOG>(lane, (int)(g.gl + 1));
+  runs.b1 = Q + lane_run<SLOG>(lane, -(int)g.gg);
+  runs.b0 = Q + lane_run<SLOG>(lane, -(int)(g.gg + g.rg));
   uint64_t em[S];
   uint32_t cnt = 0, det = 0;
-#pragma unroll
-  for (uint32_t q = 0; q < S; ++q) {
+  static_for<S>([&](auto qc) {
+    constexpr uint32_t q = decltype(qc)::value;
     const uint32_t cq = c0 + q * 64, i = cq + lane;
     const bool in = work && (full || i < n);
     bool hit;
     if (work && ca && cq >= g.gg + g.rg && (uint64_t)cq + 64 + g.gl + g.rl <= n) {
-      hit = ca_hit_inner<SLOG>(P, i, g, n_ref);
+      hit = ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8);
     } else {
       Decision d{};
       if (in) d = decide<SLOG>(P, n, i, g);
@@ -264,7 +310,7 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
     em[q] = __ballot(in && (hit || g.dense));
     cnt += (uint32_t)__popcll(em[q]);
     if (g.dense) det += (uint32_t)__popcll(__ballot(in && hit));
-  }
+  });
   if (!g.dense) det = cnt;
   uint64_t k = 0, K = cnt;
   uint32_t DET = det;
@@ -321,7 +367,7 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
 // slots, row_count[r] = the row's event count.
 template <typename Cell, bool MULTI, uint32_t SLOG>
 __device__ __forceinline__ void cfar_rows_tile(const CfarJob& J, const CfarRowGeo& G, uint32_t r0, uint64_t* lds) {
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const uint32_t rl = wave / G.W, wr = wave % G.W;
   const uint32_t r = r0 + rl;
   const bool valid = r < J.n_rows;
@@ -336,7 +382,7 @@ __device__ __forceinline__ void cfar_rows_tile(const CfarJob& J, const CfarRowGe
 template <typename Cell, bool MULTI, uint32_t SLOG>
 __device__ __forceinline__ void cfar_gather_tile(const CfarJob& J, const CfarRowGeo& G, uint32_t r0,
                                                  uint64_t* lds) {
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const uint32_t rl = wave / G.W, wr = wave % G.W;
   const uint32_t r = r0 + rl;
   const bool valid = r < J.n_rows;

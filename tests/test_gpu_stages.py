@@ -44,6 +44,30 @@ def test_pfb_random_and_extreme(gpu, orc, P, T, nb, ch):
         _eq(gpu.pfb(x, h, P, T, ch, nb, flags=False)[0], yo, "y without the flag buffer")
 
 
+@pytest.mark.parametrize("P,T,nb,ch", [(32, 16, 257, 2), (64, 15, 255, 1), (96, 16, 513, 1), (32, 1, 3, 3),
+                                       (128, 9, 256, 2), (2048, 16, 7, 1)])
+def test_pfb_beat_counts_and_history(gpu, orc, P, T, nb, ch):
+    """Beat counts around powers of two (255/256/257/513), a run shorter than
+    T, one tap, random and extreme values with and without the flag buffer,
+    and a random extreme imported history whose final state is compared too."""
+    rng = np.random.default_rng(P * 7 + T + nb)
+    for x, h in [(stim.rand_cx(rng, ch * nb * P), stim.rand_cx(rng, P * T)),
+                 (stim.rand_extreme(rng, ch * nb * P), stim.rand_extreme(rng, P * T))]:
+        yg, fg = gpu.pfb(x, h, P, T, ch, nb)
+        yo, fo = orc.pfb(x, h, P, T, ch, nb)
+        _eq(yg, yo, "y")
+        _eq(fg, fo, "flags")
+        _eq(gpu.pfb(x, h, P, T, ch, nb, flags=False)[0], yo, "y without the flag buffer")
+        if T > 1:
+            hist0 = stim.rand_extreme(rng, ch * (T - 1) * P)
+            hg, ho = hist0.copy(), hist0.copy()
+            yg, fg = gpu.pfb(x, h, P, T, ch, nb, hist=hg)
+            yo, fo = orc.pfb(x, h, P, T, ch, nb, hist=ho)
+            _eq(yg, yo, "y with history")
+            _eq(fg, fo, "flags with history")
+            _eq(hg, ho, "history out")
+
+
 def test_pfb_long_run_chunks(gpu, orc):
     """Many beats per channel so the launch splits beats into chunks with warm-up."""
     rng = np.random.default_rng(5)
