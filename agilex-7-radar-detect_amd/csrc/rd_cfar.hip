@@ -410,33 +410,54 @@ __device__ __forceinline__ void cfar_gather_tile(const CfarJob& J, const CfarRow
 }
 
 // Pass 2, one block of 1024: exclusive scan of the row counts into 64-bit
-// offsets; offs[n_rows] = total. Each thread sums a contiguous run of rows,
-// the waves scan with shuffles and one barrier combines the 16 wave totals.
+// offsets; offs[n_rows] = total. Passes of up to 1024 * kScanPer rows: each
+// thread takes a contiguous run of at most kScanPer rows, all its loads issued
+// at once and kept in registers, the waves scan their run sums with shuffles,
+// and one barrier combines the 16 wave totals.
+constexpr uint32_t kScanPer = 16;
+
 __device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
   __shared__ uint64_t wsum[16];
   const uint32_t n_rows = J.n_rows;
   const uint32_t* __restrict__ row_count = J.row_count;
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t per = (n_rows + 1023) / 1024;
-  const uint32_t r0 = t * per;
-  uint64_t s = 0;
-  for (uint32_t q = 0; q < per; ++q)
-    if (r0 + q < n_rows) s += row_count[r0 + q];
-  const uint64_t x = wave_incscan(s, lane);
-  if (lane == 63) wsum[wave] = x;
-  __syncthreads();
-  uint64_t wb = 0;
-  for (uint32_t w = 0; w < wave; ++w) wb += wsum[w];
-  uint64_t run = wb + x - s;
-  for (uint32_t q = 0; q < per; ++q) {
-    if (r0 + q < n_rows) {
-      J.offs[r0 + q] = run;
-      run += row_count[r0 + q];
+  uint64_t carry = 0;
+  for (uint32_t c0 = 0; c0 < n_rows; c0 += 1024 * kScanPer) {
+    const uint32_t rows = min(n_rows - c0, 1024u * kScanPer);
+    const uint32_t per = (rows + 1023) / 1024;
+    const uint32_t r0 = c0 + t * per;
+    const uint32_t lim = r0 < c0 + rows ? min(per, c0 + rows - r0) : 0u;  // rows of this thread
+    const uint32_t* rc = row_count + r0;
+    uint64_t* of = J.offs + r0;
+    uint32_t v[kScanPer];
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+      v[q] = q < lim ? rc[q] : 0u;
+      s += v[q];
     }
+    const uint64_t x = wave_incscan(s, lane);
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint64_t wb = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 16; ++w) {
+      const uint64_t ws = wsum[w];
+      wb += w < wave ? ws : 0;
+      all += ws;
+    }
+    uint64_t run = carry + wb + x - s;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+      if (q < lim) of[q] = run;
+      run += v[q];
+    }
+    carry += all;
+    __syncthreads();  // wsum is rewritten by the next pass
   }
-  if (t == 1023) {
-    J.offs[n_rows] = wb + x;
-    if (J.total) *J.total = wb + x;
+  if (t == 0) {
+    J.offs[n_rows] = carry;
+    if (J.total) *J.total = carry;
   }
 }
 

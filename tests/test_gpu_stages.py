@@ -243,6 +243,21 @@ def test_cfar_u64(gpu, orc, cfg, n):
     _eq(eg, eo, "events")
 
 
+def test_cfar_many_rows_across_calls(gpu, orc):
+    """Thousands of rows (hundreds of row tiles, whose event offsets chain from
+    tile to tile) in consecutive calls of different sizes on one context, so
+    each call starts from the state the previous one left."""
+    rng = np.random.default_rng(11)
+    cfg = [1, 0, 0, 2, 2, 8, 8, 0x500]
+    for rows, n in [(5000, 100), (1200, 64), (5003, 100), (300, 4096), (4096, 1024)]:
+        cells = rng.integers(0, 2**31, rows * n).astype(np.uint32)
+        eg, rcg, tot, _ = gpu.cfar(cells, n, cfg, 31, 63, frame_mod=7)
+        eo, rco = orc.cfar(cells.astype(np.uint64), n, cfg, 31, 63, frame_ids=np.arange(rows) % 7)
+        _eq(rcg, rco, f"row_count rows={rows} n={n}")
+        assert tot == eo.shape[0]
+        _eq(eg, eo, f"events rows={rows} n={n}")
+
+
 def test_cfar_u32_frame_mod_and_capacity(gpu, orc):
     import rd_amd
 
