@@ -465,21 +465,27 @@ __device__ __forceinline__ uint32_t U2(s16x2 v) { return __builtin_bit_cast(uint
 // Saturation evidence of one packed sub-stage: bits 15 / 31 of `ev` mark an
 // upward clamp in the low / high half; mx, mn bound the twiddle products
 // before saturation.
-struct PkFlags {
+// ON = false: the caller keeps no stage flags, and the evidence is not formed
+// (the data path is the same).
+template <bool ON>
+struct PkFlagsT {
+  static constexpr bool on = ON;
   uint32_t ev = 0;
   int32_t mx = -32768, mn = 32767;
   __device__ __forceinline__ uint32_t word() const {
+    if constexpr (!ON) return 0u;
     return ((ev & 0x80008000u) != 0u || mx > 32767 ? 2u : 0u) | (mn < -32768 ? 1u : 0u);
   }
 };
 
-__device__ __forceinline__ void bfly_pk(uint32_t& a, uint32_t& b, PkFlags& f) {
+template <class F>
+__device__ __forceinline__ void bfly_pk(uint32_t& a, uint32_t& b, F& f) {
   const uint32_t x = a ^ b;
   const s16x2 sx = S2(x) >> (s16x2){1, 1};
   const uint32_t hs = U2(S2(a & b) + sx);
   const uint32_t hd = U2(S2(a | ~b) + sx + (s16x2){1, 1});
   const uint32_t ts = x & hs & 0x00010001u, td = x & hd & 0x00010001u;
-  f.ev |= U2(S2(hd) + S2(td)) & ~hd;
+  if constexpr (F::on) f.ev |= U2(S2(hd) + S2(td)) & ~hd;
   a = U2(S2(hs) + S2(ts));
   b = U2(__builtin_elementwise_add_sat(S2(hd), S2(td)));
 }
@@ -487,10 +493,11 @@ __device__ __forceinline__ void bfly_pk(uint32_t& a, uint32_t& b, PkFlags& f) {
 // -j * (re + j im) = im - j re: the halves swap and the new high half is
 // negated with saturation (fft_ref.hpp:322-331); the wrapping and the
 // saturating difference disagree (in bit 31) exactly when re = -32768.
-__device__ __forceinline__ uint32_t negj_pk(uint32_t c, PkFlags& f) {
+template <class F>
+__device__ __forceinline__ uint32_t negj_pk(uint32_t c, F& f) {
   const s16x2 lo = S2(c >> 16), hi = S2(c << 16);
   const s16x2 r = __builtin_elementwise_sub_sat(lo, hi);
-  f.ev |= U2(lo - hi) ^ U2(r);
+  if constexpr (F::on) f.ev |= U2(lo - hi) ^ U2(r);
   return U2(r);
 }
 
@@ -508,12 +515,15 @@ __device__ __forceinline__ uint32_t mul_w0_pk(uint32_t x) {
 
 // x * w on a packed x (cmul_tw's arithmetic), saturated into a packed word
 // by v_cvt_pk_i16_i32 (both halves clamped to int16).
-__device__ __forceinline__ uint32_t cmul_pk(uint32_t x, uint32_t wc, uint32_t ws, PkFlags& f) {
+template <class F>
+__device__ __forceinline__ uint32_t cmul_pk(uint32_t x, uint32_t wc, uint32_t ws, F& f) {
   const int32_t re = dot2(x, wc, (int32_t)x >> 16);
   const int32_t im = dot2_0(x, ws);
   const int32_t yr = rne15(re), yi = rne15(im);
-  f.mx = max(f.mx, max(yr, yi));
-  f.mn = min(f.mn, min(yr, yi));
+  if constexpr (F::on) {
+    f.mx = max(f.mx, max(yr, yi));
+    f.mn = min(f.mn, min(yr, yi));
+  }
   return U2(__builtin_amdgcn_cvt_pk_i16(yr, yi));
 }
 
@@ -549,9 +559,10 @@ __device__ __forceinline__ void fft_stage_twiddles(const uint32_t* __restrict__ 
 
 // One tile: frames [grp*FPB, +FPB) of job J, twiddles already in LDS. Every
 // thread of the block calls it (it holds block barriers).
-template <int LOG2N, int SPC, bool ALL1>
+template <int LOG2N, int SPC, bool ALL1, bool FL>
 __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t* smem) {
   using G = FftGeo<LOG2N>;
+  using PkFlags = PkFlagsT<FL>;
   constexpr uint32_t N = G::N;
   constexpr uint32_t M = N / SPC;
   constexpr uint32_t NL = LOG2N - (SPC == 2 ? 1 : 0);
@@ -753,9 +764,10 @@ constexpr int fft_ppt(int log2n, bool all1) { return (all1 && log2n >= 6) ? 16 :
 // (exponent r * {0, 2, 1, 3}[q2]); the arithmetic of every sub-stage is the
 // radix-4 pass's. Leftover sub-stages run as one radix-4 pass over the
 // thread's four groups, then the final pass as in fft_tile.
-template <int LOG2N, int SPC>
+template <int LOG2N, int SPC, bool FL>
 __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32_t* smem) {
   using G = FftGeo<LOG2N, 16>;
+  using PkFlags = PkFlagsT<FL>;
   constexpr uint32_t N = G::N;
   constexpr uint32_t M = N / SPC;
   constexpr uint32_t NL = LOG2N - (SPC == 2 ? 1 : 0);
@@ -1003,21 +1015,23 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
 template <int LOG2N, bool ALL1>
 using FftGeoOf = FftGeo<LOG2N, fft_ppt(LOG2N, ALL1)>;
 
-template <int LOG2N, int SPC, bool ALL1>
+// FL: the stage flags are formed (the packed all-ones path forms them only
+// when a caller keeps them; the general path always does).
+template <int LOG2N, int SPC, bool ALL1, bool FL = true>
 __device__ __forceinline__ void fft_tile_any(const FftJob& J, uint32_t grp, uint32_t* smem) {
   if constexpr (fft_ppt(LOG2N, ALL1) == 16)
-    fft_tile16<LOG2N, SPC>(J, grp, smem);
+    fft_tile16<LOG2N, SPC, FL>(J, grp, smem);
   else
-    fft_tile<LOG2N, SPC, ALL1>(J, grp, smem);
+    fft_tile<LOG2N, SPC, ALL1, FL>(J, grp, smem);
 }
 
 // Single CPI: one block per tile.
-template <int LOG2N, int SPC, bool ALL1>
+template <int LOG2N, int SPC, bool ALL1, bool FL>
 __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB)) void fft_kernel(
     FftJob J, const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
   fft_stage_twiddles<LOG2N>(tw4096, smem);
-  fft_tile_any<LOG2N, SPC, ALL1>(J, blockIdx.x, smem);
+  fft_tile_any<LOG2N, SPC, ALL1, FL>(J, blockIdx.x, smem);
 }
 
 // Batched: one block per frame group of a class of CPIs (same length and lane
@@ -1034,20 +1048,22 @@ __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>
   fft_tile_any<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
 }
 
-template <int LOG2N, int SPC, bool ALL1>
+template <int LOG2N, int SPC, bool ALL1, bool FL>
 static void launch_fft_g(hipStream_t s, const uint32_t* tw4096, const FftJob& J) {
   using G = FftGeoOf<LOG2N, ALL1>;
   const dim3 grid((J.n_frames + G::FPB - 1) / G::FPB), block(G::TPF * G::FPB);
-  fft_kernel<LOG2N, SPC, ALL1><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
+  fft_kernel<LOG2N, SPC, ALL1, FL><<<grid, block, G::lds_bytes, s>>>(J, tw4096);
 }
 
 template <int LOG2N, int SPC>
 static hipError_t launch_fft_n(hipStream_t s, const uint32_t* tw4096, const FftJob& J) {
   constexpr uint32_t used = (1u << LOG2N) - 1u;
-  if ((J.sched & used) == used)
-    launch_fft_g<LOG2N, SPC, true>(s, tw4096, J);
+  if ((J.sched & used) != used)
+    launch_fft_g<LOG2N, SPC, false, true>(s, tw4096, J);
+  else if (J.stage_flags)
+    launch_fft_g<LOG2N, SPC, true, true>(s, tw4096, J);
   else
-    launch_fft_g<LOG2N, SPC, false>(s, tw4096, J);
+    launch_fft_g<LOG2N, SPC, true, false>(s, tw4096, J);
   return hipGetLastError();
 }
 

@@ -270,14 +270,17 @@ class Ctx:
             hist[...] = dh.download(np.uint32, hist.size)
         return y, f
 
-    def fft(self, x, n, spc=2, sched=0xFFFFFFFF, reorder=1):
+    def fft(self, x, n, spc=2, sched=0xFFFFFFFF, reorder=1, flags=True):
+        """rd_fft_run -> (out, bins, stage flags or None when flags=False)."""
         x = np.asarray(x, np.uint32)
         frames = x.size // n
         dx = self.put(x)
-        do, db, dfl = self.buf(4 * x.size), self.buf(4 * x.size), self.buf(4 * max(frames, 1))
+        do, db = self.buf(4 * x.size), self.buf(4 * x.size)
+        dfl = self.buf(4 * max(frames, 1)) if flags else None
         self.lib.check("rd_fft_run", self.L.rd_fft_run(self.h, dx.ptr, n, spc, sched, reorder, frames, do.ptr, db.ptr,
-                                                       dfl.ptr))
-        return do.download(np.uint32, x.size), db.download(np.uint32, x.size), dfl.download(np.uint32, frames)
+                                                       dfl.ptr if dfl else None))
+        return (do.download(np.uint32, x.size), db.download(np.uint32, x.size),
+                dfl.download(np.uint32, frames) if dfl else None)
 
     def align(self, X, n_chan, n_frames, n_bins):
         dX = self.put(np.asarray(X, np.uint32))

@@ -53,19 +53,23 @@ def algo_chain_bytes(sc, n_events: int) -> int:
             + 22 * n_events)
 
 
-def stage_kernels(ctx, ch, sc):
-    """Per-stage launchers (device buffers of `ch`), for the per-kernel timing."""
+def stage_kernels(ctx, ch, sc, lean=True):
+    """Per-stage launchers (device buffers of `ch`), for the per-kernel timing.
+    lean: the stages as the timed (lean) chain graph runs them, without the
+    PFB and FFT flag outputs."""
     L, h, b = ctx.L, ctx.h, ch.b
     S = sc.CH * sc.N * sc.F
     cols = sc.N * sc.F
+    pfb_flags = None if lean or not b["pfb_flags"] else b["pfb_flags"].ptr
+    fft_flags = None if lean or not b["fft_flags"] else b["fft_flags"].ptr
 
     def pfb():
         ctx.lib.check("pfb", L.rd_pfb_run(h, b["iq"].ptr, b["taps"].ptr, sc.N, sc.T, sc.CH, sc.F, None, b["u"].ptr,
-                                          b["pfb_flags"].ptr if b["pfb_flags"] else None))
+                                          pfb_flags))
 
     def fft():
         ctx.lib.check("fft", L.rd_fft_run(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, 1, sc.CH * sc.F, None, b["X"].ptr,
-                                          b["fft_flags"].ptr if b["fft_flags"] else None))
+                                          fft_flags))
 
     def bf():
         # the chain's beamformer: FFT layout in, power + aligned copy out (lean outputs)
@@ -76,10 +80,10 @@ def stage_kernels(ctx, ch, sc):
         ctx.lib.check("cov", L.rd_cov_run(h, b["Xa"].ptr, sc.CH, sc.F, sc.N, b["R"].ptr))
 
     # algorithmic bytes per launch of each stage kernel (its compulsory reads + writes)
-    flag_b = 1 if b["pfb_flags"] else 0
+    flag_b = 1 if pfb_flags else 0
     stages = {
         "pfb": (pfb, S * (4 + 4 + flag_b) + 4 * sc.N * sc.T),
-        "fft": (fft, S * 8 + 4 * sc.CH * sc.F),
+        "fft": (fft, S * 8 + (4 * sc.CH * sc.F if fft_flags else 0)),
         "bf": (bf, S * 4 + 4 * sc.B * sc.CH + S * 4 + sc.B * cols * 4),
         "cov": (cov, S * 4 + 16 * sc.N * sc.CH * (sc.CH + 1) // 2),
     }

@@ -116,6 +116,10 @@ def test_fft_random(gpu, orc, n, spc):
                 o = orc.fft(x, n, spc, sched, reorder)
                 for a, b, w in zip(g, o, ("out", "bins", "stage_flags")):
                     _eq(a, b, f"{w} n={n} spc={spc} sched={sched:#x} reorder={reorder}")
+                # without a stage-flag buffer (the all-ones path then forms no flags)
+                g = gpu.fft(x, n, spc, sched, reorder, flags=False)
+                for a, b, w in zip(g[:2], o[:2], ("out", "bins")):
+                    _eq(a, b, f"{w} without flags n={n} spc={spc} sched={sched:#x} reorder={reorder}")
 
 
 def twiddle_saturation_frame(n):
@@ -137,6 +141,9 @@ def test_fft_twiddle_saturation(gpu, orc, n, spc):
     g = gpu.fft(x, n, spc, 0xFFFFFFFF, 1)
     for a, b, w in zip(g, o, ("out", "bins", "stage_flags")):
         _eq(a, b, f"{w} n={n} spc={spc}")
+    g = gpu.fft(x, n, spc, 0xFFFFFFFF, 1, flags=False)
+    for a, b, w in zip(g[:2], o[:2], ("out", "bins")):
+        _eq(a, b, f"{w} without flags n={n} spc={spc}")
 
 
 def test_fft_rejects_invalid_geometry(gpu):
