@@ -50,9 +50,17 @@ const uint32_t* host_twiddles() {
   return table;
 }
 
-rd_status grow(void** buf, size_t* have, size_t need) {
+// A context buffer of at least `need` bytes. An outgrown buffer is freed at
+// once unless a graph of the context is alive (it may have captured the
+// pointer); then it is kept until the context is destroyed.
+rd_status grow(rd_ctx* c, void** buf, size_t* have, size_t need) {
   if (*have >= need) return RD_OK;
-  if (*buf) RD_TRY(hipFree(*buf));
+  if (*buf) {
+    if (c->live_graphs > 0)
+      c->retired.push_back(*buf);
+    else
+      RD_TRY(hipFree(*buf));
+  }
   *buf = nullptr;
   *have = 0;
   RD_TRY(hipMalloc(buf, need));
@@ -147,6 +155,7 @@ rd_status rd_destroy(rd_ctx* c) {
   if (c->total_dev) (void)hipFree(c->total_dev);
   if (c->total_host) (void)hipHostFree(c->total_host);
   if (c->dop) (void)hipFree(c->dop);
+  for (void* p : c->retired) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return RD_OK;
@@ -229,7 +238,7 @@ rd_status rd_bf_run(rd_ctx* c, const uint32_t* Xa, const uint32_t* W, uint32_t n
   rd_status st = enter(c);
   if (st != RD_OK) return st;
   if (!Xa || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
-  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(n_ant, n_beams));
+  st = grow(c, &c->prep, &c->prep_bytes, rd::bf_prep_bytes(n_ant, n_beams));
   if (st != RD_OK) return st;
   return fin(rd::launch_bf(c->stream, Xa, W, n_ant, n_beams, n_cols, y, flags, power, acc, c->prep));
 }
@@ -249,7 +258,7 @@ rd_status rd_doppler_run(rd_ctx* c, const uint32_t* ring, uint32_t rows, uint32_
   if (st != RD_OK) return st;
   if (!ring || !out || depth < 1 || newest >= depth || n_dop > depth) return RD_E_ARG;
   if (!pow2(n_dop) || n_dop < 8 || n_dop > 4096 || (spc != 1 && spc != 2) || n_dop / spc < 4) return RD_E_GEOM;
-  st = grow(&c->dop, &c->dop_bytes, rd::doppler_scratch_bytes(rows, n_bins, n_dop));
+  st = grow(c, &c->dop, &c->dop_bytes, rd::doppler_scratch_bytes(rows, n_bins, n_dop));
   if (st != RD_OK) return st;
   return fin(rd::launch_doppler(c->stream, c->tw, ring, rows, depth, n_bins, newest, n_dop, spc, scale_sched, shift,
                                 out, stage_flags, c->dop));
@@ -281,7 +290,7 @@ rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_
   if (!cells || !cfg || !row_count || (cell_bytes != 4 && cell_bytes != 8) || n_bins < 1 || n_bins > 8191)
     return RD_E_ARG;
   if (!events && events_cap) return RD_E_ARG;
-  st = grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_rows));
+  st = grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_rows));
   if (st != RD_OK) return st;
   RD_TRY(rd::launch_cfar(c->stream, cells, cell_bytes, n_bins, n_rows, *cfg, max_guard, max_ref, frame_ids,
                          frame_mod, events, events_cap, row_count, c->scratch, c->total_dev));
@@ -302,9 +311,9 @@ rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
   if (st != RD_OK) return st;
   st = rd::check_cpi_bufs(b);
   if (st != RD_OK) return st;
-  st = grow(&c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
+  st = grow(c, &c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
   if (st != RD_OK) return st;
-  return grow(&c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_beams * d->n_frames));
+  return grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_beams * d->n_frames));
 }
 
 // The chain's launches on the context stream (prepared CPI).

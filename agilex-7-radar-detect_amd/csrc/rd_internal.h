@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/rd_amd.h"
 
 // The context behind rd_ctx* (rd_capi.hip).
@@ -24,6 +26,7 @@ struct rd_ctx {
   int live_graphs = 0;             // graphs and batches hold the context's stream
   void* dop = nullptr;             // slow-time transform columns
   size_t dop_bytes = 0;
+  std::vector<void*> retired;      // outgrown buffers a live graph may still use
 };
 
 namespace rd {

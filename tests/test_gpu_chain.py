@@ -114,3 +114,37 @@ def test_chain_fed_from_an_iq_ascii_scene(gpu, orc, tmp_path):
     assert n == exp["n_events"]
     for k in ("power", "R", "events"):
         assert np.array_equal(got[k], exp[k]), k
+
+
+def test_chain_graph_survives_context_buffer_growth(gpu, orc):
+    """A graph keeps the context scratch it captured: a later call that needs a
+    larger CFAR scratch on the same context leaves that buffer alive (until
+    the context is destroyed), so replays after the growth stay exact."""
+    ctx = rd_amd.Ctx(gpu.lib)
+    try:
+        sc = rd_scenario.config("tiny")
+        taps, W = sc.taps(), sc.weights()
+        iq = sc.iq(seed=21)
+        ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg())
+        ch.load(iq, taps, W)
+        g = ch.graph()
+        exp = orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), 31, 63, keep=("power", "events"))
+        rng = np.random.default_rng(3)
+        rows, n = 3000, 64  # far more CFAR rows than the chain's: the scratch grows
+        cells = rng.integers(0, 2**31, rows * n).astype(np.uint32)
+        cfg = [1, 0, 0, 2, 2, 8, 8, 0x500]
+        for rep in range(2):
+            g.launch()
+            ctx.sync()
+            eg, _, tot, _ = ctx.cfar(cells, n, cfg, 31, 63, frame_mod=5)
+            eo, _ = orc.cfar(cells.astype(np.uint64), n, cfg, 31, 63, frame_ids=np.arange(rows) % 5)
+            assert tot == eo.shape[0] and np.array_equal(eg, eo), rep
+            g.launch()
+            ctx.sync()
+            k = int(ch.total.download(np.uint64, 1)[0])
+            got = ch.fetch(("power",), n_events=k)
+            assert k == exp["n_events"] and np.array_equal(got["events"], exp["events"]), rep
+            assert np.array_equal(got["power"], exp["power"]), rep
+        g.close()
+    finally:
+        ctx.close()
