@@ -478,7 +478,7 @@ class Chain:
         rc = self.ctx.L.rd_chain_graph(self.ctx.h, C.byref(self.desc), C.byref(bufs),
                                        total_ptr if total_ptr is not None else self.total.ptr, C.byref(g))
         self.ctx.lib.check("rd_chain_graph", rc)
-        return Graph(self.ctx, g.value)
+        return Graph(self.ctx, g.value, keep=self)
 
     def fetch(self, names=("u", "pfb_flags", "X", "fft_flags", "Xa", "y", "yflags", "power", "R"), n_events=None):
         CH, N, F, B = (self.shape[k] for k in ("CH", "N", "F", "B"))
@@ -499,8 +499,11 @@ class Chain:
 
 
 class Graph:
-    def __init__(self, ctx: Ctx, h):
-        self.ctx, self.h = ctx, h
+    """A captured chain. `keep` (the Chain) stays referenced while the graph
+    lives: the graph replays into its device buffers."""
+
+    def __init__(self, ctx: Ctx, h, keep=None):
+        self.ctx, self.h, self._keep = ctx, h, keep
         ctx.graphs.add(self)
 
     def launch(self):

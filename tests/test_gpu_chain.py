@@ -148,3 +148,27 @@ def test_chain_graph_survives_context_buffer_growth(gpu, orc):
         g.close()
     finally:
         ctx.close()
+
+
+def test_graph_keeps_its_chain_alive(gpu, orc):
+    """A graph replays into its chain's device buffers, so the Python graph
+    object holds the chain: dropping every other reference to the chain and
+    collecting garbage leaves the buffers in place."""
+    import gc
+
+    sc = rd_scenario.config("tiny")
+    taps, W = sc.taps(), sc.weights()
+    iq = sc.iq(seed=31)
+    ch = rd_amd.Chain(gpu, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg())
+    ch.load(iq, taps, W)
+    g = ch.graph()
+    total = ch.total
+    del ch
+    gc.collect()
+    g.launch()
+    gpu.sync()
+    exp = orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), 31, 63, keep=("events",))
+    assert int(total.download(np.uint64, 1)[0]) == exp["n_events"]
+    got = g._keep.fetch((), n_events=exp["n_events"])
+    assert np.array_equal(got["events"], exp["events"])
+    g.close()
