@@ -638,6 +638,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--workload", default="medium", choices=["tiny", "medium", "large"])
     ap.add_argument("--ring", type=int, default=32, help="distinct input CPIs cycled through")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="contexts (one HIP stream each) the timed CPIs are dealt to round-robin")
     ap.add_argument("--cpu-seconds", type=float, default=3.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
@@ -745,11 +747,53 @@ def main():
     def run_stream(i):
         ch.enqueue(iq_ptr=ring[i % args.ring].ptr, total_ptr=totals.ptr + 8 * (i % args.ring))
 
-    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay)")
+    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay, one stream)")
     timed(run_graph, args.warmup)
     dt, host_dt, gpu_dt = timed(run_graph, args.steps)
-    ms_per_step = dt * 1e3 / args.steps
-    value = args.steps * sc.samples / dt / 1e6
+    one_ms = dt * 1e3 / args.steps
+
+    # the same lean graphs on args.streams contexts (one HIP stream each),
+    # CPIs dealt round-robin, so one CPI's short tail kernels overlap the
+    # next CPI's stages; every context has its own chain buffers and totals
+    n_str = max(1, args.streams)
+    ctxs = [ctx] + [rd_amd.Ctx(ctx.lib) for _ in range(n_str - 1)]
+    chains, s_graphs, s_totals = [ch], [graphs], [totals]
+    for c in ctxs[1:]:
+        chk = rd_amd.Chain(c, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
+                           rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True)
+        chk.load(sc.iq(seed=0), taps, W)
+        tk = c.buf(8 * args.ring)
+        chains.append(chk)
+        s_totals.append(tk)
+        s_graphs.append([chk.graph(iq_ptr=ring[i].ptr, total_ptr=tk.ptr + 8 * i, lean=True)
+                         for i in range(args.ring)])
+
+    def timed_streams(steps):
+        for c in ctxs:
+            c.sync()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            s_graphs[i % n_str][(i // n_str) % args.ring].launch()
+        for c in ctxs:
+            c.sync()
+        return time.perf_counter() - t0
+
+    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay over {n_str} streams)")
+    timed_streams(args.warmup)
+    dt_s = timed_streams(args.steps)
+    ms_per_step = dt_s * 1e3 / args.steps
+    value = args.steps * sc.samples / dt_s / 1e6
+    # every context saw the same CPIs: their event totals must agree
+    tot0 = totals.download(np.uint64, args.ring)
+    used = min(args.ring, -(-args.steps // n_str))
+    for tk in s_totals[1:]:
+        if not np.array_equal(tk.download(np.uint64, args.ring)[:used], tot0[:used]):
+            sys.exit("event totals differ between the streams' contexts")
+    for gl in s_graphs[1:]:
+        for g in gl:
+            g.close()
+    for c in ctxs[1:]:
+        c.close()
     note("full-output graphs")
     timed(run_graph_full, min(args.warmup, 8))
     f_dt, f_host, f_gpu = timed(run_graph_full, args.steps)
@@ -786,8 +830,10 @@ def main():
         "data": "synthetic (seeded complex Gaussian noise + plane-wave targets, ring of distinct CPIs)",
         "config": {"workload": f"{sc.name}: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, "
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
-                   "ring": args.ring, "impl": "one hipGraph per CPI: PFB, FFT, BF+power MFMA (writes the aligned "
-                                              "copy), cov MFMA, CFAR rows/scan/gather",
+                   "ring": args.ring, "streams": n_str,
+                   "impl": "one hipGraph per CPI: PFB, FFT, BF+power MFMA (writes the aligned copy), cov MFMA, "
+                           f"CFAR rows/scan/gather; CPIs dealt round-robin to {n_str} contexts (one HIP stream "
+                           "each)",
                    "outputs": "events + covariance timed; every stage output verified bit-exact on a separate "
                               "CPI each run (timing.graph_full_outputs also writes them all)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -795,8 +841,10 @@ def main():
                      "valu_frac": valu_frac(prof["valu_insts"], per[dom]["ms"]) if prof else None},
         "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                   "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "timing": {"graph": {"wall_ms": round(ms_per_step, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
-                             "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4)},
+        "timing": {"graph_streams": {"streams": n_str, "wall_ms": round(ms_per_step, 4)},
+                   "graph": {"wall_ms": round(one_ms, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
+                             "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4),
+                             "msamples_s": round(args.steps * sc.samples / dt / 1e6, 2)},
                    "graph_full_outputs": {"wall_ms": round(f_dt * 1e3 / args.steps, 4),
                                           "gpu_ms": round(f_gpu * 1e3 / args.steps, 4),
                                           "msamples_s": round(args.steps * sc.samples / f_dt / 1e6, 2)},
