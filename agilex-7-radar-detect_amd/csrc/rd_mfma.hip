@@ -62,18 +62,18 @@ __device__ __forceinline__ int64_t combine(v4i hh, v4i x, v4i ll, int r) {
 }
 
 // round_sat(v, 15, 16) of v = 65536*hh + 256*x + ll + 128*s + c in 32-bit
-// arithmetic. With 128*s = 2^15*(s >> 8) + 128*(s & 255):
-//   floor(v / 2^15) = 2*hh + (s >> 8) + floor(R / 2^15),  v mod 2^15 = R mod 2^15,
-//   R = 256*x + ll + 128*(s & 255) + c,
+// arithmetic. With 128*s = 2^15*(s >> 8) + 128*(s & 255), v = 2^15*A + R:
+//   A = 2*hh + (s >> 8),  R = 256*x + ll + 128*(s & 255) + c,
 // and R fits in int32 for the beamformer's ranges (|x| <= 2^22, |ll| <= 2^21,
-// |c| <= 2^22). Round half to even on the remainder, then saturate.
+// |c| <= 2^22). Round half to even is floor((v + 0x3FFF + p) / 2^15) with p
+// the parity of floor(v / 2^15) = A + (R >> 15), that is
+// A + ((R + 0x3FFF + p) >> 15); then saturate.
 __device__ __forceinline__ int32_t rne15_sat_parts(int32_t hh, int32_t x, int32_t ll, int32_t s, int32_t c,
                                                    uint32_t& fl) {
   const int32_t R = (x << 8) + ll + ((s & 255) << 7) + c;
-  int32_t q = 2 * hh + (s >> 8) + (R >> 15);
-  const int32_t rem = R & 0x7FFF;
-  q += (rem + (q & 1)) > 0x4000 ? 1 : 0;
-  return sat16(q, fl);
+  const int32_t A = 2 * hh + (s >> 8);
+  const int32_t p = ((R >> 15) ^ A) & 1;
+  return sat16(A + ((R + 0x3FFF + p) >> 15), fl);
 }
 
 // Beamformer outputs as a compile-time set (bits: 1 y, 2 flags, 4 power,
@@ -299,6 +299,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
         ill = mfma8(aim_l, bil[s], ill);
       }
       if (!col_ok) continue;
+      const size_t o0 = (size_t)(bt * 16 + 4 * grp) * cols + col;  // beam 4 grp of the tile, this column
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const uint32_t b = bt * 16 + 4 * grp + r;
@@ -307,7 +308,7 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
         uint32_t fr = 0, fi = 0;
         const int32_t yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
         const int32_t yi = rne15_sat_parts(ihh[r], ix[r], ill[r], sa + sb_im, -kconst, fi);
-        const size_t o = (size_t)b * cols + col;
+        const size_t o = o0 + (size_t)r * cols;
         if (out & kBfOutY) J.y[o] = cpack(yr, yi);
         if (out & kBfOutF) J.flags[o] = (uint8_t)(fr | (fi << 2));
         if (out & kBfOutP) J.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);

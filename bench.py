@@ -785,7 +785,7 @@ def main():
     value = args.steps * sc.samples / dt_s / 1e6
     # every context saw the same CPIs: their event totals must agree
     tot0 = totals.download(np.uint64, args.ring)
-    used = min(args.ring, -(-args.steps // n_str))
+    used = min(args.ring, args.steps // n_str)  # ring slots every context filled
     for tk in s_totals[1:]:
         if not np.array_equal(tk.download(np.uint64, args.ring)[:used], tot0[:used]):
             sys.exit("event totals differ between the streams' contexts")
