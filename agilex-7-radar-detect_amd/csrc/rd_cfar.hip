@@ -347,7 +347,19 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
     const uint32_t e = e0 + lane;
     if (e < cnt) {
       const uint32_t i = lst[e];
-      const Decision d = decide<SLOG>(P, n, i, g);
+      Decision d;
+      if (ca && !g.dense) {
+        // a CA event without the dense mode is a detection, so its window
+        // fits the row: the sums need no clamping
+        d.cut = P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)];
+        const uint64_t lead = g.rl ? P[pidx<SLOG>(i + g.gl + g.rl + 1)] - P[pidx<SLOG>(i + g.gl + 1)] : 0ull;
+        const uint64_t lag = g.rg ? P[pidx<SLOG>(i - g.gg)] - P[pidx<SLOG>(i - g.gg - g.rg)] : 0ull;
+        d.sum = lead + lag;
+        d.n = n_ref;
+        d.eval = d.hit = true;
+      } else {
+        d = decide<SLOG>(P, n, i, g);
+      }
       const uint64_t pos = k + e;
       const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
       if (pos < room)
