@@ -206,6 +206,8 @@ rd_status rd_event_flits(rd_ctx* c, const uint64_t* events, uint64_t n_events, c
   if (st != RD_OK) return st;
   if (packet_w < 36 || packet_w > 512 || dest > 15 || src > 31 || vc > 3) return RD_E_ARG;
   if (n_events && (!events || !flits)) return RD_E_ARG;
+  // One work-item per flit; a dispatch holds fewer than 2^32 work-items.
+  if (n_events > (1ull << 31) || n_events * rd_event_flit_count(packet_w) > (1ull << 31)) return RD_E_ARG;
   return rd::launch_event_flits(c->stream, events, n_events, n_events_dev, packet_w, dest, src, vc, seq0, flits) ==
                  hipSuccess
              ? RD_OK
@@ -220,7 +222,7 @@ rd_status rd_event_writeback(rd_ctx* c, const uint32_t* flits, uint64_t n_packet
   if (packet_w < 36 || packet_w > 512 || port > 15 || ring_words == 0 || ring_words > (1ull << 48) || !stats)
     return RD_E_ARG;
   if (n_packets && (!flits || !ring || (reinterpret_cast<uintptr_t>(ring) & 15u))) return RD_E_ARG;
-  if (n_packets > (1ull << 40)) return RD_E_ARG;
+  if (n_packets > (1ull << 31)) return RD_E_ARG;  // one work-item per packet
   return rd::launch_event_writeback(c->stream, flits, n_packets, n_packets_dev, packet_w, port, seq0, ring,
                                     ring_words, addr0, stats) == hipSuccess
              ? RD_OK

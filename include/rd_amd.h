@@ -245,7 +245,8 @@ rd_status rd_doppler_run(rd_ctx* ctx, const uint32_t* ring, uint32_t rows, uint3
  * reference's Flit::w words. n_events_dev (nullable, device): the count is
  * min(n_events, *n_events_dev), e.g. the chain's event total, so the call can
  * follow rd_chain_enqueue in the same stream (and graph) without a host round
- * trip. packet_w in [36, 512]; dest < 16, src < 32, vc < 4. */
+ * trip. packet_w in [36, 512]; dest < 16, src < 32, vc < 4;
+ * n_events * rd_event_flit_count(packet_w) <= 2^31. */
 uint32_t rd_event_flit_count(uint32_t packet_w);
 uint32_t rd_flit_words(uint32_t packet_w);
 rd_status rd_event_flits(rd_ctx* ctx, const uint64_t* events, uint64_t n_events, const uint64_t* n_events_dev,
@@ -272,7 +273,7 @@ rd_status rd_event_flits(rd_ctx* ctx, const uint64_t* events, uint64_t n_events,
  * L, type != detection, header vc != flit vc; a payload flit with sof, with a
  * vc other than the header flit's, or whose eof is not set exactly on the
  * last flit (one each). packet_w in [36, 512], port < 16, ring_words in
- * [1, 2^48], n_packets <= 2^40. */
+ * [1, 2^48], n_packets <= 2^31. */
 #define RD_WB_NSTATS 5
 rd_status rd_event_writeback(rd_ctx* ctx, const uint32_t* flits, uint64_t n_packets, const uint64_t* n_packets_dev,
                              uint32_t packet_w, uint32_t port, uint32_t seq0, uint32_t* ring, uint64_t ring_words,

@@ -123,3 +123,7 @@ def test_event_writeback_rejects_bad_arguments(gpu):
         rc = gpu.L.rd_event_writeback(gpu.h, d.ptr, 1, None, pw, port, 0, d.ptr + off, rw, 0, st.ptr)
         assert rc == rd_amd.RD_E_ARG, (pw, port, rw, off)
     assert gpu.L.rd_event_writeback(gpu.h, d.ptr, 1, None, 512, 0, 0, d.ptr, 2, 0, None) == rd_amd.RD_E_ARG
+    # Counts past one dispatch's work-item range are refused before any launch.
+    assert gpu.L.rd_event_writeback(gpu.h, d.ptr, 2**31 + 1, None, 512, 0, 0, d.ptr, 2, 0, st.ptr) == rd_amd.RD_E_ARG
+    assert gpu.L.rd_event_flits(gpu.h, d.ptr, 2**30 + 1, None, 512, 0, 0, 0, 0, d.ptr) == rd_amd.RD_E_ARG
+    assert gpu.L.rd_event_flits(gpu.h, d.ptr, 2**62, None, 512, 0, 0, 0, 0, d.ptr) == rd_amd.RD_E_ARG
