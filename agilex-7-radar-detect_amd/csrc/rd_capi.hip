@@ -278,6 +278,7 @@ rd_status rd_cov_integrate(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint3
   if (st != RD_OK) return st;
   if (!Xa || !R || n_chan < 1 || n_chan > 64 || n_frames < 1 || mode > 1 || exp_k > 63 || acc_w < 2 || acc_w > 64)
     return RD_E_ARG;
+  if ((uint64_t)n_bins * n_chan * (n_chan + 1) / 2 > (1ull << 31)) return RD_E_ARG;  // one work-item per entry
   return fin(rd::launch_cov_integrate(c->stream, Xa, n_chan, n_frames, n_bins, mode, exp_k, acc_w, R, sat_flags));
 }
 

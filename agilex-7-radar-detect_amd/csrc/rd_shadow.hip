@@ -244,6 +244,7 @@ rd_status rd_sh_pfb(void* stream, const float* x, const float* h, uint32_t n_row
   if (!x || !h || !u || n_fft < 1 || taps < 1 || n_frames < 1) return RD_E_ARG;
   const uint64_t total = (uint64_t)n_rows * n_frames * n_fft;
   if (total == 0) return RD_OK;
+  if (total > (1ull << 31)) return RD_E_ARG;  // one work-item per output
   sh_pfb_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, (hipStream_t)stream>>>(x, h, n_fft, taps, n_frames, total, u);
   return fin(hipGetLastError());
 }
@@ -253,6 +254,7 @@ rd_status rd_sh_pfb_bwd(void* stream, const float* g_u, const float* h, uint32_t
   if (!g_u || !h || !g_x || n_fft < 1 || taps < 1 || n_frames < 1) return RD_E_ARG;
   const uint64_t total = (uint64_t)n_rows * n_frames * n_fft;
   if (total == 0) return RD_OK;
+  if (total > (1ull << 31)) return RD_E_ARG;  // one work-item per output
   sh_pfb_bwd_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, (hipStream_t)stream>>>(g_u, h, n_fft, taps, n_frames,
                                                                                        total, g_x);
   return fin(hipGetLastError());
@@ -275,8 +277,10 @@ rd_status rd_sh_fft(void* stream, const float* in, const float* tw, uint32_t n_f
 rd_status rd_sh_bf_power(void* stream, const float* X, const float* W, uint32_t n_chan, uint32_t n_beams,
                          uint64_t n_cols, uint32_t n_cpi, float* y, float* p) {
   if (!X || !W || !p || n_chan < 1 || n_chan > 64 || n_beams < 1 || n_beams > 64) return RD_E_ARG;
+  if (n_cols > (1ull << 31)) return RD_E_ARG;
   const uint64_t total = n_cols * n_cpi;
   if (total == 0) return RD_OK;
+  if (total > (1ull << 31)) return RD_E_ARG;  // one work-item per column
   const dim3 grid((uint32_t)((total + 255) / 256));
   const size_t lds = (size_t)n_beams * n_chan * sizeof(float2);
   hipStream_t s = (hipStream_t)stream;

@@ -178,7 +178,8 @@ rd_status rd_cov_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t 
  * acc_w = 64 block mode equals rd_cov_run. R as rd_cov_run's; sat_flags
  * (nullable) [n_bins][pairs], re in bits [1:0], im in [3:2]. The exponential
  * accumulator is not cleared at a window close (covar_model.hpp:266-268), so R
- * is its state after the n_frames frames. exp_k <= 63, acc_w 2..64. */
+ * is its state after the n_frames frames. exp_k <= 63, acc_w 2..64,
+ * n_bins * n_chan * (n_chan + 1) / 2 <= 2^31. */
 rd_status rd_cov_integrate(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                            uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R, uint8_t* sat_flags);
 

@@ -28,7 +28,8 @@
 extern "C" {
 #endif
 
-/* x [n_rows][n_frames*n_fft], h [n_fft][taps] -> u [n_rows][n_frames][n_fft]. */
+/* x [n_rows][n_frames*n_fft], h [n_fft][taps] -> u [n_rows][n_frames][n_fft].
+ * n_rows * n_frames * n_fft <= 2^31 (here and in the adjoint). */
 rd_status rd_sh_pfb(void* stream, const float* x, const float* h, uint32_t n_rows, uint32_t n_fft, uint32_t taps,
                     uint32_t n_frames, float* u);
 /* grad u [n_rows][n_frames][n_fft] -> grad x [n_rows][n_frames*n_fft] (the adjoint of rd_sh_pfb). */
@@ -44,7 +45,7 @@ rd_status rd_sh_fft(void* stream, const float* in, const float* tw, uint32_t n_f
 
 /* n_cpi independent CPIs sharing the weights: X [n_cpi][n_chan][n_cols],
  * W [n_beams][n_chan] -> y (nullable) / p [n_cpi][n_beams][n_cols].
- * n_chan, n_beams <= 64. */
+ * n_chan, n_beams <= 64; n_cols * n_cpi <= 2^31. */
 rd_status rd_sh_bf_power(void* stream, const float* X, const float* W, uint32_t n_chan, uint32_t n_beams,
                          uint64_t n_cols, uint32_t n_cpi, float* y, float* p);
 /* Backward of rd_sh_bf_power through p (y is recomputed from X and W):
