@@ -96,7 +96,7 @@ rd_status check_cpi(const rd_cpi_desc* d) {
 rd_status check_cpi_bufs(const rd_cpi_bufs* b) {
   if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
     return RD_E_ARG;
-  if ((uintptr_t)b->u & 15u) return RD_E_ARG;
+  if (((uintptr_t)b->u | (uintptr_t)b->X) & 15u) return RD_E_ARG;
   if (!b->events && b->events_cap) return RD_E_ARG;
   return RD_OK;
 }
@@ -219,7 +219,7 @@ rd_status rd_fft_run(rd_ctx* c, const uint32_t* in, uint32_t n_fft, uint32_t spc
                      uint32_t n_frames, uint32_t* out, uint32_t* bins, uint32_t* stage_flags) {
   rd_status st = enter(c);
   if (st != RD_OK) return st;
-  if (!in || ((uintptr_t)in & 15u)) return RD_E_ARG;
+  if (!in || (((uintptr_t)in | (uintptr_t)out | (uintptr_t)bins) & 15u)) return RD_E_ARG;
   if (!pow2(n_fft) || n_fft < 8 || n_fft > 4096 || (spc != 1 && spc != 2) || n_fft / spc < 4) return RD_E_GEOM;
   return fin(rd::launch_fft(c->stream, c->tw, in, n_fft, spc, scale_sched, reorder, n_frames, out, bins,
                             stage_flags));

@@ -445,18 +445,33 @@ __device__ __forceinline__ Cv cmul_tw(Cv x, uint32_t wc, uint32_t ws, StageFlags
 
 __device__ __forceinline__ uint32_t pad(uint32_t p) { return p + (p >> 5); }
 
+// The words at b + {0, O1, O2, O3} of a padded frame. With every O a multiple
+// of 32, pad(b + O) = pad(b) + O + O / 32, so the four reads share one address.
+template <uint32_t O1, uint32_t O2, uint32_t O3>
+__device__ __forceinline__ uint4 lds_quad(const uint32_t* v, uint32_t b) {
+  if constexpr (O1 % 32 == 0 && O2 % 32 == 0 && O3 % 32 == 0) {
+    const uint32_t* q = v + pad(b);
+    return make_uint4(q[0], q[O1 + O1 / 32], q[O2 + O2 / 32], q[O3 + O3 / 32]);
+  } else {
+    return make_uint4(v[pad(b)], v[pad(b + O1)], v[pad(b + O2)], v[pad(b + O3)]);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // The all-ones schedule (every sub-stage scales by 1, the chain's) runs its
 // butterflies on the packed words themselves, both components per instruction
-// (v_pk_* on the int16 halves), with x = a ^ b and s = x >> 1 (arithmetic):
-//   floor((a + b) / 2) = (a & b) + s
-//   floor((a - b) / 2) = (a | ~b) + s + 1        (the ceiling average of a, ~b)
-//   round to nearest even: + (x & floor & 1)     (odd sum with an odd floor)
-// The rounded half-sum always fits in int16. The rounded half-difference
-// reaches +32768 only for (32767, -32768): a saturating add clamps it, and the
-// sign flip of the wrapping add (bit 15 of wrapped & ~floor) is the sat_pos
-// evidence. These identities hold for every pair of int16 values (checked
-// exhaustively when they were written).
+// (v_pk_* on the int16 halves), with x = a ^ b and s = x >> 1 (arithmetic),
+// from a + b = 2(a | b) - x and a - b = x - 2(~a & b):
+//   ceil((a + b) / 2)  = (a | b) - s
+//   floor((a - b) / 2) = s - (~a & b)
+// x & 1 marks an odd sum (and difference). Rounded to nearest even, the
+// half-sum is the ceiling with bit 0 cleared where x is odd, and the
+// half-difference is the floor plus (x & floor & 1). The half-sum always fits
+// in int16. The rounded half-difference reaches +32768 only for
+// (32767, -32768): a saturating add clamps it, and the sign flip of the
+// wrapping add (bit 15 of wrapped & ~floor) is the sat_pos evidence. These
+// identities hold for every pair of int16 values (checked exhaustively against
+// rne1 and the clamp when they were written).
 // ----------------------------------------------------------------------------
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s16x2 S2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
@@ -480,14 +495,14 @@ struct PkFlagsT {
 
 template <class F>
 __device__ __forceinline__ void bfly_pk(uint32_t& a, uint32_t& b, F& f) {
-  const uint32_t x = a ^ b;
+  const uint32_t x = a ^ b, odd = x & 0x00010001u;
   const s16x2 sx = S2(x) >> (s16x2){1, 1};
-  const uint32_t hs = U2(S2(a & b) + sx);
-  const uint32_t hd = U2(S2(a | ~b) + sx + (s16x2){1, 1});
-  const uint32_t ts = x & hs & 0x00010001u, td = x & hd & 0x00010001u;
-  if constexpr (F::on) f.ev |= U2(S2(hd) + S2(td)) & ~hd;
-  a = U2(S2(hs) + S2(ts));
-  b = U2(__builtin_elementwise_add_sat(S2(hd), S2(td)));
+  const uint32_t cs = U2(S2(a | b) - sx);   // ceil((a + b) / 2)
+  const uint32_t fd = U2(sx - S2(~a & b));  // floor((a - b) / 2)
+  const uint32_t td = odd & fd;
+  if constexpr (F::on) f.ev |= U2(S2(fd) + S2(td)) & ~fd;
+  a = cs & ~odd;
+  b = U2(__builtin_elementwise_add_sat(S2(fd), S2(td)));
 }
 
 // -j * (re + j im) = im - j re: the halves swap and the new high half is
@@ -995,18 +1010,35 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
   frame_sync();
   if (!active) return;
 
+  // Each thread stores four runs of four consecutive outputs k = 4c + i, one
+  // 16-byte store each. The source of output k is a base plus a constant:
+  //   bins, spc 1:         bitrev(k)              = bitrev(4c) + {0,2,1,3}[i] * N/4
+  //   bins, spc 2:         2 bitrev(k % M) + k / M = its value at 4c + {0,2,1,3}[i] * M/2
+  //   out, reorder, spc 1: as bins
+  //   out, reorder, spc 2: 2 bitrev(k / 2) + k % 2 = 2 bitrev(2c) + {0, 1, M, M+1}[i]
+  //   out, natural order:  4c + i
   const size_t fo = (size_t)frame * N;
+  constexpr uint32_t Q = SPC == 1 ? N / 4 : M / 2;
+  if (bins) {
 #pragma unroll
-  for (int rr = 0; rr < 16; ++rr) {
-    const uint32_t k = g + rr * TPF;
-    if (bins) {
-      const uint32_t src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
-      bins[fo + k] = v[pad(src)];
+    for (uint32_t t = 0; t < 4; ++t) {
+      const uint32_t k = 4 * (g + t * TPF);
+      const uint32_t b = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
+      *reinterpret_cast<uint4*>(bins + fo + k) = lds_quad<2 * Q, Q, 3 * Q>(v, b);
     }
-    if (out) {
-      uint32_t src = k;
-      if (reorder) src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k >> 1, NL) + (k & 1);
-      out[fo + k] = v[pad(src)];
+  }
+  if (out) {
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+      const uint32_t k = 4 * (g + t * TPF);
+      uint4 o;
+      if (!reorder)
+        o = lds_quad<1, 2, 3>(v, k);
+      else if (SPC == 1)
+        o = lds_quad<2 * Q, Q, 3 * Q>(v, bitrev(k, NL));
+      else
+        o = lds_quad<1, M, M + 1>(v, 2 * bitrev(k >> 1, NL));
+      *reinterpret_cast<uint4*>(out + fo + k) = o;
     }
   }
   if (J.stage_flags && g == 0) J.stage_flags[frame] = sfl[lf];

@@ -79,6 +79,7 @@ typedef struct {
  *   R         [n_fft][n_chan*(n_chan+1)/2][2] int64 covariance re/im, pairs i<=j row-major; nullable
  *   events    [events_cap][3] uint64        packed 176-bit events, rows (beam, frame) in order
  *   row_count [n_beams*n_frames]            events per row (required)
+ * u and X are 16-byte aligned.
  */
 typedef struct {
   const uint32_t* iq;
@@ -136,6 +137,7 @@ rd_status rd_pfb_run(rd_ctx* ctx, const uint32_t* x, const uint32_t* coeff, uint
  * (model/cpp/fft/fft_ref.hpp:409-412) for n_frames frames of n_fft samples.
  * out: Result::out (beat order), bins: Result::bins (natural order),
  * stage_flags: Result::stage_flags packed 2 bits per sub-stage (fft_vectors.hpp:27-29).
+ * in, out and bins are 16-byte aligned (out, bins nullable).
  * Extends the reference's N <= 1024 to N <= 4096 with the same twiddle rule. */
 rd_status rd_fft_run(rd_ctx* ctx, const uint32_t* in, uint32_t n_fft, uint32_t spc, uint32_t scale_sched,
                      int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins, uint32_t* stage_flags);

@@ -155,6 +155,15 @@ def test_fft_rejects_invalid_geometry(gpu):
         assert e.value.code == rd_amd.RD_E_GEOM
 
 
+def test_fft_rejects_misaligned_buffers(gpu):
+    import rd_amd
+
+    d = gpu.buf(4 * 1024 + 64)
+    for i, o, b in [(d.ptr + 4, d.ptr, d.ptr), (d.ptr, d.ptr + 4, None), (d.ptr, None, d.ptr + 8)]:
+        rc = gpu.L.rd_fft_run(gpu.h, i, 256, 2, 0xFFFFFFFF, 1, 4, o, b, None)
+        assert rc == rd_amd.RD_E_ARG, (i - d.ptr, o and o - d.ptr, b and b - d.ptr)
+
+
 # ---------------------------------------------------------------------------- align
 def test_align(gpu, orc):
     rng = np.random.default_rng(1)
