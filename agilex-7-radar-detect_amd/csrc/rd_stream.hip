@@ -845,6 +845,11 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     } else if (L4 >= 32) {
 #pragma unroll
       for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e & 3) * L16) + (e >> 2) * (L4 + L4 / 32);
+    } else if (LL <= 5) {
+      // the block (length <= 32, aligned) lies inside one 32-word row
+      const uint32_t pb = pad(base);
+#pragma unroll
+      for (uint32_t e = 0; e < 16; ++e) ad[e] = pb + (e >> 2) * L4 + (e & 3) * L16;
     } else {
 #pragma unroll
       for (uint32_t e = 0; e < 16; ++e) ad[e] = pad(base + (e >> 2) * L4 + (e & 3) * L16);
@@ -870,11 +875,18 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
       bfly_pk(p[8 + q2], p[12 + q2], fb);
     }
     {
+      // exponent k*i << TSH with i = q2*L16 + r is k*q2*N/16 + (k*r << TSH);
+      // for N >= 512 the first term is a multiple of 32 and pads to a constant
       const uint32_t TSH = LOG2N - LL;
+      constexpr uint32_t N16 = N / 16, P16 = N16 + N16 / 32;
+      const uint32_t b1 = pad((2 * r) << TSH), b2 = pad(r << TSH), b3 = pad((3 * r) << TSH);
 #pragma unroll
       for (uint32_t q2 = 0; q2 < 4; ++q2) {
         const uint32_t i = q2 * L16 + r;
-        const uint32_t t1 = pad((2 * i) << TSH), t2 = pad(i << TSH), t3 = pad((3 * i) << TSH);
+        const bool fold = N16 % 32 == 0;
+        const uint32_t t1 = fold ? b1 + 2 * q2 * P16 : pad((2 * i) << TSH);
+        const uint32_t t2 = fold ? b2 + q2 * P16 : pad(i << TSH);
+        const uint32_t t3 = fold ? b3 + 3 * q2 * P16 : pad((3 * i) << TSH);
         p[q2] = mul_w0_pk(p[q2]);
         p[4 + q2] = cmul_pk(p[4 + q2], twc[t1], tws[t1], fb);
         p[8 + q2] = cmul_pk(p[8 + q2], twc[t2], tws[t2], fb);
@@ -978,17 +990,26 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
   } else {
     uint32_t a0[4], a1[4], t0[4], t1[4];
     PkFlags f, fm;
+    // With j = g + t*TPF (2*TPF = 2^(NL-2)): 2j is even, so pad(2j+1) =
+    // pad(2j) + 1; M and, for N >= 256, 2*TPF are multiples of 32; and
+    // bitrev(2j) = bitrev(2g) + bitrev(t) (two bits) with bitrev(2g) a
+    // multiple of 4, bitrev(2j+1) = bitrev(2j) + M/2 (a multiple of 32 for
+    // N >= 128). Those cases take constant offsets from per-thread bases.
+    constexpr bool FOLD = N >= 256;
+    const uint32_t pa = pad(2 * g), pm = pad(bitrev(2 * g, NL));
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {
       const uint32_t j = g + t * TPF;  // lane0[2j], lane0[2j+1], lane1[2j], lane1[2j+1]
-      a0[t] = v[pad(2 * j)];
-      a1[t] = v[pad(2 * j + 1)];
-      uint32_t b0 = v[pad(M + 2 * j)], b1 = v[pad(M + 2 * j + 1)];
+      const uint32_t a = FOLD ? pa + t * (2 * TPF + 2 * TPF / 32) : pad(2 * j);
+      a0[t] = v[a];
+      a1[t] = v[a + 1];
+      uint32_t b0 = v[a + M + M / 32], b1 = v[a + M + M / 32 + 1];
       if (LAST_R2) {
         bfly_pk(a0[t], a1[t], f);
         bfly_pk(b0, b1, f);
       }
-      const uint32_t m0 = pad(bitrev(2 * j, NL)), m1 = pad(bitrev(2 * j + 1, NL));
+      const uint32_t m0 = FOLD ? pm + ((t & 1) << 1 | t >> 1) : pad(bitrev(2 * j, NL));
+      const uint32_t m1 = FOLD ? m0 + M / 2 + M / 64 : pad(bitrev(2 * j + 1, NL));
       t0[t] = cmul_pk(b0, twc[m0], tws[m0], fm);
       t1[t] = cmul_pk(b1, twc[m1], tws[m1], fm);
       bfly_pk(a0[t], t0[t], fm);
@@ -999,11 +1020,12 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     frame_sync();
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {  // beat order: beats[2j] = lo, beats[2j+1] = hi
-      const uint32_t j = g + t * TPF;
-      v[pad(4 * j)] = a0[t];
-      v[pad(4 * j + 1)] = t0[t];
-      v[pad(4 * j + 2)] = a1[t];
-      v[pad(4 * j + 3)] = t1[t];
+      const uint32_t j = g + t * TPF;   // 4j % 32 <= 28: the four words share a row
+      const uint32_t o = pad(4 * j);
+      v[o] = a0[t];
+      v[o + 1] = t0[t];
+      v[o + 2] = a1[t];
+      v[o + 3] = t1[t];
     }
   }
   if (fword) atomicOr(&sfl[lf], fword);
