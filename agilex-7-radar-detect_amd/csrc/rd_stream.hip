@@ -563,12 +563,18 @@ struct FftGeo {
   static constexpr size_t lds_bytes = (size_t)(2 * NT + FPB + FPB * NP) * sizeof(uint32_t);
 };
 
-template <int LOG2N>
+// BS: the block size (a multiple of 64); entry i + k*BS pads to pad(i) +
+// k*(BS + BS/32).
+template <int LOG2N, uint32_t BS>
 __device__ __forceinline__ void fft_stage_twiddles(const uint32_t* __restrict__ tw4096, uint32_t* tw) {
-  for (uint32_t i = threadIdx.x; i < FftGeo<LOG2N>::N; i += blockDim.x) {
-    const uint32_t w = tw4096[i << (12 - LOG2N)];
-    tw[pad(i)] = tw_conj_compl(w);
-    tw[FftGeo<LOG2N>::NT + pad(i)] = tw_swap(w);
+  constexpr uint32_t N = FftGeo<LOG2N>::N, NT = FftGeo<LOG2N>::NT;
+  const uint32_t i = threadIdx.x, pi = pad(i);
+#pragma unroll
+  for (uint32_t k = 0; k < (N + BS - 1) / BS; ++k) {
+    if (N % BS != 0 && i + k * BS >= N) break;
+    const uint32_t w = tw4096[(i + k * BS) << (12 - LOG2N)];
+    tw[pi + k * (BS + BS / 32)] = tw_conj_compl(w);
+    tw[NT + pi + k * (BS + BS / 32)] = tw_swap(w);
   }
 }
 
@@ -813,15 +819,27 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
 
   if (g == 0) sfl[lf] = 0;
   if (active) {
+    // Sample x = 4c + r goes to lane slot (x & 1) * M + x / 2 (spc 2) or x.
+    // With c = g + t*TPF: 2c and 4c are even, so the small offsets stay in
+    // their 32-word row; M and, for N >= 256, 2*TPF are multiples of 32.
+    constexpr bool FOLD = N >= 256;
+    constexpr uint32_t CS = SPC == 2 ? 2 : 4;  // slot stride of c
+    const uint32_t pb = pad(CS * g);
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {
       const uint32_t c = g + t * TPF;  // four consecutive samples 4c .. 4c+3
       const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * c);
-      const uint32_t w4[4] = {q4.x, q4.y, q4.z, q4.w};
-#pragma unroll
-      for (uint32_t r = 0; r < 4; ++r) {
-        const uint32_t x = 4 * c + r;  // lane q of spc lanes is x[spc*t' + q]
-        v[pad(SPC == 2 ? ((x & 1) * M + (x >> 1)) : x)] = w4[r];
+      const uint32_t o = FOLD ? pb + t * (CS * TPF + CS * TPF / 32) : pad(CS * c);
+      if constexpr (SPC == 2) {
+        v[o] = q4.x;
+        v[o + M + M / 32] = q4.y;
+        v[o + 1] = q4.z;
+        v[o + 1 + M + M / 32] = q4.w;
+      } else {
+        v[o] = q4.x;
+        v[o + 1] = q4.y;
+        v[o + 2] = q4.z;
+        v[o + 3] = q4.w;
       }
     }
   }
@@ -1084,7 +1102,7 @@ template <int LOG2N, int SPC, bool ALL1, bool FL>
 __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB)) void fft_kernel(
     FftJob J, const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
-  fft_stage_twiddles<LOG2N>(tw4096, smem);
+  fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
   fft_tile_any<LOG2N, SPC, ALL1, FL>(J, blockIdx.x, smem);
 }
 
@@ -1098,7 +1116,7 @@ __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>
   const uint32_t t = blockIdx.x;
   if (t >= n_tiles) return;
   const FftJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
-  fft_stage_twiddles<LOG2N>(tw4096, smem);
+  fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
   fft_tile_any<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
 }
 
