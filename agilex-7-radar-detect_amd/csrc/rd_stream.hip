@@ -35,6 +35,15 @@ __device__ __forceinline__ int32_t rne15_wide(int32_t ah, int32_t al) {
   const int64_t t = v + 0x3FFF + ((v >> 15) & 1);
   return (int32_t)(t >> 15);
 }
+// The same for |al| < 2^29 - 2^14 in 32-bit arithmetic, for a caller that
+// saturates the result to 16 bits: ah is clamped to +-1.5*2^22 first. An ah
+// beyond that puts |v| above 1.25*2^30, past the saturation bound 2^30 - 2^14,
+// with the same sign, and the clamped sum stays below 2^31.
+__device__ __forceinline__ int32_t rne15_clamp(int32_t ah, int32_t al) {
+  constexpr int32_t kH = 3 << 21;
+  const int32_t v = (min(max(ah, -kH), kH) << 8) + al;
+  return (v + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)v, 15, 1)) >> 15;
+}
 __device__ __forceinline__ uint32_t pk16(int32_t lo, int32_t hi) {
   return ((uint32_t)hi << 16) | ((uint32_t)lo & 0xFFFFu);
 }
@@ -56,7 +65,8 @@ __device__ __forceinline__ uint32_t pk16(int32_t lo, int32_t hi) {
 // (and im alike). Each bracket is one packed int16 dot product
 // (v_dot2_i32_i16) per tap into an int32 partial that cannot overflow for
 // T <= 64 (|partial| < 2^30); the two partials are recombined exactly before
-// the single rounding.
+// the single rounding. With real taps (hi = 0) two taps share one dot product
+// against a pair of consecutive samples (PfbRealWindow), chosen per wave.
 // ============================================================================
 
 struct Digits {
@@ -134,7 +144,9 @@ struct PfbWindow {
       aih = dot2(hi[d], tb[k], aih);
       ail = dot2(lo[d], tb[k], ail);
     }
-    const int32_t qr = rne15_wide(arh, arl + sum_xi), qi = rne15_wide(aih, ail);
+    // |arl|, |ail| < T * 2^24 and |sum_xi| <= T * 2^15: within rne15_clamp's range
+    static_assert(T <= 16, "register window sized for T <= 16");
+    const int32_t qr = rne15_clamp(arh, arl + sum_xi), qi = rne15_clamp(aih, ail);
     if constexpr (FLAGS) {
       uint32_t fr = 0, fi = 0;
       const int32_t yr = sat16(qr, fr), yi = sat16(qi, fi);
@@ -146,36 +158,62 @@ struct PfbWindow {
   }
 };
 
-// One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
-// FLAGS: the job has a flag buffer (otherwise the flags are not formed).
-template <int T, bool FLAGS>
-__device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
-  const uint32_t P = J.P, n_beats = J.n_beats;
-  const uint32_t p = bx * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const uint32_t m0 = z * J.chunk;
-  const uint32_t m1 = min(m0 + J.chunk, n_beats);
-  const size_t span = (size_t)n_beats * P;
-  const uint32_t* xc = J.x + (size_t)c * span + p;
-  const uint32_t* hc = J.hist ? J.hist + (size_t)c * (T - 1) * P : nullptr;
-  uint32_t* yc = J.y + (size_t)c * span + p;
-  uint8_t* fc = J.flags ? J.flags + (size_t)c * span + p : nullptr;
-
-  uint32_t ta[T], tb[T];
-#pragma unroll
-  for (int k = 0; k < T; ++k) {
-    const uint32_t h = J.coeff[(size_t)p * T + k];  // (hr, hi)
-    ta[k] = h ^ 0xFFFF0000u;                        // (hr, ~hi)
-    tb[k] = (h >> 16) | (h << 16);                  // (hi, hr)
+// Real taps (every h_p[k] with a zero imaginary part, the windowed-sinc
+// channelizer's case): re = sum hr*xr and im = sum hr*xi, so two taps share one
+// packed dot product against a pair of consecutive samples. Slot j keeps the
+// digit pairs formed when its sample n arrived, (xrh[n], xrh[n-1]) and
+// (xih[n], xih[n-1]), and the same for the low digits; the beat in slot j
+// reads slots j, j-2, ..., j-T+2 against the tap pairs (h[2i], h[2i+1]):
+// T/2 dot products per digit plane instead of T, and no sum_xi term.
+template <int T>
+struct PfbRealWindow {
+  uint32_t rh[T], ih[T], rl[T], il[T];
+  uint32_t ph = 0, pl = 0;  // digit words of the previous sample
+  __device__ __forceinline__ void put(int j, uint32_t v) {
+    const uint32_t h = digits_hi(v), l = digits_lo(v);
+    rh[j] = __builtin_amdgcn_perm(ph, h, 0x05040100u);  // (h.re, prev.re)
+    ih[j] = __builtin_amdgcn_perm(ph, h, 0x07060302u);  // (h.im, prev.im)
+    rl[j] = __builtin_amdgcn_perm(pl, l, 0x05040100u);
+    il[j] = __builtin_amdgcn_perm(pl, l, 0x07060302u);
+    ph = h;
+    pl = l;
   }
-  // Slot (m - m0) mod T holds x_p[m]; the warm-up fills slots 1..T-1 with
-  // x_p[m0-T+1 .. m0-1], and slot 0 starts empty (its first sample replaces
-  // nothing).
-  PfbWindow<T> w;
+  template <bool FLAGS>
+  __device__ __forceinline__ uint32_t out(int j, const uint32_t (&tp)[T / 2], uint8_t* f) const {
+    int32_t arh = dot2_0(rh[j], tp[0]), arl = dot2_0(rl[j], tp[0]);
+    int32_t aih = dot2_0(ih[j], tp[0]), ail = dot2_0(il[j], tp[0]);
 #pragma unroll
-  for (int k = 0; k < T; ++k) w.hi[k] = w.lo[k] = 0u;
+    for (int i = 1; i < T / 2; ++i) {
+      const int d = (j - 2 * i + T) % T;
+      arh = dot2(rh[d], tp[i], arh);
+      arl = dot2(rl[d], tp[i], arl);
+      aih = dot2(ih[d], tp[i], aih);
+      ail = dot2(il[d], tp[i], ail);
+    }
+    // |arl|, |ail| < T * 2^23: within rne15_clamp's range
+    static_assert(T <= 16, "register window sized for T <= 16");
+    const int32_t qr = rne15_clamp(arh, arl), qi = rne15_clamp(aih, ail);
+    if constexpr (FLAGS) {
+      uint32_t fr = 0, fi = 0;
+      const int32_t yr = sat16(qr, fr), yi = sat16(qi, fi);
+      *f = (uint8_t)(fr | (fi << 2));
+      return cpack(yr, yi);
+    } else {
+      return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pk_i16(qr, qi));  // both halves clamped
+    }
+  }
+};
+
+// The beat loop of one branch over its chunk [m0, m1) for either window:
+// slot (m - m0) mod T holds x_p[m]. The warm-up feeds x_p[m0-T+1 .. m0-1]
+// into slots 1..T-1 oldest first (the real window pairs each sample with the
+// one before it), and slot 0 starts empty.
+template <int T, bool FLAGS, class Win, class Out>
+__device__ __forceinline__ void pfb_window_run(const uint32_t* xc, const uint32_t* hc, uint32_t P, uint32_t p,
+                                               uint32_t m0, uint32_t m1, uint32_t* yc, uint8_t* fc, Win& w,
+                                               Out out) {
 #pragma unroll
-  for (int k = 1; k < T; ++k) w.put(T - k, pfb_sample(xc - p, hc, P, p, (int64_t)m0 - k));
+  for (int k = T - 1; k >= 1; --k) w.put(T - k, pfb_sample(xc - p, hc, P, p, (int64_t)m0 - k));
 
   // Raw samples of the current block of T beats; as soon as sample j enters
   // the window its register is reloaded with sample j of the next block, so
@@ -192,7 +230,7 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
       w.put(j, cur[j]);
       cur[j] = xc[(size_t)min(mb + T + j, m1 - 1) * P];
       uint8_t f;
-      yc[o] = w.template out<FLAGS>(j, ta, tb, &f);
+      yc[o] = out(j, &f);
       if (FLAGS) fc[o] = f;
     }
   }
@@ -202,8 +240,54 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
     const size_t o = (size_t)(mb + j) * P;
     w.put(j, cur[j]);
     uint8_t f;
-    yc[o] = w.template out<FLAGS>(j, ta, tb, &f);
+    yc[o] = out(j, &f);
     if (FLAGS) fc[o] = f;
+  }
+}
+
+// One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
+// FLAGS: the job has a flag buffer (otherwise the flags are not formed). A
+// wave whose branches all have real taps takes the real window.
+template <int T, bool FLAGS>
+__device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
+  const uint32_t P = J.P, n_beats = J.n_beats;
+  const uint32_t p = bx * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const uint32_t m0 = z * J.chunk;
+  const uint32_t m1 = min(m0 + J.chunk, n_beats);
+  const size_t span = (size_t)n_beats * P;
+  const uint32_t* xc = J.x + (size_t)c * span + p;
+  const uint32_t* hc = J.hist ? J.hist + (size_t)c * (T - 1) * P : nullptr;
+  uint32_t* yc = J.y + (size_t)c * span + p;
+  uint8_t* fc = J.flags ? J.flags + (size_t)c * span + p : nullptr;
+
+  uint32_t h[T], imag = 0;
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    h[k] = J.coeff[(size_t)p * T + k];  // (hr, hi)
+    imag |= h[k] >> 16;
+  }
+  if (__builtin_amdgcn_ballot_w64(imag != 0) == 0) {
+    uint32_t tp[T / 2];
+#pragma unroll
+    for (int i = 0; i < T / 2; ++i) tp[i] = __builtin_amdgcn_perm(h[2 * i + 1], h[2 * i], 0x05040100u);
+    PfbRealWindow<T> w;
+#pragma unroll
+    for (int k = 0; k < T; ++k) w.rh[k] = w.ih[k] = w.rl[k] = w.il[k] = 0u;
+    pfb_window_run<T, FLAGS>(xc, hc, P, p, m0, m1, yc, fc, w,
+                             [&](int j, uint8_t* f) { return w.template out<FLAGS>(j, tp, f); });
+  } else {
+    uint32_t ta[T], tb[T];
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+      ta[k] = h[k] ^ 0xFFFF0000u;            // (hr, ~hi)
+      tb[k] = (h[k] >> 16) | (h[k] << 16);  // (hi, hr)
+    }
+    PfbWindow<T> w;
+#pragma unroll
+    for (int k = 0; k < T; ++k) w.hi[k] = w.lo[k] = 0u;
+    pfb_window_run<T, FLAGS>(xc, hc, P, p, m0, m1, yc, fc, w,
+                             [&](int j, uint8_t* f) { return w.template out<FLAGS>(j, ta, tb, f); });
   }
 }
 
@@ -242,16 +326,21 @@ __device__ __forceinline__ void pfb_tile(const PfbJob& J, uint32_t bx, uint32_t 
     pfb_any_tile(J, bx, c, z);
 }
 
+// Waves per SIMD asked of the register-window kernels: at T = 16 the real
+// window's 64 pair registers need about 142 VGPRs (three waves) to run
+// unspilled.
+constexpr int pfb_min_waves(int T, bool flags) { return flags ? 1 : (T == 16 ? 3 : 4); }
+
 // Single CPI: grid (nbx, n_chan, nz). T = 0 selects the generic kernel.
 template <int T, bool FLAGS>
-__global__ __launch_bounds__(256, FLAGS ? 1 : 4) void pfb_kernel(PfbJob J) {
+__global__ __launch_bounds__(256, pfb_min_waves(T, FLAGS)) void pfb_kernel(PfbJob J) {
   pfb_tile<T, FLAGS>(J, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Batched: one block per tile of a class of CPIs (block -> job via the tile
 // table); a class holds CPIs that all keep, or all drop, the PFB flags.
 template <int T, bool FLAGS>
-__global__ __launch_bounds__(256, FLAGS ? 1 : 4) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, pfb_min_waves(T, FLAGS)) void pfb_batch_kernel(const PfbJob* __restrict__ jobs,
                                                                        const uint32_t* __restrict__ tile_job,
                                                                        uint32_t n_tiles) {
   const uint32_t t = blockIdx.x;

@@ -76,6 +76,35 @@ def test_pfb_long_run_chunks(gpu, orc):
     _eq(gpu.pfb(x, h, P, T, ch, nb)[0], orc.pfb(x, h, P, T, ch, nb)[0], "y")
 
 
+@pytest.mark.parametrize("P,T,nb,ch", [(1024, 16, 40, 2), (256, 8, 33, 2), (128, 4, 257, 1), (2048, 16, 600, 1),
+                                       (96, 16, 7, 2)])
+@pytest.mark.parametrize("mix", ["real", "mixed"])
+def test_pfb_real_taps(gpu, orc, P, T, nb, ch, mix):
+    """Taps with zero imaginary parts take the paired-tap window per wave; in
+    "mixed" a few branches keep complex taps, so their waves take the complex
+    window while the others stay real. Random and extreme values (-32768 in
+    taps and samples), with and without flags, with an imported history, and
+    a beat count that splits into chunks."""
+    rng = np.random.default_rng(P * 3 + T + nb)
+    for x, h in [(stim.rand_cx(rng, ch * nb * P), stim.rand_cx(rng, P * T)),
+                 (stim.rand_extreme(rng, ch * nb * P), stim.rand_extreme(rng, P * T))]:
+        h = h & np.uint32(0xFFFF)  # real taps
+        if mix == "mixed":
+            hb = h.reshape(P, T)
+            hb[::37] = stim.rand_extreme(rng, hb[::37].size).reshape(hb[::37].shape)
+        yg, fg = gpu.pfb(x, h, P, T, ch, nb)
+        yo, fo = orc.pfb(x, h, P, T, ch, nb)
+        _eq(yg, yo, "y")
+        _eq(fg, fo, "flags")
+        _eq(gpu.pfb(x, h, P, T, ch, nb, flags=False)[0], yo, "y without the flag buffer")
+        hist0 = stim.rand_extreme(rng, ch * (T - 1) * P)
+        hg, ho = hist0.copy(), hist0.copy()
+        yg, fg = gpu.pfb(x, h, P, T, ch, nb, hist=hg)
+        yo, fo = orc.pfb(x, h, P, T, ch, nb, hist=ho)
+        _eq(yg, yo, "y with history")
+        _eq(hg, ho, "final history")
+
+
 @pytest.mark.parametrize("T", [16, 5])
 def test_pfb_history_carry(gpu, orc, T):
     rng = np.random.default_rng(T)
