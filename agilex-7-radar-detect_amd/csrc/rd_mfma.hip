@@ -76,6 +76,19 @@ __device__ __forceinline__ int32_t rne15_sat_parts(int32_t hh, int32_t x, int32_
   return sat16(A + ((R + 0x3FFF + p) >> 15), fl);
 }
 
+// The same when R = 256*x + ll + 128*s + c fits in int32 whole (at most two
+// K steps, CH <= 32: |x| <= 2^21, |ll| <= 2^20, |s| <= 2^22 + 32, |c| <= 2^21,
+// so |R| < 2^30.1):
+// v = 2^15*(2*hh) + R, the parity of floor(v / 2^15) is bit 15 of R, and the
+// rounding is 2*hh + ((R + 0x3FFF + bit15(R)) >> 15). sr = 128*s + c, split
+// by the caller into a per-column and a per-beam term.
+__device__ __forceinline__ int32_t rne15_sat_small(int32_t hh, int32_t x, int32_t ll, int32_t s7c_col,
+                                                   int32_t s7c_beam, uint32_t& fl) {
+  const int32_t R = (x << 8) + ll + s7c_col + s7c_beam;
+  const int32_t t = (R + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)R, 15, 1)) >> 15;
+  return sat16(2 * hh + t, fl);
+}
+
 // Beamformer outputs as a compile-time set (bits: 1 y, 2 flags, 4 power,
 // 8 exact accumulators); -1 reads the set from the job's pointers.
 constexpr int kBfOutY = 1, kBfOutF = 2, kBfOutP = 4, kBfOutA = 8;
@@ -306,8 +319,14 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
         if (!FULL && b >= B) continue;
         const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
         uint32_t fr = 0, fi = 0;
-        const int32_t yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
-        const int32_t yi = rne15_sat_parts(ihh[r], ix[r], ill[r], sa + sb_im, -kconst, fi);
+        int32_t yr, yi;
+        if constexpr (STEPS <= 2) {
+          yr = rne15_sat_small(rhh[r], rx[r], rll[r], sb_re << 7, (sa << 7) + wis - kconst, fr);
+          yi = rne15_sat_small(ihh[r], ix[r], ill[r], sb_im << 7, (sa << 7) - kconst, fi);
+        } else {
+          yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
+          yi = rne15_sat_parts(ihh[r], ix[r], ill[r], sa + sb_im, -kconst, fi);
+        }
         const size_t o = o0 + (size_t)r * cols;
         if (out & kBfOutY) J.y[o] = cpack(yr, yi);
         if (out & kBfOutF) J.flags[o] = (uint8_t)(fr | (fi << 2));
