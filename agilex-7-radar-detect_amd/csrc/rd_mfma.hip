@@ -534,15 +534,28 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
   const bool row_ok = ch < CH && rbin < bins_here;
   int32_t srow_r = 0, srow_i = 0;  // this thread's frame sums of its row (its tasks)
   uint32_t buf0[kMaxT][4], buf1[kMaxT][4];
+  // A task's row advances by whole frames: one pointer per task at the
+  // split's first frame of its group, then uniform frame offsets (no 64-bit
+  // index products per load).
+  const size_t fstride = (size_t)N * CH;  // words per frame
+  const uint32_t* tptr[kMaxT];
+  uint32_t tm0[kMaxT];
+#pragma unroll
+  for (uint32_t q = 0; q < kMaxT; ++q) {
+    const uint32_t g = (tsplit + q * stride) >> 6;
+    tm0[q] = fs * sps * kCovFS + 4 * g;
+    tptr[q] = Xa + (size_t)tm0[q] * fstride + (size_t)(k_base + rbin) * CH + ch;
+  }
   auto fetch = [&](uint32_t st, uint32_t (&nxt)[kMaxT][4]) {
+    const size_t so = (size_t)st * kCovFS * fstride;
 #pragma unroll
     for (uint32_t q = 0; q < kMaxT; ++q) {
-      const uint32_t task = tsplit + q * stride, g = task >> 6;
+      const uint32_t task = tsplit + q * stride;
+      const bool ok = task < kTasks && row_ok && st < sps;
 #pragma unroll
       for (uint32_t t = 0; t < 4; ++t) {
-        const uint32_t m = (fs * sps + st) * kCovFS + 4 * g + t;
-        nxt[q][t] = (task < kTasks && row_ok && st < sps && m < F) ? Xa[((size_t)m * N + k_base + rbin) * CH + ch]
-                                                                     : 0u;
+        const uint32_t m = tm0[q] + st * kCovFS + t;
+        nxt[q][t] = (ok && m < F) ? tptr[q][so + t * fstride] : 0u;
       }
     }
   };
