@@ -36,11 +36,17 @@ __device__ __forceinline__ v4i mfma8(uint64_t a, uint64_t b, v4i c) {
   return __builtin_amdgcn_mfma_i32_16x16x32_i8((long)a, (long)b, c, 0, 0, 0);
 }
 
+// v_perm selectors taking byte `byte` of two words: [p0.b, p1.b, 0, 0] and
+// [0, 0, p2.b, p3.b].
+__device__ __forceinline__ uint32_t sel01(uint32_t byte) { return 0x0C0C0000u | ((byte + 4) << 8) | byte; }
+__device__ __forceinline__ uint32_t sel23(uint32_t byte) { return ((byte + 4) << 24) | (byte << 16) | 0x0C0Cu; }
+__device__ __forceinline__ uint32_t gather4s(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, uint32_t s01,
+                                             uint32_t s23) {
+  return __builtin_amdgcn_perm(p1, p0, s01) | __builtin_amdgcn_perm(p3, p2, s23);
+}
 // Bytes `byte` of four words, gathered into one word (word q -> byte q).
 __device__ __forceinline__ uint32_t gather4(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, uint32_t byte) {
-  const uint32_t s01 = 0x0C0C0000u | ((byte + 4) << 8) | byte;        // [p0.b, p1.b, 0, 0]
-  const uint32_t s23 = ((byte + 4) << 24) | (byte << 16) | 0x0C0Cu;   // [0, 0, p2.b, p3.b]
-  return __builtin_amdgcn_perm(p1, p0, s01) | __builtin_amdgcn_perm(p3, p2, s23);
+  return gather4s(p0, p1, p2, p3, sel01(byte), sel23(byte));
 }
 
 // Digits of 8 packed complex values: H/L of the real part (bytes 1/0) and of the
@@ -266,26 +272,31 @@ __device__ __forceinline__ void bf_tile(const BfJob& J, uint64_t tile, const uin
         p[q] = ok ? xt[j * rs + a0 + q] : 0u;
         vmask |= ok ? (0xFFull << (8 * q)) : 0ull;
       }
-      const Dig8 d = digits8(p, vmask);
-      int32_t sr = 0, si = 0;
+      // only the digits this lane's K rows use: the real part's bytes 1/0 or
+      // the imaginary part's 3/2 (per-lane v_perm selectors), and its sum
+      const uint32_t bh = imag ? 3u : 1u, bl = bh - 1u;
+      const uint32_t sh01 = sel01(bh), sh23 = sel23(bh), sl01 = sel01(bl), sl23 = sel23(bl);
+      const uint64_t h = ((uint64_t)gather4s(p[0], p[1], p[2], p[3], sh01, sh23) |
+                          ((uint64_t)gather4s(p[4], p[5], p[6], p[7], sh01, sh23) << 32)) &
+                         vmask;
+      const uint64_t l = (((uint64_t)gather4s(p[0], p[1], p[2], p[3], sl01, sl23) |
+                           ((uint64_t)gather4s(p[4], p[5], p[6], p[7], sl01, sl23) << 32)) ^
+                          0x8080808080808080ull) &
+                         vmask;
+      int32_t sp = 0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        sr += cre(p[q]);
-        si += cim(p[q]);
-      }
-      if (imag) {
-        ps_i += si;
-        bih[s] = d.hi;
-        bil[s] = d.li;
-        brh[s] = ~d.hi & vmask;
-        brl[s] = ~d.li & vmask;
-      } else {
-        ps_r += sr;
-        bih[s] = d.hr;
-        bil[s] = d.lr;
-        brh[s] = d.hr;
-        brl[s] = d.lr;
-      }
+      for (int q = 0; q < 8; ++q) sp += __builtin_amdgcn_sbfe((int32_t)p[q], imag ? 16u : 0u, 16u);
+      // Bim = [Xr; Xi], Bre = [Xr; ~Xi]: the complement of masked digits is
+      // an XOR with the mask
+      const uint64_t inv = imag ? vmask : 0ull;
+      bih[s] = h;
+      bil[s] = l;
+      brh[s] = h ^ inv;
+      brl[s] = l ^ inv;
+      if (imag)
+        ps_i += sp;
+      else
+        ps_r += sp;
     }
     // column sums over all channels: the four lane groups of a column
     ps_r += __shfl_xor(ps_r, 16);
