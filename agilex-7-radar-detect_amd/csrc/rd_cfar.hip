@@ -248,8 +248,19 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
   {
     Cell v[S];
     const Cell* row = static_cast<const Cell*>(J.cells) + (size_t)r * n;
+    constexpr uint32_t NV = sizeof(Cell) * S / 16;  // 16-byte words of a lane's segment
+    if (NV > 0 && work && full && (reinterpret_cast<uintptr_t>(row) & 15u) == 0) {
+      // a whole segment in a 16-byte aligned row: S*sizeof(Cell) is a multiple
+      // of 16 and b0 a multiple of S, so the segment is 16-byte words
+      uint4 q[NV > 0 ? NV : 1];
+      const uint4* src = reinterpret_cast<const uint4*>(row + b0);
 #pragma unroll
-    for (uint32_t j = 0; j < S; ++j) v[j] = (work && (full || b0 + j < n)) ? row[b0 + j] : Cell(0);
+      for (uint32_t k = 0; k < NV; ++k) q[k] = src[k];
+      __builtin_memcpy(v, q, sizeof(v));
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < S; ++j) v[j] = (work && (full || b0 + j < n)) ? row[b0 + j] : Cell(0);
+    }
     uint64_t run = 0;
 #pragma unroll
     for (uint32_t j = 0; j < S; ++j) {
