@@ -811,6 +811,16 @@ def main():
     for name, (fn, nbytes) in stages.items():
         ms = time_stage(torch, stream, fn, args.stage_reps)
         per[name] = dict(ms=round(ms, 5), algo_bytes=nbytes, gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    # the same PFB launch with complex taps (the imaginary part of every tap
+    # set to 1): the general four-product window instead of the real-tap one
+    taps_cx = ctx.put(np.asarray(sc.taps(), np.uint32) | np.uint32(1 << 16))
+    u_cx = ctx.buf(4 * sc.CH * sc.N * sc.F)
+
+    def pfb_cx():
+        ctx.lib.check("pfb", ctx.L.rd_pfb_run(ctx.h, ch.b["iq"].ptr, taps_cx.ptr, sc.N, sc.T, sc.CH, sc.F, None,
+                                              u_cx.ptr, None))
+
+    pfb_cx_ms = time_stage(torch, stream, pfb_cx, args.stage_reps)
     dom = max(per, key=lambda k: per[k]["ms"])
     ach = per[dom]["gbs"]
     prof = profiled(dom, sc.name)
@@ -842,6 +852,7 @@ def main():
         "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                   "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "timing": {"graph_streams": {"streams": n_str, "wall_ms": round(ms_per_step, 4)},
+                   "pfb_complex_taps_ms": round(pfb_cx_ms, 5),
                    "graph": {"wall_ms": round(one_ms, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
                              "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4),
                              "msamples_s": round(args.steps * sc.samples / dt / 1e6, 2)},
