@@ -37,8 +37,9 @@ __device__ __forceinline__ int32_t rne15_wide(int32_t ah, int32_t al) {
 }
 // The same for |al| < 2^29 - 2^14 in 32-bit arithmetic, for a caller that
 // saturates the result to 16 bits: ah is clamped to +-1.5*2^22 first. An ah
-// beyond that puts |v| above 1.25*2^30, past the saturation bound 2^30 - 2^14,
-// with the same sign, and the clamped sum stays below 2^31.
+// beyond that gives |256*ah| > 1.5*2^30, so |v| > 2^30 + 2^14, past the
+// saturation bound 2^30 - 2^14, with the sign of ah (the clamped value keeps
+// both), and the clamped sum stays below 2^31.
 __device__ __forceinline__ int32_t rne15_clamp(int32_t ah, int32_t al) {
   constexpr int32_t kH = 3 << 21;
   const int32_t v = (min(max(ah, -kH), kH) << 8) + al;
