@@ -86,8 +86,9 @@ __device__ __forceinline__ int32_t rne15_sat_parts(int32_t hh, int32_t x, int32_
 // K steps, CH <= 32: |x| <= 2^21, |ll| <= 2^20, |s| <= 2^22 + 32, |c| <= 2^21,
 // so |R| < 2^30.1):
 // v = 2^15*(2*hh) + R, the parity of floor(v / 2^15) is bit 15 of R, and the
-// rounding is 2*hh + ((R + 0x3FFF + bit15(R)) >> 15). sr = 128*s + c, split
-// by the caller into a per-column and a per-beam term.
+// rounding is 2*hh + ((R + 0x3FFF + bit15(R)) >> 15). The caller passes
+// 128*s + c split as s7c_col = 128 * (the column's sum term) and
+// s7c_beam = 128 * (the beam's sum term) + c.
 __device__ __forceinline__ int32_t rne15_sat_small(int32_t hh, int32_t x, int32_t ll, int32_t s7c_col,
                                                    int32_t s7c_beam, uint32_t& fl) {
   const int32_t R = (x << 8) + ll + s7c_col + s7c_beam;
