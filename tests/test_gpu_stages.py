@@ -202,7 +202,8 @@ def test_align(gpu, orc):
 
 
 # ---------------------------------------------------------------------------- beamformer
-@pytest.mark.parametrize("CH,B", [(4, 4), (16, 16), (64, 64), (7, 3), (33, 17), (64, 128), (1, 1), (48, 20)])
+@pytest.mark.parametrize("CH,B", [(4, 4), (16, 16), (64, 64), (7, 3), (33, 17), (64, 128), (1, 1), (48, 20),
+                                  (32, 32), (17, 5), (32, 128)])
 def test_bf_random_and_extreme(gpu, orc, CH, B):
     rng = np.random.default_rng(CH * 131 + B)
     cols = 1000
@@ -234,11 +235,13 @@ def test_bf_unit_weights_passthrough(gpu):
 
 
 @pytest.mark.parametrize("CH,B,F,N", [(16, 16, 5, 1024), (64, 64, 2, 4096), (4, 4, 16, 64), (7, 3, 3, 100),
-                                      (33, 20, 2, 512), (48, 17, 1, 256)])
+                                      (33, 20, 2, 512), (48, 17, 1, 256), (32, 24, 3, 256), (16, 16, 2, 64)])
 def test_bf_stage_matches_align_then_bf(gpu, orc, CH, B, F, N):
     rng = np.random.default_rng(CH + B + F + N)
     for X, W in [(stim.rand_cx(rng, CH * F * N), stim.rand_cx(rng, B * CH)),
-                 (stim.rand_extreme(rng, CH * F * N), stim.rand_extreme(rng, B * CH))]:
+                 (stim.rand_extreme(rng, CH * F * N), stim.rand_extreme(rng, B * CH)),
+                 (np.full(CH * F * N, vecio.pack(-32768, -32768), np.uint32),
+                  np.full(B * CH, vecio.pack(-32768, -32768), np.uint32))]:
         y, fl, pw, Xa = gpu.bf_stage(X, W, CH, B, F, N)
         xa = orc.align(X, CH, F, N)
         _eq(Xa, xa, "aligned copy")
