@@ -49,6 +49,8 @@ const char* const kStageName[S_COUNT] = {"pfb", "fft", "bf", "cov", "cfar_rows",
 // Events staged per CFAR row in a batch; rows with more are recomputed by the
 // gather pass.
 constexpr uint32_t kBatchCfarSlot = 64;
+// A row emits at most n_bins events plus its summary.
+uint32_t batch_cfar_slot(uint32_t n_bins) { return n_bins + 1 < kBatchCfarSlot ? n_bins + 1 : kBatchCfarSlot; }
 
 struct Shape {
   uint32_t a = 0;     // PFB: target lanes per CPI for beat chunking (0: one chunk);
@@ -408,7 +410,7 @@ rd_status build(rd_batch& B, uint32_t flags) {
   for (uint32_t i = 0; i < n; ++i) {
     off[i] = total;
     const size_t rows = (size_t)B.d[i].n_beams * B.d[i].n_frames;
-    total += align256((rows + 1) * 8 + rows * kBatchCfarSlot * 24);
+    total += align256((rows + 1) * 8 + rows * batch_cfar_slot(B.d[i].n_fft) * 24);
   }
   RD_TRY(hipMalloc(&B.cfar_scratch, total));
   B.cfar.resize(n);
@@ -419,7 +421,7 @@ rd_status build(rd_batch& B, uint32_t flags) {
     uint64_t* offs = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(B.cfar_scratch) + off[i]);
     B.cfar[i] = rd::CfarJob{b.power, nullptr, offs + rows + 1, b.row_count, offs, b.events, B.totals_dev + i,
                             b.events ? b.events_cap : 0, rd::cfar_geom(d.cfar, d.max_guard, d.max_ref), d.n_fft,
-                            rows, d.n_frames, kBatchCfarSlot, 0};
+                            rows, d.n_frames, batch_cfar_slot(d.n_fft), 0};
   }
   RD_TRY(hipMalloc(&B.scan_jobs, n * sizeof(rd::CfarJob)));
   RD_TRY(hipMemcpy(B.scan_jobs, B.cfar.data(), n * sizeof(rd::CfarJob), hipMemcpyHostToDevice));

@@ -258,6 +258,8 @@ rd_status rd_doppler_run(rd_ctx* c, const uint32_t* ring, uint32_t rows, uint32_
   if (st != RD_OK) return st;
   if (!ring || !out || depth < 1 || newest >= depth || n_dop > depth) return RD_E_ARG;
   if (!pow2(n_dop) || n_dop < 8 || n_dop > 4096 || (spc != 1 && spc != 2) || n_dop / spc < 4) return RD_E_GEOM;
+  // rows is one grid dimension and rows * n_bins the transform's frame count (uint32)
+  if (rows > 65535 || (uint64_t)rows * n_bins > 0xFFFFFFFFull) return RD_E_ARG;
   st = grow(c, &c->dop, &c->dop_bytes, rd::doppler_scratch_bytes(rows, n_bins, n_dop));
   if (st != RD_OK) return st;
   return fin(rd::launch_doppler(c->stream, c->tw, ring, rows, depth, n_bins, newest, n_dop, spc, scale_sched, shift,
@@ -291,10 +293,10 @@ rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_
   if (!cells || !cfg || !row_count || (cell_bytes != 4 && cell_bytes != 8) || n_bins < 1 || n_bins > 8191)
     return RD_E_ARG;
   if (!events && events_cap) return RD_E_ARG;
-  st = grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_rows));
+  st = grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_bins, n_rows));
   if (st != RD_OK) return st;
   RD_TRY(rd::launch_cfar(c->stream, cells, cell_bytes, n_bins, n_rows, *cfg, max_guard, max_ref, frame_ids,
-                         frame_mod, events, events_cap, row_count, c->scratch, c->total_dev));
+                         frame_mod, events, events_cap, row_count, c->scratch, c->scratch_bytes, c->total_dev));
   RD_TRY(hipMemcpyAsync(c->total_host, c->total_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
   RD_TRY(hipStreamSynchronize(c->stream));
   const uint64_t tot = n_rows ? *c->total_host : 0;
@@ -314,11 +316,13 @@ rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
   if (st != RD_OK) return st;
   st = grow(c, &c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
   if (st != RD_OK) return st;
-  return grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_beams * d->n_frames));
+  return grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_fft, d->n_beams * d->n_frames));
 }
 
-// The chain's launches on the context stream (prepared CPI).
-rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+// The chain's launches on the context stream (prepared CPI); the CFAR pass
+// keeps its row offsets and staged events in `scratch` (scratch_bytes).
+rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev, void* scratch,
+                       size_t scratch_bytes) {
   const uint32_t CH = d->n_chan, N = d->n_fft, F = d->n_frames, B = d->n_beams;
   const uint64_t cols = (uint64_t)F * N;
   hipStream_t s = c->stream;
@@ -328,7 +332,7 @@ rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, ui
   RD_TRY(rd::launch_bf_chmajor(s, b->X, b->weights, CH, B, cols, b->y, b->yflags, b->power, b->Xa));
   if (b->R) RD_TRY(rd::launch_cov(s, b->Xa, CH, F, N, b->R));
   RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
-                         b->events ? b->events_cap : 0, b->row_count, c->scratch,
+                         b->events ? b->events_cap : 0, b->row_count, scratch, scratch_bytes,
                          n_events_dev ? n_events_dev : c->total_dev));
   return RD_OK;
 }
@@ -344,7 +348,7 @@ struct rd_graph {
 rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
   rd_status st = chain_prepare(c, d, b);
   if (st != RD_OK) return st;
-  return chain_launch(c, d, b, n_events_dev);
+  return chain_launch(c, d, b, n_events_dev, c->scratch, c->scratch_bytes);
 }
 
 rd_status rd_chain_graph(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev,
@@ -357,7 +361,7 @@ rd_status rd_chain_graph(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, 
   if (!g) return RD_E_ARG;
   g->ctx = c;
   RD_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  st = chain_launch(c, d, b, n_events_dev ? n_events_dev : c->total_dev);
+  st = chain_launch(c, d, b, n_events_dev ? n_events_dev : c->total_dev, c->scratch, c->scratch_bytes);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (st != RD_OK || e != hipSuccess) {

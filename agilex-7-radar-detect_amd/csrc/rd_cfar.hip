@@ -525,8 +525,26 @@ __global__ __launch_bounds__(1024) void cfar_scan_batch_kernel(const CfarJob* __
   cfar_scan_block(J);
 }
 
-size_t cfar_scratch_bytes(uint32_t n_rows) {
-  return ((size_t)n_rows + 1) * sizeof(uint64_t) + (size_t)n_rows * kCfarSlot * 3 * sizeof(uint64_t);
+// Staging bytes a single-CPI launch asks for at most; with more rows the slot
+// per row shrinks (down to none: every row is then recomputed by the gather).
+constexpr size_t kCfarStageBudget = (size_t)512 << 20;
+
+// Staging slot per row for a scratch of `bytes` after the row offsets: at
+// most kCfarSlot, and never more than a row can emit (n events + the summary).
+uint32_t cfar_slot_for(uint32_t n, uint32_t n_rows, size_t bytes) {
+  const size_t offs = ((size_t)n_rows + 1) * sizeof(uint64_t);
+  size_t slot = n + 1u < kCfarSlot ? n + 1u : kCfarSlot;
+  if (n_rows == 0) return (uint32_t)slot;
+  const size_t fit = bytes > offs ? (bytes - offs) / ((size_t)n_rows * 3 * sizeof(uint64_t)) : 0;
+  return (uint32_t)(fit < slot ? fit : slot);
+}
+
+size_t cfar_scratch_bytes(uint32_t n, uint32_t n_rows) {
+  const size_t offs = ((size_t)n_rows + 1) * sizeof(uint64_t);
+  const size_t per_row = (size_t)(n_rows ? n_rows : 1) * 3 * sizeof(uint64_t);
+  size_t slot = n + 1u < kCfarSlot ? n + 1u : kCfarSlot;
+  if (slot * per_row > kCfarStageBudget) slot = kCfarStageBudget / per_row;
+  return offs + slot * per_row;
 }
 
 // Rows per block for W waves per row: 4 waves in all (one row of 8 waves).
@@ -597,13 +615,15 @@ static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, uint32_t W) 
 hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n, uint32_t n_rows,
                        const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                        uint32_t frame_mod, uint64_t* events, uint64_t cap, uint32_t* row_count, void* scratch,
-                       uint64_t* total_dev) {
+                       size_t scratch_bytes, uint64_t* total_dev) {
   if (n_rows == 0) return hipSuccess;
+  if (scratch_bytes < ((size_t)n_rows + 1) * sizeof(uint64_t)) return hipErrorInvalidValue;
   const uint32_t W = cfar_single_waves(n, n_rows);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   uint64_t* stage = offs + n_rows + 1;
   const CfarJob J{cells, frame_ids, stage, row_count, offs, events, total_dev, cap,
-                  cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, kCfarSlot, 0};
+                  cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, cfar_slot_for(n, n_rows, scratch_bytes),
+                  0};
   return cell_bytes == 4 ? launch_cfar_cell<uint32_t>(s, J, W) : launch_cfar_cell<uint64_t>(s, J, W);
 }
 

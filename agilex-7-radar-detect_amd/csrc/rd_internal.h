@@ -130,12 +130,16 @@ hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_ch
                                 uint32_t n_bins, uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R,
                                 uint8_t* sat_flags);
 
-// CFAR (rd_cfar.hip). scratch must hold cfar_scratch_bytes(n_rows).
-size_t cfar_scratch_bytes(uint32_t n_rows);
+// CFAR (rd_cfar.hip). scratch holds scratch_bytes: the row offsets
+// ((n_rows + 1) * 8 bytes, required) then as many staged events per row as
+// fit (rows with more are recomputed by the gather pass).
+// cfar_scratch_bytes(n_bins, n_rows) is the size a context allocates.
+size_t cfar_scratch_bytes(uint32_t n_bins, uint32_t n_rows);
+uint32_t cfar_slot_for(uint32_t n_bins, uint32_t n_rows, size_t scratch_bytes);
 hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
                        const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                        uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
-                       void* scratch, uint64_t* total_dev);
+                       void* scratch, size_t scratch_bytes, uint64_t* total_dev);
 
 // Detection packets (rd_packet.hip): n_cap events, or min(n_cap, *n_dev) when
 // n_dev (device) is set.

@@ -532,8 +532,13 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
 
   const uint32_t nsteps = (F + kCovFS - 1) / kCovFS;
   const uint32_t sps = (nsteps + fs_n - 1) / fs_n;  // steps per split
-  v4i acc[6];
-  for (int z = 0; z < 6; ++z) acc[z] = v4i{0, 0, 0, 0};
+  // Re: HH, HL, LL, LH; Im: HH, HL, LL, LH. Every digit product is at most
+  // 2^14 in magnitude and a frame adds two of each kind (real and imaginary
+  // K rows), so one accumulator grows by at most 2^15 per frame and stays
+  // inside int32 for every F <= 65535 (HL and LH summed in one accumulator
+  // would not past 32767 frames).
+  v4i acc[8];
+  for (int z = 0; z < 8; ++z) acc[z] = v4i{0, 0, 0, 0};
   int32_t nfr = 0;
 
   // Conversion tasks of a step: (row = lane, group g of 4 frames), g = 0..7,
@@ -624,12 +629,12 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
       const uint64_t bl = *reinterpret_cast<const uint64_t*>(pb + (pre + 1) * kCovPlane);
       acc[0] = mfma8(reah, bh, acc[0]);
       acc[1] = mfma8(reah, bl, acc[1]);
-      acc[1] = mfma8(real_, bh, acc[1]);
+      acc[3] = mfma8(real_, bh, acc[3]);
       acc[2] = mfma8(real_, bl, acc[2]);
-      acc[3] = mfma8(imah, bh, acc[3]);
-      acc[4] = mfma8(imah, bl, acc[4]);
-      acc[4] = mfma8(imal, bh, acc[4]);
-      acc[5] = mfma8(imal, bl, acc[5]);
+      acc[4] = mfma8(imah, bh, acc[4]);
+      acc[5] = mfma8(imah, bl, acc[5]);
+      acc[7] = mfma8(imal, bh, acc[7]);
+      acc[6] = mfma8(imal, bl, acc[6]);
     }
   };
   for (uint32_t st = 0; st < sps; st += 2) {
@@ -652,8 +657,8 @@ __device__ __forceinline__ void cov_tile(const CovJob& J, uint32_t tile, uint32_
     const uint32_t il = bin * chp + 16 * ti + 4 * grp + r;
     const int32_t ir = sums[2 * il], ii = sums[2 * il + 1];
     const int64_t tis = (int64_t)ir + ii, uis = (int64_t)ii - ir - (int64_t)nfr;
-    part[2 * r] = combine(acc[0], acc[1], acc[2], r) + 128 * (tis + tjs) - kconst;
-    part[2 * r + 1] = combine(acc[3], acc[4], acc[5], r) + 128 * (uis + tjs) - kconst + b_i;
+    part[2 * r] = combine(acc[0], acc[1], acc[2], r) + 256 * (int64_t)acc[3][r] + 128 * (tis + tjs) - kconst;
+    part[2 * r + 1] = combine(acc[4], acc[5], acc[6], r) + 256 * (int64_t)acc[7][r] + 128 * (uis + tjs) - kconst + b_i;
   }
   if (fs_n > 1) {
     if (wave < jobs * fs_n)

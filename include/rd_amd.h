@@ -231,7 +231,8 @@ rd_status rd_graph_destroy(rd_graph* graph);
  * newest = frames - 1. out [rows][n_dop][n_bins]; with `shift` the Doppler
  * axis is fftshifted (bin k at row (k + n_dop/2) mod n_dop, range_doppler.py:
  * 262-265). stage_flags (nullable) [rows][n_bins], packed as rd_fft_run's.
- * n_dop: power of two, 8..4096, n_dop/spc >= 4, n_dop <= depth. */
+ * n_dop: power of two, 8..4096, n_dop/spc >= 4, n_dop <= depth; rows <= 65535
+ * and rows * n_bins < 2^32 (RD_E_ARG otherwise). */
 rd_status rd_doppler_run(rd_ctx* ctx, const uint32_t* ring, uint32_t rows, uint32_t depth, uint32_t n_bins,
                          uint32_t newest, uint32_t n_dop, uint32_t spc, uint32_t scale_sched, int shift, uint32_t* out,
                          uint32_t* stage_flags);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -60,20 +61,40 @@ struct Flags {
   static Flags from_packed(unsigned p) { return Flags{(p & 2u) != 0, (p & 1u) != 0}; }
 };
 
-// One shared context per process (device 0 unless rd_amd::Device::use() picks another).
+// One shared context per process, created on first use: device 0 unless
+// rd_amd::Device::use() picked another before that. Asking for a different
+// device once the context exists throws (the mirror never moves devices).
 class Device {
  public:
-  static rd_ctx* ctx() { return instance().ctx_; }
-  static void use(int device) { instance(device); }
+  static rd_ctx* ctx() {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.ctx) check(rd_create(s.device, &s.ctx), "rd_create");
+    return s.ctx;
+  }
+  static void use(int device) {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (s.ctx && device != s.device)
+      throw Error("rd_amd::Device::use(" + std::to_string(device) + ") after the context was created on device " +
+                      std::to_string(s.device),
+                  RD_E_ARG);
+    s.device = device;
+  }
 
  private:
-  explicit Device(int device) { check(rd_create(device, &ctx_), "rd_create"); }
-  ~Device() { rd_destroy(ctx_); }
-  static Device& instance(int device = 0) {
-    static Device d(device);
-    return d;
+  struct State {
+    std::mutex mu;
+    int device = 0;
+    rd_ctx* ctx = nullptr;
+    ~State() {
+      if (ctx) rd_destroy(ctx);
+    }
+  };
+  static State& state() {
+    static State s;
+    return s;
   }
-  rd_ctx* ctx_ = nullptr;
 };
 
 // Device buffer with host copies.
