@@ -36,6 +36,7 @@ sys.path[:0] = [str(ROOT), str(ROOT / "agilex-7-radar-detect_amd")]
 
 import rd_amd  # noqa: E402
 import rd_scenario  # noqa: E402
+import rd_workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 VALU_SIMDS = 1024  # 256 CUs x 4 SIMDs
@@ -176,7 +177,7 @@ def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
     ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
                       keep_y=False, keep_flags=False, keep_R=True)
     note("large: generating 2 CPIs on the device")
-    ring = [device_iq(torch, sc, 7001 + i) for i in range(2)]
+    ring = [rd_workloads.device_iq(torch, sc, 7001 + i) for i in range(2)]
     iq0 = ring[0].cpu().numpy().view(np.uint32)
     ch.load(iq0, taps, W)
     totals = ctx.buf(16)
@@ -239,97 +240,12 @@ def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
 
 
 # ----------------------------------------------------------------------------- batched leg
-def device_iq(torch, sc, seed, chan_err=None):
-    """A scene's I/Q generated on the device (same construction as
-    rd_scenario.Scene.iq: complex Gaussian noise plus plane-wave tones on the
-    array, rounded half-to-even and saturated to int16, packed {im, re}).
-    chan_err (complex [CH], optional) multiplies each channel before
-    quantisation: receiver gain/phase errors."""
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    CH, L = sc.CH, sc.N * sc.F
-    re = torch.randn((CH, L), generator=g, device="cuda") * sc.sigma
-    im = torch.randn((CH, L), generator=g, device="cuda") * sc.sigma
-    if sc.targets:
-        rng = np.random.default_rng(seed)
-        a = np.arange(CH)
-        steer = np.empty((CH, len(sc.targets)), np.complex64)
-        freqs = np.empty(len(sc.targets), np.float64)
-        for i, (k, ang, snr) in enumerate(sc.targets):
-            amp = sc.sigma * 10 ** (snr / 20.0)
-            steer[:, i] = amp * np.exp(1j * (np.pi * a * np.sin(np.deg2rad(ang)) + rng.uniform(0, 2 * np.pi)))
-            freqs[i] = k / sc.N
-        t = torch.arange(L, device="cuda", dtype=torch.float64)
-        ph = torch.outer(torch.from_numpy(2 * np.pi * freqs).cuda(), t)  # exact phase in f64, then f32 tones
-        tones = torch.polar(torch.ones_like(ph, dtype=torch.float32), ph.float())
-        sig = torch.from_numpy(steer).cuda() @ tones
-        re += sig.real
-        im += sig.imag
-        del ph, tones, sig
-    if chan_err is not None:
-        e = torch.from_numpy(np.asarray(chan_err, np.complex64)).cuda()[:, None]
-        re, im = re * e.real - im * e.imag, re * e.imag + im * e.real
-    rq = torch.clamp(torch.round(re), -32768, 32767).to(torch.int32)
-    iq_ = torch.clamp(torch.round(im), -32768, 32767).to(torch.int32)
-    return ((iq_ << 16) | (rq & 0xFFFF)).reshape(-1)
-
-
-class BatchSet:
-    """Device buffers (torch allocations) and descriptors of an uneven batch."""
-
-    def __init__(self, torch, scenes, seed0, events_per_row=64):
-        self.scenes = scenes
-        self.t = []
-        self.descs, self.bufs, self.caps = [], [], []
-        taps_of, w_of = {}, {}
-        dev = "cuda"
-        for i, sc in enumerate(scenes):
-            S, P = sc.samples, sc.B * sc.F * sc.N
-            if sc.N not in taps_of:
-                taps_of[sc.N] = torch.from_numpy(sc.taps().view(np.int32)).to(dev)
-            key = (sc.CH, sc.B)
-            if key not in w_of:
-                w_of[key] = torch.from_numpy(sc.weights().view(np.int32)).to(dev)
-            cap = sc.B * sc.F * events_per_row
-            d = dict(iq=device_iq(torch, sc, seed0 + i), taps=taps_of[sc.N], weights=w_of[key],
-                     u=torch.empty(S, dtype=torch.int32, device=dev), X=torch.empty(S, dtype=torch.int32, device=dev),
-                     Xa=torch.empty(S, dtype=torch.int32, device=dev),
-                     power=torch.empty(P, dtype=torch.int32, device=dev),
-                     R=torch.empty(sc.N * sc.CH * (sc.CH + 1), dtype=torch.int64, device=dev),
-                     events=torch.empty(3 * cap, dtype=torch.int64, device=dev),
-                     row_count=torch.empty(sc.B * sc.F, dtype=torch.int32, device=dev))
-            self.t.append(d)
-            self.caps.append(cap)
-            self.descs.append(rd_amd.CpiDesc(sc.CH, sc.N, sc.T, sc.F, sc.B, sc.spc, sc.sched, rd_scenario.MAX_GUARD,
-                                             rd_scenario.MAX_REF, rd_amd.CfarCfg.of(sc.cfar_cfg())))
-            self.bufs.append(rd_amd.CpiBufs(**{k: v.data_ptr() for k, v in d.items()}, events_cap=cap))
-            if (i + 1) % 64 == 0:
-                note(f"  batch: {i + 1}/{len(scenes)} CPIs generated")
-        torch.cuda.synchronize()
-
-    @property
-    def samples(self) -> int:
-        return sum(sc.samples for sc in self.scenes)
-
-    def fit_event_caps(self, torch, counts):
-        """Grow the event buffers of CPIs whose first pass counted more events
-        than fit (the count is exact even when the buffer is short)."""
-        for i, n in enumerate(counts.tolist()):
-            if n > self.caps[i]:
-                cap = int(n * 1.25) + 64
-                self.t[i]["events"] = torch.empty(3 * cap, dtype=torch.int64, device="cuda")
-                self.caps[i] = cap
-                self.bufs[i].events = self.t[i]["events"].data_ptr()
-                self.bufs[i].events_cap = cap
-        torch.cuda.synchronize()
-
-
 def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
     """256 uneven CPIs: one batched graph launch vs one CPI per launch, both
     checked against each other on every CPI and against the CPU on a sample."""
     scenes = rd_scenario.uneven_batch(n_cpi)
     note(f"batch: generating {n_cpi} uneven CPIs on the device")
-    bs = BatchSet(torch, scenes, seed0=77_000)
+    bs = rd_workloads.BatchSet(torch, scenes, seed0=77_000)
     L, h = ctx.L, ctx.h
     tot = np.zeros(1, np.uint64)
     totals_dev = torch.zeros(n_cpi, dtype=torch.int64, device="cuda")
@@ -337,12 +253,12 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
 
     def plain_sync():  # the drop-in per-CPI call: chain + event count back to the host
         for d, b in zip(bs.descs, bs.bufs):
-            lib_check(ctx, "rd_chain_run", L.rd_chain_run(h, rd_amd.C.byref(d), rd_amd.C.byref(b),
+            ctx.lib.check("rd_chain_run", L.rd_chain_run(h, rd_amd.C.byref(d), rd_amd.C.byref(b),
                                                           tot.ctypes.data_as(rd_amd.C.POINTER(rd_amd.C.c_uint64))))
 
     def plain_async():  # one CPI per launch sequence, no host round trip between CPIs
         for i, (d, b) in enumerate(zip(bs.descs, bs.bufs)):
-            lib_check(ctx, "rd_chain_enqueue", L.rd_chain_enqueue(h, rd_amd.C.byref(d), rd_amd.C.byref(b),
+            ctx.lib.check("rd_chain_enqueue", L.rd_chain_enqueue(h, rd_amd.C.byref(d), rd_amd.C.byref(b),
                                                                   totals_dev.data_ptr() + 8 * i))
         ctx.sync()
 
@@ -367,9 +283,9 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
     t_batch = wall(batch.launch, reps)
     totals = batch.events()
     note("batch: verifying every CPI against its one-CPI launch")
-    bad_cpis = verify_batch_vs_plain(torch, ctx, bs, batch, totals)
+    bad_cpis = rd_workloads.verify_batch_vs_plain(torch, ctx, bs, batch, totals)
     note(f"batch: oracle check on {oracle_cpis} CPIs")
-    orc_bad, orc_n = verify_batch_vs_oracle(torch, ctx, bs, totals, oracle_cpis)
+    orc_bad, orc_n = rd_workloads.verify_batch_vs_oracle(torch, ctx, bs, totals, oracle_cpis)
     samples = bs.samples
     plan = batch.plan().strip().splitlines()
     batch.close()
@@ -389,57 +305,6 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
         "mismatching_cpis": bad_cpis, "oracle_cpis": orc_n, "oracle_mismatches": orc_bad,
         "events": int(np.sum(totals)),
     }
-
-
-def lib_check(ctx, fn, rc):
-    ctx.lib.check(fn, rc)
-
-
-def verify_batch_vs_plain(torch, ctx, bs, batch, totals):
-    """Each CPI's batched outputs (already in its buffers) against a fresh
-    one-CPI launch: outputs are copied aside, the CPI is rerun alone, compared."""
-    bad = 0
-    tot = np.zeros(1, np.uint64)
-    names = ("u", "X", "Xa", "power", "R", "row_count")
-    for i, (d, b) in enumerate(zip(bs.descs, bs.bufs)):
-        t = bs.t[i]
-        n = int(totals[i])
-        saved = {k: t[k].clone() for k in names}
-        saved["events"] = t["events"][: 3 * n].clone()
-        torch.cuda.synchronize()
-        for k in names + ("events",):
-            t[k].zero_()
-        torch.cuda.synchronize()
-        ctx.lib.check("rd_chain_run", ctx.L.rd_chain_run(ctx.h, rd_amd.C.byref(d), rd_amd.C.byref(b),
-                                                         tot.ctypes.data_as(rd_amd.C.POINTER(rd_amd.C.c_uint64))))
-        ok = int(tot[0]) == n and all(torch.equal(saved[k], t[k]) for k in names)
-        ok = ok and torch.equal(saved["events"], t["events"][: 3 * n])
-        bad += 0 if ok else 1
-    return bad
-
-
-def verify_batch_vs_oracle(torch, ctx, bs, totals, k):
-    """The k smallest CPIs of the batch against the CPU chain (power, covariance, events)."""
-    if k <= 0:
-        return None, 0
-    import oracle
-
-    orc = oracle.load("oracle")
-    order = sorted(range(len(bs.scenes)), key=lambda i: bs.scenes[i].samples)[:k]
-    bad = 0
-    for i in order:
-        sc, t = bs.scenes[i], bs.t[i]
-        iq = t["iq"].cpu().numpy().view(np.uint32)
-        o = orc.chain(iq, sc.taps(), sc.weights(), sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
-                      rd_scenario.MAX_REF, keep=("power", "R", "events"), threads=min(16, os.cpu_count() or 1))
-        n = int(totals[i])
-        bad += int(np.count_nonzero(t["power"].cpu().numpy().view(np.uint32) != o["power"]))
-        bad += int(np.count_nonzero(t["R"].cpu().numpy() != o["R"]))
-        if n != o["n_events"]:
-            bad += abs(n - o["n_events"]) + 1
-        else:
-            bad += int(np.count_nonzero(t["events"][: 3 * n].cpu().numpy().view(np.uint64).reshape(n, 3) != o["events"]))
-    return bad, len(order)
 
 
 # ----------------------------------------------------------------------------- FP32 shadow leg
@@ -483,7 +348,7 @@ def shadow_leg(torch, ctx, n_cpi, steps, fd_dirs):
         sc = rd_scenario.Scene(f"c5_{q}", CH=C, N=N, T=T, F=F, B=B, sigma=256.0, guard=guard, ref=ref, targets=tg,
                                beam_angles=angles, seed=9000 + q)
         scenes.append(sc)
-        iqs.append(device_iq(torch, sc, 9000 + q, chan_err=err))
+        iqs.append(rd_workloads.device_iq(torch, sc, 9000 + q, chan_err=err))
         for k, ang, _ in tg:
             care[q, :, :, k - 2:k + 3] = 0.0
         for k, ang, _ in tg:
