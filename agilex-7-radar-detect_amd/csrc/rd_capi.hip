@@ -70,6 +70,39 @@ rd_status grow(rd_ctx* c, void** buf, size_t* have, size_t need) {
 
 bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
+constexpr size_t kStaticScratchBytes = (size_t)RD_STATIC_SCRATCH_BYTES;
+static_assert(kStaticScratchBytes % 256 == 0, "whole 256-byte lines");
+
+}  // namespace
+
+namespace rd_static {
+// Scratch of the allocation-free chain entry point (rd_chain_enqueue_static):
+// the CFAR pass's row offsets, then as many staged events per row as fit.
+// One copy per GPU, alive until the process ends. In host code `scratch` only
+// names the symbol; its device address on each GPU is taken once, at the
+// first rd_create on that device, and kept in static_scratch_dev[].
+__device__ __align__(256) uint64_t scratch[kStaticScratchBytes / sizeof(uint64_t)];
+}  // namespace rd_static
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex static_scratch_mu;
+void* static_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `device`.
+rd_status static_scratch_for(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return RD_E_NODEV;
+  std::lock_guard<std::mutex> lk(static_scratch_mu);
+  if (!static_scratch_dev[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(rd_static::scratch)) != hipSuccess || !ptr) return RD_E_HIP;
+    static_scratch_dev[device] = ptr;
+  }
+  *out = static_scratch_dev[device];
+  return RD_OK;
+}
+
 rd_status fin(hipError_t e) { return e == hipSuccess ? RD_OK : RD_E_HIP; }
 
 rd_status enter(rd_ctx* ctx) { return rd::ctx_enter(ctx); }
@@ -119,6 +152,7 @@ const char* rd_status_str(rd_status s) {
     case RD_E_HIP: return "HIP runtime error";
     case RD_E_CAPACITY: return "event buffer too small";
     case RD_E_NODEV: return "no such device";
+    case RD_E_SCRATCH: return "input exceeds the static scratch of the allocation-free entry point";
   }
   return "unknown";
 }
@@ -137,6 +171,7 @@ rd_status rd_create(int device, rd_ctx** out) {
       hipMalloc(&c->total_dev, sizeof(uint64_t)) != hipSuccess ||
       hipHostMalloc(&c->total_host, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
     st = RD_E_HIP;
+  if (st == RD_OK) st = static_scratch_for(device, &c->static_scratch);
   if (st != RD_OK) {
     rd_destroy(c);
     return st;
@@ -349,6 +384,21 @@ rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b
   rd_status st = chain_prepare(c, d, b);
   if (st != RD_OK) return st;
   return chain_launch(c, d, b, n_events_dev, c->scratch, c->scratch_bytes);
+}
+
+rd_status rd_chain_enqueue_static(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  st = rd::check_cpi(d);
+  if (st != RD_OK) return st;
+  st = rd::check_cpi_bufs(b);
+  if (st != RD_OK) return st;
+  if (!c->static_scratch) return RD_E_HIP;
+  // the row offsets must fit; staged events take what is left (rows with
+  // more are recomputed by the CFAR gather)
+  const uint64_t rows = (uint64_t)d->n_beams * d->n_frames;
+  if ((rows + 1) * sizeof(uint64_t) > kStaticScratchBytes) return RD_E_SCRATCH;
+  return chain_launch(c, d, b, n_events_dev, c->static_scratch, kStaticScratchBytes);
 }
 
 rd_status rd_chain_graph(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev,

@@ -24,6 +24,7 @@ struct rd_ctx {
   uint64_t* total_dev = nullptr;
   uint64_t* total_host = nullptr;  // pinned
   int live_graphs = 0;             // graphs and batches hold the context's stream
+  void* static_scratch = nullptr;  // this device's copy of the library's static scratch array
   void* dop = nullptr;             // slow-time transform columns
   size_t dop_bytes = 0;
   std::vector<void*> retired;      // outgrown buffers a live graph may still use

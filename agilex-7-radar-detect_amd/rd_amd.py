@@ -16,7 +16,9 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "librd_amd.so"
 
-RD_OK, RD_E_ARG, RD_E_GEOM, RD_E_HIP, RD_E_CAPACITY, RD_E_NODEV = 0, -1, -2, -3, -4, -5
+RD_OK, RD_E_ARG, RD_E_GEOM, RD_E_HIP, RD_E_CAPACITY, RD_E_NODEV, RD_E_SCRATCH = 0, -1, -2, -3, -4, -5, -6
+RD_STATIC_SCRATCH_BYTES = 32 << 20
+RD_STATIC_MAX_ROWS = RD_STATIC_SCRATCH_BYTES // 8 - 1
 
 EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
@@ -24,6 +26,7 @@ EXPORTS = [
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
     "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
+    "rd_chain_enqueue_static",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -87,6 +90,7 @@ class Lib:
                                   C.POINTER(u64)]
         L.rd_chain_run.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), C.POINTER(u64)]
         L.rd_chain_enqueue.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
+        L.rd_chain_enqueue_static.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
         L.rd_chain_graph.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp, C.POINTER(vp)]
         L.rd_graph_launch.argtypes = [vp]
         L.rd_graph_destroy.argtypes = [vp]
@@ -454,14 +458,17 @@ class Chain:
         self.ctx.lib.check("rd_chain_run", rc)
         return tot.value
 
-    def enqueue(self, iq_ptr=None, total_ptr=None):
+    def enqueue(self, iq_ptr=None, total_ptr=None, static=False):
+        """rd_chain_enqueue, or with static=True the allocation-free
+        rd_chain_enqueue_static (the CFAR scratch in the library's static array)."""
         bufs = self.bufs
         if iq_ptr is not None:
             bufs = CpiBufs.from_buffer_copy(self.bufs)
             bufs.iq = iq_ptr
-        rc = self.ctx.L.rd_chain_enqueue(self.ctx.h, C.byref(self.desc), C.byref(bufs),
-                                         total_ptr if total_ptr is not None else self.total.ptr)
-        self.ctx.lib.check("rd_chain_enqueue", rc)
+        fn = "rd_chain_enqueue_static" if static else "rd_chain_enqueue"
+        rc = getattr(self.ctx.L, fn)(self.ctx.h, C.byref(self.desc), C.byref(bufs),
+                                     total_ptr if total_ptr is not None else self.total.ptr)
+        self.ctx.lib.check(fn, rc)
 
     def graph(self, iq_ptr=None, total_ptr=None, lean=False) -> "Graph":
         """The whole chain for this CPI geometry captured as one HIP graph.

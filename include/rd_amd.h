@@ -37,7 +37,8 @@ typedef enum {
   RD_E_GEOM = -2,     /* geometry the reference rejects (e.g. fft::Config::valid) */
   RD_E_HIP = -3,      /* a HIP runtime call failed */
   RD_E_CAPACITY = -4, /* event buffer too small (count still reported) */
-  RD_E_NODEV = -5     /* no such device */
+  RD_E_NODEV = -5,    /* no such device */
+  RD_E_SCRATCH = -6   /* input exceeds the static scratch of rd_chain_enqueue_static */
 } rd_status;
 
 typedef struct rd_ctx rd_ctx;
@@ -207,6 +208,25 @@ rd_status rd_chain_run(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* 
 /* Same as rd_chain_run without the final synchronisation: the event total is
  * left in device memory at *n_events_dev (uint64). For timing loops. */
 rd_status rd_chain_enqueue(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_dev);
+
+/* Allocation-free variant of rd_chain_enqueue, for callers that cannot
+ * allocate device memory during the call: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. No counterpart in the
+ * reference (its models allocate freely with std::vector).
+ * Same arguments, launches and results as rd_chain_enqueue, but the CFAR
+ * pass keeps its row offsets and staged events in the library's static
+ * scratch array (RD_STATIC_SCRATCH_BYTES per GPU, held until the process
+ * ends) instead of the context's growable buffer, and the call allocates
+ * nothing. Limit: (n_beams * n_frames + 1) * 8 <= RD_STATIC_SCRATCH_BYTES,
+ * i.e. at most RD_STATIC_MAX_ROWS CFAR rows; larger CPIs get RD_E_SCRATCH.
+ * Every call on one GPU shares that GPU's array (all contexts on the device):
+ * calls whose work overlaps in time on one GPU race, so serialise them (one
+ * stream, or events between streams). Nothing in the array needs to start at
+ * zero; each call rewrites what it reads. */
+#define RD_STATIC_SCRATCH_BYTES (32u << 20)
+#define RD_STATIC_MAX_ROWS (RD_STATIC_SCRATCH_BYTES / 8u - 1u)
+rd_status rd_chain_enqueue_static(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs,
+                                  uint64_t* n_events_dev);
 
 /* ---- graphs --------------------------------------------------------------- */
 typedef struct rd_graph rd_graph;
