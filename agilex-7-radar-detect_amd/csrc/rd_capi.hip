@@ -190,6 +190,7 @@ rd_status rd_destroy(rd_ctx* c) {
   if (c->total_dev) (void)hipFree(c->total_dev);
   if (c->total_host) (void)hipHostFree(c->total_host);
   if (c->dop) (void)hipFree(c->dop);
+  if (c->integ) (void)hipFree(c->integ);
   for (void* p : c->retired) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -334,9 +335,103 @@ rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_
                          frame_mod, events, events_cap, row_count, c->scratch, c->scratch_bytes, c->total_dev));
   RD_TRY(hipMemcpyAsync(c->total_host, c->total_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
   RD_TRY(hipStreamSynchronize(c->stream));
-  const uint64_t tot = n_rows ? *c->total_host : 0;
+  const uint64_t tot = *c->total_host;
   if (n_events_host) *n_events_host = tot;
   return tot > events_cap ? RD_E_CAPACITY : RD_OK;
+}
+
+rd_status rd_integ_reset(const rd_integ_cfg* cfg, rd_integ_ctl* ctl) {
+  if (!cfg || !ctl || cfg->mode > 1) return RD_E_ARG;
+  ctl->active = *cfg;
+  ctl->count = 0;
+  ctl->window_id = 0;
+  return RD_OK;
+}
+
+extern "C++" {
+namespace {
+
+// Plans a control stream on the host (ctl advanced only on success), uploads
+// its control words and runs `launch` on them; synchronous.
+template <class L>
+rd_status integ_call(rd_ctx* c, rd_integ_ctl* ctl, const rd_integ_step* steps, uint32_t n_steps,
+                     rd_integ_window* windows, uint32_t cap, uint32_t* n_windows, L launch) {
+  if (!ctl || (!steps && n_steps) || !n_windows || (cap && !windows) || ctl->active.mode > 1) return RD_E_ARG;
+  for (uint32_t s = 0; s < n_steps; ++s)
+    if (steps[s].cfg.mode > 1) return RD_E_ARG;
+  rd_integ_ctl next = *ctl;
+  std::vector<uint2> words(n_steps ? n_steps : 1);
+  const uint32_t nw = rd::integ_plan(next, steps, n_steps, words.data(), windows, cap);
+  rd_status st = grow(c, &c->integ, &c->integ_bytes, words.size() * sizeof(uint2));
+  if (st != RD_OK) return st;
+  RD_TRY(hipMemcpyAsync(c->integ, words.data(), words.size() * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
+  RD_TRY(launch(static_cast<const uint2*>(c->integ)));
+  RD_TRY(hipStreamSynchronize(c->stream));
+  *ctl = next;
+  *n_windows = nw;
+  return nw > cap ? RD_E_CAPACITY : RD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+rd_status rd_integ_run(rd_ctx* c, uint32_t acc_w, rd_integ_ctl* ctl, const rd_integ_step* steps, uint32_t n_steps,
+                       const int64_t* data, uint64_t n_lanes, rd_integ_lane* lanes, int64_t* results,
+                       rd_integ_window* windows, uint32_t windows_cap, uint32_t* n_windows) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (acc_w > 64 || (n_lanes && (!lanes || (n_steps && !data))) || (windows_cap && n_lanes && !results) ||
+      n_lanes > (1ull << 39))
+    return RD_E_ARG;
+  return integ_call(c, ctl, steps, n_steps, windows, windows_cap, n_windows, [&](const uint2* w) {
+    return rd::launch_integ(c->stream, w, n_steps, data, n_lanes, lanes, results, windows_cap, acc_w);
+  });
+}
+
+rd_status rd_cov_integrate_windows(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                                   uint32_t acc_w, rd_integ_ctl* ctl, const rd_integ_step* steps,
+                                   rd_integ_lane* lanes, int64_t* results, rd_integ_window* windows,
+                                   uint32_t windows_cap, uint32_t* n_windows) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !lanes || n_chan < 1 || n_chan > 64 || acc_w > 64 || (windows_cap && !results)) return RD_E_ARG;
+  if ((uint64_t)n_bins * n_chan * (n_chan + 1) / 2 > (1ull << 31)) return RD_E_ARG;  // one work-item per pair
+  return integ_call(c, ctl, steps, n_frames, windows, windows_cap, n_windows, [&](const uint2* w) {
+    return rd::launch_cov_integ(c->stream, w, Xa, n_chan, n_frames, n_bins, lanes, results, windows_cap, acc_w);
+  });
+}
+
+rd_status rd_pfb_acc_run(rd_ctx* c, const uint32_t* x, const uint32_t* coeff, uint32_t phases, uint32_t taps,
+                         uint32_t n_chan, uint32_t n_beats, const uint32_t* hist, int64_t* acc) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!x || !coeff || !acc || phases < 1 || taps < 1 || taps > 64) return RD_E_ARG;
+  if ((uint64_t)n_chan * n_beats * phases > (1ull << 39)) return RD_E_ARG;
+  return fin(rd::launch_pfb_acc(c->stream, x, coeff, phases, taps, n_chan, n_beats, hist, acc));
+}
+
+rd_status rd_cfar_decide(rd_ctx* c, const uint64_t* cells, uint32_t n_bins, uint32_t n_rows, const rd_cfar_cfg* cfg,
+                         rd_cfar_decision* out) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!cells || !cfg || !out || (uint64_t)n_bins * n_rows > (1ull << 39)) return RD_E_ARG;
+  return fin(rd::launch_cfar_decide(c->stream, cells, n_bins, n_rows, *cfg, out));
+}
+
+rd_status rd_cfar_enqueue(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
+                          const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
+                          uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
+                          uint64_t* n_events_dev) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!cells || !cfg || !row_count || !n_events_dev || (cell_bytes != 4 && cell_bytes != 8) || n_bins < 1 ||
+      n_bins > 8191)
+    return RD_E_ARG;
+  if (!events && events_cap) return RD_E_ARG;
+  st = grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(n_bins, n_rows));
+  if (st != RD_OK) return st;
+  return fin(rd::launch_cfar(c->stream, cells, cell_bytes, n_bins, n_rows, *cfg, max_guard, max_ref, frame_ids,
+                             frame_mod, events, events_cap, row_count, c->scratch, c->scratch_bytes, n_events_dev));
 }
 
 namespace {
@@ -384,6 +479,39 @@ rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b
   rd_status st = chain_prepare(c, d, b);
   if (st != RD_OK) return st;
   return chain_launch(c, d, b, n_events_dev, c->scratch, c->scratch_bytes);
+}
+
+rd_status rd_chain_enqueue_plain(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
+  rd_status st = chain_prepare(c, d, b);
+  if (st != RD_OK) return st;
+  const uint32_t CH = d->n_chan, N = d->n_fft, F = d->n_frames, B = d->n_beams;
+  const uint64_t cols = (uint64_t)F * N;
+  hipStream_t s = c->stream;
+  RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
+  RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
+  RD_TRY(rd::launch_align(s, b->X, CH, cols, b->Xa));
+  RD_TRY(rd::launch_bf_plain(s, b->Xa, b->weights, CH, B, cols, b->y, b->yflags, b->power));
+  if (b->R) RD_TRY(rd::launch_cov_plain(s, b->Xa, CH, F, N, b->R));
+  RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
+                         b->events ? b->events_cap : 0, b->row_count, c->scratch, c->scratch_bytes,
+                         n_events_dev ? n_events_dev : c->total_dev));
+  return RD_OK;
+}
+
+rd_status rd_bf_plain_run(rd_ctx* c, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                          uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
+  return fin(rd::launch_bf_plain(c->stream, Xa, W, n_ant, n_beams, n_cols, y, flags, power));
+}
+
+rd_status rd_cov_plain_run(rd_ctx* c, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                           int64_t* R) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!Xa || !R || n_chan < 1 || n_chan > 64 || n_frames > 65535) return RD_E_ARG;
+  return fin(rd::launch_cov_plain(c->stream, Xa, n_chan, n_frames, n_bins, R));
 }
 
 rd_status rd_chain_enqueue_static(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {

@@ -616,14 +616,17 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                        const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                        uint32_t frame_mod, uint64_t* events, uint64_t cap, uint32_t* row_count, void* scratch,
                        size_t scratch_bytes, uint64_t* total_dev) {
-  if (n_rows == 0) return hipSuccess;
   if (scratch_bytes < ((size_t)n_rows + 1) * sizeof(uint64_t)) return hipErrorInvalidValue;
-  const uint32_t W = cfar_single_waves(n, n_rows);
   uint64_t* offs = static_cast<uint64_t*>(scratch);
   uint64_t* stage = offs + n_rows + 1;
   const CfarJob J{cells, frame_ids, stage, row_count, offs, events, total_dev, cap,
                   cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, cfar_slot_for(n, n_rows, scratch_bytes),
                   0};
+  if (n_rows == 0) {  // no rows: the scan alone writes the zero total
+    cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
+    return hipGetLastError();
+  }
+  const uint32_t W = cfar_single_waves(n, n_rows);
   return cell_bytes == 4 ? launch_cfar_cell<uint32_t>(s, J, W) : launch_cfar_cell<uint64_t>(s, J, W);
 }
 

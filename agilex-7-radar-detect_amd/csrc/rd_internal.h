@@ -27,6 +27,8 @@ struct rd_ctx {
   void* static_scratch = nullptr;  // this device's copy of the library's static scratch array
   void* dop = nullptr;             // slow-time transform columns
   size_t dop_bytes = 0;
+  void* integ = nullptr;           // Integrator control words of the current call
+  size_t integ_bytes = 0;
   std::vector<void*> retired;      // outgrown buffers a live graph may still use
 };
 
@@ -130,6 +132,27 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
 hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,
                                 uint32_t n_bins, uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R,
                                 uint8_t* sat_flags);
+
+// The same contractions without matrix cores (rd_plain.hip): the plain-GPU
+// comparison point. Xa aligned [cols][n_ant] / [n_frames][n_bins][n_chan].
+hipError_t launch_bf_plain(hipStream_t s, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                           uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power);
+hipError_t launch_cov_plain(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                            int64_t* R);
+
+// Exact per-element model calls (rd_exact.hip). integ_plan turns a control
+// stream into one control word per step (advancing ctl) and fills the
+// windows it emits (at most cap); it returns their number.
+uint32_t integ_plan(rd_integ_ctl& ctl, const rd_integ_step* steps, uint32_t n_steps, uint2* words,
+                    rd_integ_window* windows, uint32_t cap);
+hipError_t launch_integ(hipStream_t s, const uint2* ctl, uint32_t n_steps, const int64_t* data, uint64_t n_lanes,
+                        rd_integ_lane* lanes, int64_t* results, uint32_t cap, uint32_t acc_w);
+hipError_t launch_cov_integ(hipStream_t s, const uint2* ctl, const uint32_t* Xa, uint32_t CH, uint32_t F, uint32_t N,
+                            rd_integ_lane* lanes, int64_t* results, uint32_t cap, uint32_t acc_w);
+hipError_t launch_pfb_acc(hipStream_t s, const uint32_t* x, const uint32_t* coeff, uint32_t P, uint32_t T,
+                          uint32_t n_chan, uint32_t n_beats, const uint32_t* hist, int64_t* acc);
+hipError_t launch_cfar_decide(hipStream_t s, const uint64_t* cells, uint32_t n, uint32_t n_rows,
+                              const rd_cfar_cfg& cfg, rd_cfar_decision* out);
 
 // CFAR (rd_cfar.hip). scratch holds scratch_bytes: the row offsets
 // ((n_rows + 1) * 8 bytes, required) then as many staged events per row as

@@ -26,7 +26,8 @@ EXPORTS = [
     "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
     "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
-    "rd_chain_enqueue_static",
+    "rd_chain_enqueue_static", "rd_bf_plain_run", "rd_cov_plain_run", "rd_chain_enqueue_plain", "rd_cfar_enqueue",
+    "rd_integ_reset", "rd_integ_run", "rd_cov_integrate_windows", "rd_pfb_acc_run", "rd_cfar_decide",
 ]
 RD_BATCH_AUTOTUNE = 1
 
@@ -51,6 +52,39 @@ class CpiBufs(C.Structure):
                 ("pfb_flags", C.c_void_p), ("X", C.c_void_p), ("fft_flags", C.c_void_p), ("Xa", C.c_void_p),
                 ("y", C.c_void_p), ("yflags", C.c_void_p), ("power", C.c_void_p), ("R", C.c_void_p),
                 ("events", C.c_void_p), ("events_cap", C.c_uint64), ("row_count", C.c_void_p)]
+
+
+class IntegCfg(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("window_len", "mode", "exp_k", "enable")]
+
+
+class IntegStep(C.Structure):
+    _fields_ = [("cfg", IntegCfg), ("valid", C.c_uint32), ("flush", C.c_uint32), ("sat_clear", C.c_uint32)]
+
+
+class IntegWindow(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("step", "window_id", "sample_count", "flushed", "truncated")]
+
+
+class IntegCtl(C.Structure):
+    _fields_ = [("active", IntegCfg), ("count", C.c_uint32), ("window_id", C.c_uint32)]
+
+
+INTEG_LANE_DTYPE = np.dtype([("acc", np.int64), ("flags", np.uint32), ("sat_count", np.uint32)])
+CFAR_DECISION_DTYPE = np.dtype([("cut_power", np.uint64), ("sum_lead", np.uint64), ("sum_lag", np.uint64),
+                                ("sel_sum", np.uint64), ("sel_n", np.uint32), ("window_ok", np.uint8),
+                                ("evaluable", np.uint8), ("detected", np.uint8), ("suppressed", np.uint8),
+                                ("take_lead", np.uint8), ("pad", np.uint8, 7)])
+
+
+def integ_steps(steps7):
+    """A ctypes array of IntegStep from rows (window_len, mode, exp_k, enable,
+    valid, flush, sat_clear)."""
+    rows = np.asarray(steps7, np.uint32).reshape(-1, 7)
+    arr = (IntegStep * max(1, rows.shape[0]))()
+    for i, r in enumerate(rows.tolist()):
+        arr[i] = IntegStep(IntegCfg(*r[:4]), r[4], r[5], r[6])
+    return arr
 
 
 class RdError(RuntimeError):
@@ -88,9 +122,19 @@ class Lib:
         L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
                                   C.POINTER(u64)]
+        L.rd_integ_reset.argtypes = [C.POINTER(IntegCfg), C.POINTER(IntegCtl)]
+        L.rd_integ_run.argtypes = [vp, u32, C.POINTER(IntegCtl), vp, u32, vp, u64, vp, vp, vp, u32, C.POINTER(u32)]
+        L.rd_cov_integrate_windows.argtypes = [vp, vp, u32, u32, u32, u32, C.POINTER(IntegCtl), vp, vp, vp, vp, u32,
+                                               C.POINTER(u32)]
+        L.rd_pfb_acc_run.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp]
+        L.rd_cfar_decide.argtypes = [vp, vp, u32, u32, C.POINTER(CfarCfg), vp]
+        L.rd_cfar_enqueue.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp, vp]
         L.rd_chain_run.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), C.POINTER(u64)]
         L.rd_chain_enqueue.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
         L.rd_chain_enqueue_static.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
+        L.rd_chain_enqueue_plain.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp]
+        L.rd_bf_plain_run.argtypes = [vp, vp, vp, u32, u32, u64, vp, vp, vp]
+        L.rd_cov_plain_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_chain_graph.argtypes = [vp, C.POINTER(CpiDesc), C.POINTER(CpiBufs), vp, C.POINTER(vp)]
         L.rd_graph_launch.argtypes = [vp]
         L.rd_graph_destroy.argtypes = [vp]
@@ -314,6 +358,23 @@ class Ctx:
         return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
                 da.download(np.uint32, CH * cols))
 
+    def bf_plain(self, Xa, W, CH, B):
+        """rd_bf_plain_run -> (y, flags, power)."""
+        Xa = np.asarray(Xa, np.uint32)
+        cols = Xa.size // CH
+        dX, dW = self.put(Xa), self.put(np.asarray(W, np.uint32))
+        dy, df, dp = self.buf(4 * B * cols), self.buf(B * cols), self.buf(4 * B * cols)
+        self.lib.check("rd_bf_plain_run", self.L.rd_bf_plain_run(self.h, dX.ptr, dW.ptr, CH, B, cols, dy.ptr, df.ptr,
+                                                                 dp.ptr))
+        return dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols)
+
+    def cov_plain(self, Xa, CH, F, N):
+        dX = self.put(np.asarray(Xa, np.uint32))
+        npairs = CH * (CH + 1) // 2
+        dR = self.buf(16 * N * npairs)
+        self.lib.check("rd_cov_plain_run", self.L.rd_cov_plain_run(self.h, dX.ptr, CH, F, N, dR.ptr))
+        return dR.download(np.int64, 2 * N * npairs)
+
     def cov(self, Xa, CH, F, N):
         dX = self.put(np.asarray(Xa, np.uint32))
         npairs = CH * (CH + 1) // 2
@@ -417,6 +478,83 @@ def _ctx_cov_integrate(self, Xa, CH, F, N, mode, exp_k, acc_w):
 Ctx.cov_integrate = _ctx_cov_integrate
 
 
+def _ctx_integ_run(self, acc_w, ctl, steps7, data, lanes, cap=64):
+    """rd_integ_run: data [n_steps][n_lanes] int64 (host), lanes a DevBuf of
+    INTEG_LANE_DTYPE records carried across calls, ctl an IntegCtl (advanced in
+    place). Returns (results [n_windows][n_lanes], windows [n_windows][5],
+    n_windows)."""
+    data = np.ascontiguousarray(data, np.int64)
+    n_steps = data.shape[0]
+    n_lanes = data.shape[1] if data.ndim == 2 else 0
+    steps = integ_steps(steps7)
+    dd = self.put(data) if data.size else None
+    res = self.buf(8 * max(cap, 1) * max(n_lanes, 1))
+    wins = (IntegWindow * max(cap, 1))()
+    nw = C.c_uint32()
+    rc = self.L.rd_integ_run(self.h, acc_w, C.byref(ctl), steps, n_steps, dd.ptr if dd else None, n_lanes,
+                             lanes.ptr, res.ptr, wins, cap, C.byref(nw))
+    self.lib.check("rd_integ_run", rc, ok=(RD_OK, RD_E_CAPACITY))
+    k = min(nw.value, cap)
+    w = np.array([[x.step, x.window_id, x.sample_count, x.flushed, x.truncated] for x in wins[:k]],
+                 np.uint32).reshape(k, 5)
+    return res.download(np.int64, k * n_lanes).reshape(k, n_lanes), w, nw.value
+
+
+Ctx.integ_run = _ctx_integ_run
+
+
+def _ctx_cov_integrate_windows(self, Xa, CH, F, N, acc_w, ctl, steps7, lanes, cap=64):
+    """rd_cov_integrate_windows on host Xa; lanes a DevBuf of N*pairs*2
+    INTEG_LANE_DTYPE records. Returns (results [n_windows][N*pairs*2],
+    windows [n_windows][5], n_windows)."""
+    dX = self.put(np.ascontiguousarray(Xa, np.uint32))
+    n_lanes = N * CH * (CH + 1)
+    steps = integ_steps(steps7)
+    res = self.buf(8 * max(cap, 1) * n_lanes)
+    wins = (IntegWindow * max(cap, 1))()
+    nw = C.c_uint32()
+    rc = self.L.rd_cov_integrate_windows(self.h, dX.ptr, CH, F, N, acc_w, C.byref(ctl), steps, lanes.ptr, res.ptr,
+                                         wins, cap, C.byref(nw))
+    self.lib.check("rd_cov_integrate_windows", rc, ok=(RD_OK, RD_E_CAPACITY))
+    k = min(nw.value, cap)
+    w = np.array([[x.step, x.window_id, x.sample_count, x.flushed, x.truncated] for x in wins[:k]],
+                 np.uint32).reshape(k, 5)
+    return res.download(np.int64, k * n_lanes).reshape(k, n_lanes), w, nw.value
+
+
+Ctx.cov_integrate_windows = _ctx_cov_integrate_windows
+
+
+def _ctx_pfb_acc(self, x, coeff, P, T, n_chan, n_beats, hist=None):
+    """rd_pfb_acc_run -> acc [n_chan][n_beats][P][2] int64."""
+    dx, dc = self.put(np.asarray(x, np.uint32)), self.put(np.asarray(coeff, np.uint32))
+    dh = self.put(np.asarray(hist, np.uint32)) if hist is not None else None
+    n = n_chan * n_beats * P
+    da = self.buf(16 * max(n, 1))
+    self.lib.check("rd_pfb_acc_run", self.L.rd_pfb_acc_run(self.h, dx.ptr, dc.ptr, P, T, n_chan, n_beats,
+                                                           dh.ptr if dh else None, da.ptr))
+    return da.download(np.int64, 2 * n)
+
+
+Ctx.pfb_acc = _ctx_pfb_acc
+
+
+def _ctx_cfar_decide(self, cells, n, cfg8):
+    """rd_cfar_decide on uint64 cells [rows][n] -> structured array of
+    CFAR_DECISION_DTYPE records [rows*n]."""
+    cells = np.ascontiguousarray(cells, np.uint64)
+    rows = cells.size // n
+    dc = self.put(cells)
+    do = self.buf(48 * max(rows * n, 1))
+    cfg = CfarCfg.of(cfg8)
+    self.lib.check("rd_cfar_decide", self.L.rd_cfar_decide(self.h, dc.ptr, n, rows, C.byref(cfg), do.ptr))
+    raw = do.download(np.uint8, 48 * rows * n)
+    return raw.view(CFAR_DECISION_DTYPE)
+
+
+Ctx.cfar_decide = _ctx_cfar_decide
+
+
 class Chain:
     """Device buffers and descriptor for one CPI geometry (reused across runs)."""
 
@@ -458,14 +596,19 @@ class Chain:
         self.ctx.lib.check("rd_chain_run", rc)
         return tot.value
 
-    def enqueue(self, iq_ptr=None, total_ptr=None, static=False):
-        """rd_chain_enqueue, or with static=True the allocation-free
-        rd_chain_enqueue_static (the CFAR scratch in the library's static array)."""
+    def enqueue(self, iq_ptr=None, total_ptr=None, static=False, plain=False, lean=False):
+        """rd_chain_enqueue; with static=True the allocation-free
+        rd_chain_enqueue_static (the CFAR scratch in the library's static
+        array); with plain=True rd_chain_enqueue_plain (no matrix cores).
+        lean: the optional per-stage outputs are not written (as graph())."""
         bufs = self.bufs
-        if iq_ptr is not None:
+        if iq_ptr is not None or lean:
             bufs = CpiBufs.from_buffer_copy(self.bufs)
+        if iq_ptr is not None:
             bufs.iq = iq_ptr
-        fn = "rd_chain_enqueue_static" if static else "rd_chain_enqueue"
+        if lean:
+            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = None
+        fn = "rd_chain_enqueue_static" if static else ("rd_chain_enqueue_plain" if plain else "rd_chain_enqueue")
         rc = getattr(self.ctx.L, fn)(self.ctx.h, C.byref(self.desc), C.byref(bufs),
                                      total_ptr if total_ptr is not None else self.total.ptr)
         self.ctx.lib.check(fn, rc)

@@ -148,3 +148,25 @@ def verify_batch_vs_oracle(torch, ctx, bs, totals, k):
         else:
             bad += int(np.count_nonzero(t["events"][: 3 * n].cpu().numpy().view(np.uint64).reshape(n, 3) != o["events"]))
     return bad, len(order)
+
+
+def covariance_trace_check(torch, Xa, R, CH, F, N):
+    """The covariance-trace invariant checked in FP64 on the GPU: for every bin
+    k, tr R[k] = sum_c sum_m power(X_c[m][k]) and every diagonal entry is real
+    (covar::cross_power(x, x) = power(x) + 0j, covar_model.hpp:99-102, 137-151).
+    Every term is an integer below 2^31 and every sum below 2^53, so the float64
+    sums are exact and the check is equality. Xa [F][N][CH] packed, R as the
+    chain writes it ([N][pairs][2] int64)."""
+    x = torch.from_numpy(np.ascontiguousarray(Xa).view(np.int32)).cuda().reshape(F, N, CH)
+    re = ((x << 16) >> 16).double()
+    im = (x >> 16).double()
+    p = (re * re + im * im).sum(dim=(0, 2))
+    del x, re, im
+    npairs = CH * (CH + 1) // 2
+    r = torch.from_numpy(np.ascontiguousarray(R)).cuda().reshape(N, npairs, 2)
+    diag = torch.tensor([i * CH - i * (i - 1) // 2 for i in range(CH)], device="cuda")
+    tr = r[:, diag, 0].double().sum(1)
+    err = float((tr - p).abs().max())
+    im_nz = int((r[:, diag, 1] != 0).sum())
+    return dict(bins=N, frames=F, channels=CH, max_abs_err=err, diag_imag_nonzero=im_nz,
+                ok=bool(err == 0.0 and im_nz == 0))

@@ -1,24 +1,36 @@
 #!/usr/bin/env python3
 """Throughput of the bit-exact PFB -> FFT -> align -> beamformer/power ->
-covariance -> CFAR -> event-packing chain on one MI355X.
+covariance -> CFAR -> event-packing chain on one MI355X: the split-int8 MFMA
+path (batched where there are many CPIs) against the plain GPU path (no matrix
+cores, one CPI per launch) and the CPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload medium]
 
-A step is one CPI of the workload through the whole chain (one HIP graph launch),
-inputs already resident in HBM. Steps cycle through a ring of distinct input
-CPIs (default 32, 537 MB for the medium workload) so the input stream is not
-served from the 256 MB last-level cache. One JSON line is printed:
+A step is one pass over a ring of distinct input CPIs of the workload
+(default 32 medium CPIs, 537 MB, more than the 256 MB last-level cache), each
+CPI one HIP graph launch of the whole chain, the CPIs of a pass dealt
+round-robin to --streams contexts (one HIP stream each) so that every input
+CPI is read once per step. Inputs are resident in HBM before the timed region.
+One JSON line is printed:
 
-  value      whole-job Msamples/s (one sample = one complex int16 input sample
-             of one channel), wall clock over exactly K steps
-  roofline   the dominant kernel: algorithmic bytes per launch / its mean
-             launch time (HIP events on the library's stream) vs 8 TB/s HBM
-  chain      the fused-chain algorithmic bytes (input + taps + weights +
-             covariance + events) per CPI over the chain's time per CPI
+  value       whole-job Msamples/s (one sample = one complex int16 input sample
+              of one channel), wall clock over exactly K steps
+  roofline    the dominant kernel of the chain (largest mean launch time, HIP
+              events around each stage launch on the library's stream over the
+              ring's distinct CPIs): the fused chain's algorithmic bytes per CPI
+              (input, taps, weights, Hermitian covariance, events) over that
+              kernel's mean launch time, against 8 TB/s; `traffic` is the
+              kernel's HBM bytes per launch from the committed rocprofv3 counter
+              summary, `valu_frac_chain` the chain's VALU issue fraction
+  plain_gpu   the same chain with the plain (no matrix cores) beamformer and
+              covariance and a stand-alone alignment, one CPI per launch
   cpu_baseline  the reference's C++ models (oracle/_ref) or the CPU
-             restatement, on this host's cores, on a bounded sample of CPIs
-  mismatches bit-exact comparison of one CPI's power, covariance and events
-             against the CPU run (must be 0)
+              restatement, on this host's cores, on a bounded sample of CPIs
+  mismatches  bit-exact comparison of one CPI's power, covariance and events
+              against the CPU run (must be 0)
+  large / batch / shadow  the large workload (MFMA vs plain), the 256-CPI
+              uneven batch (one graph launch vs per-CPI launches, MFMA and
+              plain) and the FP32 shadow
 """
 from __future__ import annotations
 
@@ -39,8 +51,13 @@ import rd_scenario  # noqa: E402
 import rd_workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_OPS = 5.0e15
 VALU_SIMDS = 1024  # 256 CUs x 4 SIMDs
 VALU_CLOCK_HZ = 2.4e9
+STAGES = ("pfb", "fft", "bf", "cov", "cfar")
+KERNEL_STAGES = ("pfb", "fft", "bf", "cov")  # one kernel per launch (CFAR is rows + scan + gather)
+SPIN_CYCLES = 20_000_000  # shader-clock cycles of the spin ahead of each timed group (about 8 ms)
+PLAIN_STAGES = ("pfb", "fft", "align", "bf_plain", "cov_plain", "cfar")
 
 
 def note(msg: str) -> None:
@@ -54,55 +71,172 @@ def algo_chain_bytes(sc, n_events: int) -> int:
             + 22 * n_events)
 
 
-def stage_kernels(ctx, ch, sc, lean=True):
-    """Per-stage launchers (device buffers of `ch`), for the per-kernel timing.
-    lean: the stages as the timed (lean) chain graph runs them, without the
-    PFB and FFT flag outputs."""
-    L, h, b = ctx.L, ctx.h, ch.b
+def stage_launchers(ctx, sc, plain=False):
+    """The chain's stage launches on the library stream, as the lean chain
+    graph runs them (no PFB/FFT flags, no beam samples): each takes the chain
+    whose buffers it uses and the input CPI's device pointer. With each
+    kernel's own algorithmic bytes per launch (its compulsory reads + writes)."""
+    L, h = ctx.L, ctx.h
     S = sc.CH * sc.N * sc.F
     cols = sc.N * sc.F
-    pfb_flags = None if lean or not b["pfb_flags"] else b["pfb_flags"].ptr
-    fft_flags = None if lean or not b["fft_flags"] else b["fft_flags"].ptr
+    cfg = rd_amd.CfarCfg.of(sc.cfar_cfg())
+    chk = ctx.lib.check
 
-    def pfb():
-        ctx.lib.check("pfb", L.rd_pfb_run(h, b["iq"].ptr, b["taps"].ptr, sc.N, sc.T, sc.CH, sc.F, None, b["u"].ptr,
-                                          pfb_flags))
+    def pfb(ch, iq):
+        b = ch.b
+        chk("pfb", L.rd_pfb_run(h, iq, b["taps"].ptr, sc.N, sc.T, sc.CH, sc.F, None, b["u"].ptr, None))
 
-    def fft():
-        ctx.lib.check("fft", L.rd_fft_run(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, 1, sc.CH * sc.F, None, b["X"].ptr,
-                                          fft_flags))
+    def fft(ch, iq):
+        b = ch.b
+        chk("fft", L.rd_fft_run(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, 1, sc.CH * sc.F, None, b["X"].ptr, None))
 
-    def bf():
-        # the chain's beamformer: FFT layout in, power + aligned copy out (lean outputs)
-        ctx.lib.check("bf", L.rd_bf_stage(h, b["X"].ptr, b["weights"].ptr, sc.CH, sc.B, sc.F, sc.N, None, None,
-                                          b["power"].ptr, b["Xa"].ptr))
+    def bf(ch, iq):  # FFT layout in, power + aligned copy out
+        b = ch.b
+        chk("bf", L.rd_bf_stage(h, b["X"].ptr, b["weights"].ptr, sc.CH, sc.B, sc.F, sc.N, None, None, b["power"].ptr,
+                                b["Xa"].ptr))
 
-    def cov():
-        ctx.lib.check("cov", L.rd_cov_run(h, b["Xa"].ptr, sc.CH, sc.F, sc.N, b["R"].ptr))
+    def cov(ch, iq):
+        chk("cov", L.rd_cov_run(h, ch.b["Xa"].ptr, sc.CH, sc.F, sc.N, ch.b["R"].ptr))
 
-    # algorithmic bytes per launch of each stage kernel (its compulsory reads + writes)
-    flag_b = 1 if pfb_flags else 0
-    stages = {
-        "pfb": (pfb, S * (4 + 4 + flag_b) + 4 * sc.N * sc.T),
-        "fft": (fft, S * 8 + (4 * sc.CH * sc.F if fft_flags else 0)),
-        "bf": (bf, S * 4 + 4 * sc.B * sc.CH + S * 4 + sc.B * cols * 4),
-        "cov": (cov, S * 4 + 16 * sc.N * sc.CH * (sc.CH + 1) // 2),
+    def align(ch, iq):
+        chk("align", L.rd_align_run(h, ch.b["X"].ptr, sc.CH, sc.F, sc.N, ch.b["Xa"].ptr))
+
+    def bf_plain(ch, iq):
+        b = ch.b
+        chk("bf_plain", L.rd_bf_plain_run(h, b["Xa"].ptr, b["weights"].ptr, sc.CH, sc.B, cols, None, None,
+                                          b["power"].ptr))
+
+    def cov_plain(ch, iq):
+        chk("cov_plain", L.rd_cov_plain_run(h, ch.b["Xa"].ptr, sc.CH, sc.F, sc.N, ch.b["R"].ptr))
+
+    def cfar(ch, iq):  # rows, scan and gather; the total stays on the device
+        b = ch.b
+        chk("cfar", L.rd_cfar_enqueue(h, b["power"].ptr, 4, sc.N, sc.B * sc.F, rd_amd.C.byref(cfg),
+                                      rd_scenario.MAX_GUARD, rd_scenario.MAX_REF, None, sc.F, b["events"].ptr,
+                                      ch.events_cap, b["row_count"].ptr, ch.total.ptr))
+
+    cov_bytes = S * 4 + 16 * sc.N * sc.CH * (sc.CH + 1) // 2
+    bf_bytes = S * 4 + 4 * sc.B * sc.CH + sc.B * cols * 4
+    table = {
+        "pfb": (pfb, S * 8 + 4 * sc.N * sc.T),
+        "fft": (fft, S * 8),
+        "bf": (bf, bf_bytes + S * 4),
+        "cov": (cov, cov_bytes),
+        "align": (align, S * 8),
+        "bf_plain": (bf_plain, bf_bytes),
+        "cov_plain": (cov_plain, cov_bytes),
+        "cfar": (cfar, sc.B * cols * 4),
     }
-    return stages
+    return {k: table[k] for k in (PLAIN_STAGES if plain else STAGES)}
 
 
-def time_stage(torch, stream, fn, reps: int) -> float:
-    """Mean launch duration in ms of `fn` over `reps` back-to-back launches,
-    by HIP events recorded on the library's stream."""
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    fn()
-    e0.record(stream)
-    for _ in range(reps):
-        fn()
-    e1.record(stream)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps
+def slot_chains(ctx, sc, taps, W, n):
+    """n lean chains of the workload (no flags, no beam samples), one per ring
+    slot, so every stage launch of the per-stage timing reads inputs that no
+    launch just before it touched."""
+    out = []
+    for _ in range(n):
+        c = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
+                         rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True,
+                         events_cap=sc.B * sc.F * 64)
+        c.b["taps"].upload(taps)
+        c.b["weights"].upload(W)
+        out.append(c)
+    return out
+
+
+def ring_stage_times(torch, ctx, chains, sc, iq_ptrs, plain=False, passes=1):
+    """Mean launch duration (ms) of every stage: for each stage in chain order,
+    HIP events on the library's stream around its launches for every ring CPI
+    back to back (each CPI with its own chain buffers, so the stage reads what
+    the previous stage wrote for that CPI, a whole ring's stage ago), divided
+    by the number of launches; `passes` times over the ring."""
+    stages = stage_launchers(ctx, sc, plain)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    acc = {n: 0.0 for n in stages}
+    for fn, _ in stages.values():  # warm every kernel once
+        fn(chains[0], iq_ptrs[0])
+    ctx.sync()
+    for _ in range(passes):
+        evs = []
+        for n, (fn, _) in stages.items():
+            # a bounded clock spin first, so that every launch of the group is
+            # queued before the GPU reaches it: the events then time the
+            # kernels back to back, not the host's launch rate
+            with torch.cuda.stream(stream):
+                torch.cuda._sleep(SPIN_CYCLES)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for c, p in zip(chains, iq_ptrs):
+                fn(c, p)
+            e1.record(stream)
+            evs.append((n, e0, e1))
+            ctx.sync()
+        for n, e0, e1 in evs:
+            acc[n] += e0.elapsed_time(e1) / len(iq_ptrs)
+    out = {}
+    for n, (_, nbytes) in stages.items():
+        ms = acc[n] / passes
+        out[n] = dict(ms=round(ms, 5), algo_bytes=nbytes, gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    return out
+
+
+def profiled(stage: str, workload: str):
+    """Counters per dispatch of a stage kernel (HBM bytes, VALU and MFMA wave
+    instructions) from the committed rocprofv3 counter summary of that
+    workload, `profiles/chain_<workload>_counters.md` (None when absent)."""
+    path = ROOT / "profiles" / f"chain_{workload}_counters.md"
+    if not path.exists():
+        return None
+    head = None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) < 4:
+            continue
+        if cells[0] == "stage":
+            head = cells
+            continue
+        if head and cells[0] == stage:
+            row = dict(zip(head, cells))
+            try:
+                return dict(hbm_bytes=int(row["HBM bytes"]), valu_insts=int(row["SQ_INSTS_VALU"]),
+                            mfma_insts=int(row.get("SQ_INSTS_MFMA", "0") or 0), avg_us=float(row["avg us"]))
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
+def valu_frac(valu_insts: float, ms: float) -> float:
+    """Fraction of the VALU issue rate a launch used: wave instructions x 4
+    cycles over 1024 SIMDs x 2.4 GHz x the launch time."""
+    return round(valu_insts * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * ms * 1e-3), 4)
+
+
+def mfma_ops(sc):
+    """int8 MFMA work of the beamformer and covariance per CPI as issued: four
+    digit products x (re, im) GEMMs x 2 ops per MAC over the padded K."""
+    pairs = sc.CH * (sc.CH + 1) // 2
+    cols = sc.N * sc.F
+    return {"bf": 32 * sc.B * sc.CH * cols, "cov": 32 * pairs * sc.F * sc.N}
+
+
+def lean_bufs(ch, iq_ptr=None):
+    b = rd_amd.CpiBufs.from_buffer_copy(ch.bufs)
+    if iq_ptr is not None:
+        b.iq = iq_ptr
+    b.pfb_flags = b.fft_flags = b.y = b.yflags = None
+    return b
+
+
+def check_cpi(ch, n_ev, ref):
+    """Mismatching values of one CPI's power, covariance and events."""
+    got = ch.fetch(("power", "R"), n_events=n_ev)
+    mism = int(np.count_nonzero(got["power"] != ref["power"]) + np.count_nonzero(got["R"] != ref["R"]))
+    if n_ev != ref["n_events"]:
+        mism += abs(n_ev - ref["n_events"]) + 1
+    else:
+        mism += int(np.count_nonzero(got["events"] != ref["events"]))
+    return mism
 
 
 def cpu_baseline(sc, iq, taps, W, budget_s: float):
@@ -144,34 +278,16 @@ def cpu_model() -> str:
     return "unknown"
 
 
+
+
 # ----------------------------------------------------------------------------- large leg
-def profiled(stage: str, workload: str):
-    """HBM bytes and VALU wave-instructions per dispatch of a stage kernel, from
-    the committed rocprofv3 counter summary of that workload (None when absent)."""
-    path = ROOT / "profiles" / f"chain_{workload}_hbm_counters.md"
-    if not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 8 and cells[0] == stage:
-            try:
-                return dict(hbm_bytes=int(cells[5]), valu_insts=int(cells[7]))
-            except ValueError:
-                return None
-    return None
-
-
-def valu_frac(valu_insts: int, ms: float) -> float:
-    """Fraction of the VALU issue rate a launch used: wave instructions x 4
-    cycles over 1024 SIMDs x 2.4 GHz x the launch time."""
-    return round(valu_insts * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * ms * 1e-3), 4)
-
-
-def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
+def large_leg(torch, ctx, steps: int, verify: bool):
     """The large workload (64 ch x 4096-pt x 16 taps x 512 frames, 64 beams):
-    graph replay over two device-generated CPIs, per-stage times with the
-    beamformer and covariance MFMA rates, and one CPI checked against the
-    oracle on the host's cores."""
+    the MFMA chain as graph replays over two device-generated CPIs against the
+    plain chain (no matrix cores) enqueued per CPI on the same CPIs, per-stage
+    times of both, the beamformer's and covariance's int8 MFMA rate as issued,
+    one CPI checked against the oracle on the host's cores and its covariance
+    trace checked in FP64 on the GPU."""
     sc = rd_scenario.config("large")
     taps, W = sc.taps(), sc.weights()
     ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
@@ -180,39 +296,62 @@ def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
     ring = [rd_workloads.device_iq(torch, sc, 7001 + i) for i in range(2)]
     iq0 = ring[0].cpu().numpy().view(np.uint32)
     ch.load(iq0, taps, W)
-    totals = ctx.buf(16)
+    totals = ctx.buf(32)
     graphs = [ch.graph(iq_ptr=ring[i].data_ptr(), total_ptr=totals.ptr + 8 * i, lean=True) for i in range(2)]
-    stream = torch.cuda.ExternalStream(ctx.stream)
-    for i in range(2):
-        graphs[i].launch()
+    for g in graphs:
+        g.launch()
     ctx.sync()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for i in range(steps):
-        graphs[i % 2].launch()
-    e1.record(stream)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / steps
+
+    def wall(fn, n):
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(n):
+            fn(i)
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    ms = wall(lambda i: graphs[i % 2].launch(), steps)
+    # the plain chain, one CPI per launch sequence, on the same CPIs
+    pbufs = [lean_bufs(ch, ring[i].data_ptr()) for i in range(2)]
+
+    def plain(i):
+        ctx.lib.check("rd_chain_enqueue_plain", ctx.L.rd_chain_enqueue_plain(
+            ctx.h, rd_amd.C.byref(ch.desc), rd_amd.C.byref(pbufs[i % 2]), totals.ptr + 16 + 8 * (i % 2)))
+
+    plain(0)
+    plain(1)
+    plain_ms = wall(plain, max(2, steps // 2))
+    tot = totals.download(np.uint64, 4)
+    if not np.array_equal(tot[:2], tot[2:]):
+        sys.exit(f"large: plain and MFMA event totals differ: {tot}")
     note("large: per-stage kernel timing")
-    per = {}
-    for name, (fn, nbytes) in stage_kernels(ctx, ch, sc).items():
-        t = time_stage(torch, stream, fn, stage_reps)
-        per[name] = dict(ms=round(t, 4), algo_bytes=nbytes, gbs=round(nbytes / (t * 1e-3) / 1e9, 1))
-    pairs = sc.CH * (sc.CH + 1) // 2
-    cols = sc.N * sc.F
-    # int8 MFMA work as issued: 4 digit products x (re, im) x 2 ops per MAC
-    mfma = {"bf": 32 * sc.B * sc.CH * cols, "cov": 32 * pairs * sc.F * sc.N}
-    for k, ops in mfma.items():
-        per[k]["mfma_int8_tops"] = round(ops / (per[k]["ms"] * 1e-3) / 1e12, 1)
-        per[k]["mfma_frac_of_5pops"] = round(ops / (per[k]["ms"] * 1e-3) / 5e15, 4)
+    ptrs = [r.data_ptr() for r in ring]
+    sch = slot_chains(ctx, sc, taps, W, len(ptrs))
+    per = ring_stage_times(torch, ctx, sch, sc, ptrs, passes=2)
+    per_plain = ring_stage_times(torch, ctx, sch, sc, ptrs, plain=True)
+    del sch
+    for k, ops in mfma_ops(sc).items():
+        per[k]["mfma_int8_tops_as_issued"] = round(ops / (per[k]["ms"] * 1e-3) / 1e12, 1)
+        per[k]["mfma_frac_of_5pops_as_issued"] = round(ops / (per[k]["ms"] * 1e-3) / INT8_PEAK_OPS, 4)
+    for k in STAGES:
+        prof = profiled(k, "large")
+        if prof:
+            per[k]["valu_frac"] = valu_frac(prof["valu_insts"], per[k]["ms"])
+            if prof["mfma_insts"]:
+                per[k]["mfma_wave_insts"] = prof["mfma_insts"]
     for g in graphs:
         g.close()
     res = dict(workload=f"large: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, {sc.B} beams, "
                         f"CA-CFAR {sc.ref}/{sc.guard}",
                cpi_samples=sc.samples, steps=steps, ms_per_cpi=round(ms, 4),
-               msamples_s=round(sc.samples / (ms * 1e-3) / 1e6, 1), stages=per, mismatches=None)
+               msamples_s=round(sc.samples / (ms * 1e-3) / 1e6, 1),
+               plain_gpu=dict(ms_per_cpi=round(plain_ms, 4), msamples_s=round(sc.samples / (plain_ms * 1e-3) / 1e6, 1),
+                              impl="rd_chain_enqueue_plain per CPI: stand-alone alignment, VALU beamformer and "
+                                   "covariance (int64 sums), same PFB/FFT/CFAR", stages=per_plain),
+               mfma_split_over_plain=round(plain_ms / ms, 3),
+               bf_cov_mfma_over_plain=round((per_plain["align"]["ms"] + per_plain["bf_plain"]["ms"]
+                                             + per_plain["cov_plain"]["ms"]) / (per["bf"]["ms"] + per["cov"]["ms"]), 3),
+               stages=per, mismatches=None)
     if verify:
         import oracle
 
@@ -220,29 +359,26 @@ def large_leg(torch, ctx, steps: int, verify: bool, stage_reps: int):
         ch.enqueue()
         ctx.sync()
         n_ev = int(ch.total.download(np.uint64, 1)[0])
-        got = ch.fetch(("power", "R"), n_events=n_ev)
         threads = max(1, min(16, os.cpu_count() or 1))
         t0 = time.perf_counter()
         ref = oracle.load("oracle").chain(iq0, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(),
                                           rd_scenario.MAX_GUARD, rd_scenario.MAX_REF, keep=("power", "R", "events"),
                                           threads=threads)
         cpu_s = time.perf_counter() - t0
-        mism = int(np.count_nonzero(got["power"] != ref["power"]) + np.count_nonzero(got["R"] != ref["R"]))
-        if n_ev != ref["n_events"]:
-            mism += abs(n_ev - ref["n_events"]) + 1
-        else:
-            mism += int(np.count_nonzero(got["events"] != ref["events"]))
-        res.update(mismatches=mism, events_per_cpi=n_ev,
+        res.update(mismatches=check_cpi(ch, n_ev, ref), events_per_cpi=n_ev,
                    oracle_cpu=dict(seconds=round(cpu_s, 2), threads=threads,
                                    msamples_s=round(sc.samples / cpu_s / 1e6, 2), kind="port"))
+        f = ch.fetch(("Xa", "R"))
+        res["covariance_trace_fp64"] = rd_workloads.covariance_trace_check(torch, f["Xa"], f["R"], sc.CH, sc.F, sc.N)
     del ring
     return res
 
 
 # ----------------------------------------------------------------------------- batched leg
 def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
-    """256 uneven CPIs: one batched graph launch vs one CPI per launch, both
-    checked against each other on every CPI and against the CPU on a sample."""
+    """256 uneven CPIs: one batched graph launch of the MFMA chain vs one CPI
+    per launch (MFMA chain, and the plain chain without matrix cores), checked
+    against each other on every CPI and against the CPU on a sample."""
     scenes = rd_scenario.uneven_batch(n_cpi)
     note(f"batch: generating {n_cpi} uneven CPIs on the device")
     bs = rd_workloads.BatchSet(torch, scenes, seed0=77_000)
@@ -262,6 +398,21 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
                                                                   totals_dev.data_ptr() + 8 * i))
         ctx.sync()
 
+    plain_totals = torch.zeros(n_cpi, dtype=torch.int64, device="cuda")
+    def lean():  # the current buffers (event buffers may have been regrown) without the optional outputs
+        out = []
+        for b in bs.bufs:
+            lb = rd_amd.CpiBufs.from_buffer_copy(b)
+            lb.pfb_flags = lb.fft_flags = lb.y = lb.yflags = None
+            out.append(lb)
+        return out
+
+    def plain_gpu():  # no matrix cores, one CPI per launch sequence
+        for i, (d, b) in enumerate(zip(bs.descs, plain_bufs)):
+            ctx.lib.check("rd_chain_enqueue_plain", L.rd_chain_enqueue_plain(h, rd_amd.C.byref(d), rd_amd.C.byref(b),
+                                                                              plain_totals.data_ptr() + 8 * i))
+        ctx.sync()
+
     def wall(fn, n):
         ctx.sync()
         t0 = time.perf_counter()
@@ -275,6 +426,11 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
     bs.fit_event_caps(torch, totals_dev.cpu().numpy())
     t_async = wall(plain_async, reps)
     t_sync = wall(plain_sync, max(1, reps // 2))
+    note("batch: plain GPU (no matrix cores) one-CPI-per-launch passes")
+    plain_bufs = lean()
+    t_plain_gpu = wall(plain_gpu, max(1, reps // 2))
+    if not torch.equal(plain_totals, totals_dev):
+        sys.exit("batch: the plain chain's event totals differ from the MFMA chain's")
     note(f"batch: creating the batch graph (autotune={autotune})")
     t0 = time.perf_counter()
     batch = rd_amd.Batch(ctx, bs.descs, bs.bufs, autotune=autotune, keep=bs)
@@ -295,12 +451,15 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
         "samples": samples,
         "batched": {"ms": round(t_batch * 1e3, 3), "msamples_s": round(samples / t_batch / 1e6, 1),
                     "impl": "one hipGraph launch: per-stage shape classes as persistent kernels over all CPIs"},
-        "plain_async": {"ms": round(t_async * 1e3, 3), "msamples_s": round(samples / t_async / 1e6, 1),
+        "mfma_per_cpi_async": {"ms": round(t_async * 1e3, 3), "msamples_s": round(samples / t_async / 1e6, 1),
                         "impl": "rd_chain_enqueue per CPI on one stream, one sync at the end"},
-        "plain_sync": {"ms": round(t_sync * 1e3, 3), "msamples_s": round(samples / t_sync / 1e6, 1),
+        "mfma_per_cpi_sync": {"ms": round(t_sync * 1e3, 3), "msamples_s": round(samples / t_sync / 1e6, 1),
                        "impl": "rd_chain_run per CPI (event count returned to the host each CPI)"},
-        "speedup_vs_plain_async": round(t_async / t_batch, 3),
-        "speedup_vs_plain_sync": round(t_sync / t_batch, 3),
+        "plain_gpu": {"ms": round(t_plain_gpu * 1e3, 3), "msamples_s": round(samples / t_plain_gpu / 1e6, 1),
+                      "impl": "rd_chain_enqueue_plain per CPI (no matrix cores), one sync at the end"},
+        "speedup_vs_plain_gpu": round(t_plain_gpu / t_batch, 3),
+        "speedup_vs_mfma_per_cpi_async": round(t_async / t_batch, 3),
+        "speedup_vs_mfma_per_cpi_sync": round(t_sync / t_batch, 3),
         "autotune": bool(autotune), "create_s": round(t_create, 2), "plan": plan,
         "mismatching_cpis": bad_cpis, "oracle_cpis": orc_n, "oracle_mismatches": orc_bad,
         "events": int(np.sum(totals)),
@@ -496,18 +655,20 @@ def shadow_leg(torch, ctx, n_cpi, steps, fd_dirs):
     }
 
 
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=320)
-    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (passes over the ring)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="medium", choices=["tiny", "medium", "large"])
-    ap.add_argument("--ring", type=int, default=32, help="distinct input CPIs cycled through")
+    ap.add_argument("--ring", type=int, default=32, help="distinct input CPIs; one step is one pass over them")
     ap.add_argument("--streams", type=int, default=3,
-                    help="contexts (one HIP stream each) the timed CPIs are dealt to round-robin")
+                    help="contexts (one HIP stream each) the CPIs of a pass are dealt to round-robin")
     ap.add_argument("--cpu-seconds", type=float, default=3.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--stage-passes", type=int, default=2, help="passes over the ring for the per-stage timing")
     ap.add_argument("--batch-cpis", type=int, default=256, help="CPIs of the uneven batch leg (0: skip it)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--batch-oracle", type=int, default=3, help="smallest batch CPIs also checked on the CPU")
@@ -518,6 +679,7 @@ def main():
     ap.add_argument("--shadow-only", action="store_true", help="run only the FP32 shadow leg")
     ap.add_argument("--large-steps", type=int, default=10, help="graph replays of the large-workload leg (0: skip it)")
     ap.add_argument("--large-verify", type=int, default=1, help="check one large CPI against the oracle (0: skip)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-GPU timing of the main workload")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
@@ -554,144 +716,154 @@ def main():
         return
     sc = rd_scenario.config(args.workload)
     taps, W = sc.taps(), sc.weights()
+    R = args.ring
+    n_str = max(1, min(args.streams, R))
     ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
                       keep_y=True, keep_flags=True, keep_R=True)
-    note(f"workload {sc.name}: generating {args.ring} input CPIs")
-    ch.load(sc.iq(seed=0), taps, W)
-    ring = []
-    for i in range(args.ring):
-        ring.append(ctx.put(sc.iq(seed=1000 + i)))
-        if sc.samples > 50_000_000:
-            note(f"  input CPI {i + 1}/{args.ring} resident")
-    totals = ctx.buf(8 * args.ring)
+    note(f"workload {sc.name}: generating {R} input CPIs")
+    ch.b["taps"].upload(taps)
+    ch.b["weights"].upload(W)
+    if sc.samples > 50_000_000:  # large CPIs are generated on the device (same construction as Scene.iq)
+        ring = [rd_workloads.device_iq(torch, sc, 1000 + i) for i in range(R)]
+        ring_ptrs = [t.data_ptr() for t in ring]
+        iq0 = None if args.no_cpu else ring[0].cpu().numpy().view(np.uint32)
+    else:
+        ring = [ctx.put(sc.iq(seed=1000 + i)) for i in range(R)]
+        ring_ptrs = [r.ptr for r in ring]
+        iq0 = sc.iq(seed=1000)
+    totals = ctx.buf(8 * R)
 
     # correctness of one CPI against the CPU run (also the timed CPU baseline)
-    iq0 = sc.iq(seed=1000)
     cpu = None
     if not args.no_cpu:
         note("CPU baseline")
         cpu, cpu_out = cpu_baseline(sc, iq0, taps, W, args.cpu_seconds)
-    ch.enqueue(iq_ptr=ring[0].ptr)
+    ch.enqueue(iq_ptr=ring_ptrs[0])
     ctx.sync()
     n_ev = int(ch.total.download(np.uint64, 1)[0])
-    got = ch.fetch(("power", "R"), n_events=n_ev)
-    mismatches = None
-    if cpu is not None:
-        mismatches = int(np.count_nonzero(got["power"] != cpu_out["power"]) + np.count_nonzero(got["R"] != cpu_out["R"]))
-        if n_ev != cpu_out["n_events"]:
-            mismatches += abs(n_ev - cpu_out["n_events"]) + 1
-        else:
-            mismatches += int(np.count_nonzero(got["events"] != cpu_out["events"]))
+    mismatches = check_cpi(ch, n_ev, cpu_out) if cpu is not None else None
+    f = ch.fetch(("Xa", "R"))
+    trace = rd_workloads.covariance_trace_check(torch, f["Xa"], f["R"], sc.CH, sc.F, sc.N)
+    del f
 
-    stream = torch.cuda.ExternalStream(ctx.stream)
-    note(f"capturing {args.ring} chain graphs (lean and full-output)")
-    graphs = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i, lean=True) for i in range(args.ring)]
-    graphs_full = [ch.graph(iq_ptr=ring[i].ptr, total_ptr=totals.ptr + 8 * i) for i in range(args.ring)]
-
-    def timed(launch_one, steps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        ctx.sync()
+    def timed(step, steps):
+        """Wall clock of `steps` calls of step(), bracketed by a synchronisation
+        of every context and of torch on both sides."""
+        for c in ctxs:
+            c.sync()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        e0.record(stream)
-        for i in range(steps):
-            launch_one(i)
-        t_host = time.perf_counter() - t0
-        e1.record(stream)
-        ctx.sync()
+        for _ in range(steps):
+            step()
+        for c in ctxs:
+            c.sync()
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, t_host, e0.elapsed_time(e1) * 1e-3
+        return time.perf_counter() - t0
 
-    def run_graph(i):
-        graphs[i % args.ring].launch()
-
-    def run_graph_full(i):
-        graphs_full[i % args.ring].launch()
-
-    def run_stream(i):
-        ch.enqueue(iq_ptr=ring[i % args.ring].ptr, total_ptr=totals.ptr + 8 * (i % args.ring))
-
-    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay, one stream)")
-    timed(run_graph, args.warmup)
-    dt, host_dt, gpu_dt = timed(run_graph, args.steps)
-    one_ms = dt * 1e3 / args.steps
-
-    # the same lean graphs on args.streams contexts (one HIP stream each),
-    # CPIs dealt round-robin, so one CPI's short tail kernels overlap the
-    # next CPI's stages; every context has its own chain buffers and totals
-    n_str = max(1, args.streams)
+    # the headline: one lean graph per ring slot, slot j captured on context
+    # j % n_str with its own chain buffers, so a step reads each input CPI once
     ctxs = [ctx] + [rd_amd.Ctx(ctx.lib) for _ in range(n_str - 1)]
-    chains, s_graphs, s_totals = [ch], [graphs], [totals]
+    chains, s_totals = [ch], [totals]
     for c in ctxs[1:]:
         chk = rd_amd.Chain(c, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
                            rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True)
-        chk.load(sc.iq(seed=0), taps, W)
-        tk = c.buf(8 * args.ring)
+        chk.b["taps"].upload(taps)
+        chk.b["weights"].upload(W)
         chains.append(chk)
-        s_totals.append(tk)
-        s_graphs.append([chk.graph(iq_ptr=ring[i].ptr, total_ptr=tk.ptr + 8 * i, lean=True)
-                         for i in range(args.ring)])
+        s_totals.append(c.buf(8 * R))
+    note(f"capturing {R} chain graphs over {n_str} streams")
+    slot_graphs = [chains[j % n_str].graph(iq_ptr=ring_ptrs[j], total_ptr=s_totals[j % n_str].ptr + 8 * j, lean=True)
+                   for j in range(R)]
 
-    def timed_streams(steps):
-        for c in ctxs:
-            c.sync()
-        t0 = time.perf_counter()
-        for i in range(steps):
-            s_graphs[i % n_str][(i // n_str) % args.ring].launch()
-        for c in ctxs:
-            c.sync()
-        return time.perf_counter() - t0
+    def step_streams():
+        for g in slot_graphs:
+            g.launch()
 
-    note(f"warmup {args.warmup} + {args.steps} timed steps (graph replay over {n_str} streams)")
-    timed_streams(args.warmup)
-    dt_s = timed_streams(args.steps)
-    ms_per_step = dt_s * 1e3 / args.steps
-    value = args.steps * sc.samples / dt_s / 1e6
-    # every context saw the same CPIs: their event totals must agree
-    tot0 = totals.download(np.uint64, args.ring)
-    used = min(args.ring, args.steps // n_str)  # ring slots every context filled
-    for tk in s_totals[1:]:
-        if not np.array_equal(tk.download(np.uint64, args.ring)[:used], tot0[:used]):
-            sys.exit("event totals differ between the streams' contexts")
-    for gl in s_graphs[1:]:
-        for g in gl:
-            g.close()
+    note(f"warmup {args.warmup} + {args.steps} timed steps of {R} CPIs (graph replay over {n_str} streams)")
+    timed(step_streams, args.warmup)
+    dt = timed(step_streams, args.steps)
+    ms_per_step = dt * 1e3 / args.steps
+    ms_per_cpi = ms_per_step / R
+    value = args.steps * R * sc.samples / dt / 1e6
+    slot_tot = np.array([s_totals[j % n_str].download(np.uint64, R)[j] for j in range(R)], np.uint64)
+    for g in slot_graphs:
+        g.close()
     for c in ctxs[1:]:
         c.close()
-    note("full-output graphs")
-    timed(run_graph_full, min(args.warmup, 8))
-    f_dt, f_host, f_gpu = timed(run_graph_full, args.steps)
-    note("stream-enqueue comparison")
-    timed(run_stream, min(args.warmup, 8))
-    s_dt, s_host, s_gpu = timed(run_stream, args.steps)
-    totals_ok = totals.download(np.uint64, args.ring)
-    if not np.all(totals_ok[:min(args.ring, args.steps)] > 0):
-        sys.exit("a timed CPI produced no events (summaries are always emitted)")
+    ctxs = [ctx]
 
-    # per-kernel launch durations (HIP events on the library's stream)
-    note("per-stage kernel timing")
-    stages = stage_kernels(ctx, ch, sc)
-    per = {}
-    for name, (fn, nbytes) in stages.items():
-        ms = time_stage(torch, stream, fn, args.stage_reps)
-        per[name] = dict(ms=round(ms, 5), algo_bytes=nbytes, gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
-    # the same PFB launch with complex taps (the imaginary part of every tap
-    # set to 1): the general four-product window instead of the real-tap one
-    taps_cx = ctx.put(np.asarray(sc.taps(), np.uint32) | np.uint32(1 << 16))
-    u_cx = ctx.buf(4 * sc.CH * sc.N * sc.F)
+    # one stream: lean graphs, full-output graphs, lean stream enqueue
+    note("one-stream comparisons: lean graphs, full-output graphs, stream enqueue")
+    g1 = [ch.graph(iq_ptr=ring_ptrs[j], total_ptr=totals.ptr + 8 * j, lean=True) for j in range(R)]
+    timed(lambda: [g.launch() for g in g1], 1)
+    t_graph = timed(lambda: [g.launch() for g in g1], args.steps)
+    one_tot = totals.download(np.uint64, R)
+    if not np.array_equal(one_tot, slot_tot) or not np.all(one_tot > 0):
+        sys.exit("event totals differ between the streams' contexts and the one-stream run")
+    for g in g1:
+        g.close()
+    gf = [ch.graph(iq_ptr=ring_ptrs[j], total_ptr=totals.ptr + 8 * j) for j in range(R)]
+    timed(lambda: [g.launch() for g in gf], 1)
+    t_full = timed(lambda: [g.launch() for g in gf], args.steps)
+    for g in gf:
+        g.close()
+    bufs_l = [lean_bufs(ch, p) for p in ring_ptrs]
 
-    def pfb_cx():
-        ctx.lib.check("pfb", ctx.L.rd_pfb_run(ctx.h, ch.b["iq"].ptr, taps_cx.ptr, sc.N, sc.T, sc.CH, sc.F, None,
-                                              u_cx.ptr, None))
+    def enqueue_pass(fn):
+        for j in range(R):
+            ctx.lib.check(fn, getattr(ctx.L, fn)(ctx.h, rd_amd.C.byref(ch.desc), rd_amd.C.byref(bufs_l[j]),
+                                                 totals.ptr + 8 * j))
 
-    pfb_cx_ms = time_stage(torch, stream, pfb_cx, args.stage_reps)
-    dom = max(per, key=lambda k: per[k]["ms"])
-    ach = per[dom]["gbs"]
-    prof = profiled(dom, sc.name)
+    timed(lambda: enqueue_pass("rd_chain_enqueue"), 1)
+    t_stream = timed(lambda: enqueue_pass("rd_chain_enqueue"), args.steps)
+    plain = None
+    if not args.no_plain:
+        note("plain GPU chain (no matrix cores), one CPI per launch")
+        timed(lambda: enqueue_pass("rd_chain_enqueue_plain"), 1)
+        if not np.array_equal(totals.download(np.uint64, R), one_tot):
+            sys.exit("the plain chain's event totals differ from the MFMA chain's")
+        t_plain = timed(lambda: enqueue_pass("rd_chain_enqueue_plain"), max(1, args.steps // 2))
+        plain_ms = t_plain * 1e3 / max(1, args.steps // 2) / R
+        plain = dict(ms_per_cpi=round(plain_ms, 5), msamples_s=round(sc.samples / (plain_ms * 1e-3) / 1e6, 1),
+                     impl="rd_chain_enqueue_plain per CPI on one stream over the ring: PFB, FFT, stand-alone "
+                          "alignment, VALU beamformer + power and covariance (int64 sums), CFAR",
+                     mfma_split_over_plain=round(plain_ms / ms_per_cpi, 3),
+                     mfma_over_plain_same_launch=round(t_plain / max(1, args.steps // 2) / (t_stream / args.steps),
+                                                       3))
+
+    # per-kernel launch durations over the ring's distinct CPIs
+    note("per-stage kernel timing over the ring")
+    sch = slot_chains(ctx, sc, taps, W, R)
+    per = ring_stage_times(torch, ctx, sch, sc, ring_ptrs, passes=args.stage_passes)
+    if plain is not None:
+        plain["stages"] = ring_stage_times(torch, ctx, sch, sc, ring_ptrs, plain=True, passes=1)
+    del sch
+    for k, ops in mfma_ops(sc).items():
+        per[k]["mfma_int8_tops_as_issued"] = round(ops / (per[k]["ms"] * 1e-3) / 1e12, 2)
+        per[k]["mfma_frac_of_5pops_as_issued"] = round(ops / (per[k]["ms"] * 1e-3) / INT8_PEAK_OPS, 4)
+    chain_valu = 0
+    profs = {k: profiled(k, sc.name) for k in STAGES}
+    for k, prof in profs.items():
+        if prof:
+            per[k]["valu_frac"] = valu_frac(prof["valu_insts"], per[k]["ms"])
+            per[k]["hbm_bytes_counted"] = prof["hbm_bytes"]
+            if prof["mfma_insts"]:
+                per[k]["mfma_wave_insts"] = prof["mfma_insts"]
+            chain_valu += prof["valu_insts"]
+    dom = max(KERNEL_STAGES, key=lambda k: per[k]["ms"])
     chain_bytes = algo_chain_bytes(sc, n_ev)
+    ach = chain_bytes / (per[dom]["ms"] * 1e-3) / 1e9
+    prof = profs.get(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": prof["hbm_bytes"] if prof else None,
+                "algo_bytes_per_launch": chain_bytes, "kernel_ms": per[dom]["ms"],
+                "kernel_own_bytes": per[dom]["algo_bytes"],
+                "kernel_own_frac": round(per[dom]["gbs"] / HBM_PEAK_GBS, 4),
+                "valu_frac_kernel": per[dom].get("valu_frac"),
+                "valu_frac_chain": (valu_frac(chain_valu, ms_per_cpi) if all(profs.values()) else None),
+                "chain_valu_wave_insts_per_cpi": chain_valu or None}
     out = {
-        "metric": "bit-exact Msamples/s PFB->CFAR (plain GPU, one CPI per graph launch)",
+        "metric": "bit-exact Msamples/s PFB→CFAR: MFMA-split+batched vs plain GPU vs CPU",
         "value": round(value, 2),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -705,31 +877,27 @@ def main():
         "data": "synthetic (seeded complex Gaussian noise + plane-wave targets, ring of distinct CPIs)",
         "config": {"workload": f"{sc.name}: {sc.CH} ch x {sc.N}-pt PFB/FFT x {sc.T} taps x {sc.F} frames, "
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
-                   "ring": args.ring, "streams": n_str,
-                   "impl": "one hipGraph per CPI: PFB, FFT, BF+power MFMA (writes the aligned copy), cov MFMA, "
-                           f"CFAR rows/scan/gather; CPIs dealt round-robin to {n_str} contexts (one HIP stream "
-                           "each)",
+                   "cpis_per_step": R, "ms_per_cpi": round(ms_per_cpi, 5), "streams": n_str,
+                   "impl": "one lean hipGraph per CPI (PFB, FFT, BF+power split-int8 MFMA writing the aligned "
+                           "copy, covariance split-int8 MFMA, CFAR rows/scan/gather); a step is one pass over "
+                           f"{R} distinct input CPIs dealt round-robin to {n_str} contexts (one HIP stream each)",
                    "outputs": "events + covariance timed; every stage output verified bit-exact on a separate "
                               "CPI each run (timing.graph_full_outputs also writes them all)"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": prof["hbm_bytes"] if prof else None,
-                     "valu_frac": valu_frac(prof["valu_insts"], per[dom]["ms"]) if prof else None},
-        "chain": {"algo_bytes_per_cpi": chain_bytes, "achieved_gbs": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-                  "frac": round(chain_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "timing": {"graph_streams": {"streams": n_str, "wall_ms": round(ms_per_step, 4)},
-                   "pfb_complex_taps_ms": round(pfb_cx_ms, 5),
-                   "graph": {"wall_ms": round(one_ms, 4), "host_ms": round(host_dt * 1e3 / args.steps, 4),
-                             "gpu_ms": round(gpu_dt * 1e3 / args.steps, 4),
-                             "msamples_s": round(args.steps * sc.samples / dt / 1e6, 2)},
-                   "graph_full_outputs": {"wall_ms": round(f_dt * 1e3 / args.steps, 4),
-                                          "gpu_ms": round(f_gpu * 1e3 / args.steps, 4),
-                                          "msamples_s": round(args.steps * sc.samples / f_dt / 1e6, 2)},
-                   "stream": {"wall_ms": round(s_dt * 1e3 / args.steps, 4), "host_ms": round(s_host * 1e3 / args.steps, 4),
-                              "gpu_ms": round(s_gpu * 1e3 / args.steps, 4),
-                              "msamples_s": round(args.steps * sc.samples / s_dt / 1e6, 2)}},
+        "roofline": roofline,
+        "chain": {"algo_bytes_per_cpi": chain_bytes,
+                  "achieved_gbs": round(chain_bytes / (ms_per_cpi * 1e-3) / 1e9, 1),
+                  "frac": round(chain_bytes / (ms_per_cpi * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "plain_gpu": plain,
+        "timing": {"graph_one_stream": {"ms_per_cpi": round(t_graph * 1e3 / args.steps / R, 5),
+                                        "msamples_s": round(args.steps * R * sc.samples / t_graph / 1e6, 1)},
+                   "graph_full_outputs": {"ms_per_cpi": round(t_full * 1e3 / args.steps / R, 5),
+                                          "msamples_s": round(args.steps * R * sc.samples / t_full / 1e6, 1)},
+                   "stream_enqueue": {"ms_per_cpi": round(t_stream * 1e3 / args.steps / R, 5),
+                                      "msamples_s": round(args.steps * R * sc.samples / t_stream / 1e6, 1)}},
         "stages": per,
         "cpu_baseline": cpu,
         "mismatches": mismatches,
+        "covariance_trace_fp64": trace,
         "events_per_cpi": n_ev,
     }
     if args.calib_copy_mb:
@@ -741,10 +909,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out["calib_copy_bytes"] = nbytes
+        del src, dst
+    del ring
     if args.batch_cpis:
-        for g in graphs + graphs_full:
-            g.close()
-        graphs, graphs_full = [], []
         out["batch"] = batched_leg(torch, ctx, args.batch_cpis, args.batch_reps, not args.no_autotune,
                                    args.batch_oracle)
     if args.shadow_cpis:
@@ -752,10 +919,22 @@ def main():
         out["shadow"] = shadow_leg(torch, ctx, args.shadow_cpis, args.shadow_steps, 2)
     if args.large_steps and sc.name != "large":
         torch.cuda.empty_cache()
-        out["large"] = large_leg(torch, ctx, args.large_steps, bool(args.large_verify), max(2, args.stage_reps // 10))
+        out["large"] = large_leg(torch, ctx, args.large_steps, bool(args.large_verify))
+    out["three_way"] = {
+        "medium_msamples_s": {"mfma_split": out["value"], "plain_gpu": plain["msamples_s"] if plain else None,
+                              "cpu": cpu["value"] if cpu else None},
+        "large_msamples_s": ({"mfma_split": out["large"]["msamples_s"],
+                              "plain_gpu": out["large"]["plain_gpu"]["msamples_s"],
+                              "cpu_oracle": out["large"].get("oracle_cpu", {}).get("msamples_s")}
+                             if "large" in out else None),
+        "large_mfma_split_over_plain": out["large"]["mfma_split_over_plain"] if "large" in out else None,
+        "batch_msamples_s": ({"mfma_split_batched": out["batch"]["batched"]["msamples_s"],
+                              "mfma_split_per_cpi": out["batch"]["mfma_per_cpi_async"]["msamples_s"],
+                              "plain_gpu_per_cpi": out["batch"]["plain_gpu"]["msamples_s"]}
+                             if "batch" in out else None),
+        "batch_mfma_batched_over_plain_per_cpi": out["batch"]["speedup_vs_plain_gpu"] if "batch" in out else None,
+    }
     print(json.dumps(out), flush=True)
-    for g in graphs + graphs_full:
-        g.close()
     ctx.close()
 
 

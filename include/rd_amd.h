@@ -43,6 +43,47 @@ typedef enum {
 
 typedef struct rd_ctx rd_ctx;
 
+/* ---- Integrator control (covar::Integrator, model/cpp/covariance/covar_model.hpp:159-262)
+ * covar::Config: window_len (0 counts as 1), mode 0 block / 1 exponential,
+ * exp_k the exponential shift, enable. */
+typedef struct {
+  uint32_t window_len, mode, exp_k, enable;
+} rd_integ_cfg;
+/* The control inputs of one Integrator::step edge (cfg ports, valid_in, flush,
+ * sat_clear), shared by every lane of a call. */
+typedef struct {
+  rd_integ_cfg cfg;
+  uint32_t valid, flush, sat_clear;
+} rd_integ_step;
+/* covar::Result without the value (covar_model.hpp:173-184): the window a step
+ * emitted, the same for every lane of a call. */
+typedef struct {
+  uint32_t step, window_id, sample_count, flushed, truncated;
+} rd_integ_window;
+/* The Integrator's registers that do not depend on the data (the active
+ * configuration, the sample count, the window id): host memory, carried from
+ * call to call like the model object. */
+typedef struct {
+  rd_integ_cfg active;
+  uint32_t count, window_id;
+} rd_integ_ctl;
+/* One lane's data registers (device memory, carried from call to call): the
+ * accumulator, its sticky saturation flags {sat_pos<<1 | sat_neg} and the
+ * saturating count of flagged steps (Integrator::acc, sat_flags, sat_count). */
+typedef struct {
+  int64_t acc;
+  uint32_t flags;
+  uint32_t sat_count;
+} rd_integ_lane;
+
+/* One cfar::decide result (model/cpp/cfar/cfar_model.hpp:223-234). */
+typedef struct {
+  uint64_t cut_power, sum_lead, sum_lag, sel_sum;
+  uint32_t sel_n;
+  uint8_t window_ok, evaluable, detected, suppressed, take_lead;
+  uint8_t pad[7];
+} rd_cfar_decision;
+
 /* CFAR configuration as written to the register plane
  * (model/cpp/cfar/cfar_model.hpp:112-128). mode: 0 CA, 1 GO.
  * out_mode: 0 events (DETECT + SUMMARY), 1 dense (one event per bin + SUMMARY).
@@ -186,6 +227,55 @@ rd_status rd_cov_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t 
 rd_status rd_cov_integrate(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                            uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R, uint8_t* sat_flags);
 
+/* ---- the Integrator over a control stream --------------------------------
+ * Many covar::Integrator(acc_w) instances ("lanes") stepped together:
+ * Integrator::reset(cfg) is rd_integ_reset(cfg, &ctl) plus zeroed lanes, and
+ * n_steps calls of Integrator::step(steps[s].cfg, steps[s].valid, data,
+ * steps[s].flush, steps[s].sat_clear, &result) (covar_model.hpp:191-262) are
+ * one call, every lane with its own data and the same control inputs.
+ * Each step adds (block) or moves by floor((x - acc) / 2^exp_k) (exponential),
+ * saturates to acc_w bits with sticky flags, closes a window when its sample
+ * count reaches window_len, emits a result on a close or a flush of a
+ * non-empty window, clears the sticky flags on sat_clear or flush, and
+ * latches a new configuration only at a window boundary. The emitted windows
+ * (the same for every lane) are written to windows[0 .. min(n, cap)) on the
+ * host and their values to results [cap][n_lanes] on the device; *n_windows
+ * receives n (RD_E_CAPACITY when n > cap, values past cap not written). ctl
+ * and lanes are updated in place, so consecutive calls continue the stream.
+ * Synchronous. acc_w <= 64 (below 2 every value saturates to 0, as
+ * fxp::max_of/min_of do), mode 0 or 1. */
+rd_status rd_integ_reset(const rd_integ_cfg* cfg, rd_integ_ctl* ctl);
+/* data [n_steps][n_lanes] int64 (device), lanes [n_lanes] (device). */
+rd_status rd_integ_run(rd_ctx* ctx, uint32_t acc_w, rd_integ_ctl* ctl, const rd_integ_step* steps, uint32_t n_steps,
+                       const int64_t* data, uint64_t n_lanes, rd_integ_lane* lanes, int64_t* results,
+                       rd_integ_window* windows, uint32_t windows_cap, uint32_t* n_windows);
+/* The lanes of a covariance (covar::Engine's accumulators,
+ * covar_model.hpp:318-416): lane ((k * pairs + p) * 2 + c) integrates
+ * component c (0 re, 1 im) of covar::cross_power(X_i, X_j) of bin k, pair
+ * p = (i <= j) row-major; step m is frame m of Xa [n_frames][n_bins][n_chan].
+ * lanes [n_bins][pairs][2], results [windows_cap][n_bins][pairs][2] (device).
+ * With steps all valid, no flush or clear and window_len = n_frames, the
+ * one window's result is rd_cov_run's R (at acc_w = 64). */
+rd_status rd_cov_integrate_windows(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,
+                                   uint32_t n_bins, uint32_t acc_w, rd_integ_ctl* ctl, const rd_integ_step* steps,
+                                   rd_integ_lane* lanes, int64_t* results, rd_integ_window* windows,
+                                   uint32_t windows_cap, uint32_t* n_windows);
+
+/* pfb::LaneModel::accumulate (model/cpp/pfb/pfb_model.hpp:135-150) for every
+ * output of rd_pfb_run: the exact int64 sums of cmul_q15_re_raw and
+ * cmul_q15_im_raw over the taps, before rounding. acc [n_chan][n_beats]
+ * [phases][2]; hist as rd_pfb_run's (read only, nullable). */
+rd_status rd_pfb_acc_run(rd_ctx* ctx, const uint32_t* x, const uint32_t* coeff, uint32_t phases, uint32_t taps,
+                         uint32_t n_chan, uint32_t n_beats, const uint32_t* hist, int64_t* acc);
+
+/* cfar::decide(bins, i, cfg) (model/cpp/cfar/cfar_model.hpp:243-285) for every
+ * bin i of every row: cells [n_rows][n_bins] uint64 magnitudes, cfg used as
+ * given (no clamp: cfar::run_frame clamps before it calls decide), out
+ * [n_rows][n_bins], including the partial window sums of bins whose window
+ * leaves the row. */
+rd_status rd_cfar_decide(rd_ctx* ctx, const uint64_t* cells, uint32_t n_bins, uint32_t n_rows,
+                         const rd_cfar_cfg* cfg, rd_cfar_decision* out);
+
 /* CA/GO-CFAR over rows of n_bins cells. Replaces cfar::run_frame(bins, cfg,
  * max_guard, max_ref, frame_id) per row (model/cpp/cfar/cfar_model.hpp:295-340),
  * with the events packed by the 176-bit layout of cfar_model.hpp:75-84 into three
@@ -199,6 +289,15 @@ rd_status rd_cfar_run(rd_ctx* ctx, const void* cells, uint32_t cell_bytes, uint3
                       const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
                       uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
                       uint64_t* n_events_host);
+/* rd_cfar_run without the final synchronisation: the event total is written
+ * to device memory at *n_events_dev (uint64, required; 0 when n_rows is 0)
+ * and the call returns once the work is queued. The capacity is not checked
+ * on the host: a total above events_cap means the events past the cap were
+ * not written. */
+rd_status rd_cfar_enqueue(rd_ctx* ctx, const void* cells, uint32_t cell_bytes, uint32_t n_bins, uint32_t n_rows,
+                          const rd_cfar_cfg* cfg, uint32_t max_guard, uint32_t max_ref, const uint32_t* frame_ids,
+                          uint32_t frame_mod, uint64_t* events, uint64_t events_cap, uint32_t* row_count,
+                          uint64_t* n_events_dev);
 
 /* The composed chain for one CPI: PFB -> FFT -> alignment -> beamformer+power ->
  * covariance -> CFAR -> event packing, every stage as above, all on the
@@ -228,13 +327,37 @@ rd_status rd_chain_enqueue(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bu
 rd_status rd_chain_enqueue_static(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs,
                                   uint64_t* n_events_dev);
 
+/* ---- plain (no matrix cores) contractions --------------------------------
+ * The "plain GPU" comparison point of the chain's measurement. The same values,
+ * bit for bit, as rd_bf_run / rd_cov_run, computed without MFMA: each
+ * int16 x int16 product in a 32-bit multiply, summed in int64, rounded once,
+ * tiles staged in LDS. They replace the same reference calls: bf::dot per beam
+ * and column plus covar::power (model/cpp/beamformer/beamformer_model.hpp:162-179,
+ * model/cpp/covariance/covar_model.hpp:99-102), and covar::cross_power summed
+ * by an acc_w = 64 block-mode Integrator (covar_model.hpp:137-151, 186-262).
+ * rd_bf_plain_run: Xa [n_cols][n_ant] -> y/flags/power [n_beams][n_cols] (each
+ * nullable). rd_cov_plain_run: Xa [n_frames][n_bins][n_chan] -> R as rd_cov_run.
+ * rd_chain_enqueue_plain: rd_chain_enqueue's chain with the stand-alone
+ * alignment (rd_align_run) and these two kernels in place of the fused MFMA
+ * beamformer and the MFMA covariance; same arguments and results. */
+rd_status rd_bf_plain_run(rd_ctx* ctx, const uint32_t* Xa, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                          uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power);
+rd_status rd_cov_plain_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
+                           int64_t* R);
+rd_status rd_chain_enqueue_plain(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs,
+                                 uint64_t* n_events_dev);
+
 /* ---- graphs --------------------------------------------------------------- */
 typedef struct rd_graph rd_graph;
 
 /* Records the launches of rd_chain_enqueue(ctx, desc, bufs, n_events_dev) into an
  * executable HIP graph (buffers and geometry are fixed at capture). Each
  * rd_graph_launch then replays the whole chain with one launch on the context's
- * stream. No graph node is a memset. */
+ * stream. No graph node is a memset.
+ * Lifetime: the graph holds the raw device pointers of bufs (and
+ * n_events_dev): every one of those buffers must stay allocated until
+ * rd_graph_destroy returns. Freeing one while the graph lives and launching
+ * it again writes to freed memory. */
 rd_status rd_chain_graph(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_dev,
                          rd_graph** out);
 rd_status rd_graph_launch(rd_graph* graph);
@@ -343,7 +466,10 @@ rd_status rd_write_iq_ascii(const char* path, const uint32_t* iq, uint32_t n_ant
  * job tables and the graph. With RD_BATCH_AUTOTUNE it first times candidate
  * tile and grid shapes per class on the given buffers (running the chain's
  * stages on them, so inputs must be loaded) and keeps the fastest; otherwise
- * it uses fixed heuristics. The buffers of different CPIs must not overlap. */
+ * it uses fixed heuristics. The buffers of different CPIs must not overlap.
+ * Lifetime: the batch's graph and job tables hold the raw device pointers of
+ * every CPI's buffers: they must stay allocated until rd_batch_destroy
+ * returns (refill them in place between launches instead). */
 typedef struct rd_batch rd_batch;
 
 #define RD_BATCH_AUTOTUNE 1u

@@ -210,6 +210,54 @@ class PfbModel {
   DevVec<uint32_t> hist_;
 };
 
+// pfb::LaneModel (pfb_model.hpp:117-183): one branch. The history stays on the
+// host (most recent sample first, taps - 1 entries) and travels with each call.
+class LaneModel {
+ public:
+  LaneModel() = default;
+  LaneModel(unsigned taps, const Complex* coeff)
+      : taps_(taps), coeff_(coeff, coeff + taps), history_(taps ? taps - 1 : 0, Complex{}) {}
+  void reset() { history_.assign(history_.size(), Complex{}); }
+  void set_coeff(const Complex* coeff) { coeff_.assign(coeff, coeff + taps_); }
+  unsigned taps() const { return taps_; }
+  const std::vector<Complex>& coeff() const { return coeff_; }
+
+  // The exact accumulators of one sample, before rounding (rd_pfb_acc_run).
+  void accumulate(Complex x, int64_t* acc_re, int64_t* acc_im) const {
+    const uint32_t xp = x.packed();
+    const std::vector<uint32_t> c = pack_all(coeff_), h = pack_all(history_);
+    DevVec<uint32_t> dx(&xp, 1), dc(c.data(), c.size()), dh(h.data(), h.size());
+    DevVec<int64_t> da(2);
+    check(rd_pfb_acc_run(Device::ctx(), dx.ptr(), dc.ptr(), 1, taps_, 1, 1, h.empty() ? nullptr : dh.ptr(), da.ptr()),
+          "rd_pfb_acc_run");
+    const std::vector<int64_t> a = da.get(2);
+    *acc_re = a[0];
+    *acc_im = a[1];
+  }
+
+  // One sample through the branch (rd_pfb_run with the history imported and
+  // exported); the history advances after the evaluation.
+  Complex step(Complex x, Flags* f_re = nullptr, Flags* f_im = nullptr) {
+    const uint32_t xp = x.packed();
+    const std::vector<uint32_t> c = pack_all(coeff_), h = pack_all(history_);
+    DevVec<uint32_t> dx(&xp, 1), dc(c.data(), c.size()), dh(h.data(), h.size()), dy(1);
+    DevVec<uint8_t> df(1);
+    check(rd_pfb_run(Device::ctx(), dx.ptr(), dc.ptr(), 1, taps_, 1, 1, h.empty() ? nullptr : dh.ptr(), dy.ptr(),
+                     df.ptr()),
+          "rd_pfb_run");
+    if (!h.empty()) history_ = unpack_all(dh.get(h.size()));
+    const uint8_t f = df.get(1)[0];
+    if (f_re) *f_re = Flags::from_packed(f & 3u);
+    if (f_im) *f_im = Flags::from_packed((f >> 2) & 3u);
+    return Complex::from_packed(dy.get(1)[0]);
+  }
+
+ private:
+  unsigned taps_ = 0;
+  std::vector<Complex> coeff_;
+  std::vector<Complex> history_;  // history_[k-1] == x[n-k]
+};
+
 }  // namespace pfb
 
 // ---------------------------------------------------------------------------
@@ -392,6 +440,108 @@ inline std::vector<Cross> covariance(const std::vector<Complex>& xa, unsigned n_
   return out;
 }
 
+// covar::Integrator (covar_model.hpp:157-288) over rd_integ_run: the control
+// registers stay on the host, the accumulator, sticky flags and event count
+// on the device. step() is one clock edge (one launch); run() takes a whole
+// control stream with its data in one launch.
+enum class Mode : unsigned { kBlock = 0, kExp = 1 };
+
+struct Config {
+  unsigned window_len = 1;
+  Mode mode = Mode::kBlock;
+  unsigned exp_k = 0;
+  bool enable = true;
+  bool operator==(const Config& o) const {
+    return window_len == o.window_len && mode == o.mode && exp_k == o.exp_k && enable == o.enable;
+  }
+};
+
+struct Result {
+  int64_t acc = 0;
+  unsigned window_id = 0;
+  unsigned sample_count = 0;
+  bool flushed = false;
+  bool truncated = false;
+  bool operator==(const Result& o) const {
+    return acc == o.acc && window_id == o.window_id && sample_count == o.sample_count && flushed == o.flushed &&
+           truncated == o.truncated;
+  }
+  bool operator!=(const Result& o) const { return !(*this == o); }
+};
+
+struct Step {
+  Config cfg;
+  bool valid = true;
+  int64_t data = 0;
+  bool flush = false;
+  bool sat_clear = false;
+};
+
+class Integrator {
+ public:
+  Integrator(unsigned acc_w, unsigned window_w) : acc_w_(acc_w), window_w_(window_w), lane_(1) { reset(Config{}); }
+
+  void reset(const Config& cfg) {
+    const rd_integ_cfg c = to_c(cfg);
+    check(rd_integ_reset(&c, &ctl_), "rd_integ_reset");
+    const rd_integ_lane z{0, 0, 0};
+    lane_.put(&z, 1);
+  }
+
+  bool step(const Config& cfg, bool valid_in, int64_t data, bool flush, bool sat_clear, Result* out) {
+    std::vector<std::pair<size_t, Result>> r = run({Step{cfg, valid_in, data, flush, sat_clear}});
+    if (!r.empty() && out) *out = r[0].second;
+    return !r.empty();
+  }
+
+  // Every step of `steps` in one launch; returns (step index, result) of the
+  // windows they emit, in order.
+  std::vector<std::pair<size_t, Result>> run(const std::vector<Step>& steps) {
+    const uint32_t n = (uint32_t)steps.size();
+    std::vector<rd_integ_step> cs(n ? n : 1);
+    std::vector<int64_t> d(n ? n : 1);
+    for (uint32_t s = 0; s < n; ++s) {
+      cs[s] = rd_integ_step{to_c(steps[s].cfg), steps[s].valid, steps[s].flush, steps[s].sat_clear};
+      d[s] = steps[s].data;
+    }
+    const uint32_t cap = n ? n : 1;
+    DevVec<int64_t> dd(d.data(), d.size()), dr(cap);
+    std::vector<rd_integ_window> w(cap);
+    uint32_t nw = 0;
+    check(rd_integ_run(Device::ctx(), acc_w_, &ctl_, cs.data(), n, dd.ptr(), 1, lane_.ptr(), dr.ptr(), w.data(),
+                       cap, &nw),
+          "rd_integ_run");
+    const std::vector<int64_t> v = dr.get(nw);
+    std::vector<std::pair<size_t, Result>> out(nw);
+    for (uint32_t k = 0; k < nw; ++k)
+      out[k] = {w[k].step, Result{v[k], w[k].window_id, w[k].sample_count, w[k].flushed != 0, w[k].truncated != 0}};
+    return out;
+  }
+
+  int64_t acc() const { return lane().acc; }
+  unsigned count() const { return ctl_.count; }
+  unsigned window_id() const { return ctl_.window_id; }
+  Config active() const {
+    return Config{ctl_.active.window_len, ctl_.active.mode ? Mode::kExp : Mode::kBlock, ctl_.active.exp_k,
+                  ctl_.active.enable != 0};
+  }
+  unsigned effective_len() const { return ctl_.active.window_len == 0 ? 1u : ctl_.active.window_len; }
+  Flags sat_flags() const { return Flags::from_packed(lane().flags); }
+  bool sat_sticky() const { return sat_flags().any(); }
+  uint32_t sat_count() const { return lane().sat_count; }
+  unsigned acc_w() const { return acc_w_; }
+  unsigned window_w() const { return window_w_; }
+
+ private:
+  static rd_integ_cfg to_c(const Config& c) {
+    return rd_integ_cfg{c.window_len, (unsigned)c.mode, c.exp_k, c.enable ? 1u : 0u};
+  }
+  rd_integ_lane lane() const { return lane_.get(1)[0]; }
+  unsigned acc_w_, window_w_;
+  rd_integ_ctl ctl_{};
+  DevVec<rd_integ_lane> lane_;
+};
+
 }  // namespace covar
 
 // ---------------------------------------------------------------------------
@@ -441,6 +591,37 @@ inline Event unpack_event(const uint64_t* w) {
   e.eof = field(177, 1) != 0;
   return e;
 }
+
+// cfar::Decision / cfar::decide (cfar_model.hpp:223-285): cfg as given, no clamp.
+struct Decision {
+  bool window_ok = false, evaluable = false, detected = false, suppressed = false;
+  uint64_t cut_power = 0, sum_lead = 0, sum_lag = 0, sel_sum = 0;
+  unsigned sel_n = 0;
+  bool take_lead = false;
+};
+
+inline rd_cfar_cfg to_c(const Config& cfg) {
+  return rd_cfar_cfg{cfg.enable ? 1u : 0u, (unsigned)cfg.mode, (unsigned)cfg.out_mode, cfg.guard_lead,
+                     cfg.guard_lag, cfg.ref_lead, cfg.ref_lag, cfg.alpha};
+}
+
+// Every bin's decision in one launch.
+inline std::vector<Decision> decide_all(const std::vector<uint64_t>& bins, const Config& cfg) {
+  const unsigned n = (unsigned)bins.size();
+  const rd_cfar_cfg c = to_c(cfg);
+  DevVec<uint64_t> dc(bins.data(), n ? n : 1);
+  DevVec<rd_cfar_decision> dd(n ? n : 1);
+  check(rd_cfar_decide(Device::ctx(), dc.ptr(), n, 1, &c, dd.ptr()), "rd_cfar_decide");
+  const std::vector<rd_cfar_decision> r = dd.get(n);
+  std::vector<Decision> out(n);
+  for (unsigned i = 0; i < n; ++i)
+    out[i] = Decision{r[i].window_ok != 0, r[i].evaluable != 0, r[i].detected != 0, r[i].suppressed != 0,
+                      r[i].cut_power,      r[i].sum_lead,       r[i].sum_lag,       r[i].sel_sum,
+                      r[i].sel_n,          r[i].take_lead != 0};
+  return out;
+}
+
+inline Decision decide(const std::vector<uint64_t>& bins, size_t i, const Config& cfg) { return decide_all(bins, cfg)[i]; }
 
 // cfar::run_frame (cfar_model.hpp:295-340)
 inline std::vector<Event> run_frame(const std::vector<uint64_t>& bins, const Config& cfg, unsigned max_guard,

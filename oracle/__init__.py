@@ -194,6 +194,69 @@ class Oracle:
                                  C.c_uint(acc_w), _ptr(R), _ptr(fl), C.c_uint(threads))
         return R, fl
 
+    def _integ_out(self, n_lanes, cap):
+        return dict(acc=np.zeros(n_lanes, np.int64), flags=np.zeros(n_lanes, np.uint32),
+                    count=np.zeros(n_lanes, np.uint32), results=np.zeros(max(cap, 1) * n_lanes, np.int64),
+                    windows=np.zeros((max(cap, 1), 5), np.uint32), ctl=np.zeros(6, np.uint32))
+
+    def integ_run(self, acc_w, cfg4, steps7, data, cap=64):
+        """covar::Integrator lanes from reset(cfg4) over a control stream
+        steps7 [n_steps][7] = (window_len, mode, exp_k, enable, valid, flush,
+        sat_clear) with data [n_steps][n_lanes] int64. Returns dict(acc, flags,
+        count per lane, results [n_windows][n_lanes], windows [n_windows][5] =
+        (step, window_id, sample_count, flushed, truncated), ctl (active cfg
+        (4), count, window_id), n_windows)."""
+        steps7 = np.ascontiguousarray(steps7, np.uint32).reshape(-1, 7)
+        data = np.ascontiguousarray(data, np.int64).reshape(steps7.shape[0], -1)
+        n_lanes = data.shape[1]
+        o = self._integ_out(n_lanes, cap)
+        f = self._f("integ_run")
+        f.restype = C.c_uint32
+        nw = f(C.c_uint(acc_w), _ptr(np.ascontiguousarray(cfg4, np.uint32)), _ptr(steps7),
+               C.c_uint32(steps7.shape[0]), _ptr(data), C.c_size_t(n_lanes), _ptr(o["acc"]), _ptr(o["flags"]),
+               _ptr(o["count"]), _ptr(o["results"]), _ptr(o["windows"]), C.c_uint32(cap), _ptr(o["ctl"]))
+        k = min(nw, cap)
+        o.update(n_windows=nw, results=o["results"][: k * n_lanes].reshape(k, n_lanes), windows=o["windows"][:k])
+        return o
+
+    def cov_integrate_windows(self, Xa, CH, F, N, acc_w, cfg4, steps7, cap=64, threads=1):
+        """The Integrator lanes of a covariance: lane ((k*pairs + p)*2 + c) is
+        component c of cross_power(X_i, X_j) of bin k, frame m = step m. Same
+        return layout as integ_run (results [n_windows][N*pairs*2])."""
+        Xa = np.ascontiguousarray(Xa, np.uint32)
+        steps7 = np.ascontiguousarray(steps7, np.uint32).reshape(-1, 7)
+        assert steps7.shape[0] == F
+        n_lanes = N * CH * (CH + 1)
+        o = self._integ_out(n_lanes, cap)
+        f = self._f("cov_integrate_windows")
+        f.restype = C.c_uint32
+        nw = f(_ptr(Xa), C.c_uint(CH), C.c_uint(F), C.c_uint(N), C.c_uint(acc_w),
+               _ptr(np.ascontiguousarray(cfg4, np.uint32)), _ptr(steps7), _ptr(o["acc"]), _ptr(o["flags"]),
+               _ptr(o["count"]), _ptr(o["results"]), _ptr(o["windows"]), C.c_uint32(cap), _ptr(o["ctl"]),
+               C.c_uint(threads))
+        k = min(nw, cap)
+        o.update(n_windows=nw, results=o["results"][: k * n_lanes].reshape(k, n_lanes), windows=o["windows"][:k])
+        return o
+
+    def pfb_acc(self, x, coeff, P, T, n_chan, n_beats, hist=None):
+        """pfb::LaneModel::accumulate for every output: acc [n_chan][n_beats][P][2] int64."""
+        acc = np.zeros(n_chan * n_beats * P * 2, np.int64)
+        h = None if hist is None else np.ascontiguousarray(hist, np.uint32)
+        self._f("pfb_acc")(_ptr(np.ascontiguousarray(x, np.uint32)), _ptr(np.ascontiguousarray(coeff, np.uint32)),
+                           C.c_uint(P), C.c_uint(T), C.c_uint(n_chan), C.c_uint(n_beats), _ptr(h), _ptr(acc))
+        return acc
+
+    def cfar_decide(self, bins, n, cfg8):
+        """cfar::decide for every bin of every row (configuration as given, no
+        clamp): [rows*n][8] u64 = (cut_power, sum_lead, sum_lag, sel_sum, sel_n,
+        window_ok, evaluable | detected<<1 | suppressed<<2, take_lead)."""
+        bins = np.ascontiguousarray(bins, np.uint64)
+        rows = bins.size // n
+        dec = np.zeros(rows * n * 8, np.uint64)
+        self._f("cfar_decide")(_ptr(bins), C.c_uint(n), C.c_uint(rows), _ptr(np.ascontiguousarray(cfg8, np.uint32)),
+                               _ptr(dec))
+        return dec.reshape(rows * n, 8)
+
     def event_flits(self, events, packet_w, dest=0, src=0, vc=0, seq0=0):
         """Detection packets of packed events [n][3] -> flits [n][L][fw] u32."""
         ev = np.ascontiguousarray(events, np.uint64).reshape(-1, 3)

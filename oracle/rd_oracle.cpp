@@ -892,3 +892,219 @@ int64_t rdo_chain_run(const uint32_t* iq, const uint32_t* taps, const uint32_t* 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// covar::Integrator over a control stream — covar_model.hpp:186-262. Every
+// lane is one Integrator(acc_w) started by reset(cfg0) and stepped with the
+// same (cfg, valid, flush, sat_clear) per step and its own data. steps7 is
+// [n_steps][7] = {window_len, mode, exp_k, enable, valid, flush, sat_clear}.
+// Outputs per lane: the live acc, the sticky flags {pos<<1|neg}, the
+// saturating flag-event count; per emitted window k (k < cap): results
+// [k][n_lanes] and win5[k] = {step, window_id, sample_count, flushed,
+// truncated}; ctl6 = the live {active cfg (4), count, window_id}. Returns the
+// number of windows emitted (every lane emits the same ones).
+// ---------------------------------------------------------------------------
+namespace {
+struct ICfg {
+  u32 window_len, mode, exp_k, enable;
+};
+struct ILane {
+  unsigned acc_w;
+  ICfg active;
+  i64 acc = 0;
+  u32 count = 0, window_id = 0, sticky = 0, events = 0;
+  // one clock edge; returns true and fills the window fields when it emits
+  bool step(const ICfg& cfg, bool valid, i64 x, bool flush, bool sat_clear, i64* out, u32* w5, u32 s) {
+    const u32 wl = active.window_len == 0 ? 1u : active.window_len;
+    const bool accept = valid && active.enable;
+    i64 raw = acc;
+    if (accept) raw = active.mode == 1 ? wadd(acc, floor_shift(wsub(x, acc), active.exp_k)) : wadd(acc, x);
+    const unsigned f = accept ? flags_of(raw, acc_w) : 0u;
+    const i64 next = clamp_w(raw, acc_w);
+    const u32 cnext = accept ? count + 1u : count;
+    const bool close = accept && cnext >= wl;
+    const bool emit = close || (flush && !close && cnext != 0u);
+    if (emit) {
+      *out = next;
+      w5[0] = s;
+      w5[1] = window_id;
+      w5[2] = cnext;
+      w5[3] = flush ? 1u : 0u;
+      w5[4] = cnext != wl ? 1u : 0u;
+    }
+    if (sat_clear || flush) {
+      sticky = 0;
+      events = 0;
+    } else if (accept) {
+      sticky |= f;
+      if (f && events != 0xFFFFFFFFu) ++events;
+    }
+    const bool boundary = close || flush || (count == 0u && !accept);
+    if (flush) {
+      acc = 0;
+      count = 0;
+      window_id = 0;
+    } else if (close) {
+      acc = active.mode == 1 ? next : 0;
+      count = 0;
+      window_id = (window_id + 1u) & 0xFFFFu;
+    } else {
+      acc = next;
+      count = cnext;
+    }
+    if (boundary) active = cfg;
+    return emit;
+  }
+};
+
+// Runs one lane over the steps with its data source get(s).
+template <class G>
+u32 run_lane(ILane& L, const u32* steps7, u32 n_steps, G get, int64_t* results, size_t n_lanes, size_t lane,
+             u32* win5, u32 cap) {
+  u32 nw = 0;
+  for (u32 s = 0; s < n_steps; ++s) {
+    const u32* t = steps7 + 7 * (size_t)s;
+    i64 out = 0;
+    u32 w5[5];
+    if (L.step(ICfg{t[0], t[1], t[2], t[3]}, t[4] != 0, get(s), t[5] != 0, t[6] != 0, &out, w5, s)) {
+      if (nw < cap) {
+        if (results) results[(size_t)nw * n_lanes + lane] = out;
+        if (win5 && lane == 0) std::memcpy(win5 + 5 * (size_t)nw, w5, sizeof(w5));
+      }
+      ++nw;
+    }
+  }
+  return nw;
+}
+}  // namespace
+
+extern "C" {
+
+uint32_t rdo_integ_run(unsigned acc_w, const uint32_t* cfg4, const uint32_t* steps7, uint32_t n_steps,
+                       const int64_t* data, size_t n_lanes, int64_t* lane_acc, uint32_t* lane_flags,
+                       uint32_t* lane_count, int64_t* results, uint32_t* win5, uint32_t cap, uint32_t* ctl6) {
+  u32 nw = 0;
+  for (size_t l = 0; l < n_lanes; ++l) {
+    ILane L{acc_w, ICfg{cfg4[0], cfg4[1], cfg4[2], cfg4[3]}};
+    nw = run_lane(L, steps7, n_steps, [&](u32 s) { return (i64)data[(size_t)s * n_lanes + l]; }, results, n_lanes,
+                  l, win5, cap);
+    lane_acc[l] = L.acc;
+    lane_flags[l] = L.sticky;
+    lane_count[l] = L.events;
+    if (l == 0 && ctl6) {
+      const u32 c[6] = {L.active.window_len, L.active.mode, L.active.exp_k, L.active.enable, L.count, L.window_id};
+      std::memcpy(ctl6, c, sizeof(c));
+    }
+  }
+  return nw;
+}
+
+// The same with the cross powers of a covariance as the lanes: lane
+// ((k * pairs + p) * 2 + c) integrates component c of cross_power(X_i, X_j)
+// of bin k, pair p = (i <= j) row-major, frame m = step m.
+uint32_t rdo_cov_integrate_windows(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w,
+                                   const uint32_t* cfg4, const uint32_t* steps7, int64_t* lane_acc,
+                                   uint32_t* lane_flags, uint32_t* lane_count, int64_t* results, uint32_t* win5,
+                                   uint32_t cap, uint32_t* ctl6, unsigned threads) {
+  const size_t npairs = (size_t)CH * (CH + 1) / 2, lanes = 2 * (size_t)N * npairs;
+  std::vector<u32> nws(N, 0);
+  parallel_for(N, threads, [&](size_t k) {
+    size_t pi = 0;
+    for (unsigned i = 0; i < CH; ++i)
+      for (unsigned j = i; j < CH; ++j, ++pi)
+        for (unsigned c = 0; c < 2; ++c) {
+          const size_t l = (k * npairs + pi) * 2 + c;
+          ILane L{acc_w, ICfg{cfg4[0], cfg4[1], cfg4[2], cfg4[3]}};
+          auto get = [&](u32 m) {
+            i64 re, im;
+            const size_t base = ((size_t)m * N + k) * CH;
+            cross_swap(unpack(Xa[base + i]), unpack(Xa[base + j]), &re, &im);
+            return c ? im : re;
+          };
+          nws[k] = run_lane(L, steps7, F, get, results, lanes, l, l == 0 ? win5 : nullptr, cap);
+          lane_acc[l] = L.acc;
+          lane_flags[l] = L.sticky;
+          lane_count[l] = L.events;
+          if (l == 0 && ctl6) {
+            const u32 cc[6] = {L.active.window_len, L.active.mode, L.active.exp_k, L.active.enable, L.count,
+                               L.window_id};
+            std::memcpy(ctl6, cc, sizeof(cc));
+          }
+        }
+  });
+  return N ? nws[0] : 0;
+}
+
+// pfb::LaneModel::accumulate — pfb_model.hpp:135-150: the exact accumulators
+// of every output sample, acc [n_chan][n_beats][P][2]; hist (nullable)
+// [n_chan][T-1][P] most recent beat first seeds x[m-k] for m-k < 0.
+void rdo_pfb_acc(const uint32_t* x, const uint32_t* coeff, unsigned P, unsigned T, unsigned n_chan,
+                 unsigned n_beats, const uint32_t* hist, int64_t* acc) {
+  for (unsigned c = 0; c < n_chan; ++c)
+    for (unsigned m = 0; m < n_beats; ++m)
+      for (unsigned p = 0; p < P; ++p) {
+        i64 re = 0, im = 0;
+        for (unsigned k = 0; k < T; ++k) {
+          u32 sw = 0;
+          if (k <= m)
+            sw = x[((size_t)c * n_beats + (m - k)) * P + p];
+          else if (hist)
+            sw = hist[((size_t)c * (T - 1) + (k - m - 1)) * P + p];
+          const Cx s = unpack(sw), h = unpack(coeff[(size_t)p * T + k]);
+          re = wadd(re, cm_re(s, h));
+          im = wadd(im, cm_im(s, h));
+        }
+        const size_t o = (((size_t)c * n_beats + m) * P + p) * 2;
+        acc[o] = re;
+        acc[o + 1] = im;
+      }
+}
+
+// cfar::decide — cfar_model.hpp:243-285 — for every bin of every row with the
+// configuration as given (no clamp). dec [n_rows*n][8] u64 = {cut_power,
+// sum_lead, sum_lag, sel_sum, sel_n, window_ok, evaluable | detected<<1 |
+// suppressed<<2, take_lead}.
+void rdo_cfar_decide(const uint64_t* bins, unsigned n, unsigned n_rows, const uint32_t* cfg8, uint64_t* dec) {
+  const CfarCfg c{cfg8[0], cfg8[1], cfg8[2], cfg8[3], cfg8[4], cfg8[5], cfg8[6], cfg8[7]};
+  for (size_t r = 0; r < n_rows; ++r)
+    for (size_t i = 0; i < n; ++i) {
+      const uint64_t* row = bins + r * n;
+      u64 lead = 0, lag = 0;
+      bool ok = true;
+      for (u32 j = 1; j <= c.ref_lead; ++j) {
+        const i64 k = (i64)i + (i64)(u32)(c.guard_lead + j);
+        if (k >= (i64)n) {
+          ok = false;
+          break;
+        }
+        lead += row[k];
+      }
+      if (ok)
+        for (u32 j = 1; j <= c.ref_lag; ++j) {
+          const i64 k = (i64)i - (i64)(u32)(c.guard_lag + j);
+          if (k < 0) {
+            ok = false;
+            break;
+          }
+          lag += row[k];
+        }
+      const bool go = c.mode == 1;
+      const bool take = go ? lead * (u64)c.ref_lag >= lag * (u64)c.ref_lead : true;
+      const u64 sel = go ? (take ? lead : lag) : lead + lag;
+      const u32 seln = go ? (take ? c.ref_lead : c.ref_lag) : c.ref_lead + c.ref_lag;
+      const bool refs = go ? (c.ref_lead != 0 && c.ref_lag != 0) : (c.ref_lead + c.ref_lag) != 0;
+      const bool eval = ok && c.enable && refs;
+      const bool det = eval && ((row[i] * (u64)seln) << 8) > (u64)c.alpha * sel;
+      uint64_t* d = dec + (r * n + i) * 8;
+      d[0] = row[i];
+      d[1] = lead;
+      d[2] = lag;
+      d[3] = sel;
+      d[4] = seln;
+      d[5] = ok;
+      d[6] = (eval ? 1u : 0u) | (det ? 2u : 0u) | (!eval ? 4u : 0u);
+      d[7] = take;
+    }
+}
+
+}  // extern "C"

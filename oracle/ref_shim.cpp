@@ -385,3 +385,137 @@ int rdr_scoreboard(const uint32_t* flits, uint64_t n_packets, unsigned packet_w,
 }
 
 }  // extern "C"
+
+namespace {
+covar::Config icfg_of(const uint32_t* c) {
+  covar::Config o;
+  o.window_len = c[0];
+  o.mode = c[1] == 1 ? covar::Mode::kExp : covar::Mode::kBlock;
+  o.exp_k = c[2];
+  o.enable = c[3] != 0;
+  return o;
+}
+
+// One reference Integrator per lane, stepped with the control stream; see
+// rdo_integ_run for the layouts.
+template <class G>
+uint32_t ref_lane(covar::Integrator& I, const uint32_t* steps7, uint32_t n_steps, G get, int64_t* results,
+                  size_t n_lanes, size_t lane, uint32_t* win5, uint32_t cap) {
+  uint32_t nw = 0;
+  for (uint32_t s = 0; s < n_steps; ++s) {
+    const uint32_t* t = steps7 + 7 * (size_t)s;
+    covar::Result r{};
+    if (I.step(icfg_of(t), t[4] != 0, get(s), t[5] != 0, t[6] != 0, &r)) {
+      if (nw < cap) {
+        if (results) results[(size_t)nw * n_lanes + lane] = r.acc;
+        if (win5 && lane == 0) {
+          const uint32_t w[5] = {s, r.window_id, r.sample_count, r.flushed ? 1u : 0u, r.truncated ? 1u : 0u};
+          std::memcpy(win5 + 5 * (size_t)nw, w, sizeof(w));
+        }
+      }
+      ++nw;
+    }
+  }
+  return nw;
+}
+
+void ctl_of(const covar::Integrator& I, uint32_t* ctl6) {
+  const covar::Config& a = I.active();
+  const uint32_t c[6] = {a.window_len, a.mode == covar::Mode::kExp ? 1u : 0u, a.exp_k, a.enable ? 1u : 0u,
+                         I.count(), I.window_id()};
+  std::memcpy(ctl6, c, sizeof(c));
+}
+}  // namespace
+
+extern "C" {
+
+uint32_t rdr_integ_run(unsigned acc_w, const uint32_t* cfg4, const uint32_t* steps7, uint32_t n_steps,
+                       const int64_t* data, size_t n_lanes, int64_t* lane_acc, uint32_t* lane_flags,
+                       uint32_t* lane_count, int64_t* results, uint32_t* win5, uint32_t cap, uint32_t* ctl6) {
+  uint32_t nw = 0;
+  for (size_t l = 0; l < n_lanes; ++l) {
+    covar::Integrator I(acc_w, 16);
+    I.reset(icfg_of(cfg4));
+    nw = ref_lane(I, steps7, n_steps, [&](uint32_t s) { return (fxp::wide_t)data[(size_t)s * n_lanes + l]; },
+                  results, n_lanes, l, win5, cap);
+    lane_acc[l] = I.acc();
+    lane_flags[l] = I.sat_flags().packed();
+    lane_count[l] = I.sat_count();
+    if (l == 0 && ctl6) ctl_of(I, ctl6);
+  }
+  return nw;
+}
+
+uint32_t rdr_cov_integrate_windows(const uint32_t* Xa, unsigned CH, unsigned F, unsigned N, unsigned acc_w,
+                                   const uint32_t* cfg4, const uint32_t* steps7, int64_t* lane_acc,
+                                   uint32_t* lane_flags, uint32_t* lane_count, int64_t* results, uint32_t* win5,
+                                   uint32_t cap, uint32_t* ctl6, unsigned threads) {
+  const size_t npairs = (size_t)CH * (CH + 1) / 2, lanes = 2 * (size_t)N * npairs;
+  std::vector<uint32_t> nws(N, 0);
+  par(N, threads, [&](size_t k) {
+    size_t pi = 0;
+    for (unsigned i = 0; i < CH; ++i)
+      for (unsigned j = i; j < CH; ++j, ++pi)
+        for (unsigned c = 0; c < 2; ++c) {
+          const size_t l = (k * npairs + pi) * 2 + c;
+          covar::Integrator I(acc_w, 16);
+          I.reset(icfg_of(cfg4));
+          auto get = [&](uint32_t m) {
+            const size_t base = ((size_t)m * N + k) * CH;
+            const covar::Cross x = covar::cross_power(fxp::Complex::from_packed(Xa[base + i]),
+                                                      fxp::Complex::from_packed(Xa[base + j]));
+            return c ? x.im : x.re;
+          };
+          nws[k] = ref_lane(I, steps7, F, get, results, lanes, l, l == 0 ? win5 : nullptr, cap);
+          lane_acc[l] = I.acc();
+          lane_flags[l] = I.sat_flags().packed();
+          lane_count[l] = I.sat_count();
+          if (l == 0 && ctl6) ctl_of(I, ctl6);
+        }
+  });
+  return N ? nws[0] : 0;
+}
+
+// pfb::LaneModel::accumulate per branch; the history is seeded by stepping the
+// model through the hist beats, oldest first.
+void rdr_pfb_acc(const uint32_t* x, const uint32_t* coeff, unsigned P, unsigned T, unsigned n_chan,
+                 unsigned n_beats, const uint32_t* hist, int64_t* acc) {
+  for (unsigned c = 0; c < n_chan; ++c)
+    for (unsigned p = 0; p < P; ++p) {
+      std::vector<fxp::Complex> h(T);
+      for (unsigned k = 0; k < T; ++k) h[k] = fxp::Complex::from_packed(coeff[(size_t)p * T + k]);
+      pfb::LaneModel L(T, h.data());
+      if (hist)
+        for (unsigned j = T - 1; j-- > 0;) L.step(fxp::Complex::from_packed(hist[((size_t)c * (T - 1) + j) * P + p]));
+      for (unsigned m = 0; m < n_beats; ++m) {
+        const fxp::Complex s = fxp::Complex::from_packed(x[((size_t)c * n_beats + m) * P + p]);
+        fxp::wide_t re = 0, im = 0;
+        L.accumulate(s, &re, &im);
+        const size_t o = (((size_t)c * n_beats + m) * P + p) * 2;
+        acc[o] = re;
+        acc[o + 1] = im;
+        L.step(s);
+      }
+    }
+}
+
+void rdr_cfar_decide(const uint64_t* bins, unsigned n, unsigned n_rows, const uint32_t* cfg8, uint64_t* dec) {
+  const cfar::Config c = cfg_of(cfg8);
+  for (size_t r = 0; r < n_rows; ++r) {
+    std::vector<uint64_t> row(bins + r * n, bins + (r + 1) * n);
+    for (size_t i = 0; i < n; ++i) {
+      const cfar::Decision d = cfar::decide(row, i, c);
+      uint64_t* o = dec + (r * n + i) * 8;
+      o[0] = d.cut_power;
+      o[1] = d.sum_lead;
+      o[2] = d.sum_lag;
+      o[3] = d.sel_sum;
+      o[4] = d.sel_n;
+      o[5] = d.window_ok;
+      o[6] = (d.evaluable ? 1u : 0u) | (d.detected ? 2u : 0u) | (d.suppressed ? 4u : 0u);
+      o[7] = d.take_lead;
+    }
+  }
+}
+
+}  // extern "C"

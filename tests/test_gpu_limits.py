@@ -80,3 +80,33 @@ def test_batch_cov_full_scale_long_frames(gpu, orc, autotune):
         assert int(tot) == o["n_events"]
         _eq(got["events"], o["events"], f"events CH={CH}")
     b.close()
+
+
+def test_cfar_enqueue_matches_cfar_run(gpu, orc):
+    """rd_cfar_enqueue (total left on the device) equals rd_cfar_run and the
+    oracle; with no rows it writes a zero total."""
+    import ctypes as C
+
+    rng = np.random.default_rng(5)
+    n, rows = 1024, 37
+    cells = rng.integers(0, 1 << 20, n * rows).astype(np.uint32)
+    cfg8 = [1, 0, 0, 4, 4, 16, 16, 0x200]
+    ev, rc, tot, st = gpu.cfar(cells, n, cfg8, 31, 63)
+    dc = gpu.put(cells)
+    cap = rows * (n + 1)
+    dev, drc, dtot = gpu.buf(24 * cap), gpu.buf(4 * rows), gpu.buf(8)
+    cfg = rd_amd.CfarCfg.of(cfg8)
+    gpu.lib.check("rd_cfar_enqueue", gpu.L.rd_cfar_enqueue(gpu.h, dc.ptr, 4, n, rows, C.byref(cfg), 31, 63, None, 0,
+                                                            dev.ptr, cap, drc.ptr, dtot.ptr))
+    gpu.sync()
+    t = int(dtot.download(np.uint64, 1)[0])
+    assert t == tot
+    _eq(dev.download(np.uint64, 3 * t).reshape(t, 3), ev, "events")
+    _eq(drc.download(np.uint32, rows), rc, "row counts")
+    o = orc.cfar(cells.astype(np.uint64), n, cfg8, 31, 63)
+    _eq(ev, o[0], "events vs oracle")
+    dtot.upload(np.array([12345], np.uint64))
+    gpu.lib.check("rd_cfar_enqueue", gpu.L.rd_cfar_enqueue(gpu.h, dc.ptr, 4, n, 0, C.byref(cfg), 31, 63, None, 0,
+                                                            dev.ptr, cap, drc.ptr, dtot.ptr))
+    gpu.sync()
+    assert int(dtot.download(np.uint64, 1)[0]) == 0
