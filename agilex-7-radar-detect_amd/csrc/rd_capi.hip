@@ -340,6 +340,29 @@ rd_status rd_cfar_run(rd_ctx* c, const void* cells, uint32_t cell_bytes, uint32_
   return tot > events_cap ? RD_E_CAPACITY : RD_OK;
 }
 
+rd_status rd_fxp_eval(rd_ctx* c, const rd_fxp_op* ops, uint64_t n, int64_t* y, uint32_t* flags, uint32_t* forms) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if ((n && (!ops || !y || !flags)) || n > (1ull << 39)) return RD_E_ARG;
+  return fin(rd::launch_fxp_eval(c->stream, ops, n, y, flags, forms));
+}
+
+rd_status rd_fxp_accum(rd_ctx* c, uint32_t acc_w, const int64_t* x, uint32_t n, int64_t* y, uint32_t* step_flags,
+                       uint32_t* sticky, uint32_t* count) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (n && (!x || !y || !step_flags || !sticky || !count)) return RD_E_ARG;
+  return fin(rd::launch_fxp_accum(c->stream, acc_w, x, n, y, step_flags, sticky, count));
+}
+
+rd_status rd_fxp_cmult(rd_ctx* c, const uint32_t* a, const uint32_t* b, uint64_t n, int64_t* p4, int64_t* p3,
+                       uint32_t* y, uint8_t* flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if ((n && (!a || !b || !p4 || !p3 || !y || !flags)) || n > (1ull << 39)) return RD_E_ARG;
+  return fin(rd::launch_fxp_cmult(c->stream, a, b, n, p4, p3, y, flags));
+}
+
 rd_status rd_integ_reset(const rd_integ_cfg* cfg, rd_integ_ctl* ctl) {
   if (!cfg || !ctl || cfg->mode > 1) return RD_E_ARG;
   ctl->active = *cfg;

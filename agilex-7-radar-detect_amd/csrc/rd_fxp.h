@@ -52,6 +52,50 @@ __device__ __forceinline__ int32_t sat16_i64(int64_t r, uint32_t& fl) {
   return r > 32767 ? 32767 : (r < -32768 ? -32768 : (int32_t)r);
 }
 
+// ---------------------------------------------------------------------------
+// General forms, any width w and shift s (fxp.hpp:87-231): wrapping 64-bit
+// arithmetic, w-bit range endpoints (w >= 64: the full range; w < 2: {0}),
+// floor shift, round half to even / half up on the floor remainder, round
+// first then saturate with the flags of the rounded value. The stage kernels
+// use the specialised forms above; these define what the specialisations must
+// equal and back the element-wise fxp entry points.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int64_t fx_add(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+__host__ __device__ __forceinline__ int64_t fx_sub(int64_t a, int64_t b) { return (int64_t)((uint64_t)a - (uint64_t)b); }
+__host__ __device__ __forceinline__ int64_t fx_neg(int64_t a) { return (int64_t)(0ull - (uint64_t)a); }
+__host__ __device__ __forceinline__ int64_t fx_top(uint32_t w) {
+  return w >= 64 ? INT64_MAX : (w < 2 ? 0 : (int64_t)((1ull << (w - 1)) - 1ull));
+}
+__host__ __device__ __forceinline__ int64_t fx_bottom(uint32_t w) {
+  return w >= 64 ? INT64_MIN : (w < 2 ? 0 : fx_neg((int64_t)(1ull << (w - 1))));
+}
+__host__ __device__ __forceinline__ uint32_t fx_flags(int64_t v, uint32_t w) {
+  return (v > fx_top(w) ? 2u : 0u) | (v < fx_bottom(w) ? 1u : 0u);
+}
+__host__ __device__ __forceinline__ int64_t fx_sat(int64_t v, uint32_t w) {
+  return v > fx_top(w) ? fx_top(w) : (v < fx_bottom(w) ? fx_bottom(w) : v);
+}
+__host__ __device__ __forceinline__ int64_t fx_floor_shift(int64_t v, uint32_t s) {
+  return s == 0 ? v : (s >= 64 ? (v < 0 ? -1 : 0) : (v >> s));
+}
+__host__ __device__ __forceinline__ int64_t fx_rne(int64_t v, uint32_t s) {
+  if (s == 0) return v;
+  if (s >= 64) return 0;
+  const int64_t q = v >> s;
+  const uint64_t rem = (uint64_t)v & ((1ull << s) - 1ull), half = 1ull << (s - 1);
+  return rem > half ? fx_add(q, 1) : (rem < half ? q : fx_add(q, q & 1));
+}
+__host__ __device__ __forceinline__ int64_t fx_rhu(int64_t v, uint32_t s) {
+  if (s == 0) return v;
+  if (s >= 64) return 0;
+  return fx_add(v, (int64_t)(1ull << (s - 1))) >> s;
+}
+__host__ __device__ __forceinline__ int64_t fx_round_sat(int64_t v, uint32_t s, uint32_t w, uint32_t& fl) {
+  const int64_t r = fx_rne(v, s);
+  fl = fx_flags(r, w);
+  return fx_sat(r, w);
+}
+
 __host__ __device__ __forceinline__ uint32_t bitrev(uint32_t v, uint32_t w) {
 #ifdef __HIP_DEVICE_COMPILE__
   return w == 0 ? 0u : (__brev(v) >> (32 - w));

@@ -154,6 +154,14 @@ hipError_t launch_pfb_acc(hipStream_t s, const uint32_t* x, const uint32_t* coef
 hipError_t launch_cfar_decide(hipStream_t s, const uint64_t* cells, uint32_t n, uint32_t n_rows,
                               const rd_cfar_cfg& cfg, rd_cfar_decision* out);
 
+// Element-wise fixed-point primitives (rd_fxpgate.hip).
+hipError_t launch_fxp_eval(hipStream_t s, const rd_fxp_op* ops, uint64_t n, int64_t* y, uint32_t* flags,
+                           uint32_t* forms);
+hipError_t launch_fxp_accum(hipStream_t s, uint32_t acc_w, const int64_t* x, uint32_t n, int64_t* y,
+                            uint32_t* step_flags, uint32_t* sticky, uint32_t* count);
+hipError_t launch_fxp_cmult(hipStream_t s, const uint32_t* a, const uint32_t* b, uint64_t n, int64_t* p4,
+                            int64_t* p3, uint32_t* y, uint8_t* flags);
+
 // CFAR (rd_cfar.hip). scratch holds scratch_bytes: the row offsets
 // ((n_rows + 1) * 8 bytes, required) then as many staged events per row as
 // fit (rows with more are recomputed by the gather pass).

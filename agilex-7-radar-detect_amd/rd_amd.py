@@ -28,7 +28,13 @@ EXPORTS = [
     "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
     "rd_chain_enqueue_static", "rd_bf_plain_run", "rd_cov_plain_run", "rd_chain_enqueue_plain", "rd_cfar_enqueue",
     "rd_integ_reset", "rd_integ_run", "rd_cov_integrate_windows", "rd_pfb_acc_run", "rd_cfar_decide",
+    "rd_fxp_eval", "rd_fxp_accum", "rd_fxp_cmult",
 ]
+FXP_OPS = {"sat": 0, "trunc": 1, "round_even": 2, "round": 2, "round_half_up": 3, "round_sat": 4, "add_sat": 5,
+           "sub_sat": 6, "neg_sat": 7, "mul_q15": 8, "mul_q15_rs": 9, "cmul_re": 10, "cmul_im": 11, "acc_w": 12,
+           "growth_bits": 13}
+FXP_OP_DTYPE = np.dtype([("a", np.int64), ("b", np.int64), ("op", np.uint32), ("sh", np.uint32), ("w", np.uint32),
+                         ("pad", np.uint32)])
 RD_BATCH_AUTOTUNE = 1
 
 
@@ -122,6 +128,9 @@ class Lib:
         L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
                                   C.POINTER(u64)]
+        L.rd_fxp_eval.argtypes = [vp, vp, u64, vp, vp, vp]
+        L.rd_fxp_accum.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp]
+        L.rd_fxp_cmult.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp]
         L.rd_integ_reset.argtypes = [C.POINTER(IntegCfg), C.POINTER(IntegCtl)]
         L.rd_integ_run.argtypes = [vp, u32, C.POINTER(IntegCtl), vp, u32, vp, u64, vp, vp, vp, u32, C.POINTER(u32)]
         L.rd_cov_integrate_windows.argtypes = [vp, vp, u32, u32, u32, u32, C.POINTER(IntegCtl), vp, vp, vp, vp, u32,
@@ -553,6 +562,42 @@ def _ctx_cfar_decide(self, cells, n, cfg8):
 
 
 Ctx.cfar_decide = _ctx_cfar_decide
+
+
+def _ctx_fxp_eval(self, ops):
+    """rd_fxp_eval on records (FXP_OP_DTYPE) -> (y int64, flags u32, forms u32)."""
+    ops = np.ascontiguousarray(ops, FXP_OP_DTYPE)
+    n = ops.size
+    d = self.put(ops.view(np.uint8))
+    dy, df, dm = self.buf(8 * max(n, 1)), self.buf(4 * max(n, 1)), self.buf(4 * max(n, 1))
+    self.lib.check("rd_fxp_eval", self.L.rd_fxp_eval(self.h, d.ptr, n, dy.ptr, df.ptr, dm.ptr))
+    return dy.download(np.int64, n), df.download(np.uint32, n), dm.download(np.uint32, n)
+
+
+def _ctx_fxp_accum(self, acc_w, xs):
+    """rd_fxp_accum from a clear -> (y, step_flags, sticky, count) per step."""
+    x = np.ascontiguousarray(xs, np.int64)
+    n = x.size
+    dx = self.put(x)
+    b = [self.buf(8 * max(n, 1))] + [self.buf(4 * max(n, 1)) for _ in range(3)]
+    self.lib.check("rd_fxp_accum", self.L.rd_fxp_accum(self.h, acc_w, dx.ptr, n, *[t.ptr for t in b]))
+    return (b[0].download(np.int64, n),) + tuple(t.download(np.uint32, n) for t in b[1:])
+
+
+def _ctx_fxp_cmult(self, a, b):
+    """rd_fxp_cmult -> (p4 [n][2], p3 [n][2], y packed, flags f_re | f_im << 2)."""
+    a, b = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32)
+    n = a.size
+    da, db = self.put(a), self.put(b)
+    d4, d3, dy, df = self.buf(16 * max(n, 1)), self.buf(16 * max(n, 1)), self.buf(4 * max(n, 1)), self.buf(max(n, 1))
+    self.lib.check("rd_fxp_cmult", self.L.rd_fxp_cmult(self.h, da.ptr, db.ptr, n, d4.ptr, d3.ptr, dy.ptr, df.ptr))
+    return (d4.download(np.int64, 2 * n).reshape(n, 2), d3.download(np.int64, 2 * n).reshape(n, 2),
+            dy.download(np.uint32, n), df.download(np.uint8, n))
+
+
+Ctx.fxp_eval = _ctx_fxp_eval
+Ctx.fxp_accum = _ctx_fxp_accum
+Ctx.fxp_cmult = _ctx_fxp_cmult
 
 
 class Chain:

@@ -70,14 +70,33 @@ def read_counters(dirs):
     return out
 
 
-def read_stats(d):
-    """{kernel name: (calls, average ns)} from *kernel_stats.csv."""
+def read_stats(d, full=False):
+    """{kernel name: (calls, average ns)} from *kernel_stats.csv (full: also
+    min, max and total ns)."""
     out = {}
     for f in _files(d, "*kernel_stats.csv"):
         with _open(f) as fh:
             for row in csv.DictReader(fh):
-                out[row["Name"]] = (int(row["Calls"]), float(row["AverageNs"]))
+                v = (int(row["Calls"]), float(row["AverageNs"]))
+                if full:
+                    v += (float(row["MinNs"]), float(row["MaxNs"]), float(row["TotalDurationNs"]))
+                out[row["Name"]] = v
     return out
+
+
+def list_table(d, title, commands):
+    """The library's kernels of a kernel trace, by total time."""
+    st = read_stats(d, full=True)
+    rows = sorted(((n, v) for n, v in st.items() if "rd::" in n), key=lambda kv: -kv[1][4])
+    lines = [f"# {title}", ""]
+    if commands:
+        lines += ["Command (run from `/tmp` with `TMPDIR=/tmp` on one MI355X; `<repo>` is the repository root, "
+                  "`<out>` an output directory):", "", "```"] + commands + ["```", "",
+                  "Per-kernel durations from that run's `kernel_stats.csv` (every dispatch of the run).", ""]
+    lines += ["| kernel | calls | avg us | min us | max us | total ms |", "|---|---|---|---|---|---|"]
+    for n, (c, avg, mn, mx, tot) in rows:
+        lines.append(f"| `{short(n)}` | {c} | {avg / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | {tot / 1e6:.3f} |")
+    return "\n".join(lines)
 
 
 def short(name: str) -> str:
@@ -99,7 +118,11 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--title", default="")
     ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--list", action="store_true", help="only the kernel-trace table of the library's kernels")
     a = ap.parse_args()
+    if a.list:
+        print(list_table(a.trace, a.title, a.command))
+        return
     stats = read_stats(a.trace)
     ctr = read_counters(a.pmc)
     copy = [v for (n, g), (v, c) in ctr.items() if "copyBuffer" in n]

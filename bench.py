@@ -467,194 +467,64 @@ def batched_leg(torch, ctx, n_cpi, reps, autotune, oracle_cpis):
 
 
 # ----------------------------------------------------------------------------- FP32 shadow leg
-def hard_cfar_hits(torch, power, N, guard, ref, alpha):
-    """The bit-exact CA-CFAR decision (cfar_model.hpp:296-301) on int power
-    rows [..., N] in exact int64 torch ops: hit iff (C * n) << 8 > alpha * S."""
-    P = power.to(torch.int64)
-    lo, hi = guard + ref, N - guard - ref
-    S = torch.zeros_like(P[..., lo:hi])
-    for d in range(guard + 1, guard + ref + 1):
-        S += P[..., lo + d: hi + d] + P[..., lo - d: hi - d]
-    return ((P[..., lo:hi] * (2 * ref)) << 8) > alpha * S
-
-
-def shadow_leg(torch, ctx, n_cpi, steps, fd_dirs):
+def shadow_leg(torch, ctx, n_cpi, steps, fd_dirs, n_train=None, pos_weight=512.0, lr=3e-3):
     """The differentiable FP32 shadow on 64 CPIs of 32 ch x 1024-pt x 128
-    frames: a training step's forward + backward timed, gradients checked
-    against float64 (plain autograd and central finite differences), the
-    weights trained, quantised to Q1.15 (channel gains folded in) and rerun
-    through the bit-exact batched chain."""
+    frames (rd_workloads.ShadowProblem): one training step (forward + backward
+    over every CPI) timed, gradients checked against float64 (plain autograd
+    and central finite differences), the weights and per-channel gains trained
+    on the first 3/4 of the CPIs, quantised to Q1.15 (gains folded in) and both
+    weight sets rerun through the bit-exact batched chain, scored on the
+    training and the held-out CPIs."""
     import rd_shadow as sh
 
-    C, N, T, F, B = 32, 1024, 16, 128, 32
-    guard, ref, alpha_q = 4, 16, 0x800
-    alpha, tau = alpha_q / 256.0, 0.1
-    rng = np.random.default_rng(55)
-    err = 10 ** (rng.uniform(-1, 1, C) / 20) * np.exp(1j * np.deg2rad(rng.uniform(-10, 10, C)))
-    angles = list(np.rad2deg(np.arcsin(np.linspace(-0.9, 0.9, B))))
-    sin_b = np.sin(np.deg2rad(angles))
-    mask = torch.zeros((n_cpi, B, F, N), dtype=torch.float32, device="cuda")
-    # a target's own response outside its cell (its bin in the other beams, and
-    # the neighbouring bins the PFB channel overlap reaches) is neither a
-    # detection to reward nor a false alarm to penalise
-    care = torch.ones((n_cpi, B, F, N), dtype=torch.float32, device="cuda")
-    scenes, iqs = [], []
+    n_train = n_cpi * 3 // 4 if n_train is None else n_train
     note(f"shadow: generating {n_cpi} CPIs (32 ch x 1024 x 128, channel gain/phase errors)")
-    for q in range(n_cpi):
-        nt = int(rng.integers(1, 9))
-        tg = [(int(rng.integers(guard + ref + 1, N - guard - ref - 1)), float(rng.uniform(-55, 55)),
-               float(rng.uniform(-30, -10))) for _ in range(nt)]
-        sc = rd_scenario.Scene(f"c5_{q}", CH=C, N=N, T=T, F=F, B=B, sigma=256.0, guard=guard, ref=ref, targets=tg,
-                               beam_angles=angles, seed=9000 + q)
-        scenes.append(sc)
-        iqs.append(rd_workloads.device_iq(torch, sc, 9000 + q, chan_err=err))
-        for k, ang, _ in tg:
-            care[q, :, :, k - 2:k + 3] = 0.0
-        for k, ang, _ in tg:
-            bb = int(np.argmin(np.abs(sin_b - np.sin(np.deg2rad(ang)))))
-            mask[q, bb, :, k] = 1.0
-            care[q, bb, :, k] = 1.0
-    lo = guard + ref
-    mask_v = mask[..., lo:N - lo].contiguous()
-    care_v = care[..., lo:N - lo].contiguous()
-    del mask, care
-    taps_w, W0_w = scenes[0].taps(), scenes[0].weights()
-
-    def to_cx(words_i32):  # packed int32 words -> complex64 / 2^15
-        w = words_i32.to(torch.int32)
-        re = ((w << 16) >> 16).float()
-        im = (w >> 16).float()
-        return torch.complex(re, im) / 32768.0
-
-    x = torch.stack([to_cx(t).reshape(C, F * N) for t in iqs])
-    h = to_cx(torch.from_numpy(taps_w.view(np.int32)).cuda()).reshape(N, T).contiguous()
-    W = to_cx(torch.from_numpy(W0_w.view(np.int32)).cuda()).reshape(B, C).contiguous().requires_grad_()
-    gain = torch.ones(C, dtype=torch.complex64, device="cuda").requires_grad_()
-
-    def loss32(W_=None, g_=None):
-        p = sh.FrontEnd.apply(x, W if W_ is None else W_, gain if g_ is None else g_, h)
-        return sh.loss_fn(sh.soft_cfar_logits(p, N, guard, ref, alpha, tau), mask_v, care_v)
-
-    # timing of one training step (forward + backward)
+    pb = rd_workloads.ShadowProblem(torch, n_cpi, n_train, pos_weight=pos_weight)
+    W = pb.W0.clone().requires_grad_()
+    gain = torch.ones(pb.C, dtype=torch.complex64, device="cuda").requires_grad_()
     for _ in range(2):
-        loss32().backward()
-    W.grad = None
-    gain.grad = None
+        pb.loss32(W, gain).backward()
     torch.cuda.synchronize()
     reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
         W.grad = None
         gain.grad = None
-        loss32().backward()
+        pb.loss32(W, gain).backward()
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / reps
-    samples = n_cpi * C * N * F
-
-    # gradient checks against float64
     note("shadow: float64 gradient checks")
-    W.grad = None
-    gain.grad = None
-    L32 = loss32()
-    L32.backward()
-    gW32, gg32 = W.grad.detach().clone(), gain.grad.detach().clone()
-    x64, h64, m64, c64 = x.to(torch.complex128), h.to(torch.complex128), mask_v.double(), care_v.double()
-
-    def loss64(W_, g_):
-        p = sh.reference(x64, W_, g_, h64)
-        return sh.loss_fn(sh.soft_cfar_logits(p, N, guard, ref, alpha, tau), m64, c64)
-
-    W64 = W.detach().to(torch.complex128).requires_grad_()
-    g64 = gain.detach().to(torch.complex128).requires_grad_()
-    L64 = loss64(W64, g64)
-    gW64, gg64 = torch.autograd.grad(L64, (W64, g64))
-    g32 = torch.cat([gW32.reshape(-1), gg32]).to(torch.complex128)
-    g64v = torch.cat([gW64.reshape(-1), gg64])
-    autograd_rel = float((g32 - g64v).abs().norm() / g64v.abs().norm())
-    fd_rel = []
-    gen = torch.Generator(device="cpu").manual_seed(5)
-    eps = 1e-5
-    with torch.no_grad():
-        for _ in range(fd_dirs):
-            vW = torch.complex(torch.randn(B, C, generator=gen, dtype=torch.float64),
-                               torch.randn(B, C, generator=gen, dtype=torch.float64)).cuda()
-            vg = torch.complex(torch.randn(C, generator=gen, dtype=torch.float64),
-                               torch.randn(C, generator=gen, dtype=torch.float64)).cuda()
-            lp = loss64(W64 + eps * vW, g64 + eps * vg)
-            lm = loss64(W64 - eps * vW, g64 - eps * vg)
-            fd = float((lp - lm) / (2 * eps))
-            # directional derivative from the gradient: Re <g, v> over re/im parts
-            an = float((gW32.to(torch.complex128).conj() * vW).real.sum() + (gg32.to(torch.complex128).conj() * vg).real.sum())
-            fd_rel.append(abs(an - fd) / abs(fd))
-    del x64, m64, c64
-
-    # training
-    note(f"shadow: {steps} training steps")
-    opt = torch.optim.Adam([W, gain], lr=1e-3)
-    losses = []
-    for _ in range(steps):
-        opt.zero_grad()
-        L = loss32()
-        L.backward()
-        opt.step()
-        losses.append(float(L.detach()))
-
-    # quantise back and rerun the bit-exact chain (batched) with both weight sets
+    autograd_rel, fd_rel = pb.gradient_check(pb.W0, torch.ones(pb.C, dtype=torch.complex64, device="cuda"), fd_dirs)
+    note(f"shadow: {steps} training steps on {pb.n_train} CPIs")
+    Wt, gt, losses = pb.train(steps, lr)
     note("shadow: bit-exact rerun with the quantised weights")
-    Wq = sh.quantise_weights(W, gain)
+    Wq = sh.quantise_weights(Wt, gt)
     with torch.no_grad():  # the FP32 loss at the quantised weights (gains folded in, unit gains)
-        Wq_c = to_cx(torch.from_numpy(Wq.view(np.int32)).cuda()).reshape(B, C).contiguous()
-        W0_c = to_cx(torch.from_numpy(W0_w.view(np.int32)).cuda()).reshape(B, C).contiguous()
-        ones = torch.ones(C, dtype=torch.complex64, device="cuda")
-        loss_q = [float(loss32(W0_c, ones)), float(loss32(Wq_c, ones))]
-    hits = {}
-    consistency = 0
-    for name, wq in (("initial", W0_w), ("trained", Wq)):
-        wt = torch.from_numpy(np.asarray(wq, np.uint32).view(np.int32)).cuda()
-        tp = torch.from_numpy(taps_w.view(np.int32)).cuda()
-        descs, bufs, keep = [], [], []
-        cfg = np.array([1, 0, 0, guard, guard, ref, ref, alpha_q], np.uint32)
-        for q in range(n_cpi):
-            S = C * N * F
-            d = dict(iq=iqs[q], taps=tp, weights=wt, u=torch.empty(S, dtype=torch.int32, device="cuda"),
-                     X=torch.empty(S, dtype=torch.int32, device="cuda"),
-                     Xa=torch.empty(S, dtype=torch.int32, device="cuda"),
-                     power=torch.empty(B * F * N, dtype=torch.int32, device="cuda"),
-                     row_count=torch.empty(B * F, dtype=torch.int32, device="cuda"))
-            keep.append(d)
-            descs.append(rd_amd.CpiDesc(C, N, T, F, B, 2, rd_scenario.SCHED_ALL, rd_scenario.MAX_GUARD,
-                                        rd_scenario.MAX_REF, rd_amd.CfarCfg.of(cfg)))
-            bufs.append(rd_amd.CpiBufs(**{k: v.data_ptr() for k, v in d.items()}, events_cap=0))
-        torch.cuda.synchronize()
-        bt = rd_amd.Batch(ctx, descs, bufs, keep=keep)
-        bt.launch()
-        bt.events()
-        bt.close()
-        pw = torch.stack([d["power"].view(torch.int32).reshape(B, F, N) for d in keep]).to(torch.int64) & 0xFFFFFFFF
-        hit = hard_cfar_hits(torch, pw, N, guard, ref, alpha_q)
-        # the decision rule reproduces the bit-exact CFAR's per-row detection counts
-        rc = torch.stack([d["row_count"].reshape(B, F) for d in keep]).to(torch.int64) - 1
-        consistency += int((hit.sum(-1) != rc).sum())
-        t = mask_v > 0
-        fa_cells = (mask_v == 0) & (care_v > 0)
-        hits[name] = dict(pd=round(float(hit[t].float().mean()), 4), false_alarms=int(hit[fa_cells].sum()),
-                          sidelobe_hits=int(hit[care_v == 0].sum()))
-        del pw, hit, keep
+        ones = torch.ones(pb.C, dtype=torch.complex64, device="cuda")
+        held = slice(pb.n_train, pb.n_cpi)
+        loss_q = {"initial": float(pb.loss32(pb.W0, ones, held)),
+                  "trained": float(pb.loss32(pb.words_to_weights(Wq), ones, held))}
+    hits, consistency = {}, 0
+    for name, wq in (("initial", pb.W0_w), ("trained", Wq)):
+        hits[name], c = pb.bit_exact_hits(ctx, wq)
+        consistency += c
+    samples = pb.samples
+    del pb
     return {
         "workload": f"{n_cpi} CPIs x 32 ch x 1024-pt x 128 frames, FP32, 32x32 complex weights + 32 channel gains, "
-                    f"soft CA-CFAR sigmoid((p/mean - {alpha})/{tau}), BCE vs target cells",
+                    f"soft CA-CFAR sigmoid((p/mean - 8.0)/0.1), weighted BCE (targets x {pos_weight:g}) against "
+                    f"target cells, sidelobes and noise; trained on {n_train} CPIs, {n_cpi - n_train} held out",
         "samples": samples,
         "train_step_ms": round(t_step * 1e3, 2),
         "train_step_msamples_s": round(samples / t_step / 1e6, 1),
         "grad_rel_err_vs_fp64_autograd": autograd_rel,
         "grad_rel_err_vs_fp64_finite_diff": fd_rel,
+        "train_steps": steps, "lr": lr,
         "loss_first_last": [losses[0], losses[-1]] if losses else None,
-        "loss_fp32_at_q15_weights_initial_trained": loss_q,
+        "held_out_loss_fp32_at_q15_weights": loss_q,
         "bit_exact_rerun": hits,
         "cfar_rule_vs_row_counts_mismatches": consistency,
     }
-
-
 
 
 def main():
@@ -675,7 +545,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     ap.add_argument("--shadow-cpis", type=int, default=64, help="CPIs of the FP32 shadow leg (0: skip it)")
-    ap.add_argument("--shadow-steps", type=int, default=20)
+    ap.add_argument("--shadow-steps", type=int, default=60)
     ap.add_argument("--shadow-only", action="store_true", help="run only the FP32 shadow leg")
     ap.add_argument("--large-steps", type=int, default=10, help="graph replays of the large-workload leg (0: skip it)")
     ap.add_argument("--large-verify", type=int, default=1, help="check one large CPI against the oracle (0: skip)")

@@ -227,6 +227,39 @@ rd_status rd_cov_run(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t 
 rd_status rd_cov_integrate(rd_ctx* ctx, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                            uint32_t mode, uint32_t exp_k, uint32_t acc_w, int64_t* R, uint8_t* sat_flags);
 
+/* ---- fixed-point primitives, element-wise --------------------------------
+ * The operations of model/cpp/fxp/fxp.hpp (sat :175-179, trunc :185,
+ * round_even / round_half_up :191-215, round_sat :221-231, add/sub/neg_sat
+ * :237-254, mul_q15 :261-263, cmul_q15 :303-325, growth_bits / acc_w
+ * :334-352) on the device, by the device functions the stage kernels inline.
+ * One record per operation; for the multiplies a and b carry int16 operands
+ * in their low 16 bits (mul) or packed complex words in their low 32 bits
+ * (cmul). y and flags ({sat_pos<<1 | sat_neg}, 0 where the operation cannot
+ * saturate) per record; forms (nullable) per record: bit 0 set when one of the
+ * kernels' specialised forms (16-bit saturation, shift-15 rounding in 32 or
+ * 64 bits, the one-bit butterfly rounding) applies to the operands, bit 1 when
+ * it disagrees with the general form. ops, y, flags, forms: device. */
+enum {
+  RD_FXP_SAT = 0, RD_FXP_TRUNC, RD_FXP_ROUND_EVEN, RD_FXP_ROUND_HALF_UP, RD_FXP_ROUND_SAT, RD_FXP_ADD_SAT,
+  RD_FXP_SUB_SAT, RD_FXP_NEG_SAT, RD_FXP_MUL_Q15, RD_FXP_MUL_Q15_RS, RD_FXP_CMUL_RE, RD_FXP_CMUL_IM, RD_FXP_ACC_W,
+  RD_FXP_GROWTH_BITS
+};
+typedef struct {
+  int64_t a, b;
+  uint32_t op, sh, w, pad;
+} rd_fxp_op;
+rd_status rd_fxp_eval(rd_ctx* ctx, const rd_fxp_op* ops, uint64_t n, int64_t* y, uint32_t* flags, uint32_t* forms);
+/* fxp::Acc (fxp.hpp:392-418) from a clear over x[0..n): per step the
+ * saturated accumulator, the step's flags, the sticky flags and the saturating
+ * count of flagged steps (StickyFlags, fxp.hpp:358-381). Device arrays. */
+rd_status rd_fxp_accum(rd_ctx* ctx, uint32_t acc_w, const int64_t* x, uint32_t n, int64_t* y, uint32_t* step_flags,
+                       uint32_t* sticky, uint32_t* count);
+/* fxp::cmult::raw_mult4 / raw_mult3 / eval (model/cpp/fxp/cmult.hpp:85-168):
+ * the exact products [n][2] (re, im) of both forms, the rounded Q1.15 product
+ * and its flags (f_re | f_im << 2). Device arrays. */
+rd_status rd_fxp_cmult(rd_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, int64_t* p4, int64_t* p3,
+                       uint32_t* y, uint8_t* flags);
+
 /* ---- the Integrator over a control stream --------------------------------
  * Many covar::Integrator(acc_w) instances ("lanes") stepped together:
  * Integrator::reset(cfg) is rd_integ_reset(cfg, &ctl) plus zeroed lanes, and

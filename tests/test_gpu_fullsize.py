@@ -59,6 +59,10 @@ def test_large_cpi_full_size(gpu, orc, wl):
     assert tot == exp["n_events"]
     assert np.array_equal(got["events"], exp["events"]), "large: events differ"
     assert tot > sc.B * sc.F
+    # the covariance-trace invariant in FP64 on the GPU: tr R[k] is the sum of
+    # every channel's power in bin k over the frames (covar_model.hpp:99-102)
+    tr = wl.covariance_trace_check(torch, got["Xa"], got["R"], sc.CH, sc.F, sc.N)
+    assert tr["ok"], tr
 
 
 def test_uneven_batch_256_matches_per_cpi(gpu, orc, wl):

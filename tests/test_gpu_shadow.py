@@ -216,3 +216,20 @@ def test_training_then_quantise_into_the_bit_exact_chain(gpu, orc, sh):
     assert np.array_equal(got["power"], o["power"])
     assert np.array_equal(got["events"], o["events"])
     del sc
+
+
+def test_config5_gradients_at_full_size(sh):
+    """The stated configuration: 64 CPIs x 32 ch x 1024-pt x 128 frames, 32x32
+    complex weights and 32 channel gains. The FP32 gradients of the soft-CFAR
+    loss (hand-written HIP backward kernels) against float64 autograd of the
+    plain-torch model and float64 central finite differences: relative error
+    below 1e-4."""
+    import rd_workloads
+
+    pb = rd_workloads.ShadowProblem(torch, n_cpi=64, n_train=48)
+    assert pb.x.shape == (64, 32, 1024 * 128) and pb.W0.shape == (32, 32)
+    rel, fd = pb.gradient_check(pb.W0, torch.ones(32, dtype=torch.complex64, device="cuda"), fd_dirs=2)
+    assert rel < 1e-4, rel
+    assert max(fd) < 1e-4, fd
+    del pb
+    torch.cuda.empty_cache()
