@@ -496,6 +496,7 @@ rd_status rd_batch_create(rd_ctx* ctx, uint32_t n_cpi, const rd_cpi_desc* descs,
     if (st != RD_OK) return st;
     st = rd::check_cpi_bufs(&bufs[i]);
     if (st != RD_OK) return st;
+    if (!bufs[i].Xa) return RD_E_ARG;  // the batch's beamformer classes write the copy its covariance reads
   }
   rd_batch* B = new (std::nothrow) rd_batch();
   if (!B) return RD_E_ARG;

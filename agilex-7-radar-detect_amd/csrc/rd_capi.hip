@@ -127,7 +127,7 @@ rd_status check_cpi(const rd_cpi_desc* d) {
 }
 
 rd_status check_cpi_bufs(const rd_cpi_bufs* b) {
-  if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->Xa || !b->power || !b->row_count)
+  if (!b || !b->iq || !b->taps || !b->weights || !b->u || !b->X || !b->power || !b->row_count)
     return RD_E_ARG;
   if (((uintptr_t)b->u | (uintptr_t)b->X) & 15u) return RD_E_ARG;
   if (!b->events && b->events_cap) return RD_E_ARG;
@@ -191,6 +191,7 @@ rd_status rd_destroy(rd_ctx* c) {
   if (c->total_host) (void)hipHostFree(c->total_host);
   if (c->dop) (void)hipFree(c->dop);
   if (c->integ) (void)hipFree(c->integ);
+  if (c->xa) (void)hipFree(c->xa);
   for (void* p : c->retired) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -285,6 +286,19 @@ rd_status rd_bf_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t 
   if (st != RD_OK) return st;
   if (!X || !W || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128) return RD_E_ARG;
   return fin(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
+}
+
+rd_status rd_bf_cov_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                          uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
+                          uint32_t* Xa, int64_t* R) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!X || !W || !R || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128 || n_frames > 65535) return RD_E_ARG;
+  if (rd::bfcov_fits(n_ant, n_bins, n_frames, n_beams))
+    return fin(rd::launch_bfcov(c->stream, X, W, n_ant, n_beams, n_frames, n_bins, y, flags, power, Xa, R));
+  if (!Xa) return RD_E_ARG;
+  RD_TRY(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
+  return fin(rd::launch_cov(c->stream, Xa, n_ant, n_frames, n_bins, R));
 }
 
 rd_status rd_doppler_run(rd_ctx* c, const uint32_t* ring, uint32_t rows, uint32_t depth, uint32_t n_bins,
@@ -459,8 +473,20 @@ rd_status rd_cfar_enqueue(rd_ctx* c, const void* cells, uint32_t cell_bytes, uin
 
 namespace {
 
+// The beamformer and the covariance run as one kernel (no aligned copy
+// needed between them).
+bool chain_fused(const rd_cpi_desc* d, const rd_cpi_bufs* b) {
+  return b->R && rd::bfcov_fits(d->n_chan, d->n_fft, d->n_frames, d->n_beams);
+}
+
+// The path needs the aligned copy as an intermediate: the plain chain
+// always, the MFMA chain when it computes the covariance unfused.
+bool chain_needs_xa(const rd_cpi_desc* d, const rd_cpi_bufs* b, bool plain) {
+  return plain || (b->R && !chain_fused(d, b));
+}
+
 // Validates a CPI and sizes the context's scratch for it (no launches).
-rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
+rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, bool plain = false) {
   rd_status st = enter(c);
   if (st != RD_OK) return st;
   st = rd::check_cpi(d);
@@ -469,6 +495,10 @@ rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b) {
   if (st != RD_OK) return st;
   st = grow(c, &c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
   if (st != RD_OK) return st;
+  if (!b->Xa && chain_needs_xa(d, b, plain)) {
+    st = grow(c, &c->xa, &c->xa_bytes, (size_t)4 * d->n_chan * d->n_fft * d->n_frames);
+    if (st != RD_OK) return st;
+  }
   return grow(c, &c->scratch, &c->scratch_bytes, rd::cfar_scratch_bytes(d->n_fft, d->n_beams * d->n_frames));
 }
 
@@ -481,9 +511,17 @@ rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, ui
   hipStream_t s = c->stream;
   RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
   RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
-  // beamformer + power on the FFT layout; the same staged tiles give the aligned copy
-  RD_TRY(rd::launch_bf_chmajor(s, b->X, b->weights, CH, B, cols, b->y, b->yflags, b->power, b->Xa));
-  if (b->R) RD_TRY(rd::launch_cov(s, b->Xa, CH, F, N, b->R));
+  if (chain_fused(d, b)) {
+    // beamformer + power and covariance from one read of X
+    RD_TRY(rd::launch_bfcov(s, b->X, b->weights, CH, B, F, N, b->y, b->yflags, b->power, b->Xa, b->R));
+  } else {
+    // beamformer + power on the FFT layout; the same staged tiles give the
+    // aligned copy the covariance reads
+    uint32_t* xa = b->Xa ? b->Xa : (b->R ? static_cast<uint32_t*>(c->xa) : nullptr);
+    if (b->R && !xa) return RD_E_ARG;
+    RD_TRY(rd::launch_bf_chmajor(s, b->X, b->weights, CH, B, cols, b->y, b->yflags, b->power, xa));
+    if (b->R) RD_TRY(rd::launch_cov(s, xa, CH, F, N, b->R));
+  }
   RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
                          b->events ? b->events_cap : 0, b->row_count, scratch, scratch_bytes,
                          n_events_dev ? n_events_dev : c->total_dev));
@@ -505,16 +543,17 @@ rd_status rd_chain_enqueue(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b
 }
 
 rd_status rd_chain_enqueue_plain(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, uint64_t* n_events_dev) {
-  rd_status st = chain_prepare(c, d, b);
+  rd_status st = chain_prepare(c, d, b, true);
   if (st != RD_OK) return st;
   const uint32_t CH = d->n_chan, N = d->n_fft, F = d->n_frames, B = d->n_beams;
   const uint64_t cols = (uint64_t)F * N;
   hipStream_t s = c->stream;
+  uint32_t* xa = b->Xa ? b->Xa : static_cast<uint32_t*>(c->xa);
   RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
   RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
-  RD_TRY(rd::launch_align(s, b->X, CH, cols, b->Xa));
-  RD_TRY(rd::launch_bf_plain(s, b->Xa, b->weights, CH, B, cols, b->y, b->yflags, b->power));
-  if (b->R) RD_TRY(rd::launch_cov_plain(s, b->Xa, CH, F, N, b->R));
+  RD_TRY(rd::launch_align(s, b->X, CH, cols, xa));
+  RD_TRY(rd::launch_bf_plain(s, xa, b->weights, CH, B, cols, b->y, b->yflags, b->power));
+  if (b->R) RD_TRY(rd::launch_cov_plain(s, xa, CH, F, N, b->R));
   RD_TRY(rd::launch_cfar(s, b->power, 4, N, B * F, d->cfar, d->max_guard, d->max_ref, nullptr, F, b->events,
                          b->events ? b->events_cap : 0, b->row_count, c->scratch, c->scratch_bytes,
                          n_events_dev ? n_events_dev : c->total_dev));
@@ -549,6 +588,8 @@ rd_status rd_chain_enqueue_static(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_
   // more are recomputed by the CFAR gather)
   const uint64_t rows = (uint64_t)d->n_beams * d->n_frames;
   if ((rows + 1) * sizeof(uint64_t) > kStaticScratchBytes) return RD_E_SCRATCH;
+  // no allocation here: a path that needs the aligned copy needs the caller's
+  if (!b->Xa && chain_needs_xa(d, b, false)) return RD_E_ARG;
   return chain_launch(c, d, b, n_events_dev, c->static_scratch, kStaticScratchBytes);
 }
 

@@ -29,6 +29,8 @@ struct rd_ctx {
   size_t dop_bytes = 0;
   void* integ = nullptr;           // Integrator control words of the current call
   size_t integ_bytes = 0;
+  void* xa = nullptr;              // aligned copy for chains whose caller passes none
+  size_t xa_bytes = 0;
   std::vector<void*> retired;      // outgrown buffers a live graph may still use
 };
 
@@ -80,6 +82,14 @@ struct CovJob {
   uint32_t CH, F, N, tile0;
 };
 
+// Beamformer + covariance of one CPI read once from the FFT layout
+// (bf.X = X[ch][frame][bin], bf.xa_out the optional aligned copy).
+struct BfCovJob {
+  BfJob bf;
+  int64_t* R;
+  uint32_t N, F;
+};
+
 // CFAR geometry after clamping to the elaboration maxima.
 struct CfarGeom {
   uint32_t enable, go, dense, gl, gg, rl, rg, alpha, refs_ok;
@@ -127,6 +137,12 @@ hipError_t launch_bf_chmajor(hipStream_t s, const uint32_t* X, const uint32_t* W
                              uint64_t n_cols, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* xa_out);
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R);
+// Beamformer + power and covariance in one kernel over the FFT layout X; for
+// the geometries bfcov_fits accepts (at most 16 channels, bins a multiple of 4).
+bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_beams);
+hipError_t launch_bfcov(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_chan, uint32_t n_beams,
+                        uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
+                        uint32_t* xa_out, int64_t* R);
 // The Integrator frame by frame: mode 0 block, 1 exponential (shift exp_k),
 // saturating to acc_w (2..64) with sticky flags (nullable) per [bin][pair].
 hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,

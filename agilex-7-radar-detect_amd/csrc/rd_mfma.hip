@@ -791,4 +791,321 @@ hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const Cl
   return hipGetLastError();
 }
 
+// ============================================================================
+// Beamformer + power and covariance in one pass over the FFT output, for CPIs
+// of at most 16 channels. Both contractions read the same samples: the
+// beamformer contracts over channels per column (frame, bin), the covariance
+// over frames per bin. A block owns kFbNB adjacent bins over every frame and
+// walks the frames FS = 4 * WAVES at a time:
+//   * every thread loads the four channel words (4q .. 4q+3) of one column and
+//     writes their int8 digits once, as one 32-bit word per plane, into eight
+//     LDS digit planes [plane][column][16 channel bytes]:
+//       hr, lr, hi, li, ~hr, ~lr, ~hi, ~li
+//     (real/imaginary high and offset-low digits and their complements), so
+//     a plane row is the 16 channels of one column;
+//   * beamformer: wave w takes the 16 columns of frames 4w .. 4w+3; its B
+//     operand (8 channels of one column) is one 8-byte LDS read per plane;
+//   * covariance: wave w takes bin w % NB and the 16 frames 16 * (w / NB) of
+//     the step; its operands (8 frames of one channel) are the transposes of
+//     plane rows, read with ds_read_b64_tr_b8: in each 16-lane group, lane
+//     2r + h addresses half h (channels 8h .. 8h+7) of row r (frame r), and
+//     lane n receives channel n of the group's 8 rows;
+//   * the per-column channel sums (beamformer) and per-channel frame sums
+//     (covariance) of the digit identity come from extra MFMAs against an
+//     all-ones operand instead of VALU additions.
+// The covariance accumulators stay in registers over all frames (int32 is
+// exact to 65535 frames: a frame adds at most 2^15 to each); the waves of a
+// bin add theirs in LDS at the end and one wave forms the int64 results.
+// The aligned copy Xa is an optional output (16-byte stores of the staged
+// channel words), not an intermediate.
+// ============================================================================
+constexpr uint32_t kFbNB = 4;  // bins per block
+constexpr uint32_t kFbPlanes = 8;
+enum : uint32_t { kPHR = 0, kPLR, kPHI, kPLI, kPNHR, kPNLR, kPNHI, kPNLI };
+
+typedef int v2i_t __attribute__((ext_vector_type(2)));
+
+// LDS writes of one lane become visible to the other lanes of its wave.
+__device__ __forceinline__ void wave_lds_sync_mfma() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ds_read_b64_tr_b8 at LDS byte address p (8-byte aligned).
+__device__ __forceinline__ uint64_t lds_tr8(const unsigned char* p) {
+  const v2i_t r = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
+      (__attribute__((address_space(3))) v2i_t*)(__attribute__((address_space(3))) unsigned char*)p);
+  return (uint64_t)(uint32_t)r.x | ((uint64_t)(uint32_t)r.y << 32);
+}
+
+template <int WAVES>
+struct FbGeo {
+  static constexpr uint32_t FS = 4 * WAVES;                     // frames per step
+  static constexpr uint32_t COLS = FS * kFbNB;                  // columns per step
+  static constexpr uint32_t PLANE = COLS * 16;                  // bytes per plane
+  static constexpr uint32_t BUF = kFbPlanes * PLANE;            // one step's planes
+  static constexpr uint32_t RED = kFbNB * 36 * 64 * 4;          // covariance partials [bin][36][64] int32
+  static constexpr uint32_t STAGE = 2 * BUF > RED ? 2 * BUF : RED;  // double-buffered planes, aliased by RED
+  static size_t lds(uint32_t bt_n) {
+    return STAGE + bf_frag_words(bt_n, 1) * 4 + (size_t)bt_n * 32 * 4 + kFbNB * 16 * 4 * 4;
+  }
+};
+
+bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_beams) {
+  // 32-bit byte offsets from a per-step base cover every channel row
+  return n_chan >= 1 && n_chan <= 16 && n_beams >= 1 && n_beams <= 128 && n_bins % kFbNB == 0 && n_bins >= kFbNB &&
+         n_frames >= 1 && n_frames <= 65535 && (uint64_t)16 * n_frames * n_bins * 4 < (1ull << 32);
+}
+
+template <int WAVES, int OUT>
+__global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
+  using G = FbGeo<WAVES>;
+  constexpr uint32_t FS = G::FS;
+  extern __shared__ uint32_t lds[];
+  unsigned char* const sm = reinterpret_cast<unsigned char*>(lds);
+  uint32_t* frag = reinterpret_cast<uint32_t*>(sm + G::STAGE);
+  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(J.bf.bt_n, 1));
+  int32_t* sums = tail + J.bf.bt_n * 32;  // [bin][16 rows][4] digit sums of the covariance A rows
+  const int out = OUT >= 0 ? OUT : bf_out_mask(J.bf);
+  const uint32_t CH = J.bf.CH, B = J.bf.B, N = J.N, F = J.F;
+  const uint64_t cols = (uint64_t)F * N;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+
+  // XCD-aware tile order: consecutive blocks are dealt to the 8 XCDs in turn,
+  // so the blocks one XCD runs together take adjacent bins and share the L2
+  // lines of their loads and stores
+  const uint32_t nblk = gridDim.x;
+  const uint32_t tile = nblk % 8 == 0 ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+  const uint32_t k0 = tile * kFbNB;
+
+  // conversion unit of this thread: channel quad q, bin b, frame f of the step
+  const uint32_t t = threadIdx.x;
+  const uint32_t q = t & 3, cb = (t >> 2) & 3, cf = t >> 4;
+  uint32_t off[4];  // byte offsets of the unit's four channel words from the step base
+  uint32_t chmask = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    off[i] = ((4 * q + i) * (uint32_t)F * N + cf * N + cb) * 4u;
+    chmask |= 4 * q + i < CH ? 0xFFu << (8 * i) : 0u;
+  }
+  const uint32_t nsteps = (F + FS - 1) / FS;
+  auto fetch = [&](uint32_t st, uint32_t (&w)[4]) {
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(J.bf.X + (size_t)st * FS * N + k0);
+    const bool fok = st * FS + cf < F;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+      w[i] = (fok && 4 * q + i < CH) ? *reinterpret_cast<const uint32_t*>(base + off[i]) : 0u;
+  };
+  // the first step's loads are in flight while the block derives the weight digits
+  uint32_t w[4];
+  fetch(0, w);
+  bf_weights_to_lds(J.bf.W, CH, B, 16, 1, J.bf.bt_n, frag, tail);
+
+  // beamformer lane roles: column c = 16 * wave + cl (frame c / 4, bin c % 4),
+  // K rows: channel half grp & 1, real (grp < 2) or imaginary part
+  const uint32_t bc = 16 * wave + cl, bfr = bc >> 2, bbin = bc & 3;
+  const bool bim = grp >= 2;
+  const uint32_t boff = bc * 16 + 8 * (grp & 1);
+  const uint32_t p_bih = bim ? kPHI : kPHR, p_bil = bim ? kPLI : kPLR;
+  const uint32_t p_brh = bim ? kPNHI : kPHR, p_brl = bim ? kPNLI : kPLR;
+  const int32_t kconst_bf = 16384 * 2 * (int32_t)CH;
+  const uint64_t ONES = 0x0101010101010101ull;
+
+  // covariance lane roles: bin cbin, frames 16 * cks + 8 * (grp & 1) + (cl >> 1)
+  // (this lane's transpose-read row), channel half cl & 1
+  const uint32_t cbin = wave % kFbNB, cks = wave / kFbNB;
+  const uint32_t cfr = 16 * cks + 8 * (grp & 1) + (cl >> 1);
+  const uint32_t coff = (cfr * kFbNB + cbin) * 16 + 8 * (cl & 1);
+  const uint32_t p_reh = bim ? kPHI : kPHR, p_rel = bim ? kPLI : kPLR;
+  const uint32_t p_imh = bim ? kPNHR : kPHI, p_iml = bim ? kPNLR : kPLI;
+  // HH, HL, LL, LH of Re; the same of Im; acc[8] holds the frame sums of the
+  // digits of the A rows: column quarter z of row i sums operand z over the
+  // frames (z = Re high, Re low, Im high, Im low), since the all-ones B
+  // operand of the z-th sum product is nonzero only in quarter z
+  v4i acc[9];
+#pragma unroll
+  for (int z = 0; z < 9; ++z) acc[z] = v4i{0, 0, 0, 0};
+  const uint32_t cq = cl >> 2;
+
+  for (uint32_t st = 0; st < nsteps; ++st) {
+    unsigned char* P = sm + (st & 1) * G::BUF;
+    const uint32_t m0 = st * FS;
+    {  // digits of this thread's four channel words into the planes
+      const uint32_t bm = m0 + cf < F ? chmask : 0u;
+      const uint32_t hr = gather4(w[0], w[1], w[2], w[3], 1) & bm;
+      const uint32_t lr = (gather4(w[0], w[1], w[2], w[3], 0) ^ 0x80808080u) & bm;
+      const uint32_t hi = gather4(w[0], w[1], w[2], w[3], 3) & bm;
+      const uint32_t li = (gather4(w[0], w[1], w[2], w[3], 2) ^ 0x80808080u) & bm;
+      // plane word index of (column 4 * cf + cb, channels 4q..4q+3) is t
+      uint32_t* Pw = reinterpret_cast<uint32_t*>(P) + t;
+      constexpr uint32_t PW = G::PLANE / 4;
+      Pw[kPHR * PW] = hr;
+      Pw[kPLR * PW] = lr;
+      Pw[kPHI * PW] = hi;
+      Pw[kPLI * PW] = li;
+      Pw[kPNHR * PW] = ~hr & bm;
+      Pw[kPNLR * PW] = ~lr & bm;
+      Pw[kPNHI * PW] = ~hi & bm;
+      Pw[kPNLI * PW] = ~li & bm;
+      if (J.bf.xa_out && m0 + cf < F) {
+        uint32_t* xa = J.bf.xa_out + ((uint64_t)(m0 + cf) * N + k0 + cb) * CH + 4 * q;
+        if (CH == 16 && (reinterpret_cast<uintptr_t>(J.bf.xa_out) & 15u) == 0) {
+          *reinterpret_cast<uint4*>(xa) = uint4{w[0], w[1], w[2], w[3]};
+        } else {
+#pragma unroll
+          for (uint32_t i = 0; i < 4; ++i)
+            if (4 * q + i < CH) xa[i] = w[i];
+        }
+      }
+    }
+    if (st + 1 < nsteps) fetch(st + 1, w);
+    __syncthreads();  // the planes of this step are complete (and the step before last is consumed)
+
+    {  // beamformer: 16 columns x every beam tile
+      const uint32_t m = m0 + bfr;
+      const bool col_ok = m < F;
+      const uint64_t col = (uint64_t)m * N + k0 + bbin;
+      const uint64_t bih = *reinterpret_cast<const uint64_t*>(P + p_bih * G::PLANE + boff);
+      const uint64_t bil = *reinterpret_cast<const uint64_t*>(P + p_bil * G::PLANE + boff);
+      const uint64_t brh = *reinterpret_cast<const uint64_t*>(P + p_brh * G::PLANE + boff);
+      const uint64_t brl = *reinterpret_cast<const uint64_t*>(P + p_brl * G::PLANE + boff);
+      // column sums over the K rows: Im B = [Xr; Xi], Re B = [Xr; ~Xi]
+      const v4i z4 = {0, 0, 0, 0};
+      const v4i s_ih = mfma8(ONES, bih, z4), s_il = mfma8(ONES, bil, z4);
+      const v4i s_rh = mfma8(ONES, brh, z4), s_rl = mfma8(ONES, brl, z4);
+      const int32_t sb_im = 256 * s_ih[0] + s_il[0] + 256 * (int32_t)CH;
+      const int32_t sb_re = 256 * s_rh[0] + s_rl[0] + 256 * (int32_t)CH;
+      for (uint32_t bt = 0; bt < J.bf.bt_n; ++bt) {
+        const uint64_t* a = reinterpret_cast<const uint64_t*>(frag) + (size_t)bt * 256 + lane;
+        const uint64_t are_h = a[0], are_l = a[64], aim_h = a[128], aim_l = a[192];
+        v4i rhh = mfma8(are_h, brh, z4);
+        v4i rx = mfma8(are_h, brl, z4);
+        rx = mfma8(are_l, brh, rx);
+        v4i rll = mfma8(are_l, brl, z4);
+        v4i ihh = mfma8(aim_h, bih, z4);
+        v4i ix = mfma8(aim_h, bil, z4);
+        ix = mfma8(aim_l, bih, ix);
+        v4i ill = mfma8(aim_l, bil, z4);
+        if (!col_ok) continue;
+        const size_t o0 = (size_t)(bt * 16 + 4 * grp) * cols + col;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t b = bt * 16 + 4 * grp + r;
+          if (b >= B) continue;
+          const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
+          uint32_t fr = 0, fi = 0;
+          const int32_t yr = rne15_sat_small(rhh[r], rx[r], rll[r], sb_re << 7, (sa << 7) + wis - kconst_bf, fr);
+          const int32_t yi = rne15_sat_small(ihh[r], ix[r], ill[r], sb_im << 7, (sa << 7) - kconst_bf, fi);
+          const size_t o = o0 + (size_t)r * cols;
+          if (out & kBfOutY) J.bf.y[o] = cpack(yr, yi);
+          if (out & kBfOutF) J.bf.flags[o] = (uint8_t)(fr | (fi << 2));
+          if (out & kBfOutP) J.bf.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+        }
+      }
+    }
+    {  // covariance: one K step (16 frames) of bin cbin
+      const uint64_t reh = lds_tr8(P + p_reh * G::PLANE + coff);
+      const uint64_t rel = lds_tr8(P + p_rel * G::PLANE + coff);
+      const uint64_t imh = lds_tr8(P + p_imh * G::PLANE + coff);
+      const uint64_t iml = lds_tr8(P + p_iml * G::PLANE + coff);
+      acc[0] = mfma8(reh, reh, acc[0]);
+      acc[1] = mfma8(reh, rel, acc[1]);
+      acc[3] = mfma8(rel, reh, acc[3]);
+      acc[2] = mfma8(rel, rel, acc[2]);
+      acc[4] = mfma8(imh, reh, acc[4]);
+      acc[5] = mfma8(imh, rel, acc[5]);
+      acc[7] = mfma8(iml, reh, acc[7]);
+      acc[6] = mfma8(iml, rel, acc[6]);
+      acc[8] = mfma8(reh, cq == 0 ? ONES : 0ull, acc[8]);
+      acc[8] = mfma8(rel, cq == 1 ? ONES : 0ull, acc[8]);
+      acc[8] = mfma8(imh, cq == 2 ? ONES : 0ull, acc[8]);
+      acc[8] = mfma8(iml, cq == 3 ? ONES : 0ull, acc[8]);
+    }
+  }
+
+  // the waves of a bin add their partial sums: the first K step's wave
+  // stores, the others add with LDS atomics, then the first forms the results
+  __syncthreads();  // every wave is done with the planes (the area is reused)
+  int32_t* red = reinterpret_cast<int32_t*>(sm) + (size_t)cbin * 36 * 64 + lane;
+  if (cks == 0) {
+#pragma unroll
+    for (int z = 0; z < 9; ++z)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[(4 * z + r) * 64] = acc[z][r];
+  }
+  __syncthreads();
+  if (cks != 0) {
+#pragma unroll
+    for (int z = 0; z < 9; ++z)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) atomicAdd(&red[(4 * z + r) * 64], acc[z][r]);
+  }
+  __syncthreads();
+  if (cks != 0) return;
+#pragma unroll
+  for (int z = 0; z < 9; ++z)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[z][r] = red[(4 * z + r) * 64];
+
+  // frame sums of row i = 4 grp + r: T = sum_m (xr_i + xi_i) over the Re
+  // A row [Xr | Xi], U = sum_m (xi_i + ~xr_i) = sum xi - sum xr - F over the
+  // Im A row [Xi | ~Xr]; each digit entry of a valid frame carries +128
+  int32_t* sb = sums + (size_t)cbin * 64;
+  if ((cl & 3) == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sb[(4 * grp + r) * 4 + cq] = acc[8][r];
+  }
+  wave_lds_sync_mfma();
+  const int64_t Fl = F;
+  auto row_t = [&](uint32_t i) { return 256 * (int64_t)sb[4 * i] + sb[4 * i + 1] + 256 * Fl; };
+  auto row_u = [&](uint32_t i) { return 256 * (int64_t)sb[4 * i + 2] + sb[4 * i + 3] + 256 * Fl; };
+  const int64_t Tj = row_t(cl), Uj = row_u(cl);
+  const int64_t xij = (Tj + Uj + Fl) / 2;  // sum_m xi_j (exact: Tj + Uj + F = 2 sum xi_j)
+  const int64_t kconst = 16384LL * 2 * Fl;
+  const uint32_t k = k0 + cbin;
+  const size_t npairs = (size_t)CH * (CH + 1) / 2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t ii = 4 * grp + r, jj = cl;
+    if (ii >= CH || jj >= CH || ii > jj) continue;
+    const int64_t re =
+        combine(acc[0], acc[1], acc[2], r) + 256 * (int64_t)acc[3][r] + 128 * (row_t(ii) + Tj) - kconst;
+    const int64_t im =
+        combine(acc[4], acc[5], acc[6], r) + 256 * (int64_t)acc[7][r] + 128 * (row_u(ii) + Tj) - kconst + xij;
+    const size_t pair = (size_t)ii * CH - (size_t)ii * (ii - 1) / 2 + (jj - ii);
+    int64_t* o = J.R + ((size_t)k * npairs + pair) * 2;
+    o[0] = re;
+    o[1] = im;
+  }
+}
+
+template <int WAVES>
+static hipError_t launch_bfcov_w(hipStream_t s, const BfCovJob& J) {
+  const size_t lds = FbGeo<WAVES>::lds(J.bf.bt_n);
+  const dim3 grid(J.N / kFbNB);
+  const int out = bf_out_mask(J.bf);
+  if (out == kBfOutP)
+    bfcov_kernel<WAVES, kBfOutP><<<grid, 64 * WAVES, lds, s>>>(J);
+  else if (out == (kBfOutY | kBfOutF | kBfOutP))
+    bfcov_kernel<WAVES, kBfOutY | kBfOutF | kBfOutP><<<grid, 64 * WAVES, lds, s>>>(J);
+  else
+    bfcov_kernel<WAVES, -1><<<grid, 64 * WAVES, lds, s>>>(J);
+  return hipGetLastError();
+}
+
+hipError_t launch_bfcov(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_chan, uint32_t n_beams,
+                        uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
+                        uint32_t* xa_out, int64_t* R) {
+  if (!bfcov_fits(n_chan, n_bins, n_frames, n_beams) || !R) return hipErrorInvalidValue;
+  BfCovJob J{};
+  J.bf = BfJob{X, W, y, flags, power, nullptr, xa_out, (uint64_t)n_frames * n_bins, n_chan, n_beams,
+               (n_beams + 15) / 16, 0};
+  J.R = R;
+  J.N = n_bins;
+  J.F = n_frames;
+  return launch_bfcov_w<16>(s, J);
+}
+
 }  // namespace rd

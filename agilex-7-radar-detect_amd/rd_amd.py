@@ -23,7 +23,7 @@ RD_STATIC_MAX_ROWS = RD_STATIC_SCRATCH_BYTES // 8 - 1
 EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
-    "rd_bf_run", "rd_bf_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
+    "rd_bf_run", "rd_bf_stage", "rd_bf_cov_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
     "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
     "rd_chain_enqueue_static", "rd_bf_plain_run", "rd_cov_plain_run", "rd_chain_enqueue_plain", "rd_cfar_enqueue",
@@ -125,6 +125,7 @@ class Lib:
         L.rd_align_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_bf_run.argtypes = [vp, vp, vp, u32, u32, u64, vp, vp, vp, vp]
         L.rd_bf_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp]
+        L.rd_bf_cov_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]
         L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
                                   C.POINTER(u64)]
@@ -366,6 +367,26 @@ class Ctx:
                                                          da.ptr))
         return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
                 da.download(np.uint32, CH * cols))
+
+    def bf_cov_stage(self, X, W, CH, B, F, N, outputs=("y", "flags", "power", "Xa")):
+        """rd_bf_cov_stage on the FFT layout; returns (y, flags, power, Xa, R),
+        None for an output not in `outputs` (passed as NULL)."""
+        X = np.asarray(X, np.uint32)
+        cols = F * N
+        dX, dW = self.put(X), self.put(np.asarray(W, np.uint32))
+        dy = self.buf(4 * B * cols) if "y" in outputs else None
+        df = self.buf(B * cols) if "flags" in outputs else None
+        dp = self.buf(4 * B * cols) if "power" in outputs else None
+        da = self.buf(4 * CH * cols) if "Xa" in outputs else None
+        dR = self.buf(16 * N * CH * (CH + 1) // 2)
+        ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
+        self.lib.check("rd_bf_cov_stage", self.L.rd_bf_cov_stage(self.h, dX.ptr, dW.ptr, CH, B, F, N, ptr(dy),
+                                                                 ptr(df), ptr(dp), ptr(da), dR.ptr))
+        return (dy.download(np.uint32, B * cols) if dy is not None else None,
+                df.download(np.uint8, B * cols) if df is not None else None,
+                dp.download(np.uint32, B * cols) if dp is not None else None,
+                da.download(np.uint32, CH * cols) if da is not None else None,
+                dR.download(np.int64, N * CH * (CH + 1)))
 
     def bf_plain(self, Xa, W, CH, B):
         """rd_bf_plain_run -> (y, flags, power)."""
@@ -652,7 +673,7 @@ class Chain:
         if iq_ptr is not None:
             bufs.iq = iq_ptr
         if lean:
-            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = None
+            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = bufs.Xa = None
         fn = "rd_chain_enqueue_static" if static else ("rd_chain_enqueue_plain" if plain else "rd_chain_enqueue")
         rc = getattr(self.ctx.L, fn)(self.ctx.h, C.byref(self.desc), C.byref(bufs),
                                      total_ptr if total_ptr is not None else self.total.ptr)
@@ -660,15 +681,15 @@ class Chain:
 
     def graph(self, iq_ptr=None, total_ptr=None, lean=False) -> "Graph":
         """The whole chain for this CPI geometry captured as one HIP graph.
-        lean: the optional per-stage outputs (PFB/FFT/beamformer flags and the
-        beam samples y) are not written; every stage still runs and the
-        intermediates the next stage reads, the covariance and the events are
-        produced."""
+        lean: the optional per-stage outputs (PFB/FFT/beamformer flags, the
+        beam samples y and the aligned copy Xa) are not written; every stage
+        still runs and the intermediates the next stage reads, the covariance
+        and the events are produced."""
         bufs = CpiBufs.from_buffer_copy(self.bufs)
         if iq_ptr is not None:
             bufs.iq = iq_ptr
         if lean:
-            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = None
+            bufs.pfb_flags = bufs.fft_flags = bufs.y = bufs.yflags = bufs.Xa = None
         g = C.c_void_p()
         rc = self.ctx.L.rd_chain_graph(self.ctx.h, C.byref(self.desc), C.byref(bufs),
                                        total_ptr if total_ptr is not None else self.total.ptr, C.byref(g))

@@ -114,7 +114,10 @@ typedef struct {
  *   pfb_flags [n_chan][n_frames][n_fft]     nullable
  *   X         [n_chan][n_frames][n_fft]     FFT bins, natural order (required)
  *   fft_flags [n_chan][n_frames]            stage flags, sub-stage g at bits [2g+1:2g]; nullable
- *   Xa        [n_frames][n_fft][n_chan]     aligned (bin-major, antenna-minor) (required)
+ *   Xa        [n_frames][n_fft][n_chan]     aligned (bin-major, antenna-minor); nullable in the
+ *             single-CPI chain (written when set; when null the context keeps its own copy
+ *             where a path needs one, and rd_chain_enqueue_static then needs n_chan <= 16 and
+ *             n_fft a multiple of 4 or no covariance); required by rd_batch_create
  *   y         [n_beams][n_frames][n_fft]    beam samples; nullable
  *   yflags    [n_beams][n_frames][n_fft]    nullable
  *   power     [n_beams][n_frames][n_fft]    |y|^2 as uint32 (exact, <= 2^31) (required)
@@ -205,6 +208,18 @@ rd_status rd_bf_run(rd_ctx* ctx, const uint32_t* Xa, const uint32_t* W, uint32_t
  * copy Xa [n_frames][n_bins][n_ant] (nullable) from the same staged tiles. */
 rd_status rd_bf_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                       uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power, uint32_t* Xa);
+
+/* The chain's beamformer and covariance together: the values of rd_bf_stage
+ * and of rd_cov_run on its aligned copy (BeamformerModel::step,
+ * beamformer_model.hpp:145-325; covar::power and the Integrator's block-mode
+ * window sum, covar_model.hpp:99-151, 186-262), read once from X
+ * [n_ant][n_frames][n_bins]. One kernel for n_ant <= 16 with n_bins a
+ * multiple of 4 (Xa nullable: written when set); otherwise the two stages,
+ * and then Xa is required. R [n_bins][n_ant*(n_ant+1)/2][2] is required;
+ * y/flags/power are nullable. */
+rd_status rd_bf_cov_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
+                          uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
+                          uint32_t* Xa, int64_t* R);
 
 /* Covariance: R[bin][pair i<=j] = sum over frames of covar::cross_power(X_i, X_j)
  * (covar_model.hpp:137-151), integrated by a block-mode covar::Integrator with
@@ -334,7 +349,11 @@ rd_status rd_cfar_enqueue(rd_ctx* ctx, const void* cells, uint32_t cell_bytes, u
 
 /* The composed chain for one CPI: PFB -> FFT -> alignment -> beamformer+power ->
  * covariance -> CFAR -> event packing, every stage as above, all on the
- * context's stream. *n_events_host (synchronous) receives the event total. */
+ * context's stream. *n_events_host (synchronous) receives the event total.
+ * With at most 16 channels, n_fft a multiple of 4 and R set, the beamformer
+ * and the covariance are one kernel reading X once (the aligned copy Xa is
+ * then only an optional output); otherwise the beamformer writes Xa and the
+ * covariance reads it. Results are the same bits either way. */
 rd_status rd_chain_run(rd_ctx* ctx, const rd_cpi_desc* desc, const rd_cpi_bufs* bufs, uint64_t* n_events_host);
 
 /* Same as rd_chain_run without the final synchronisation: the event total is
