@@ -88,6 +88,7 @@ struct rd_batch {
   uint32_t n_cu = 0;
   void* cfar_scratch = nullptr;
   void* scan_jobs = nullptr;
+  uint32_t max_rows = 0;  // CFAR rows of the largest CPI
   uint64_t* totals_dev = nullptr;
   uint64_t* totals_host = nullptr;
   hipStream_t side = nullptr;
@@ -226,7 +227,7 @@ rd_status enqueue_all(const rd_batch& B) {
   }
   for (const Class& c : B.cls)
     if (c.stage == S_CFAR_ROWS) RD_TRY(launch_class(B, c, s));
-  RD_TRY(rd::launch_cfar_scan_batch(s, B.scan_jobs, B.n));
+  RD_TRY(rd::launch_cfar_scan_batch(s, B.scan_jobs, B.n, B.max_rows));
   for (const Class& c : B.cls)
     if (c.stage == S_CFAR_GATHER) RD_TRY(launch_class(B, c, s));
   if (any_cov) RD_TRY(hipStreamWaitEvent(s, B.ev_join, 0));
@@ -418,6 +419,7 @@ rd_status build(rd_batch& B, uint32_t flags) {
     const rd_cpi_desc& d = B.d[i];
     const rd_cpi_bufs& b = B.b[i];
     const uint32_t rows = d.n_beams * d.n_frames;
+    B.max_rows = rows > B.max_rows ? rows : B.max_rows;
     uint64_t* offs = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(B.cfar_scratch) + off[i]);
     B.cfar[i] = rd::CfarJob{b.power, nullptr, offs + rows + 1, b.row_count, offs, b.events, B.totals_dev + i,
                             b.events ? b.events_cap : 0, rd::cfar_geom(d.cfar, d.max_guard, d.max_ref), d.n_fft,

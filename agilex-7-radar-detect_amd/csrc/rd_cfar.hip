@@ -433,29 +433,32 @@ __device__ __forceinline__ void cfar_gather_tile(const CfarJob& J, const CfarRow
 }
 
 // Pass 2, one block of 1024: exclusive scan of the row counts into 64-bit
-// offsets; offs[n_rows] = total. Passes of up to 1024 * kScanPer rows: each
-// thread takes a contiguous run of at most kScanPer rows, all its loads issued
-// at once and kept in registers, the waves scan their run sums with shuffles,
-// and one barrier combines the 16 wave totals.
-constexpr uint32_t kScanPer = 16;
+// offsets; offs[n_rows] = total. Two forms.
+// Direct, for at most kScanDirectRows rows (one pass, no LDS staging; more
+// rows take more passes): each thread takes a contiguous run of at most
+// kScanDirectPer rows, its loads issued at once and kept in registers, the
+// waves scan their run sums with shuffles, and one barrier combines the 16
+// wave totals.
+constexpr uint32_t kScanDirectPer = 4;
+constexpr uint32_t kScanDirectRows = 1024 * kScanDirectPer;
 
-__device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
+__device__ __forceinline__ void cfar_scan_direct(const CfarJob& J) {
   __shared__ uint64_t wsum[16];
   const uint32_t n_rows = J.n_rows;
   const uint32_t* __restrict__ row_count = J.row_count;
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
   uint64_t carry = 0;
-  for (uint32_t c0 = 0; c0 < n_rows; c0 += 1024 * kScanPer) {
-    const uint32_t rows = min(n_rows - c0, 1024u * kScanPer);
+  for (uint32_t c0 = 0; c0 < n_rows; c0 += kScanDirectRows) {
+    const uint32_t rows = min(n_rows - c0, kScanDirectRows);
     const uint32_t per = (rows + 1023) / 1024;
     const uint32_t r0 = c0 + t * per;
     const uint32_t lim = r0 < c0 + rows ? min(per, c0 + rows - r0) : 0u;  // rows of this thread
     const uint32_t* rc = row_count + r0;
     uint64_t* of = J.offs + r0;
-    uint32_t v[kScanPer];
+    uint32_t v[kScanDirectPer];
     uint64_t s = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < kScanPer; ++q) {
+    for (uint32_t q = 0; q < kScanDirectPer; ++q) {
       v[q] = q < lim ? rc[q] : 0u;
       s += v[q];
     }
@@ -471,12 +474,85 @@ __device__ __forceinline__ void cfar_scan_block(const CfarJob& J) {
     }
     uint64_t run = carry + wb + x - s;
 #pragma unroll
-    for (uint32_t q = 0; q < kScanPer; ++q) {
+    for (uint32_t q = 0; q < kScanDirectPer; ++q) {
       if (q < lim) of[q] = run;
       run += v[q];
     }
     carry += all;
     __syncthreads();  // wsum is rewritten by the next pass
+  }
+  if (t == 0) {
+    J.offs[n_rows] = carry;
+    if (J.total) *J.total = carry;
+  }
+}
+
+// Staged, for more rows: passes of kScanRows rows, every global access
+// coalesced: the counts go through LDS (thread t loads rows t, t +
+// 1024, ...), each thread then scans a contiguous run of kScanPer of them
+// (LDS index i + i / kScanPer: one padding word per run keeps the run reads
+// free of bank conflicts), the waves scan their run sums with shuffles, one
+// barrier combines the 16 wave totals, and the offsets return through LDS to
+// coalesced stores. The next pass's counts are loaded before this pass's
+// stores.
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanRows = 1024 * kScanPer;
+constexpr uint32_t kScanPad = kScanRows + kScanRows / kScanPer;  // padded words per array
+
+__device__ __forceinline__ void cfar_scan_staged(const CfarJob& J) {
+  __shared__ uint64_t wsum[16];
+  __shared__ uint32_t cnt[kScanPad];
+  __shared__ uint64_t off[kScanPad];
+  const uint32_t n_rows = J.n_rows;
+  const uint32_t* __restrict__ row_count = J.row_count;
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  auto pad = [](uint32_t i) { return i + i / kScanPer; };
+  uint32_t ld[kScanPer];
+  auto fetch = [&](uint32_t c0) {
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+      const uint32_t r = c0 + t + 1024 * j;
+      ld[j] = r < n_rows ? row_count[r] : 0u;
+    }
+  };
+  uint64_t carry = 0;
+  fetch(0);
+  for (uint32_t c0 = 0; c0 < n_rows; c0 += kScanRows) {
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) cnt[pad(t + 1024 * j)] = ld[j];
+    __syncthreads();
+    if (c0 + kScanRows < n_rows) fetch(c0 + kScanRows);  // in flight during this pass
+    uint32_t v[kScanPer];
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+      v[q] = cnt[pad(kScanPer * t + q)];
+      sum += v[q];
+    }
+    const uint64_t x = wave_incscan(sum, lane);
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint64_t wb = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 16; ++w) {
+      const uint64_t ws = wsum[w];
+      wb += w < wave ? ws : 0;
+      all += ws;
+    }
+    uint64_t run = carry + wb + x - sum;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPer; ++q) {
+      off[pad(kScanPer * t + q)] = run;
+      run += v[q];
+    }
+    carry += all;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+      const uint32_t r = c0 + t + 1024 * j;
+      if (r < n_rows) J.offs[r] = off[pad(t + 1024 * j)];
+    }
+    __syncthreads();  // cnt, off and wsum are rewritten by the next pass
   }
   if (t == 0) {
     J.offs[n_rows] = carry;
@@ -492,7 +568,22 @@ __global__ __launch_bounds__(512) void cfar_rows_kernel(CfarJob J, CfarRowGeo G)
   cfar_rows_tile<Cell, MULTI, SLOG>(J, G, blockIdx.x * R, lds);
 }
 
-__global__ __launch_bounds__(1024) void cfar_scan_kernel(CfarJob J) { cfar_scan_block(J); }
+// STAGED selects the form; the host picks it by the row count (the staged
+// form's LDS and extra barriers only slow a one-pass scan down).
+template <bool STAGED>
+__global__ __launch_bounds__(1024) void cfar_scan_kernel(CfarJob J) {
+  if (STAGED)
+    cfar_scan_staged(J);
+  else
+    cfar_scan_direct(J);
+}
+
+static void launch_cfar_scan(hipStream_t s, const CfarJob& J) {
+  if (J.n_rows > kScanDirectRows)
+    cfar_scan_kernel<true><<<1, 1024, 0, s>>>(J);
+  else
+    cfar_scan_kernel<false><<<1, 1024, 0, s>>>(J);
+}
 
 template <typename Cell, bool MULTI, uint32_t SLOG>
 __global__ __launch_bounds__(512) void cfar_gather_kernel(CfarJob J, CfarRowGeo G) {
@@ -520,9 +611,13 @@ __global__ __launch_bounds__(512) void cfar_batch_kernel(const CfarJob* __restri
 }
 
 // Batched scan: one block per CPI.
+template <bool STAGED>
 __global__ __launch_bounds__(1024) void cfar_scan_batch_kernel(const CfarJob* __restrict__ jobs) {
   const CfarJob J = jobs[blockIdx.x];
-  cfar_scan_block(J);
+  if (STAGED)
+    cfar_scan_staged(J);
+  else
+    cfar_scan_direct(J);
 }
 
 // Staging bytes a single-CPI launch asks for at most; with more rows the slot
@@ -592,7 +687,7 @@ static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, uint32_t W) 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
+  launch_cfar_scan(s, J);
   e = hipGetLastError();
   if (e != hipSuccess || J.cap == 0) return e;
   {
@@ -623,7 +718,7 @@ hipError_t launch_cfar(hipStream_t s, const void* cells, uint32_t cell_bytes, ui
                   cfar_geom(cfg, max_guard, max_ref), n, n_rows, frame_mod, cfar_slot_for(n, n_rows, scratch_bytes),
                   0};
   if (n_rows == 0) {  // no rows: the scan alone writes the zero total
-    cfar_scan_kernel<<<1, 1024, 0, s>>>(J);
+    launch_cfar_scan(s, J);
     return hipGetLastError();
   }
   const uint32_t W = cfar_single_waves(n, n_rows);
@@ -661,9 +756,12 @@ hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_
   return launch_cfar_batch(s, n, W, rpb, L, false);
 }
 
-hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs) {
+hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs, uint32_t max_rows) {
   if (n_jobs == 0) return hipSuccess;
-  cfar_scan_batch_kernel<<<n_jobs, 1024, 0, s>>>(static_cast<const CfarJob*>(jobs));
+  if (max_rows > kScanDirectRows)
+    cfar_scan_batch_kernel<true><<<n_jobs, 1024, 0, s>>>(static_cast<const CfarJob*>(jobs));
+  else
+    cfar_scan_batch_kernel<false><<<n_jobs, 1024, 0, s>>>(static_cast<const CfarJob*>(jobs));
   return hipGetLastError();
 }
 

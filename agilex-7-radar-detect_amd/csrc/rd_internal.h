@@ -233,7 +233,7 @@ hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const Cl
 uint32_t cfar_min_waves(uint32_t n);
 uint32_t cfar_rows_per_tile_max(uint32_t n, uint32_t W);
 hipError_t launch_cfar_rows_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L);
-hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs);
+hipError_t launch_cfar_scan_batch(hipStream_t s, const void* jobs, uint32_t n_jobs, uint32_t max_rows);
 hipError_t launch_cfar_gather_batch(hipStream_t s, uint32_t n, uint32_t W, uint32_t rpb, const ClassLaunch& L);
 
 }  // namespace rd
