@@ -495,6 +495,7 @@ rd_status rd_batch_create(rd_ctx* ctx, uint32_t n_cpi, const rd_cpi_desc* descs,
   if (n_cpi == 0 || n_cpi > 65535 || !descs || !bufs || (flags & ~RD_BATCH_AUTOTUNE)) return RD_E_ARG;
   for (uint32_t i = 0; i < n_cpi; ++i) {
     st = rd::check_cpi(&descs[i]);
+    if (st == RD_OK && descs[i].x_layout != 0) st = RD_E_ARG;  // a batch keeps X in natural order
     if (st != RD_OK) return st;
     st = rd::check_cpi_bufs(&bufs[i]);
     if (st != RD_OK) return st;

@@ -123,6 +123,7 @@ rd_status check_cpi(const rd_cpi_desc* d) {
   if (d->taps < 1 || d->taps > 64 || d->n_frames < 1 || d->n_frames > 65535) return RD_E_ARG;
   if (!pow2(d->n_fft) || d->n_fft < 8 || d->n_fft > 4096 || (d->spc != 1 && d->spc != 2)) return RD_E_GEOM;
   if (d->n_fft / d->spc < 4) return RD_E_GEOM;
+  if (d->x_layout > 1) return RD_E_ARG;
   return RD_OK;
 }
 
@@ -262,6 +263,17 @@ rd_status rd_fft_run(rd_ctx* c, const uint32_t* in, uint32_t n_fft, uint32_t spc
                             stage_flags));
 }
 
+rd_status rd_fft_stage(rd_ctx* c, const uint32_t* u, uint32_t n_fft, uint32_t spc, uint32_t scale_sched,
+                       uint32_t n_chan, uint32_t n_frames, uint32_t x_layout, uint32_t* X, uint32_t* stage_flags) {
+  rd_status st = enter(c);
+  if (st != RD_OK) return st;
+  if (!u || !X || ((uintptr_t)u | (uintptr_t)X) & 15u || x_layout > 1) return RD_E_ARG;
+  if (!pow2(n_fft) || n_fft < 8 || n_fft > 4096 || (spc != 1 && spc != 2) || n_fft / spc < 4) return RD_E_GEOM;
+  if ((uint64_t)n_chan * n_frames > 0xFFFFFFFFull) return RD_E_ARG;
+  return fin(rd::launch_fft(c->stream, c->tw, u, n_fft, spc, scale_sched, 1, n_chan * n_frames, nullptr, X,
+                            stage_flags, x_layout ? n_frames : 0));
+}
+
 rd_status rd_align_run(rd_ctx* c, const uint32_t* X, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                        uint32_t* Xa) {
   rd_status st = enter(c);
@@ -290,13 +302,16 @@ rd_status rd_bf_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t 
 
 rd_status rd_bf_cov_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                           uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
-                          uint32_t* Xa, int64_t* R) {
+                          uint32_t* Xa, int64_t* R, uint32_t x_layout) {
   rd_status st = enter(c);
   if (st != RD_OK) return st;
-  if (!X || !W || !R || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128 || n_frames > 65535) return RD_E_ARG;
+  if (!X || !W || !R || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128 || n_frames > 65535 ||
+      x_layout > 1)
+    return RD_E_ARG;
   if (rd::bfcov_fits(n_ant, n_bins, n_frames, n_beams))
-    return fin(rd::launch_bfcov(c->stream, X, W, n_ant, n_beams, n_frames, n_bins, y, flags, power, Xa, R));
-  if (!Xa) return RD_E_ARG;
+    return fin(rd::launch_bfcov(c->stream, X, W, n_ant, n_beams, n_frames, n_bins, y, flags, power, Xa, R,
+                                x_layout == 1));
+  if (!Xa || x_layout != 0) return RD_E_ARG;  // the two-kernel form reads natural order
   RD_TRY(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
   return fin(rd::launch_cov(c->stream, Xa, n_ant, n_frames, n_bins, R));
 }
@@ -485,6 +500,11 @@ bool chain_needs_xa(const rd_cpi_desc* d, const rd_cpi_bufs* b, bool plain) {
   return plain || (b->R && !chain_fused(d, b));
 }
 
+// The blocked X layout is an intermediate of the fused path only.
+rd_status check_x_layout(const rd_cpi_desc* d, const rd_cpi_bufs* b, bool plain) {
+  return d->x_layout == 0 || (!plain && chain_fused(d, b)) ? RD_OK : RD_E_ARG;
+}
+
 // Validates a CPI and sizes the context's scratch for it (no launches).
 rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, bool plain = false) {
   rd_status st = enter(c);
@@ -492,6 +512,8 @@ rd_status chain_prepare(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, b
   st = rd::check_cpi(d);
   if (st != RD_OK) return st;
   st = rd::check_cpi_bufs(b);
+  if (st != RD_OK) return st;
+  st = check_x_layout(d, b, plain);
   if (st != RD_OK) return st;
   st = grow(c, &c->prep, &c->prep_bytes, rd::bf_prep_bytes(d->n_chan, d->n_beams));
   if (st != RD_OK) return st;
@@ -510,10 +532,12 @@ rd_status chain_launch(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_bufs* b, ui
   const uint64_t cols = (uint64_t)F * N;
   hipStream_t s = c->stream;
   RD_TRY(rd::launch_pfb(s, b->iq, b->taps, N, d->taps, CH, F, nullptr, b->u, b->pfb_flags));
-  RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags));
+  const bool xb = d->x_layout == 1;  // (validated: the fused path)
+  RD_TRY(rd::launch_fft(s, c->tw, b->u, N, d->spc, d->scale_sched, 1, CH * F, nullptr, b->X, b->fft_flags,
+                        xb ? F : 0));
   if (chain_fused(d, b)) {
     // beamformer + power and covariance from one read of X
-    RD_TRY(rd::launch_bfcov(s, b->X, b->weights, CH, B, F, N, b->y, b->yflags, b->power, b->Xa, b->R));
+    RD_TRY(rd::launch_bfcov(s, b->X, b->weights, CH, B, F, N, b->y, b->yflags, b->power, b->Xa, b->R, xb));
   } else {
     // beamformer + power on the FFT layout; the same staged tiles give the
     // aligned copy the covariance reads
@@ -582,6 +606,8 @@ rd_status rd_chain_enqueue_static(rd_ctx* c, const rd_cpi_desc* d, const rd_cpi_
   st = rd::check_cpi(d);
   if (st != RD_OK) return st;
   st = rd::check_cpi_bufs(b);
+  if (st != RD_OK) return st;
+  st = check_x_layout(d, b, false);
   if (st != RD_OK) return st;
   if (!c->static_scratch) return RD_E_HIP;
   // the row offsets must fit; staged events take what is left (rows with

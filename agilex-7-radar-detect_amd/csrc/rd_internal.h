@@ -60,6 +60,7 @@ struct FftJob {
   uint32_t* bins;
   uint32_t* stage_flags;
   uint32_t sched, reorder, n_frames, tile0;
+  uint32_t bins_fpc;  // 0: bins in natural order; else bin-quad blocked with this many frames per channel
 };
 
 // Beamformer of one CPI: tiles = ceil(cols / TC).
@@ -88,6 +89,7 @@ struct BfCovJob {
   BfJob bf;
   int64_t* R;
   uint32_t N, F;
+  uint32_t x_blocked;  // X bin-quad blocked [ch][N/4][F][4] instead of [ch][F][N]
 };
 
 // CFAR geometry after clamping to the elaboration maxima.
@@ -115,9 +117,12 @@ CfarGeom cfar_geom(const rd_cfar_cfg& cfg, uint32_t max_guard, uint32_t max_ref)
 // Streaming stages (rd_stream.hip)
 hipError_t launch_pfb(hipStream_t s, const uint32_t* x, const uint32_t* coeff, uint32_t P, uint32_t T,
                       uint32_t n_chan, uint32_t n_beats, uint32_t* hist, uint32_t* y, uint8_t* flags);
+// bins_fpc > 0 writes `bins` bin-quad blocked: frame f = ch * bins_fpc + m,
+// bin k at [ch][k / 4][m][k % 4] (the chain's layout for the fused beamformer
+// + covariance); `out` stays natural.
 hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t n, uint32_t spc,
                       uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
-                      uint32_t* stage_flags);
+                      uint32_t* stage_flags, uint32_t bins_fpc = 0);
 hipError_t launch_align(hipStream_t s, const uint32_t* X, uint32_t n_chan, uint64_t cols, uint32_t* Xa);
 // Slow-time transform of the last H frames of a ring [rows][depth][n_bins]
 // (newest frame in slot `newest`) -> out [rows][H][n_bins], stage_flags
@@ -142,7 +147,7 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
 bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_beams);
 hipError_t launch_bfcov(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_chan, uint32_t n_beams,
                         uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
-                        uint32_t* xa_out, int64_t* R);
+                        uint32_t* xa_out, int64_t* R, bool x_blocked = false);
 // The Integrator frame by frame: mode 0 block, 1 exponential (shift exp_k),
 // saturating to acc_w (2..64) with sticky flags (nullable) per [bin][pair].
 hipError_t launch_cov_integrate(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames,

@@ -883,16 +883,22 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   // conversion unit of this thread: channel quad q, bin b, frame f of the step
   const uint32_t t = threadIdx.x;
   const uint32_t q = t & 3, cb = (t >> 2) & 3, cf = t >> 4;
+  // X natural [ch][F][N]: a step of frames is FS * N words on from the last
+  // and a frame N words; bin-quad blocked [ch][N/4][F][4]: the block's 4 bins
+  // of every frame are one contiguous run per channel (4 words a frame)
+  const bool xb = J.x_blocked != 0;
+  const uint32_t fstride = xb ? 4u : N;
   uint32_t off[4];  // byte offsets of the unit's four channel words from the step base
   uint32_t chmask = 0;
 #pragma unroll
   for (uint32_t i = 0; i < 4; ++i) {
-    off[i] = ((4 * q + i) * (uint32_t)F * N + cf * N + cb) * 4u;
+    off[i] = ((4 * q + i) * (uint32_t)F * N + cf * fstride + cb) * 4u;
     chmask |= 4 * q + i < CH ? 0xFFu << (8 * i) : 0u;
   }
   const uint32_t nsteps = (F + FS - 1) / FS;
+  const uint32_t* const X0 = J.bf.X + (xb ? (size_t)k0 * F : (size_t)k0);  // k0 is a multiple of 4
   auto fetch = [&](uint32_t st, uint32_t (&w)[4]) {
-    const unsigned char* base = reinterpret_cast<const unsigned char*>(J.bf.X + (size_t)st * FS * N + k0);
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(X0 + (size_t)st * FS * fstride);
     const bool fok = st * FS + cf < F;
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i)
@@ -1097,9 +1103,10 @@ static hipError_t launch_bfcov_w(hipStream_t s, const BfCovJob& J) {
 
 hipError_t launch_bfcov(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_chan, uint32_t n_beams,
                         uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
-                        uint32_t* xa_out, int64_t* R) {
+                        uint32_t* xa_out, int64_t* R, bool x_blocked) {
   if (!bfcov_fits(n_chan, n_bins, n_frames, n_beams) || !R) return hipErrorInvalidValue;
   BfCovJob J{};
+  J.x_blocked = x_blocked ? 1u : 0u;
   J.bf = BfJob{X, W, y, flags, power, nullptr, xa_out, (uint64_t)n_frames * n_bins, n_chan, n_beams,
                (n_beams + 15) / 16, 0};
   J.R = R;

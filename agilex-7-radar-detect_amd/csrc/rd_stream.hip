@@ -670,6 +670,23 @@ __device__ __forceinline__ void fft_stage_twiddles(const uint32_t* __restrict__ 
 
 // One tile: frames [grp*FPB, +FPB) of job J, twiddles already in LDS. Every
 // thread of the block calls it (it holds block barriers).
+// Word offset of bin k of frame `frame` in the `bins` output: natural
+// frame * N + k, or with J.bins_fpc set, bin-quad blocked [ch][k / 4][m][k % 4]
+// for frame = ch * bins_fpc + m; `base` is the frame's part, bins_at(base, k)
+// adds the bin's (four consecutive bins from a multiple of 4 stay contiguous).
+struct BinsAt {
+  size_t base;
+  uint32_t quad;  // words per bin quad: 4 * frames per channel, or 0 (natural)
+  __device__ __forceinline__ size_t at(uint32_t k) const {
+    return quad ? base + (size_t)(k >> 2) * quad + (k & 3) : base + k;
+  }
+};
+__device__ __forceinline__ BinsAt bins_at(const FftJob& J, uint32_t frame, uint32_t N) {
+  if (!J.bins_fpc) return BinsAt{(size_t)frame * N, 0};
+  const uint32_t ch = frame / J.bins_fpc, m = frame - ch * J.bins_fpc;
+  return BinsAt{(size_t)ch * J.bins_fpc * N + 4 * m, 4 * J.bins_fpc};
+}
+
 template <int LOG2N, int SPC, bool ALL1, bool FL>
 __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t* smem) {
   using G = FftGeo<LOG2N>;
@@ -847,12 +864,13 @@ __device__ __forceinline__ void fft_tile(const FftJob& J, uint32_t grp, uint32_t
   // v holds the beats in beat order. Natural bin k: spc = 1 -> beats[bitrev(k)];
   // spc = 2 -> k = bitrev(j) + q*m reads beats[2j + q].
   const size_t fo = (size_t)frame * N;
+  const BinsAt bo = bins_at(J, frame, N);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const uint32_t k = g + r * TPF;
     if (bins) {
       const uint32_t src = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
-      bins[fo + k] = v[pad(src)];
+      bins[bo.at(k)] = v[pad(src)];
     }
     if (out) {
       uint32_t src = k;
@@ -1150,11 +1168,12 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
   const size_t fo = (size_t)frame * N;
   constexpr uint32_t Q = SPC == 1 ? N / 4 : M / 2;
   if (bins) {
+    const BinsAt bo = bins_at(J, frame, N);
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {
       const uint32_t k = 4 * (g + t * TPF);
       const uint32_t b = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
-      *reinterpret_cast<uint4*>(bins + fo + k) = lds_quad<2 * Q, Q, 3 * Q>(v, b);
+      *reinterpret_cast<uint4*>(bins + bo.at(k)) = lds_quad<2 * Q, Q, 3 * Q>(v, b);
     }
   }
   if (out) {
@@ -1193,7 +1212,11 @@ __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>
     FftJob J, const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
   fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
-  fft_tile_any<LOG2N, SPC, ALL1, FL>(J, blockIdx.x, smem);
+  // blocked bins: consecutive blocks are dealt to the 8 XCDs in turn, so
+  // adjacent frames (which share the output's cache lines) go to one XCD
+  const uint32_t nb = gridDim.x;
+  const uint32_t grp = J.bins_fpc && nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+  fft_tile_any<LOG2N, SPC, ALL1, FL>(J, grp, smem);
 }
 
 // Batched: one block per frame group of a class of CPIs (same length and lane
@@ -1274,11 +1297,12 @@ static hipError_t launch_fft_batch_spc(hipStream_t s, const uint32_t* tw, uint32
 
 hipError_t launch_fft(hipStream_t s, const uint32_t* tw4096, const uint32_t* in, uint32_t n, uint32_t spc,
                       uint32_t sched, int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins,
-                      uint32_t* stage_flags) {
+                      uint32_t* stage_flags, uint32_t bins_fpc) {
   if (n_frames == 0) return hipSuccess;
   const uint32_t log2n = ilog2u(n);
   if (log2n < 3 || log2n > 12) return hipErrorInvalidValue;
-  const FftJob J{in, out, bins, stage_flags, sched, reorder ? 1u : 0u, n_frames, 0};
+  if (bins_fpc && (n % 4 != 0 || n_frames % bins_fpc != 0)) return hipErrorInvalidValue;
+  const FftJob J{in, out, bins, stage_flags, sched, reorder ? 1u : 0u, n_frames, 0, bins_fpc};
   return spc == 2 ? launch_fft_spc<2>(s, tw4096, log2n, J) : launch_fft_spc<1>(s, tw4096, log2n, J);
 }
 

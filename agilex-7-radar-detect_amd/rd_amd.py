@@ -23,7 +23,7 @@ RD_STATIC_MAX_ROWS = RD_STATIC_SCRATCH_BYTES // 8 - 1
 EXPORTS = [
     "rd_device_count", "rd_create", "rd_destroy", "rd_status_str", "rd_sync", "rd_malloc", "rd_free", "rd_h2d",
     "rd_d2h", "rd_memset", "rd_stream", "rd_twiddle_table", "rd_pfb_run", "rd_fft_run", "rd_align_run",
-    "rd_bf_run", "rd_bf_stage", "rd_bf_cov_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
+    "rd_bf_run", "rd_bf_stage", "rd_bf_cov_stage", "rd_fft_stage", "rd_cov_run", "rd_cfar_run", "rd_chain_run", "rd_chain_enqueue", "rd_chain_graph",
     "rd_graph_launch", "rd_graph_destroy", "rd_batch_create", "rd_batch_launch", "rd_batch_events", "rd_batch_plan",
     "rd_batch_destroy", "rd_event_flit_count", "rd_flit_words", "rd_event_flits", "rd_event_writeback", "rd_doppler_run", "rd_read_coeff", "rd_read_pfb_vec", "rd_read_iq_ascii", "rd_write_iq_ascii", "rd_cov_integrate",
     "rd_chain_enqueue_static", "rd_bf_plain_run", "rd_cov_plain_run", "rd_chain_enqueue_plain", "rd_cfar_enqueue",
@@ -47,10 +47,20 @@ class CfarCfg(C.Structure):
         return cls(*[int(w) for w in words])
 
 
+def blocked_x(X, CH, F, N):
+    """X [CH][F][N] (natural) -> [CH][N/4][F][4] (bin-quad blocked), flat."""
+    return np.ascontiguousarray(np.asarray(X, np.uint32).reshape(CH, F, N // 4, 4).transpose(0, 2, 1, 3)).reshape(-1)
+
+
+def natural_x(Xb, CH, F, N):
+    """The inverse of blocked_x."""
+    return np.ascontiguousarray(np.asarray(Xb, np.uint32).reshape(CH, N // 4, F, 4).transpose(0, 2, 1, 3)).reshape(-1)
+
+
 class CpiDesc(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in
                 ("n_chan", "n_fft", "taps", "n_frames", "n_beams", "spc", "scale_sched", "max_guard", "max_ref")] + [
-        ("cfar", CfarCfg)]
+        ("cfar", CfarCfg), ("x_layout", C.c_uint32)]
 
 
 class CpiBufs(C.Structure):
@@ -125,7 +135,8 @@ class Lib:
         L.rd_align_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_bf_run.argtypes = [vp, vp, vp, u32, u32, u64, vp, vp, vp, vp]
         L.rd_bf_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp]
-        L.rd_bf_cov_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]
+        L.rd_bf_cov_stage.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, u32]
+        L.rd_fft_stage.argtypes = [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp]
         L.rd_cov_run.argtypes = [vp, vp, u32, u32, u32, vp]
         L.rd_cfar_run.argtypes = [vp, vp, u32, u32, u32, C.POINTER(CfarCfg), u32, u32, vp, u32, vp, u64, vp,
                                   C.POINTER(u64)]
@@ -340,6 +351,20 @@ class Ctx:
         return (do.download(np.uint32, x.size), db.download(np.uint32, x.size),
                 dfl.download(np.uint32, frames) if dfl else None)
 
+    def fft_stage(self, u, n, n_chan, n_frames, x_layout=0, spc=2, sched=0xFFFFFFFF, flags=True):
+        """rd_fft_stage -> (X in natural order, stage flags or None); with
+        x_layout 1 the device writes it bin-quad blocked and it is returned
+        un-blocked."""
+        u = np.asarray(u, np.uint32)
+        du, dX = self.put(u), self.buf(4 * u.size)
+        dfl = self.buf(4 * max(n_chan * n_frames, 1)) if flags else None
+        self.lib.check("rd_fft_stage", self.L.rd_fft_stage(self.h, du.ptr, n, spc, sched, n_chan, n_frames, x_layout,
+                                                           dX.ptr, dfl.ptr if dfl else None))
+        X = dX.download(np.uint32, u.size)
+        if x_layout == 1:
+            X = natural_x(X, n_chan, n_frames, n)
+        return X, (dfl.download(np.uint32, n_chan * n_frames) if dfl else None)
+
     def align(self, X, n_chan, n_frames, n_bins):
         dX = self.put(np.asarray(X, np.uint32))
         dA = self.buf(dX.nbytes)
@@ -368,10 +393,13 @@ class Ctx:
         return (dy.download(np.uint32, B * cols), df.download(np.uint8, B * cols), dp.download(np.uint32, B * cols),
                 da.download(np.uint32, CH * cols))
 
-    def bf_cov_stage(self, X, W, CH, B, F, N, outputs=("y", "flags", "power", "Xa")):
-        """rd_bf_cov_stage on the FFT layout; returns (y, flags, power, Xa, R),
-        None for an output not in `outputs` (passed as NULL)."""
+    def bf_cov_stage(self, X, W, CH, B, F, N, outputs=("y", "flags", "power", "Xa"), x_layout=0):
+        """rd_bf_cov_stage on the FFT layout (X given in natural order; with
+        x_layout 1 it is passed bin-quad blocked); returns (y, flags, power,
+        Xa, R), None for an output not in `outputs` (passed as NULL)."""
         X = np.asarray(X, np.uint32)
+        if x_layout == 1:
+            X = blocked_x(X, CH, F, N)
         cols = F * N
         dX, dW = self.put(X), self.put(np.asarray(W, np.uint32))
         dy = self.buf(4 * B * cols) if "y" in outputs else None
@@ -381,7 +409,7 @@ class Ctx:
         dR = self.buf(16 * N * CH * (CH + 1) // 2)
         ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
         self.lib.check("rd_bf_cov_stage", self.L.rd_bf_cov_stage(self.h, dX.ptr, dW.ptr, CH, B, F, N, ptr(dy),
-                                                                 ptr(df), ptr(dp), ptr(da), dR.ptr))
+                                                                 ptr(df), ptr(dp), ptr(da), dR.ptr, x_layout))
         return (dy.download(np.uint32, B * cols) if dy is not None else None,
                 df.download(np.uint8, B * cols) if df is not None else None,
                 dp.download(np.uint32, B * cols) if dp is not None else None,
@@ -625,11 +653,13 @@ class Chain:
     """Device buffers and descriptor for one CPI geometry (reused across runs)."""
 
     def __init__(self, ctx: Ctx, CH, N, T, F, B, cfar_cfg8, max_guard=31, max_ref=63, spc=2, sched=0xFFFFFFFF,
-                 keep_y=True, keep_flags=True, keep_R=True, events_cap=None):
+                 keep_y=True, keep_flags=True, keep_R=True, events_cap=None, x_layout=0):
+        """x_layout 1: the chain keeps X bin-quad blocked [CH][N/4][F][4] (the
+        fused path's fast layout); fetch() returns it in natural order."""
         self.ctx = ctx
         S = CH * N * F
         self.shape = dict(CH=CH, N=N, T=T, F=F, B=B)
-        self.desc = CpiDesc(CH, N, T, F, B, spc, sched, max_guard, max_ref, CfarCfg.of(cfar_cfg8))
+        self.desc = CpiDesc(CH, N, T, F, B, spc, sched, max_guard, max_ref, CfarCfg.of(cfar_cfg8), x_layout)
         self.events_cap = B * F * (N + 1) if events_cap is None else events_cap
         b = {}
         b["iq"] = ctx.buf(4 * S)
@@ -707,6 +737,8 @@ class Chain:
             if self.b.get(k) is not None:
                 dt, n = spec[k]
                 out[k] = self.b[k].download(dt, n)
+        if "X" in out and self.desc.x_layout == 1:
+            out["X"] = natural_x(out["X"], CH, F, N)
         out["row_count"] = self.b["row_count"].download(np.uint32, B * F)
         if n_events is not None:
             t = min(n_events, self.events_cap)

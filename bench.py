@@ -64,6 +64,12 @@ def fused(sc) -> bool:
     return sc.CH <= 16 and sc.N % 4 == 0 and sc.B <= 128 and 16 * sc.F * sc.N * 4 < 2 ** 32
 
 
+def x_layout(sc) -> int:
+    """The chain's X layout: bin-quad blocked (1) where it fuses the
+    beamformer and the covariance, which then read X as contiguous runs."""
+    return 1 if fused(sc) else 0
+
+
 def chain_stages(sc):
     """The stages of the chain as it runs the workload, in order."""
     return ("pfb", "fft", "bfcov", "cfar") if fused(sc) else ("pfb", "fft", "bf", "cov", "cfar")
@@ -100,9 +106,12 @@ def stage_launchers(ctx, sc, plain=False):
         b = ch.b
         chk("pfb", L.rd_pfb_run(h, iq, b["taps"].ptr, sc.N, sc.T, sc.CH, sc.F, None, b["u"].ptr, None))
 
-    def fft(ch, iq):
+    xl = x_layout(sc)
+
+    def fft(ch, iq):  # the chain's FFT stage, X in the chain's layout
         b = ch.b
-        chk("fft", L.rd_fft_run(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, 1, sc.CH * sc.F, None, b["X"].ptr, None))
+        chk("fft", L.rd_fft_stage(h, b["u"].ptr, sc.N, 2, 0xFFFFFFFF, sc.CH, sc.F, xl if not plain else 0,
+                                  b["X"].ptr, None))
 
     def bf(ch, iq):  # FFT layout in, power + aligned copy out
         b = ch.b
@@ -115,7 +124,7 @@ def stage_launchers(ctx, sc, plain=False):
     def bfcov(ch, iq):  # FFT layout in, power + covariance out, no aligned copy
         b = ch.b
         chk("bfcov", L.rd_bf_cov_stage(h, b["X"].ptr, b["weights"].ptr, sc.CH, sc.B, sc.F, sc.N, None, None,
-                                       b["power"].ptr, None, b["R"].ptr))
+                                       b["power"].ptr, None, b["R"].ptr, xl))
 
     def align(ch, iq):
         chk("align", L.rd_align_run(h, ch.b["X"].ptr, sc.CH, sc.F, sc.N, ch.b["Xa"].ptr))
@@ -158,7 +167,7 @@ def slot_chains(ctx, sc, taps, W, n):
     for _ in range(n):
         c = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
                          rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True,
-                         events_cap=sc.B * sc.F * 64)
+                         events_cap=sc.B * sc.F * 64, x_layout=x_layout(sc))
         c.b["taps"].upload(taps)
         c.b["weights"].upload(W)
         out.append(c)
@@ -616,7 +625,7 @@ def main():
     R = args.ring
     n_str = max(1, min(args.streams, R))
     ch = rd_amd.Chain(ctx, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD, rd_scenario.MAX_REF,
-                      keep_y=True, keep_flags=True, keep_R=True)
+                      keep_y=True, keep_flags=True, keep_R=True, x_layout=x_layout(sc))
     note(f"workload {sc.name}: generating {R} input CPIs")
     ch.b["taps"].upload(taps)
     ch.b["weights"].upload(W)
@@ -663,7 +672,7 @@ def main():
     chains, s_totals = [ch], [totals]
     for c in ctxs[1:]:
         chk = rd_amd.Chain(c, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), rd_scenario.MAX_GUARD,
-                           rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True)
+                           rd_scenario.MAX_REF, keep_y=False, keep_flags=False, keep_R=True, x_layout=x_layout(sc))
         chk.b["taps"].upload(taps)
         chk.b["weights"].upload(W)
         chains.append(chk)
@@ -706,9 +715,13 @@ def main():
         g.close()
     bufs_l = [lean_bufs(ch, p) for p in ring_ptrs]
 
+    natural = rd_amd.CpiDesc.from_buffer_copy(ch.desc)
+    natural.x_layout = 0  # the plain chain keeps X in natural order
+
     def enqueue_pass(fn):
+        d = natural if fn == "rd_chain_enqueue_plain" else ch.desc
         for j in range(R):
-            ctx.lib.check(fn, getattr(ctx.L, fn)(ctx.h, rd_amd.C.byref(ch.desc), rd_amd.C.byref(bufs_l[j]),
+            ctx.lib.check(fn, getattr(ctx.L, fn)(ctx.h, rd_amd.C.byref(d), rd_amd.C.byref(bufs_l[j]),
                                                  totals.ptr + 8 * j))
 
     timed(lambda: enqueue_pass("rd_chain_enqueue"), 1)
@@ -776,7 +789,8 @@ def main():
                                f"{sc.B} beams, CA-CFAR {sc.ref}/{sc.guard}", "cpi_samples": sc.samples,
                    "cpis_per_step": R, "ms_per_cpi": round(ms_per_cpi, 5), "streams": n_str,
                    "impl": "one lean hipGraph per CPI (PFB, FFT, "
-                           + ("beamformer+power and covariance as one split-int8 MFMA kernel reading X once"
+                           + ("beamformer+power and covariance as one split-int8 MFMA kernel reading X once, "
+                              "X kept bin-quad blocked by the FFT"
                               if fused(sc) else "BF+power split-int8 MFMA writing the aligned copy, covariance "
                                                 "split-int8 MFMA")
                            + f", CFAR rows/scan/gather); a step is one pass over {R} distinct input CPIs dealt "

@@ -104,6 +104,13 @@ typedef struct {
   uint32_t max_guard;   /* CFAR elaboration maxima (clamp)               */
   uint32_t max_ref;
   rd_cfar_cfg cfar;
+  uint32_t x_layout;    /* layout of bufs->X in the chain: 0 natural order
+                         * [n_chan][n_frames][n_fft]; 1 bin-quad blocked
+                         * [n_chan][n_fft/4][n_frames][4], which the fused
+                         * beamformer + covariance reads as contiguous runs
+                         * (faster); allowed only where the chain fuses them
+                         * (n_chan <= 16, n_fft >= 16, R set), not in
+                         * rd_chain_enqueue_plain or a batch (RD_E_ARG) */
 } rd_cpi_desc;
 
 /* Device buffers for one CPI (layouts: [a][b] is row-major, a slowest).
@@ -112,7 +119,8 @@ typedef struct {
  *   weights   [n_beams][n_chan]             beam-major
  *   u         [n_chan][n_frames][n_fft]     PFB output (required)
  *   pfb_flags [n_chan][n_frames][n_fft]     nullable
- *   X         [n_chan][n_frames][n_fft]     FFT bins, natural order (required)
+ *   X         [n_chan][n_frames][n_fft]     FFT bins, natural order (required); bin-quad blocked
+ *             [n_chan][n_fft/4][n_frames][4] when desc->x_layout is 1
  *   fft_flags [n_chan][n_frames]            stage flags, sub-stage g at bits [2g+1:2g]; nullable
  *   Xa        [n_frames][n_fft][n_chan]     aligned (bin-major, antenna-minor); nullable in the
  *             single-CPI chain (written when set; when null the context keeps its own copy
@@ -187,6 +195,15 @@ rd_status rd_pfb_run(rd_ctx* ctx, const uint32_t* x, const uint32_t* coeff, uint
 rd_status rd_fft_run(rd_ctx* ctx, const uint32_t* in, uint32_t n_fft, uint32_t spc, uint32_t scale_sched,
                      int reorder, uint32_t n_frames, uint32_t* out, uint32_t* bins, uint32_t* stage_flags);
 
+/* The chain's FFT stage (fft::transform per frame, fft_ref.hpp:250-412): u
+ * [n_chan][n_frames][n_fft] -> X in natural bin order (x_layout 0; the same
+ * as rd_fft_run's `bins` output with reorder 1) or bin-quad blocked
+ * [n_chan][n_fft/4][n_frames][4] (x_layout 1: bins 4j..4j+3 of every frame
+ * of a channel in one run, what the fused beamformer + covariance reads).
+ * stage_flags [n_chan * n_frames] nullable. u and X 16-byte aligned. */
+rd_status rd_fft_stage(rd_ctx* ctx, const uint32_t* u, uint32_t n_fft, uint32_t spc, uint32_t scale_sched,
+                       uint32_t n_chan, uint32_t n_frames, uint32_t x_layout, uint32_t* X, uint32_t* stage_flags);
+
 /* Frequency-bin alignment (corner turn): X [n_chan][n_frames][n_bins] ->
  * Xa [n_frames][n_bins][n_chan], the beamformer's bin-major, antenna-minor beat
  * layout (model/cpp/align/align_model.hpp:172-186). Pure permutation. */
@@ -216,10 +233,12 @@ rd_status rd_bf_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_
  * [n_ant][n_frames][n_bins]. One kernel for n_ant <= 16 with n_bins a
  * multiple of 4 (Xa nullable: written when set); otherwise the two stages,
  * and then Xa is required. R [n_bins][n_ant*(n_ant+1)/2][2] is required;
- * y/flags/power are nullable. */
+ * y/flags/power are nullable. x_layout: X in natural order (0) or bin-quad
+ * blocked [n_ant][n_bins/4][n_frames][4] (1, the one-kernel form only; what
+ * rd_fft_stage writes with x_layout 1, the chain's fast layout). */
 rd_status rd_bf_cov_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_t n_ant, uint32_t n_beams,
                           uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
-                          uint32_t* Xa, int64_t* R);
+                          uint32_t* Xa, int64_t* R, uint32_t x_layout);
 
 /* Covariance: R[bin][pair i<=j] = sum over frames of covar::cross_power(X_i, X_j)
  * (covar_model.hpp:137-151), integrated by a block-mode covar::Integrator with

@@ -23,8 +23,8 @@ def _eq(a, b, what):
     assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
 
 
-def _check(gpu, orc, X, W, CH, B, F, N, outputs=("y", "flags", "power", "Xa"), what=""):
-    y, fl, pw, xa, R = gpu.bf_cov_stage(X, W, CH, B, F, N, outputs=outputs)
+def _check(gpu, orc, X, W, CH, B, F, N, outputs=("y", "flags", "power", "Xa"), what="", x_layout=0):
+    y, fl, pw, xa, R = gpu.bf_cov_stage(X, W, CH, B, F, N, outputs=outputs, x_layout=x_layout)
     oxa = orc.align(X, CH, F, N)
     oy, ofl, opw, _ = orc.bf(oxa, W, CH, B, want_acc=False)
     _eq(R, orc.cov(oxa, CH, F, N)[0], f"R {what}")
@@ -59,6 +59,16 @@ def test_fused_stage_against_oracle(gpu, orc, CH, B, F, N):
         _check(gpu, orc, X, W, CH, B, F, N, what=f"{name} CH={CH} B={B} F={F} N={N}")
 
 
+@pytest.mark.parametrize("CH,B,F,N", [(16, 16, 5, 1024), (16, 16, 256, 64), (16, 16, 65, 8), (1, 1, 1, 4),
+                                      (12, 17, 33, 16), (16, 128, 3, 8), (9, 31, 200, 20)])
+def test_fused_stage_blocked_layout(gpu, orc, CH, B, F, N):
+    """X given bin-quad blocked [CH][N/4][F][4] (x_layout 1): the same bits
+    as from the natural layout, every output including the aligned copy."""
+    rng = np.random.default_rng(CH * 7 + B * 3 + F + N)
+    for name, X, W in _inputs(rng, CH, B, F, N):
+        _check(gpu, orc, X, W, CH, B, F, N, what=f"blocked {name} CH={CH} B={B} F={F} N={N}", x_layout=1)
+
+
 @pytest.mark.parametrize("CH,B,F,N", [(16, 16, 256, 1024), (16, 16, 70, 8), (7, 20, 9, 12)])
 def test_fused_stage_power_only(gpu, orc, CH, B, F, N):
     """The lean chain's output set: power and R, no beam samples, flags or
@@ -89,6 +99,9 @@ def test_unfused_geometries_take_the_two_kernels(gpu, orc, CH, B, F, N):
     _check(gpu, orc, X, W, CH, B, F, N, what=f"{name} CH={CH} B={B} F={F} N={N}")
     with pytest.raises(RuntimeError):
         gpu.bf_cov_stage(X, W, CH, B, F, N, outputs=("power",))
+    if N % 4 == 0:  # the blocked layout is the one-kernel form's only
+        with pytest.raises(RuntimeError):
+            gpu.bf_cov_stage(X, W, CH, B, F, N, x_layout=1)
 
 
 def test_fused_equals_the_two_kernels(gpu):
@@ -132,3 +145,53 @@ def test_lean_chain_without_aligned_copy(gpu, orc):
         if CH > 16:
             with pytest.raises(RuntimeError):
                 ch.enqueue(lean=True, static=True)
+
+
+@pytest.mark.parametrize("name,CH,F", [("medium", 16, 6), ("medium", 8, 20), ("tiny", 4, 16)])
+def test_chain_with_blocked_x(gpu, orc, name, CH, F):
+    """The chain with X kept bin-quad blocked (x_layout 1): power, R and the
+    events equal the oracle, and X read back in natural order equals the
+    oracle's FFT output; full outputs, the lean graph and the allocation-free
+    entry point alike."""
+    import rd_scenario
+
+    sc = rd_scenario.config(name)
+    sc.CH, sc.F = CH, F
+    taps, W = sc.taps(), sc.weights()
+    iq = sc.iq(seed=9)
+    exp = orc.chain(iq, taps, W, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), 31, 63,
+                    keep=("X", "power", "R", "events"))
+    ch = rd_amd.Chain(gpu, sc.CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), x_layout=1)
+    ch.load(iq, taps, W)
+    for how in ("run", "lean graph", "static"):
+        ch.b["power"].upload(np.zeros(sc.B * sc.F * sc.N, np.uint32))
+        if how == "run":
+            n = ch.run()
+        else:
+            if how == "lean graph":
+                g = ch.graph(lean=True)
+                g.launch()
+            else:
+                ch.enqueue(static=True)
+            gpu.sync()
+            n = int(ch.total.download(np.uint64, 1)[0])
+        got = ch.fetch(("X", "power", "R"), n_events=n)
+        assert n == exp["n_events"], how
+        for k in ("X", "power", "R", "events"):
+            _eq(got[k], exp[k], f"{k} ({how}) CH={CH}")
+        if how == "lean graph":
+            g.close()
+
+
+def test_blocked_x_only_where_the_chain_fuses(gpu):
+    """x_layout 1 is refused where the chain does not fuse (more than 16
+    channels), by the plain chain, and for layouts other than 0 and 1."""
+    import rd_scenario
+
+    sc = rd_scenario.config("medium")
+    sc.F = 4
+    for CH, layout, plain in ((32, 1, False), (16, 1, True), (16, 2, False)):
+        ch = rd_amd.Chain(gpu, CH, sc.N, sc.T, sc.F, sc.B, sc.cfar_cfg(), x_layout=layout)
+        ch.load(np.zeros(CH * sc.N * sc.F, np.uint32), sc.taps(), np.zeros(sc.B * CH, np.uint32))
+        with pytest.raises(RuntimeError):
+            ch.enqueue(plain=plain)

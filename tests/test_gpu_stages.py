@@ -151,6 +151,29 @@ def test_fft_random(gpu, orc, n, spc):
                     _eq(a, b, f"{w} without flags n={n} spc={spc} sched={sched:#x} reorder={reorder}")
 
 
+@pytest.mark.parametrize("n,spc,CH,F", [(8, 2, 3, 5), (16, 1, 2, 7), (64, 2, 4, 9), (1024, 2, 3, 6),
+                                         (4096, 2, 2, 3), (256, 1, 5, 2)])
+def test_fft_stage_layouts(gpu, orc, n, spc, CH, F):
+    """rd_fft_stage: X in natural order equals rd_fft_run's natural bins, and
+    the bin-quad blocked X [CH][n/4][F][4] holds the same words; under the
+    all-ones schedule (16 points per thread from 64 points) and others (4
+    points per thread), with and without stage flags."""
+    import rd_amd
+
+    rng = np.random.default_rng(n + CH * 31 + F)
+    x = stim.rand_extreme(rng, CH * F * n)
+    for sched in (0xFFFFFFFF, int(rng.integers(0, 2**32))):
+        o = orc.fft(x, n, spc, sched, 1)
+        for layout in (0, 1):
+            for flags in (True, False):
+                X, sf = gpu.fft_stage(x, n, CH, F, x_layout=layout, spc=spc, sched=sched, flags=flags)
+                _eq(X, o[1], f"X layout={layout} n={n} spc={spc} sched={sched:#x} flags={flags}")
+                if flags:
+                    _eq(sf, o[2], f"stage flags layout={layout} n={n}")
+        Xb = rd_amd.natural_x(gpu.fft_stage(x, n, CH, F, x_layout=0, spc=spc, sched=sched)[0], CH, F, n)
+        _eq(rd_amd.blocked_x(Xb, CH, F, n), o[1], "layout helpers round trip")
+
+
 def twiddle_saturation_frame(n):
     """A frame whose first radix-4 pass under the all-ones schedule leaves
     (-32768, -32768) at quarter 1 of every group, so the twiddles near 45 deg
