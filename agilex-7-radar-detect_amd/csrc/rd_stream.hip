@@ -1155,6 +1155,26 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     }
   }
   if (fword) atomicOr(&sfl[lf], fword);
+  const uint32_t fpc = J.bins_fpc;
+  if (bins && !out && fpc && fpc % FPB == 0) {
+    // Bin-quad blocked bins whose channels hold whole blocks of frames (so
+    // every frame of the block is active and in one channel): after one block
+    // barrier, consecutive threads take the same bin quad of consecutive
+    // frames, so each run of FPB threads stores FPB * 16 contiguous bytes.
+    __syncthreads();
+    constexpr uint32_t Q = SPC == 1 ? N / 4 : M / 2;
+    const uint32_t f0 = grp * FPB, ch = f0 / fpc;
+    uint32_t* const base = bins + (size_t)ch * fpc * N + 4 * (f0 - ch * fpc);
+#pragma unroll
+    for (uint32_t it = threadIdx.x; it < N / 4 * FPB; it += TPF * FPB) {
+      const uint32_t j = it / FPB, fl = it % FPB, k = 4 * j;
+      const uint32_t b = SPC == 1 ? bitrev(k, NL) : 2 * bitrev(k & (M - 1), NL) + (k >> NL);
+      *reinterpret_cast<uint4*>(base + (size_t)j * 4 * fpc + 4 * fl) =
+          lds_quad<2 * Q, Q, 3 * Q>(smem + 2 * NT + FPB + fl * NP, b);
+    }
+    if (J.stage_flags && g == 0) J.stage_flags[frame] = sfl[lf];
+    return;
+  }
   frame_sync();
   if (!active) return;
 
