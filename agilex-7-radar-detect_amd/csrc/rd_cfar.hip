@@ -14,11 +14,12 @@
 // bin costs two differences whatever the reference length; u64 wrap-around of
 // the prefix differences equals the reference's u64 running sums modulo 2^64.
 //
-// Three launches: per-row decisions with the events staged in a fixed slot per
-// row, a scan of the per-row counts into event offsets, and a gather that moves
-// each row's events to its offset (recomputing only rows that overflowed).
-// Batched launches run the same row bodies over the rows of many CPIs and one
-// scan block per CPI.
+// Per-row decisions with the events staged in a fixed slot per row, a scan of
+// the per-row counts into event offsets, and a gather that moves each row's
+// events to its offset and recomputes only rows that overflowed their slot
+// (for a single CPI with rows of several waves, two launches: a copy without
+// LDS, then the recomputation of the overflowed rows). Batched launches run the same row bodies over the
+// rows of many CPIs and one scan block per CPI.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -585,11 +586,56 @@ static void launch_cfar_scan(hipStream_t s, const CfarJob& J) {
     cfar_scan_kernel<false><<<1, 1024, 0, s>>>(J);
 }
 
-template <typename Cell, bool MULTI, uint32_t SLOG>
+// Pass 3 of a single CPI for rows of one wave: one block per R rows, the
+// copy or, for a row that overflowed its slot, its recomputation.
+template <typename Cell, uint32_t SLOG>
 __global__ __launch_bounds__(512) void cfar_gather_kernel(CfarJob J, CfarRowGeo G) {
   extern __shared__ uint64_t lds[];
-  const uint32_t R = blockDim.x / (64 * G.W);
-  cfar_gather_tile<Cell, MULTI, SLOG>(J, G, blockIdx.x * R, lds);
+  const uint32_t R = blockDim.x / 64;
+  cfar_gather_tile<Cell, false, SLOG>(J, G, blockIdx.x * R, lds);
+}
+
+// Pass 3 of a single CPI for rows of several waves, as two launches, so that
+// moving the staged events needs no LDS and no block barrier per row. 3a: one wave per row moves the row's staged events to its
+// offset; rows that overflowed their slot are left to 3b.
+__global__ __launch_bounds__(256) void cfar_copy_kernel(CfarJob J) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t r = blockIdx.x * 4 + wave;
+  if (r >= J.n_rows) return;
+  const uint32_t cnt = J.row_count[r];
+  if (cnt > J.slot) return;
+  const uint64_t base = J.offs[r];
+  const uint64_t room = base < J.cap ? J.cap - base : 0;
+  const uint64_t words = 3ull * min((uint64_t)cnt, room);
+  const uint64_t* __restrict__ src = J.stage + (size_t)r * J.slot * 3;
+  uint64_t* __restrict__ dst = J.events + 3 * base;
+  for (uint64_t w = lane; w < words; w += 64) dst[w] = src[w];
+}
+
+// 3b: block b looks at rows [b * blockDim, +blockDim) at once (one count per
+// thread) and recomputes the ones that overflowed their slot, one after the
+// other with all its waves (a block is one row of W waves), writing their
+// events directly at their offsets. Blocks with none return after one load.
+template <typename Cell, bool MULTI, uint32_t SLOG>
+__global__ __launch_bounds__(512) void cfar_redo_kernel(CfarJob J, CfarRowGeo G) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint64_t over_mask[8];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t r0 = blockIdx.x * blockDim.x, rt = r0 + threadIdx.x;
+  const bool over = rt < J.n_rows && J.row_count[rt] > J.slot;
+  const uint64_t m = __ballot(over);
+  if (lane == 0) over_mask[wave] = m;
+  if (!__syncthreads_or(over)) return;
+  for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+    uint64_t mm = over_mask[w];  // the same for every thread: the loop is uniform
+    while (mm) {
+      const uint32_t r = r0 + 64 * w + (uint32_t)__builtin_ctzll(mm);
+      mm &= mm - 1;
+      const uint64_t base = J.offs[r];
+      const uint64_t room = base < J.cap ? J.cap - base : 0;
+      cfar_row_run<Cell, MULTI, SLOG>(J, G.W, r, true, wave, lane, lds, J.events + 3 * base, room);
+    }
+  }
 }
 
 // Batched: one block per row tile of a class of CPIs (same row length, so the
@@ -694,20 +740,23 @@ static hipError_t launch_cfar_cell(hipStream_t s, const CfarJob& J, uint32_t W) 
   launch_cfar_scan(s, J);
   e = hipGetLastError();
   if (e != hipSuccess || J.cap == 0) return e;
-  {
-    const uint32_t Wg = cfar_min_waves(J.n);
-    const CfarRowGeo G = cfar_row_geo(J.n, Wg);
-    const uint32_t R = cfar_rows_per_tile_max(J.n, Wg);
-    const size_t lds = cfar_lds_bytes(G, R);
-    const dim3 grid((J.n_rows + R - 1) / R), block(64 * G.W * R);
-#define RD_GATHER(SL, M) cfar_gather_kernel<Cell, M, SL><<<grid, block, lds, s>>>(J, G)
-    if (G.W > 1) {
-      RD_CFAR_DISPATCH(RD_GATHER, true)
-    } else {
-      RD_CFAR_DISPATCH(RD_GATHER, false)
-    }
+  const uint32_t Wg = cfar_min_waves(J.n);
+  const CfarRowGeo G = cfar_row_geo(J.n, Wg);
+  if (Wg == 1) {  // one launch: a second costs more than the copy saves at this row length
+    const uint32_t R = cfar_rows_per_tile_max(J.n, 1);
+    const dim3 grid((J.n_rows + R - 1) / R), block(64 * R);
+#define RD_GATHER(SL, M) cfar_gather_kernel<Cell, SL><<<grid, block, cfar_lds_bytes(G, R), s>>>(J, G)
+    RD_CFAR_DISPATCH(RD_GATHER, false)
 #undef RD_GATHER
+    return hipGetLastError();
   }
+  cfar_copy_kernel<<<(J.n_rows + 3) / 4, 256, 0, s>>>(J);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 grid((J.n_rows + 64 * G.W - 1) / (64 * G.W)), block(64 * G.W);
+#define RD_REDO(SL, M) cfar_redo_kernel<Cell, M, SL><<<grid, block, cfar_lds_bytes(G, 1), s>>>(J, G)
+  RD_CFAR_DISPATCH(RD_REDO, true)
+#undef RD_REDO
   return hipGetLastError();
 }
 
