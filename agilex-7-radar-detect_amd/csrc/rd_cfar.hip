@@ -104,14 +104,39 @@ __device__ __forceinline__ uint32_t pidx(uint32_t e) {
   return SLOG ? e + (e >> SLOG) : e;
 }
 
-__device__ __forceinline__ uint64_t wave_incscan(uint64_t v, uint32_t lane) {
-  uint64_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint64_t y = __shfl_up(x, o);
-    if (lane >= (uint32_t)o) x += y;
-  }
-  return x;
+// One step of the scan below: x += (x moved between lanes by the DPP control
+// CTRL, in the rows ROWS, a 4-bit mask of the wave's rows of 16 lanes), the
+// 64-bit value kept as two 32-bit halves added with a carry; lanes outside
+// ROWS, and lanes whose source lies outside their row, add 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_add(uint32_t& lo, uint32_t& hi) {
+  const uint32_t dl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, CTRL, ROWS, 0xF, true);
+  const uint32_t dh = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, CTRL, ROWS, 0xF, true);
+  const uint32_t s = lo + dl;
+  hi = hi + dh + (s < lo ? 1u : 0u);
+  lo = s;
+}
+
+// Inclusive scan over the 64 lanes of a wave, every lane active: within each
+// row of 16 lanes by row shifts of 1, 2, 4 and 8, then the last lane of rows 0
+// and 2 added to rows 1 and 3 (row_bcast:15) and lane 31 to rows 2 and 3
+// (row_bcast:31). No LDS traffic.
+__device__ __forceinline__ uint64_t wave_incscan(uint64_t x) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  dpp_add<0x111, 0xF>(lo, hi);
+  dpp_add<0x112, 0xF>(lo, hi);
+  dpp_add<0x114, 0xF>(lo, hi);
+  dpp_add<0x118, 0xF>(lo, hi);
+  dpp_add<0x142, 0xA>(lo, hi);
+  dpp_add<0x143, 0xC>(lo, hi);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// The value of lane 63 (uniform).
+__device__ __forceinline__ uint64_t lane63(uint64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // LDS writes of one lane become visible to the other lanes of the wave.
@@ -269,21 +294,26 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
       loc[j] = run;
     }
   }
-  const uint64_t incl = wave_incscan(loc[S - 1], lane);
+  const uint64_t incl = wave_incscan(loc[S - 1]);
   uint64_t base = incl - loc[S - 1];
   if (MULTI) {
-    const uint64_t wave_total = __shfl(incl, 63);
+    const uint64_t wave_total = lane63(incl);
     if (lane == 0) stats[wr * 4 + 0] = wave_total;
     __syncthreads();
     for (uint32_t w = 0; w < wr; ++w) base += stats[w * 4 + 0];
   }
   if (work) {
-    // P[b0 + j + 1] sits at a fixed offset from the lane's base
+    // P[b0 + j + 1] sits at a fixed offset from the lane's base; a wave
+    // inside the row stores without per-entry tests
     uint64_t* Pl = P + pidx<SLOG>(b0 + 1) - 1;
+    auto off = [](uint32_t j) { return j + 1 + ((SLOG && j + 1 == S) ? 1u : 0u); };
+    if (full) {
 #pragma unroll
-    for (uint32_t j = 0; j < S; ++j) {
-      const uint32_t off = j + 1 + ((SLOG && j + 1 == S) ? 1u : 0u);
-      if (full || b0 + j + 1 <= n) Pl[off] = base + loc[j];
+      for (uint32_t j = 0; j < S; ++j) Pl[off(j)] = base + loc[j];
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < S; ++j)
+        if (b0 + j + 1 <= n) Pl[off(j)] = base + loc[j];
     }
     if (t == 0) P[0] = 0;
   }
@@ -305,23 +335,32 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
   runs.a0 = Q + lane_run<SLOG>(lane, (int)(g.gl + 1));
   runs.b1 = Q + lane_run<SLOG>(lane, -(int)g.gg);
   runs.b0 = Q + lane_run<SLOG>(lane, -(int)(g.gg + g.rg));
+  // chunks [qa, qb) of the wave, 64 bins from c0 + 64 q, have every bin's
+  // window inside the row: q >= qa is c0 + 64 q >= guard_lag + ref_lag, q < qb
+  // is c0 + 64 q + 64 + guard_lead + ref_lead <= n
+  uint32_t qa = S, qb = 0;
+  if (work && ca) {
+    const uint32_t lo = g.gg + g.rg, span = 64 + g.gl + g.rl;
+    qa = lo > c0 ? (lo - c0 + 63) / 64 : 0u;
+    qb = n >= c0 + span ? (n - c0 - span) / 64 + 1 : 0u;
+  }
   uint64_t em[S];
   uint32_t cnt = 0, det = 0;
   static_for<S>([&](auto qc) {
     constexpr uint32_t q = decltype(qc)::value;
-    const uint32_t cq = c0 + q * 64, i = cq + lane;
-    const bool in = work && (full || i < n);
-    bool hit;
-    if (work && ca && cq >= g.gg + g.rg && (uint64_t)cq + 64 + g.gl + g.rl <= n) {
-      hit = ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8);
+    if (q >= qa && q < qb) {  // every bin of the chunk is in the row
+      const bool hit = ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8);
+      em[q] = __ballot(hit || g.dense);
+      if (g.dense) det += (uint32_t)__popcll(__ballot(hit));
     } else {
+      const uint32_t i = c0 + q * 64 + lane;
+      const bool in = work && (full || i < n);
       Decision d{};
       if (in) d = decide<SLOG>(P, n, i, g);
-      hit = d.hit;
+      em[q] = __ballot(in && (d.hit || g.dense));
+      if (g.dense) det += (uint32_t)__popcll(__ballot(in && d.hit));
     }
-    em[q] = __ballot(in && (hit || g.dense));
     cnt += (uint32_t)__popcll(em[q]);
-    if (g.dense) det += (uint32_t)__popcll(__ballot(in && hit));
   });
   if (!g.dense) det = cnt;
   uint64_t k = 0, K = cnt;
@@ -350,6 +389,7 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
     uint32_t kq = 0;
 #pragma unroll
     for (uint32_t q = 0; q < S; ++q) {
+      if (em[q] == 0) continue;  // uniform: most chunks of a sparse row emit nothing
       if ((em[q] >> lane) & 1ull) lst[kq + (uint32_t)__popcll(em[q] & below)] = c0 + q * 64 + lane;
       kq += (uint32_t)__popcll(em[q]);
     }
@@ -463,7 +503,7 @@ __device__ __forceinline__ void cfar_scan_direct(const CfarJob& J) {
       v[q] = q < lim ? rc[q] : 0u;
       s += v[q];
     }
-    const uint64_t x = wave_incscan(s, lane);
+    const uint64_t x = wave_incscan(s);
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
     uint64_t wb = 0, all = 0;
@@ -530,7 +570,7 @@ __device__ __forceinline__ void cfar_scan_staged(const CfarJob& J) {
       v[q] = cnt[pad(kScanPer * t + q)];
       sum += v[q];
     }
-    const uint64_t x = wave_incscan(sum, lane);
+    const uint64_t x = wave_incscan(sum);
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
     uint64_t wb = 0, all = 0;
