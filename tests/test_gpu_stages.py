@@ -329,6 +329,40 @@ def test_cfar_many_rows_across_calls(gpu, orc):
         _eq(eg, eo, f"events rows={rows} n={n}")
 
 
+@pytest.mark.parametrize("cfg", [CFAR_CFGS[0], CFAR_CFGS[5], CFAR_CFGS[6], CFAR_CFGS[8]])
+@pytest.mark.parametrize("n", [64, 1024, 4096, 100, 5000, 2])
+def test_cfar_u32_rows_either_side_of_2_32(gpu, orc, cfg, n):
+    """u32 cells in rows whose total is below 2^32, exactly 2^32 - 1, exactly
+    2^32 and far above, with planted targets, against the oracle: the 64-bit
+    prefix carries past 2^32 inside a row."""
+    rng = np.random.default_rng(n * 31 + sum(cfg))
+    rows = 16
+    cells = np.zeros((rows, n), np.uint64)
+    for r in range(rows):
+        kind = r % 4
+        if kind == 3:
+            cells[r] = rng.integers(0, 2**31, n)
+            continue
+        cells[r] = rng.integers(0, 2**16, n)
+        cells[r, rng.integers(0, n, 3)] = rng.integers(2**22, 2**24, 3)  # targets
+        if kind == 1:  # total 2^32 - 1
+            cells[r, -1] = 0
+            cells[r, -1] = 2**32 - 1 - int(cells[r].sum())
+        elif kind == 2:  # total 2^32: the last two cells carry it
+            cells[r, -2:] = 0
+            rest = 2**32 - int(cells[r].sum())
+            cells[r, -2], cells[r, -1] = rest // 2, rest - rest // 2
+    assert all(int(cells[r].sum()) == 2**32 - 1 for r in range(1, rows, 4))
+    assert all(int(cells[r].sum()) == 2**32 for r in range(2, rows, 4))
+    c32 = cells.reshape(-1).astype(np.uint32)
+    assert np.array_equal(c32.astype(np.uint64), cells.reshape(-1))
+    eg, rcg, tot, _ = gpu.cfar(c32, n, cfg, 31, 63, frame_mod=5)
+    eo, rco = orc.cfar(cells.reshape(-1), n, cfg, 31, 63, frame_ids=np.arange(rows) % 5)
+    _eq(rcg, rco, "row_count")
+    assert tot == eo.shape[0]
+    _eq(eg, eo, "events")
+
+
 def test_cfar_u32_frame_mod_and_capacity(gpu, orc):
     import rd_amd
 
