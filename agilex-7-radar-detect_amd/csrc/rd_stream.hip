@@ -205,9 +205,49 @@ struct PfbRealWindow {
   }
 };
 
-// The beat loop of one branch over its chunk [m0, m1) for either window:
+// Real taps whose magnitudes sum to at most 65535 on the branch (the
+// windowed-sinc channelizer's taps: at most 57429 for the 1024- and 4096-branch
+// banks): every partial sum of h * x is then within 65535 * 2^15 = 2^31 - 2^15
+// for any int16 samples, so the products need no digit split and the int32
+// sums are the exact accumulator. Slot j keeps the raw component pairs
+// (xr[n], xr[n-1]) and (xi[n], xi[n-1]); an output is T/2 dot products per
+// component (half of PfbRealWindow's) and one rounding, whose bias keeps the
+// sum below 2^31.
+template <int T>
+struct PfbRealDirect {
+  uint32_t rp[T], ip[T];
+  // the previous sample's components are the low halves of the previous
+  // slot's pairs, so a loaded sample is dead once it has entered the window
+  __device__ __forceinline__ void put(int j, uint32_t v) {
+    const int q = (j + T - 1) % T;
+    rp[j] = __builtin_amdgcn_perm(rp[q], v, 0x05040100u);  // (v.re, prev.re)
+    ip[j] = __builtin_amdgcn_perm(ip[q], v, 0x05040302u);  // (v.im, prev.im)
+  }
+  template <bool FLAGS>
+  __device__ __forceinline__ uint32_t out(int j, const uint32_t (&tp)[T / 2], uint8_t* f) const {
+    int32_t ar = dot2_0(rp[j], tp[0]), ai = dot2_0(ip[j], tp[0]);
+#pragma unroll
+    for (int i = 1; i < T / 2; ++i) {
+      const int d = (j - 2 * i + T) % T;
+      ar = dot2(rp[d], tp[i], ar);
+      ai = dot2(ip[d], tp[i], ai);
+    }
+    const int32_t qr = (ar + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)ar, 15, 1)) >> 15;
+    const int32_t qi = (ai + 0x3FFF + (int32_t)__builtin_amdgcn_ubfe((uint32_t)ai, 15, 1)) >> 15;
+    if constexpr (FLAGS) {
+      uint32_t fr = 0, fi = 0;
+      const int32_t yr = sat16(qr, fr), yi = sat16(qi, fi);
+      *f = (uint8_t)(fr | (fi << 2));
+      return cpack(yr, yi);
+    } else {
+      return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pk_i16(qr, qi));  // both halves clamped
+    }
+  }
+};
+
+// The beat loop of one branch over its chunk [m0, m1) for any window:
 // slot (m - m0) mod T holds x_p[m]. The warm-up feeds x_p[m0-T+1 .. m0-1]
-// into slots 1..T-1 oldest first (the real window pairs each sample with the
+// into slots 1..T-1 oldest first (the real windows pair each sample with the
 // one before it), and slot 0 starts empty.
 template <int T, bool FLAGS, class Win, class Out>
 __device__ __forceinline__ void pfb_window_run(const uint32_t* xc, const uint32_t* hc, uint32_t P, uint32_t p,
@@ -248,7 +288,8 @@ __device__ __forceinline__ void pfb_window_run(const uint32_t* xc, const uint32_
 
 // One tile: branches [bx*blockDim, +blockDim) of channel c over beat chunk z.
 // FLAGS: the job has a flag buffer (otherwise the flags are not formed). A
-// wave whose branches all have real taps takes the real window.
+// wave whose branches all have real taps takes a real window: the direct one
+// when every branch's tap magnitudes sum to at most 65535, else the digit one.
 template <int T, bool FLAGS>
 __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint32_t c, uint32_t z) {
   const uint32_t P = J.P, n_beats = J.n_beats;
@@ -262,21 +303,30 @@ __device__ __forceinline__ void pfb_reg_tile(const PfbJob& J, uint32_t bx, uint3
   uint32_t* yc = J.y + (size_t)c * span + p;
   uint8_t* fc = J.flags ? J.flags + (size_t)c * span + p : nullptr;
 
-  uint32_t h[T], imag = 0;
+  uint32_t h[T], imag = 0, mag = 0;
 #pragma unroll
   for (int k = 0; k < T; ++k) {
     h[k] = J.coeff[(size_t)p * T + k];  // (hr, hi)
     imag |= h[k] >> 16;
+    mag += (uint32_t)abs(cre(h[k]));
   }
   if (__builtin_amdgcn_ballot_w64(imag != 0) == 0) {
     uint32_t tp[T / 2];
 #pragma unroll
     for (int i = 0; i < T / 2; ++i) tp[i] = __builtin_amdgcn_perm(h[2 * i + 1], h[2 * i], 0x05040100u);
-    PfbRealWindow<T> w;
+    if (__builtin_amdgcn_ballot_w64(mag > 65535u) == 0) {
+      PfbRealDirect<T> w;
 #pragma unroll
-    for (int k = 0; k < T; ++k) w.rh[k] = w.ih[k] = w.rl[k] = w.il[k] = 0u;
-    pfb_window_run<T, FLAGS>(xc, hc, P, p, m0, m1, yc, fc, w,
-                             [&](int j, uint8_t* f) { return w.template out<FLAGS>(j, tp, f); });
+      for (int k = 0; k < T; ++k) w.rp[k] = w.ip[k] = 0u;
+      pfb_window_run<T, FLAGS>(xc, hc, P, p, m0, m1, yc, fc, w,
+                               [&](int j, uint8_t* f) { return w.template out<FLAGS>(j, tp, f); });
+    } else {
+      PfbRealWindow<T> w;
+#pragma unroll
+      for (int k = 0; k < T; ++k) w.rh[k] = w.ih[k] = w.rl[k] = w.il[k] = 0u;
+      pfb_window_run<T, FLAGS>(xc, hc, P, p, m0, m1, yc, fc, w,
+                               [&](int j, uint8_t* f) { return w.template out<FLAGS>(j, tp, f); });
+    }
   } else {
     uint32_t ta[T], tb[T];
 #pragma unroll

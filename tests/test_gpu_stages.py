@@ -105,6 +105,56 @@ def test_pfb_real_taps(gpu, orc, P, T, nb, ch, mix):
         _eq(hg, ho, "final history")
 
 
+def _tap_magnitudes(rng, P, T, total):
+    """P rows of T magnitudes in [1, 32767], each row summing to `total`."""
+    m = np.full((P, T), total // T, np.int64)
+    m[:, : total % T] += 1
+    for _ in range(4 * T):
+        i, j = rng.integers(0, T, 2)
+        if i == j:
+            continue
+        d = np.minimum(np.minimum(rng.integers(0, 2000, P), m[:, i] - 1), 32767 - m[:, j])
+        m[:, i] -= d
+        m[:, j] += d
+    assert (m.sum(1) == total).all() and m.min() >= 1 and m.max() <= 32767
+    return m
+
+
+@pytest.mark.parametrize("T", [16, 8, 4])
+def test_pfb_direct_window_at_its_bound(gpu, orc, T):
+    """Real taps whose magnitudes sum to exactly 65535 on every branch of the
+    first wave (the direct window: no digit split, int32 sums up to
+    65535 * 32768 = 2^31 - 2^15 in magnitude) and to 65536 on one branch of
+    the second wave (the whole wave keeps the digit window). Taps all
+    positive, all negative and of random signs against samples all -32768
+    (the largest sums, saturating), all 32767, extreme and random; with and
+    without flags, with an imported history, beats split into chunks."""
+    P, nb, ch = 128, 70, 2
+    rng = np.random.default_rng(T * 11)
+    m = _tap_magnitudes(rng, P, T, 65535)
+    m[100, int(np.argmin(m[100]))] += 1  # branch 100 (second wave): 65536
+    assert m[:64].sum(1).max() == 65535 and m[100].sum() == 65536
+    signs = {"positive": np.ones((P, T), np.int64), "negative": -np.ones((P, T), np.int64),
+             "random": rng.choice([-1, 1], (P, T))}
+    S = ch * nb * P
+    samples = {"all -32768": np.full(S, vecio.pack(-32768, -32768), np.uint32),
+               "all 32767": np.full(S, vecio.pack(32767, 32767), np.uint32),
+               "extreme": stim.rand_extreme(rng, S), "random": stim.rand_cx(rng, S)}
+    for sn, sg in signs.items():
+        h = ((m * sg).reshape(-1) & 0xFFFF).astype(np.uint32)  # real taps
+        for xn, x in samples.items():
+            yg, fg = gpu.pfb(x, h, P, T, ch, nb)
+            yo, fo = orc.pfb(x, h, P, T, ch, nb)
+            _eq(yg, yo, f"y taps {sn} samples {xn}")
+            _eq(fg, fo, f"flags taps {sn} samples {xn}")
+            _eq(gpu.pfb(x, h, P, T, ch, nb, flags=False)[0], yo, f"y without flags, taps {sn} samples {xn}")
+        hist0 = stim.rand_extreme(rng, ch * (T - 1) * P)
+        hg, ho = hist0.copy(), hist0.copy()
+        x = samples["extreme"]
+        _eq(gpu.pfb(x, h, P, T, ch, nb, hist=hg)[0], orc.pfb(x, h, P, T, ch, nb, hist=ho)[0], f"y with history {sn}")
+        _eq(hg, ho, f"final history {sn}")
+
+
 @pytest.mark.parametrize("T", [16, 5])
 def test_pfb_history_carry(gpu, orc, T):
     rng = np.random.default_rng(T)
