@@ -308,9 +308,11 @@ rd_status rd_bf_cov_stage(rd_ctx* c, const uint32_t* X, const uint32_t* W, uint3
   if (!X || !W || !R || n_ant < 1 || n_ant > 64 || n_beams < 1 || n_beams > 128 || n_frames > 65535 ||
       x_layout > 1)
     return RD_E_ARG;
-  if (rd::bfcov_fits(n_ant, n_bins, n_frames, n_beams))
+  if (rd::bfcov_fits(n_ant, n_bins, n_frames, n_beams)) {
+    if ((uintptr_t)X & 15u) return RD_E_ARG;  // the one-kernel form moves X in 16-byte pieces
     return fin(rd::launch_bfcov(c->stream, X, W, n_ant, n_beams, n_frames, n_bins, y, flags, power, Xa, R,
                                 x_layout == 1));
+  }
   if (!Xa || x_layout != 0) return RD_E_ARG;  // the two-kernel form reads natural order
   RD_TRY(rd::launch_bf_chmajor(c->stream, X, W, n_ant, n_beams, (uint64_t)n_frames * n_bins, y, flags, power, Xa));
   return fin(rd::launch_cov(c->stream, Xa, n_ant, n_frames, n_bins, R));

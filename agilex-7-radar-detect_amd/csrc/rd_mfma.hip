@@ -122,10 +122,12 @@ __host__ __device__ inline size_t bf_frag_words(uint32_t bt, uint32_t steps) { r
 
 size_t bf_prep_bytes(uint32_t, uint32_t) { return 0; }
 
-// Every block derives the weight digits itself (B x CH words, L2-resident).
+// Every block derives the weight digits itself (B x CH words, L2-resident),
+// with the block's first nthr threads (whole waves).
 __device__ __forceinline__ void bf_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B, uint32_t chp,
-                                  uint32_t steps, uint32_t bt_n, uint32_t* frag, int32_t* tail) {
-  for (uint32_t f = threadIdx.x; f < bt_n * steps * 64; f += blockDim.x) {
+                                  uint32_t steps, uint32_t bt_n, uint32_t* frag, int32_t* tail,
+                                  uint32_t nthr = blockDim.x) {
+  for (uint32_t f = threadIdx.x; f < bt_n * steps * 64; f += nthr) {
     const uint32_t lane = f & 63, s = (f >> 6) % steps, bt = (f >> 6) / steps;
     const uint32_t b = bt * 16 + (lane & 15);
     const uint32_t k0 = 32 * s + 8 * (lane >> 4);
@@ -148,7 +150,7 @@ __device__ __forceinline__ void bf_weights_to_lds(const uint32_t* __restrict__ W
     o[192] = imag ? d.lr : d.li;
   }
   // per-beam sums: four threads per beam, 8 independent loads in flight each
-  for (uint32_t t = threadIdx.x; t < bt_n * 64; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < bt_n * 64; t += nthr) {
     const uint32_t b = t >> 2, part = t & 3;
     int32_t sa = 0, wis = 0;
     if (b < B)
@@ -818,9 +820,24 @@ hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const Cl
 // bin add theirs in LDS at the end and one wave forms the int64 results.
 // The aligned copy Xa is an optional output (16-byte stores of the staged
 // channel words), not an intermediate.
+// Loads: one extra wave of the block (the loader) only moves the samples,
+// global -> LDS without registers (global_load_lds_dwordx4), kFbRing steps
+// ahead into a ring of raw step buffers, and waits for them with a counted
+// s_waitcnt vmcnt before each step's barrier. The WAVES compute waves issue
+// no vector-memory loads in the frame loop, so their power stores are never
+// waited for and no load latency sits on their path.
 // ============================================================================
 constexpr uint32_t kFbNB = 4;  // bins per block
 constexpr uint32_t kFbPlanes = 8;
+// Raw step buffers of the loader: per channel pair one 1 KiB block (channel
+// 2j: frames 0..31 of the step, 16 bytes = the block's 4 bins each; then
+// channel 2j + 1), written by one global_load_lds_dwordx4; 32 bytes of
+// padding after each block put the four channel quads of the conversion reads
+// on distinct LDS banks.
+constexpr uint32_t kFbRing = 3;                   // steps in flight
+static_assert((kFbRing - 1) * 8 <= 40, "wait_vmcnt covers the loader's counts");
+constexpr uint32_t kFbBlk = 1024 + 32;            // bytes per channel-pair block
+constexpr uint32_t kFbSlot = 8 * kFbBlk;          // one step (16 channels)
 enum : uint32_t { kPHR = 0, kPLR, kPHI, kPLI, kPNHR, kPNLR, kPNHI, kPNLI };
 
 typedef int v2i_t __attribute__((ext_vector_type(2)));
@@ -847,10 +864,50 @@ struct FbGeo {
   static constexpr uint32_t BUF = kFbPlanes * PLANE;            // one step's planes
   static constexpr uint32_t RED = kFbNB * 36 * 64 * 4;          // covariance partials [bin][36][64] int32
   static constexpr uint32_t STAGE = 2 * BUF > RED ? 2 * BUF : RED;  // double-buffered planes, aliased by RED
+  static constexpr uint32_t RAW = kFbRing * kFbSlot;               // the loader's ring
+  static_assert(FS == 32, "a raw block holds 32 frames of a channel pair");
   static size_t lds(uint32_t bt_n) {
-    return STAGE + bf_frag_words(bt_n, 1) * 4 + (size_t)bt_n * 32 * 4 + kFbNB * 16 * 4 * 4;
+    return STAGE + RAW + bf_frag_words(bt_n, 1) * 4 + (size_t)bt_n * 32 * 4 + kFbNB * 16 * 4 * 4;
   }
 };
+
+// A block barrier that orders LDS only: the fences are local-address-space
+// ones, so waves with global stores in flight do not wait for them (a plain
+// __syncthreads would also make them visible, a full vmcnt drain).
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((7 << 4) | 0xC00F);  // lgkmcnt(0) only
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// global_load_lds_dwordx4: lane l's 16 bytes at src go to LDS dst + 16 l (dst
+// wave-uniform). Written as asm so the compiler's wait insertion, which would
+// otherwise treat every later LDS read of the function as a possible reader
+// of the transfer, leaves the compute waves alone; the loader counts its own
+// transfers with wait_vmcnt. No register receives data, so nothing is left
+// in flight behind the compiler's back.
+__device__ __forceinline__ void lds_dma16(const void* src, void* dst) {
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst);
+  __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 40 (the immediate is a constant:
+// vmcnt low bits 3:0, high bits 15:14; expcnt 7 and lgkmcnt 15 do not wait).
+__device__ __forceinline__ void wait_vmcnt(uint32_t n) {
+#define RD_VMW(k) \
+  case k: __builtin_amdgcn_s_waitcnt(((k) & 15) | (((k) >> 4) << 14) | (7 << 4) | (15 << 8)); break;
+  switch (n) {
+    RD_VMW(1) RD_VMW(2) RD_VMW(3) RD_VMW(4) RD_VMW(5) RD_VMW(6) RD_VMW(7) RD_VMW(8)
+    RD_VMW(9) RD_VMW(10) RD_VMW(11) RD_VMW(12) RD_VMW(13) RD_VMW(14) RD_VMW(15) RD_VMW(16)
+    RD_VMW(17) RD_VMW(18) RD_VMW(19) RD_VMW(20) RD_VMW(21) RD_VMW(22) RD_VMW(23) RD_VMW(24)
+    RD_VMW(25) RD_VMW(26) RD_VMW(27) RD_VMW(28) RD_VMW(29) RD_VMW(30) RD_VMW(31) RD_VMW(32)
+    RD_VMW(33) RD_VMW(34) RD_VMW(35) RD_VMW(36) RD_VMW(37) RD_VMW(38) RD_VMW(39) RD_VMW(40)
+    default: __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8)); break;  // vmcnt(0)
+  }
+#undef RD_VMW
+}
 
 bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_beams) {
   // 32-bit byte offsets from a per-step base cover every channel row
@@ -858,13 +915,16 @@ bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_
          n_frames >= 1 && n_frames <= 65535 && (uint64_t)16 * n_frames * n_bins * 4 < (1ull << 32);
 }
 
-template <int WAVES, int OUT>
-__global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
+// T1: the beams fill exactly one 16-beam tile (B = 16): no beam bounds, and a
+// lane's four beam constants stay in registers over the whole frame loop.
+template <int WAVES, int OUT, bool T1>
+__global__ __launch_bounds__(64 * (WAVES + 1)) void bfcov_kernel(BfCovJob J) {
   using G = FbGeo<WAVES>;
   constexpr uint32_t FS = G::FS;
   extern __shared__ uint32_t lds[];
   unsigned char* const sm = reinterpret_cast<unsigned char*>(lds);
-  uint32_t* frag = reinterpret_cast<uint32_t*>(sm + G::STAGE);
+  unsigned char* const raw = sm + G::STAGE;  // the loader's ring of raw steps
+  uint32_t* frag = reinterpret_cast<uint32_t*>(raw + G::RAW);
   int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(J.bf.bt_n, 1));
   int32_t* sums = tail + J.bf.bt_n * 32;  // [bin][16 rows][4] digit sums of the covariance A rows
   const int out = OUT >= 0 ? OUT : bf_out_mask(J.bf);
@@ -872,6 +932,7 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   const uint64_t cols = (uint64_t)F * N;
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t grp = lane >> 4, cl = lane & 15;
+  const bool loader = wave == WAVES;  // the last wave only loads
 
   // XCD-aware tile order: consecutive blocks are dealt to the 8 XCDs in turn,
   // so the blocks one XCD runs together take adjacent bins and share the L2
@@ -880,36 +941,62 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   const uint32_t tile = nblk % 8 == 0 ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
   const uint32_t k0 = tile * kFbNB;
 
-  // conversion unit of this thread: channel quad q, bin b, frame f of the step
+  // conversion unit of a compute thread: channel quad q, bin b, frame f of the step
   const uint32_t t = threadIdx.x;
-  const uint32_t q = t & 3, cb = (t >> 2) & 3, cf = t >> 4;
-  // X natural [ch][F][N]: a step of frames is FS * N words on from the last
-  // and a frame N words; bin-quad blocked [ch][N/4][F][4]: the block's 4 bins
-  // of every frame are one contiguous run per channel (4 words a frame)
+  const uint32_t q = t & 3, cb = (t >> 2) & 3, cf = (t >> 4) & (FS - 1);
+  // X natural [ch][F][N]: a frame is N words on from the last; bin-quad
+  // blocked [ch][N/4][F][4]: the block's 4 bins of every frame are one
+  // contiguous run per channel (4 words a frame)
   const bool xb = J.x_blocked != 0;
   const uint32_t fstride = xb ? 4u : N;
-  uint32_t off[4];  // byte offsets of the unit's four channel words from the step base
   uint32_t chmask = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < 4; ++i) {
-    off[i] = ((4 * q + i) * (uint32_t)F * N + cf * fstride + cb) * 4u;
-    chmask |= 4 * q + i < CH ? 0xFFu << (8 * i) : 0u;
-  }
+  for (uint32_t i = 0; i < 4; ++i) chmask |= 4 * q + i < CH ? 0xFFu << (8 * i) : 0u;
+  // the unit's four channel words in a raw step: channel 4q + i sits in block
+  // 2q + i / 2, half i % 2
+  const uint32_t roff = q * 2 * kFbBlk + cf * 16 + cb * 4;
   const uint32_t nsteps = (F + FS - 1) / FS;
   const uint32_t* const X0 = J.bf.X + (xb ? (size_t)k0 * F : (size_t)k0);  // k0 is a multiple of 4
-  auto fetch = [&](uint32_t st, uint32_t (&w)[4]) {
-    const unsigned char* base = reinterpret_cast<const unsigned char*>(X0 + (size_t)st * FS * fstride);
-    const bool fok = st * FS + cf < F;
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i)
-      w[i] = (fok && 4 * q + i < CH) ? *reinterpret_cast<const uint32_t*>(base + off[i]) : 0u;
-  };
-  // the first step's loads are in flight while the block derives the weight digits
-  uint32_t w[4];
-  fetch(0, w);
-  bf_weights_to_lds(J.bf.W, CH, B, 16, 1, J.bf.bt_n, frag, tail);
+  // the compute waves derive the weight digits while the first steps load
+  if (!loader) bf_weights_to_lds(J.bf.W, CH, B, 16, 1, J.bf.bt_n, frag, tail, 64 * WAVES);
 
-  // beamformer lane roles: column c = 16 * wave + cl (frame c / 4, bin c % 4),
+  if (loader) {
+    // lane l of block j moves channel 2j + l / 32, frame l % 32 of the step
+    // (16 bytes: the block's 4 bins); channels past CH and frames past F
+    // re-read the last ones (every block is one instruction for every lane,
+    // so the counted waits below stay exact) and are masked by the conversion
+    const uint32_t n_ld = (CH + 1) / 2;  // blocks per step
+    const uint32_t half = lane >> 5, lf = lane & 31;
+    auto issue = [&](uint32_t st) {
+      unsigned char* slot = raw + (st % kFbRing) * kFbSlot;
+      const uint32_t fr = min(st * FS + lf, F - 1);
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) {
+        if (j >= n_ld) break;
+        const uint32_t ch = min(2 * j + half, CH - 1);
+        const uint32_t* src = X0 + (size_t)ch * F * N + (size_t)fr * fstride;
+        lds_dma16(src, slot + j * kFbBlk);
+      }
+    };
+    // loads issued so far cover steps [0, next); before the barrier that
+    // releases step s, the loads of step s are complete
+    uint32_t next = 0;
+    for (; next < nsteps && next < kFbRing; ++next) issue(next);
+    wait_vmcnt((next - 1) * n_ld);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();  // releases step 0
+    for (uint32_t st = 0; st < nsteps; ++st) {
+      // step st + 1 complete: the later steps issued may stay in flight
+      wait_vmcnt(st + 1 < next ? (next - st - 2) * n_ld : 0u);
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();  // the compute waves' barrier of step st
+      __asm__ volatile("" ::: "memory");
+      // slot st % kFbRing was converted by every wave before that barrier
+      if (next < nsteps) issue(next++);
+    }
+  }
+
+  // beamformer lane roles: column c = 16 * wave + cl  // beamformer lane roles: column c = 16 * wave + cl (frame c / 4, bin c % 4),
   // K rows: channel half grp & 1, real (grp < 2) or imaginary part
   const uint32_t bc = 16 * wave + cl, bfr = bc >> 2, bbin = bc & 3;
   const bool bim = grp >= 2;
@@ -917,6 +1004,7 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   const uint32_t p_bih = bim ? kPHI : kPHR, p_bil = bim ? kPLI : kPLR;
   const uint32_t p_brh = bim ? kPNHI : kPHR, p_brl = bim ? kPNLI : kPLR;
   const int32_t kconst_bf = 16384 * 2 * (int32_t)CH;
+  const size_t lane_row = (size_t)(4 * grp) * cols;  // output row of the lane's first beam
   const uint64_t ONES = 0x0101010101010101ull;
 
   // covariance lane roles: bin cbin, frames 16 * cks + 8 * (grp & 1) + (cl >> 1)
@@ -935,10 +1023,27 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   for (int z = 0; z < 9; ++z) acc[z] = v4i{0, 0, 0, 0};
   const uint32_t cq = cl >> 2;
 
-  for (uint32_t st = 0; st < nsteps; ++st) {
+  if (!loader) __syncthreads();  // the loader's step 0 is in the ring
+  // T1: the rounding constants 128 * (beam sum) + c of the lane's beams
+  // 4 grp + r, in the accumulator layout (row r of the lane): they start the
+  // LL products' accumulation, so the epilogue does not add them
+  v4i c_re = {0, 0, 0, 0}, c_im = {0, 0, 0, 0};
+  if constexpr (T1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t sa = tail[2 * (4 * grp + r)], wis = tail[2 * (4 * grp + r) + 1];
+      c_re[r] = (sa << 7) + wis - kconst_bf;
+      c_im[r] = (sa << 7) - kconst_bf;
+    }
+  }
+  for (uint32_t st = 0; st < nsteps && !loader; ++st) {
     unsigned char* P = sm + (st & 1) * G::BUF;
     const uint32_t m0 = st * FS;
     {  // digits of this thread's four channel words into the planes
+      const unsigned char* rs = raw + (st % kFbRing) * kFbSlot + roff;
+      const uint32_t w[4] = {*reinterpret_cast<const uint32_t*>(rs), *reinterpret_cast<const uint32_t*>(rs + 512),
+                             *reinterpret_cast<const uint32_t*>(rs + kFbBlk),
+                             *reinterpret_cast<const uint32_t*>(rs + kFbBlk + 512)};
       const uint32_t bm = m0 + cf < F ? chmask : 0u;
       const uint32_t hr = gather4(w[0], w[1], w[2], w[3], 1) & bm;
       const uint32_t lr = (gather4(w[0], w[1], w[2], w[3], 0) ^ 0x80808080u) & bm;
@@ -966,8 +1071,7 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
         }
       }
     }
-    if (st + 1 < nsteps) fetch(st + 1, w);
-    __syncthreads();  // the planes of this step are complete (and the step before last is consumed)
+    lds_barrier();  // the planes of this step are complete (and the step before last is consumed)
 
     {  // beamformer: 16 columns x every beam tile
       const uint32_t m = m0 + bfr;
@@ -989,21 +1093,26 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
         v4i rhh = mfma8(are_h, brh, z4);
         v4i rx = mfma8(are_h, brl, z4);
         rx = mfma8(are_l, brh, rx);
-        v4i rll = mfma8(are_l, brl, z4);
+        v4i rll = mfma8(are_l, brl, c_re);  // + the beam constants (T1; else zero)
         v4i ihh = mfma8(aim_h, bih, z4);
         v4i ix = mfma8(aim_h, bil, z4);
         ix = mfma8(aim_l, bih, ix);
-        v4i ill = mfma8(aim_l, bil, z4);
+        v4i ill = mfma8(aim_l, bil, c_im);
         if (!col_ok) continue;
-        const size_t o0 = (size_t)(bt * 16 + 4 * grp) * cols + col;
+        const size_t o0 = (size_t)(bt * 16) * cols + lane_row + col;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t b = bt * 16 + 4 * grp + r;
-          if (b >= B) continue;
-          const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
+          int32_t cr = 0, ci = 0;  // T1: already in rll / ill
+          if constexpr (!T1) {
+            if (b >= B) continue;
+            const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
+            cr = (sa << 7) + wis - kconst_bf;
+            ci = (sa << 7) - kconst_bf;
+          }
           uint32_t fr = 0, fi = 0;
-          const int32_t yr = rne15_sat_small(rhh[r], rx[r], rll[r], sb_re << 7, (sa << 7) + wis - kconst_bf, fr);
-          const int32_t yi = rne15_sat_small(ihh[r], ix[r], ill[r], sb_im << 7, (sa << 7) - kconst_bf, fi);
+          const int32_t yr = rne15_sat_small(rhh[r], rx[r], rll[r], sb_re << 7, cr, fr);
+          const int32_t yi = rne15_sat_small(ihh[r], ix[r], ill[r], sb_im << 7, ci, fi);
           const size_t o = o0 + (size_t)r * cols;
           if (out & kBfOutY) J.bf.y[o] = cpack(yr, yi);
           if (out & kBfOutF) J.bf.flags[o] = (uint8_t)(fr | (fi << 2));
@@ -1042,7 +1151,7 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
       for (int r = 0; r < 4; ++r) red[(4 * z + r) * 64] = acc[z][r];
   }
   __syncthreads();
-  if (cks != 0) {
+  if (cks != 0 && !loader) {
 #pragma unroll
     for (int z = 0; z < 9; ++z)
 #pragma unroll
@@ -1087,24 +1196,31 @@ __global__ __launch_bounds__(64 * WAVES) void bfcov_kernel(BfCovJob J) {
   }
 }
 
-template <int WAVES>
-static hipError_t launch_bfcov_w(hipStream_t s, const BfCovJob& J) {
+template <int WAVES, bool T1>
+static hipError_t launch_bfcov_t(hipStream_t s, const BfCovJob& J) {
   const size_t lds = FbGeo<WAVES>::lds(J.bf.bt_n);
   const dim3 grid(J.N / kFbNB);
   const int out = bf_out_mask(J.bf);
   if (out == kBfOutP)
-    bfcov_kernel<WAVES, kBfOutP><<<grid, 64 * WAVES, lds, s>>>(J);
+    bfcov_kernel<WAVES, kBfOutP, T1><<<grid, 64 * (WAVES + 1), lds, s>>>(J);
   else if (out == (kBfOutY | kBfOutF | kBfOutP))
-    bfcov_kernel<WAVES, kBfOutY | kBfOutF | kBfOutP><<<grid, 64 * WAVES, lds, s>>>(J);
+    bfcov_kernel<WAVES, kBfOutY | kBfOutF | kBfOutP, T1><<<grid, 64 * (WAVES + 1), lds, s>>>(J);
   else
-    bfcov_kernel<WAVES, -1><<<grid, 64 * WAVES, lds, s>>>(J);
+    bfcov_kernel<WAVES, -1, T1><<<grid, 64 * (WAVES + 1), lds, s>>>(J);
   return hipGetLastError();
+}
+
+template <int WAVES>
+static hipError_t launch_bfcov_w(hipStream_t s, const BfCovJob& J) {
+  return J.bf.B == 16 ? launch_bfcov_t<WAVES, true>(s, J) : launch_bfcov_t<WAVES, false>(s, J);
 }
 
 hipError_t launch_bfcov(hipStream_t s, const uint32_t* X, const uint32_t* W, uint32_t n_chan, uint32_t n_beams,
                         uint32_t n_frames, uint32_t n_bins, uint32_t* y, uint8_t* flags, uint32_t* power,
                         uint32_t* xa_out, int64_t* R, bool x_blocked) {
-  if (!bfcov_fits(n_chan, n_bins, n_frames, n_beams) || !R) return hipErrorInvalidValue;
+  // the loader's 16-byte LDS loads need X 16-byte aligned
+  if (!bfcov_fits(n_chan, n_bins, n_frames, n_beams) || !R || (reinterpret_cast<uintptr_t>(X) & 15u))
+    return hipErrorInvalidValue;
   BfCovJob J{};
   J.x_blocked = x_blocked ? 1u : 0u;
   J.bf = BfJob{X, W, y, flags, power, nullptr, xa_out, (uint64_t)n_frames * n_bins, n_chan, n_beams,

@@ -231,8 +231,8 @@ rd_status rd_bf_stage(rd_ctx* ctx, const uint32_t* X, const uint32_t* W, uint32_
  * beamformer_model.hpp:145-325; covar::power and the Integrator's block-mode
  * window sum, covar_model.hpp:99-151, 186-262), read once from X
  * [n_ant][n_frames][n_bins]. One kernel for n_ant <= 16 with n_bins a
- * multiple of 4 (Xa nullable: written when set); otherwise the two stages,
- * and then Xa is required. R [n_bins][n_ant*(n_ant+1)/2][2] is required;
+ * multiple of 4 (X 16-byte aligned, else RD_E_ARG; Xa nullable: written when
+ * set); otherwise the two stages, and then Xa is required. R [n_bins][n_ant*(n_ant+1)/2][2] is required;
  * y/flags/power are nullable. x_layout: X in natural order (0) or bin-quad
  * blocked [n_ant][n_bins/4][n_frames][4] (1, the one-kernel form only; what
  * rd_fft_stage writes with x_layout 1, the chain's fast layout). */

@@ -104,6 +104,20 @@ def test_unfused_geometries_take_the_two_kernels(gpu, orc, CH, B, F, N):
             gpu.bf_cov_stage(X, W, CH, B, F, N, x_layout=1)
 
 
+def test_fused_refuses_a_misaligned_x(gpu):
+    """The one-kernel form moves X in 16-byte pieces: an X that is not 16-byte
+    aligned is refused (RD_E_ARG, nothing launched)."""
+    CH, B, F, N = 16, 16, 4, 8
+    dX = gpu.put(np.zeros(CH * F * N + 4, np.uint32))
+    dW = gpu.put(np.zeros(B * CH, np.uint32))
+    dR = gpu.buf(16 * N * CH * (CH + 1) // 2)
+    st = gpu.L.rd_bf_cov_stage(gpu.h, dX.ptr + 4, dW.ptr, CH, B, F, N, None, None, None, None, dR.ptr, 0)
+    assert st == -1, st  # RD_E_ARG
+    gpu.lib.check("rd_bf_cov_stage", gpu.L.rd_bf_cov_stage(gpu.h, dX.ptr + 16, dW.ptr, CH, B, F, N, None, None,
+                                                           None, None, dR.ptr, 0))
+    gpu.sync()
+
+
 def test_fused_equals_the_two_kernels(gpu):
     """The fused kernel and rd_bf_stage + rd_cov_run give the same bits on a
     medium-sized CPI."""
