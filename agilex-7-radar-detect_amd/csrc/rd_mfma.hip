@@ -175,6 +175,44 @@ __device__ __forceinline__ void bf_weights_to_lds(const uint32_t* __restrict__ W
   }
 }
 
+// A block barrier that orders LDS only: the fences are local-address-space
+// ones, so waves with global stores in flight do not wait for them (a plain
+// __syncthreads would also make them visible, a full vmcnt drain).
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((7 << 4) | 0xC00F);  // lgkmcnt(0) only
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// global_load_lds_dwordx4: lane l's 16 bytes at src go to LDS dst + 16 l (dst
+// wave-uniform). Written as asm so the compiler's wait insertion, which would
+// otherwise treat every later LDS read of the function as a possible reader
+// of the transfer, leaves the compute waves alone; the loader counts its own
+// transfers with wait_vmcnt. No register receives data, so nothing is left
+// in flight behind the compiler's back.
+__device__ __forceinline__ void lds_dma16(const void* src, void* dst) {
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst);
+  __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 40 (the immediate is a constant:
+// vmcnt low bits 3:0, high bits 15:14; expcnt 7 and lgkmcnt 15 do not wait).
+__device__ __forceinline__ void wait_vmcnt(uint32_t n) {
+#define RD_VMW(k) \
+  case k: __builtin_amdgcn_s_waitcnt(((k) & 15) | (((k) >> 4) << 14) | (7 << 4) | (15 << 8)); break;
+  switch (n) {
+    RD_VMW(1) RD_VMW(2) RD_VMW(3) RD_VMW(4) RD_VMW(5) RD_VMW(6) RD_VMW(7) RD_VMW(8)
+    RD_VMW(9) RD_VMW(10) RD_VMW(11) RD_VMW(12) RD_VMW(13) RD_VMW(14) RD_VMW(15) RD_VMW(16)
+    RD_VMW(17) RD_VMW(18) RD_VMW(19) RD_VMW(20) RD_VMW(21) RD_VMW(22) RD_VMW(23) RD_VMW(24)
+    RD_VMW(25) RD_VMW(26) RD_VMW(27) RD_VMW(28) RD_VMW(29) RD_VMW(30) RD_VMW(31) RD_VMW(32)
+    RD_VMW(33) RD_VMW(34) RD_VMW(35) RD_VMW(36) RD_VMW(37) RD_VMW(38) RD_VMW(39) RD_VMW(40)
+    default: __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8)); break;  // vmcnt(0)
+  }
+#undef RD_VMW
+}
+
 // LDS of a beamformer block for at most bt beam tiles and ch channels: weight
 // digits, per-beam sums, and a column tile [TC][ch + 1] (odd row stride:
 // conflict-free).
@@ -401,6 +439,197 @@ __global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_batch_kernel(const
   }
 }
 
+// ============================================================================
+// Beamformer over the FFT layout for CPIs whose channels fill 3 or 4 K steps
+// (48 or 64 channels), beams a multiple of 16 and columns a multiple of 128:
+// a persistent block of 8 compute waves and one loader wave. The loader moves
+// each 128-column tile of X (64 channel rows of 512 bytes) into an LDS ring of
+// two slots with global_load_lds_dwordx4 (one instruction per channel pair)
+// and counts its own transfers; the compute waves issue no vector-memory load
+// in the tile loop, so their power and aligned-copy stores are never waited
+// for. Per tile the compute waves copy the tile out as Xa[col][ch] (if
+// requested) and build their B fragments from the slot, meet the loader at one
+// LDS-only barrier (after which the slot is reloaded two tiles ahead), then run
+// the MFMAs and the epilogue of their 16 columns over every beam tile.
+// ============================================================================
+constexpr uint32_t kBrTC = 128;        // columns per tile
+constexpr uint32_t kBrBlk = 1024 + 16;  // bytes per channel-pair block (two rows of 128 words + padding)
+constexpr uint32_t kBrRing = 2;
+
+template <int STEPS>
+struct BrGeo {
+  static constexpr uint32_t CHP = STEPS * 16;
+  static constexpr uint32_t NBLK = CHP / 2;  // loader instructions per tile
+  static constexpr uint32_t SLOT = NBLK * kBrBlk;
+  static size_t lds(uint32_t bt_n) {
+    return (size_t)kBrRing * SLOT + bf_frag_words(bt_n, STEPS) * 4 + (size_t)bt_n * 32 * 4;
+  }
+};
+
+// Byte offset of word `col` of channel row `ch` in a slot.
+__device__ __forceinline__ uint32_t br_off(uint32_t ch, uint32_t col) {
+  return (ch >> 1) * kBrBlk + (ch & 1) * 512 + col * 4;
+}
+
+bool bf_ring_fits(const BfJob& J, uint32_t steps) {
+  return (steps == 3 || steps == 4) && J.CH == steps * 16 && J.B % 16 == 0 && J.cols % kBrTC == 0 &&
+         J.cols / kBrTC > 0 && !J.acc && (reinterpret_cast<uintptr_t>(J.X) & 15u) == 0;
+}
+
+template <int STEPS, int OUT>
+__global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
+  using G = BrGeo<STEPS>;
+  constexpr uint32_t CHP = G::CHP;
+  extern __shared__ uint32_t lds[];
+  unsigned char* const raw = reinterpret_cast<unsigned char*>(lds);
+  uint32_t* frag = reinterpret_cast<uint32_t*>(raw + kBrRing * G::SLOT);
+  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(J.bt_n, STEPS));
+  const int out = OUT >= 0 ? OUT : bf_out_mask(J);
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool loader = wave == 8;
+  const uint64_t cols = J.cols;
+  const uint64_t n_tiles = cols / kBrTC;
+  // this block's tiles: blockIdx.x + k * gridDim.x, k < nt
+  const uint32_t nt = (uint32_t)((n_tiles - blockIdx.x + gridDim.x - 1) / gridDim.x);
+  if (!loader) bf_weights_to_lds(J.W, CHP, J.B, CHP, STEPS, J.bt_n, frag, tail, 512);
+
+  if (loader) {
+    // lane l: channel 2j + l / 32 of block j, columns 4 (l % 32) .. + 3
+    const uint32_t half = lane >> 5, lc = (lane & 31) * 4;
+    auto issue = [&](uint32_t k) {
+      unsigned char* slot = raw + (k % kBrRing) * G::SLOT;
+      const uint64_t c0 = ((uint64_t)blockIdx.x + (uint64_t)k * gridDim.x) * kBrTC;
+      const uint32_t* src = J.X + (size_t)half * cols + c0 + lc;
+#pragma unroll
+      for (uint32_t j = 0; j < G::NBLK; ++j) lds_dma16(src + (size_t)(2 * j) * cols, slot + j * kBrBlk);
+    };
+    uint32_t next = 0;
+    for (; next < nt && next < kBrRing; ++next) issue(next);
+    wait_vmcnt((next - 1) * G::NBLK);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();  // tile 0 is in the ring
+    for (uint32_t k = 0; k < nt; ++k) {
+      wait_vmcnt(k + 1 < next ? (next - k - 2) * G::NBLK : 0u);  // tile k + 1 complete
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();  // the compute waves are done with slot k % kBrRing
+      __asm__ volatile("" ::: "memory");
+      if (next < nt) issue(next++);
+    }
+    return;
+  }
+
+  __syncthreads();  // weights in LDS, tile 0 in the ring
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+  const int32_t kconst = 16384 * 2 * (int32_t)CHP;
+  const uint32_t j = 16 * wave + cl;  // this lane's column of the tile
+  const size_t lane_row = (size_t)(4 * grp) * cols;
+  for (uint32_t k = 0; k < nt; ++k) {
+    const unsigned char* slot = raw + (k % kBrRing) * G::SLOT;
+    const uint64_t c0 = ((uint64_t)blockIdx.x + (uint64_t)k * gridDim.x) * kBrTC;
+    if (J.xa_out) {
+      // the aligned copy: 16-byte stores of four consecutive channels of one
+      // column, consecutive threads on consecutive quads
+      uint4* dst = reinterpret_cast<uint4*>(J.xa_out + c0 * CHP);
+      constexpr uint32_t PER = kBrTC * CHP / 4 / 512;
+#pragma unroll
+      for (uint32_t it = 0; it < PER; ++it) {
+        const uint32_t q = threadIdx.x + 512 * it, w = 4 * q, col = w / CHP, a = w % CHP;
+        const unsigned char* sp = slot + br_off(a, col);
+        dst[q] = uint4{*reinterpret_cast<const uint32_t*>(sp), *reinterpret_cast<const uint32_t*>(sp + 512),
+                       *reinterpret_cast<const uint32_t*>(sp + kBrBlk),
+                       *reinterpret_cast<const uint32_t*>(sp + kBrBlk + 512)};
+      }
+    }
+    // B fragments for every step: Bim = [Xr; Xi] digits, Bre = [Xr; ~Xi] digits
+    uint64_t bih[STEPS], bil[STEPS], brh[STEPS], brl[STEPS];
+    int32_t ps_r = 0, ps_i = 0;
+#pragma unroll
+    for (int st = 0; st < STEPS; ++st) {
+      const uint32_t k0 = 32 * st + 8 * grp;
+      const bool imag = k0 >= CHP;
+      const uint32_t a0 = imag ? k0 - CHP : k0;
+      uint32_t p[8];
+      const unsigned char* sp = slot + br_off(a0, j);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        p[q] = *reinterpret_cast<const uint32_t*>(sp + (q >> 1) * kBrBlk + (q & 1) * 512);
+      const uint32_t bh = imag ? 3u : 1u, bl = bh - 1u;
+      const uint32_t sh01 = sel01(bh), sh23 = sel23(bh), sl01 = sel01(bl), sl23 = sel23(bl);
+      const uint64_t h = (uint64_t)gather4s(p[0], p[1], p[2], p[3], sh01, sh23) |
+                         ((uint64_t)gather4s(p[4], p[5], p[6], p[7], sh01, sh23) << 32);
+      const uint64_t l = ((uint64_t)gather4s(p[0], p[1], p[2], p[3], sl01, sl23) |
+                          ((uint64_t)gather4s(p[4], p[5], p[6], p[7], sl01, sl23) << 32)) ^
+                         0x8080808080808080ull;
+      int32_t sp16 = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sp16 += __builtin_amdgcn_sbfe((int32_t)p[q], imag ? 16u : 0u, 16u);
+      const uint64_t inv = imag ? ~0ull : 0ull;
+      bih[st] = h;
+      bil[st] = l;
+      brh[st] = h ^ inv;
+      brl[st] = l ^ inv;
+      if (imag)
+        ps_i += sp16;
+      else
+        ps_r += sp16;
+    }
+    ps_r += __shfl_xor(ps_r, 16);
+    ps_r += __shfl_xor(ps_r, 32);
+    ps_i += __shfl_xor(ps_i, 16);
+    ps_i += __shfl_xor(ps_i, 32);
+    const int32_t sb_im = ps_r + ps_i;
+    const int32_t sb_re = ps_r - ps_i - (int32_t)CHP;
+    lds_barrier();  // every wave is done with this slot: the loader refills it
+
+    const uint64_t col = c0 + j;
+    for (uint32_t bt = 0; bt < J.bt_n; ++bt) {
+      v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
+      v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) {
+        const uint64_t* a = reinterpret_cast<const uint64_t*>(frag) + ((size_t)bt * STEPS + st) * 256 + lane;
+        const uint64_t are_h = a[0], are_l = a[64], aim_h = a[128], aim_l = a[192];
+        rhh = mfma8(are_h, brh[st], rhh);
+        rx = mfma8(are_h, brl[st], rx);
+        rx = mfma8(are_l, brh[st], rx);
+        rll = mfma8(are_l, brl[st], rll);
+        ihh = mfma8(aim_h, bih[st], ihh);
+        ix = mfma8(aim_h, bil[st], ix);
+        ix = mfma8(aim_l, bih[st], ix);
+        ill = mfma8(aim_l, bil[st], ill);
+      }
+      const size_t o0 = (size_t)(bt * 16) * cols + lane_row + col;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t b = bt * 16 + 4 * grp + r;
+        const int32_t sa = tail[2 * b], wis = tail[2 * b + 1];
+        uint32_t fr = 0, fi = 0;
+        const int32_t yr = rne15_sat_parts(rhh[r], rx[r], rll[r], sa + sb_re, wis - kconst, fr);
+        const int32_t yi = rne15_sat_parts(ihh[r], ix[r], ill[r], sa + sb_im, -kconst, fi);
+        const size_t o = o0 + (size_t)r * cols;
+        if (out & kBfOutY) J.y[o] = cpack(yr, yi);
+        if (out & kBfOutF) J.flags[o] = (uint8_t)(fr | (fi << 2));
+        if (out & kBfOutP) J.power[o] = (uint32_t)(yr * yr) + (uint32_t)(yi * yi);
+      }
+    }
+  }
+}
+
+template <int STEPS>
+static hipError_t launch_bf_ring(hipStream_t s, const BfJob& J) {
+  const uint64_t tiles = J.cols / kBrTC;
+  const dim3 grid((uint32_t)(tiles < 256 ? tiles : 256));  // one resident block per CU (LDS)
+  const size_t lds = BrGeo<STEPS>::lds(J.bt_n);
+  const int out = bf_out_mask(J);
+  if (out == kBfOutP)
+    bf_ring_kernel<STEPS, kBfOutP><<<grid, 64 * 9, lds, s>>>(J);
+  else if (out == (kBfOutY | kBfOutF | kBfOutP))
+    bf_ring_kernel<STEPS, kBfOutY | kBfOutF | kBfOutP><<<grid, 64 * 9, lds, s>>>(J);
+  else
+    bf_ring_kernel<STEPS, -1><<<grid, 64 * 9, lds, s>>>(J);
+  return hipGetLastError();
+}
+
 template <int STEPS>
 static hipError_t launch_bf_s(hipStream_t s, bool chmajor, const BfJob& J) {
   constexpr uint32_t TC = STEPS <= 2 ? 256 : 128;  // columns per tile
@@ -408,6 +637,9 @@ static hipError_t launch_bf_s(hipStream_t s, bool chmajor, const BfJob& J) {
   const dim3 grid((uint32_t)(tiles < 2048 ? tiles : 2048));
   const size_t lds = BfLds<STEPS, TC>::words(J.bt_n, J.CH) * sizeof(uint32_t);
   const int out = bf_out_mask(J);
+  if constexpr (STEPS == 3 || STEPS == 4) {
+    if (chmajor && bf_ring_fits(J, STEPS)) return launch_bf_ring<STEPS>(s, J);
+  }
   if (chmajor) {
     if (out == kBfOutP)
       bf_kernel<STEPS, TC, true, kBfOutP><<<grid, 256, lds, s>>>(J);
@@ -871,43 +1103,6 @@ struct FbGeo {
   }
 };
 
-// A block barrier that orders LDS only: the fences are local-address-space
-// ones, so waves with global stores in flight do not wait for them (a plain
-// __syncthreads would also make them visible, a full vmcnt drain).
-__device__ __forceinline__ void lds_barrier() {
-  __asm__ volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt((7 << 4) | 0xC00F);  // lgkmcnt(0) only
-  __builtin_amdgcn_s_barrier();
-  __asm__ volatile("" ::: "memory");
-}
-
-// global_load_lds_dwordx4: lane l's 16 bytes at src go to LDS dst + 16 l (dst
-// wave-uniform). Written as asm so the compiler's wait insertion, which would
-// otherwise treat every later LDS read of the function as a possible reader
-// of the transfer, leaves the compute waves alone; the loader counts its own
-// transfers with wait_vmcnt. No register receives data, so nothing is left
-// in flight behind the compiler's back.
-__device__ __forceinline__ void lds_dma16(const void* src, void* dst) {
-  const uint32_t m0 = __builtin_amdgcn_readfirstlane(
-      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst);
-  __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
-}
-
-// s_waitcnt vmcnt(n) for a wave-uniform n <= 40 (the immediate is a constant:
-// vmcnt low bits 3:0, high bits 15:14; expcnt 7 and lgkmcnt 15 do not wait).
-__device__ __forceinline__ void wait_vmcnt(uint32_t n) {
-#define RD_VMW(k) \
-  case k: __builtin_amdgcn_s_waitcnt(((k) & 15) | (((k) >> 4) << 14) | (7 << 4) | (15 << 8)); break;
-  switch (n) {
-    RD_VMW(1) RD_VMW(2) RD_VMW(3) RD_VMW(4) RD_VMW(5) RD_VMW(6) RD_VMW(7) RD_VMW(8)
-    RD_VMW(9) RD_VMW(10) RD_VMW(11) RD_VMW(12) RD_VMW(13) RD_VMW(14) RD_VMW(15) RD_VMW(16)
-    RD_VMW(17) RD_VMW(18) RD_VMW(19) RD_VMW(20) RD_VMW(21) RD_VMW(22) RD_VMW(23) RD_VMW(24)
-    RD_VMW(25) RD_VMW(26) RD_VMW(27) RD_VMW(28) RD_VMW(29) RD_VMW(30) RD_VMW(31) RD_VMW(32)
-    RD_VMW(33) RD_VMW(34) RD_VMW(35) RD_VMW(36) RD_VMW(37) RD_VMW(38) RD_VMW(39) RD_VMW(40)
-    default: __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8)); break;  // vmcnt(0)
-  }
-#undef RD_VMW
-}
 
 bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_beams) {
   // 32-bit byte offsets from a per-step base cover every channel row
