@@ -946,9 +946,170 @@ struct CovGeo {
 uint32_t cov_bins_per_tile(uint32_t chp) { return CovGeo(chp).nb; }
 uint32_t cov_max_splits(uint32_t chp) { return 16 / CovGeo(chp).jobs; }
 
+// ============================================================================
+// Covariance of 64-channel CPIs (one bin per block, its 10 upper-triangle
+// 16x16 channel tiles on 10 compute waves) fed by an 11th, loader wave: each
+// 32-frame step of the bin (32 runs of 256 bytes in Xa[F][N][64]) is moved into
+// a ring of raw LDS slots by global_load_lds_dwordx4 (eight instructions a
+// step) kCrRing steps ahead, with counted waits; the compute waves convert a
+// slot into digit planes (double-buffered, so one barrier a step orders both
+// the planes and the slot's release) and run the same MFMA products and exact
+// int64 recombination as cov_tile.
+// ============================================================================
+constexpr uint32_t kCrRing = 3;
+constexpr uint32_t kCrSlot = kCovFS * 64 * 4;  // one step: [32 frames][64 channels] words
+constexpr uint32_t kCrPlanes = 6 * kCovPlane;
+
+bool cov_ring_fits(uint32_t n_chan, uint32_t n_frames, uint32_t n_bins, const uint32_t* Xa) {
+  return n_chan == 64 && n_frames >= 1 && n_bins >= 1 && (reinterpret_cast<uintptr_t>(Xa) & 15u) == 0;
+}
+
+__global__ __launch_bounds__(64 * 11) void cov_ring_kernel(CovJob J) {
+  extern __shared__ uint32_t smem[];
+  unsigned char* const lds = reinterpret_cast<unsigned char*>(smem);
+  unsigned char* const raw = lds;                                   // [kCrRing][kCrSlot]
+  unsigned char* const planes0 = raw + kCrRing * kCrSlot;          // [2][6 planes]
+  int32_t* const sums = reinterpret_cast<int32_t*>(planes0 + 2 * kCrPlanes);  // [64 rows][2]
+  const uint32_t F = J.F, N = J.N;
+  const uint32_t k = blockIdx.x;  // the block's bin
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool loader = wave == 10;
+  const uint32_t nsteps = (F + kCovFS - 1) / kCovFS;
+  if (threadIdx.x < 128) sums[threadIdx.x] = 0;
+
+  if (loader) {
+    // lane l of instruction j: frame 4 j + l / 16 of the step, channels
+    // 4 (l % 16) .. + 3; frames past F re-read the last one (masked later)
+    const uint32_t fl = lane >> 4, c4 = (lane & 15) * 4;
+    auto issue = [&](uint32_t st) {
+      unsigned char* slot = raw + (st % kCrRing) * kCrSlot;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t m = min(st * kCovFS + 4 * j + fl, F - 1);
+        lds_dma16(J.Xa + ((size_t)m * N + k) * 64 + c4, slot + j * 1024);
+      }
+    };
+    uint32_t next = 0;
+    for (; next < nsteps && next < kCrRing; ++next) issue(next);
+    wait_vmcnt((next - 1) * 8);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();  // step 0 is in the ring (and sums are zero)
+    for (uint32_t st = 0; st < nsteps; ++st) {
+      wait_vmcnt(st + 1 < next ? (next - st - 2) * 8 : 0u);  // step st + 1 complete
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();  // the compute waves are done with slot st % kCrRing
+      __asm__ volatile("" ::: "memory");
+      if (next < nsteps) issue(next++);
+    }
+    __syncthreads();  // the frame sums (below)
+    return;
+  }
+
+  const uint32_t grp = lane >> 4, cl = lane & 15;
+  uint32_t u = wave, ti = 0;  // upper-triangle tile (ti, tj) of this wave
+  while (u >= 4 - ti) {
+    u -= 4 - ti;
+    ++ti;
+  }
+  const uint32_t tj = ti + u;
+  const uint32_t ra = 16 * ti + cl, rb = 16 * tj + cl;  // operand rows of this lane
+  v4i acc[8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) acc[z] = v4i{0, 0, 0, 0};
+  // conversion task of this thread (threads 0..511): row = lane (channel),
+  // frame group g = wave (4 frames)
+  const bool task = wave < 8;
+  const uint32_t g = wave;
+  int32_t srow_r = 0, srow_i = 0;
+  __syncthreads();  // step 0 is in the ring (and sums are zero)
+  for (uint32_t st = 0; st < nsteps; ++st) {
+    unsigned char* planes = planes0 + (st & 1) * kCrPlanes;
+    const uint32_t m0 = st * kCovFS;
+    if (task) {
+      const unsigned char* rs = raw + (st % kCrRing) * kCrSlot + (4 * g * 64 + lane) * 4;
+      const uint32_t w[4] = {*reinterpret_cast<const uint32_t*>(rs), *reinterpret_cast<const uint32_t*>(rs + 256),
+                             *reinterpret_cast<const uint32_t*>(rs + 512),
+                             *reinterpret_cast<const uint32_t*>(rs + 768)};
+      uint32_t bm = 0;  // byte mask of the valid frames among the four
+#pragma unroll
+      for (uint32_t t = 0; t < 4; ++t) bm |= m0 + 4 * g + t < F ? (0xFFu << (8 * t)) : 0u;
+      const uint32_t hr = gather4(w[0], w[1], w[2], w[3], 1) & bm;
+      const uint32_t lr = (gather4(w[0], w[1], w[2], w[3], 0) ^ 0x80808080u) & bm;
+      const uint32_t hi = gather4(w[0], w[1], w[2], w[3], 3) & bm;
+      const uint32_t li = (gather4(w[0], w[1], w[2], w[3], 2) ^ 0x80808080u) & bm;
+      unsigned char* o = planes + lane * kCovRow + 4 * g;
+      *reinterpret_cast<uint32_t*>(o) = hr;
+      *reinterpret_cast<uint32_t*>(o + kCovPlane) = lr;
+      *reinterpret_cast<uint32_t*>(o + 2 * kCovPlane) = hi;
+      *reinterpret_cast<uint32_t*>(o + 3 * kCovPlane) = li;
+      *reinterpret_cast<uint32_t*>(o + 4 * kCovPlane) = ~hr & bm;
+      *reinterpret_cast<uint32_t*>(o + 5 * kCovPlane) = ~lr & bm;
+#pragma unroll
+      for (uint32_t t = 0; t < 4; ++t) {
+        if ((bm >> (8 * t)) & 1u) {
+          srow_r += cre(w[t]);
+          srow_i += cim(w[t]);
+        }
+      }
+    }
+    lds_barrier();  // planes of step st complete; slot st % kCrRing consumed
+#pragma unroll
+    for (uint32_t ks = 0; ks < kCovFS / 16; ++ks) {
+      const uint32_t fb = 16 * ks + 8 * (grp & 1);
+      const bool imag = (grp >> 1) != 0;
+      const unsigned char* pa = planes + ra * kCovRow + fb;
+      const unsigned char* pb = planes + rb * kCovRow + fb;
+      const uint32_t pre = imag ? 2 : 0, pim = imag ? 4 : 2;
+      const uint64_t reah = *reinterpret_cast<const uint64_t*>(pa + pre * kCovPlane);
+      const uint64_t real_ = *reinterpret_cast<const uint64_t*>(pa + (pre + 1) * kCovPlane);
+      const uint64_t imah = *reinterpret_cast<const uint64_t*>(pa + pim * kCovPlane);
+      const uint64_t imal = *reinterpret_cast<const uint64_t*>(pa + (pim + 1) * kCovPlane);
+      const uint64_t bh = *reinterpret_cast<const uint64_t*>(pb + pre * kCovPlane);
+      const uint64_t bl = *reinterpret_cast<const uint64_t*>(pb + (pre + 1) * kCovPlane);
+      acc[0] = mfma8(reah, bh, acc[0]);
+      acc[1] = mfma8(reah, bl, acc[1]);
+      acc[3] = mfma8(real_, bh, acc[3]);
+      acc[2] = mfma8(real_, bl, acc[2]);
+      acc[4] = mfma8(imah, bh, acc[4]);
+      acc[5] = mfma8(imah, bl, acc[5]);
+      acc[7] = mfma8(imal, bh, acc[7]);
+      acc[6] = mfma8(imal, bl, acc[6]);
+    }
+  }
+  // per-row frame sums (the eight tasks of a row add up)
+  if (task && (srow_r | srow_i)) {
+    atomicAdd(&sums[2 * lane], srow_r);
+    atomicAdd(&sums[2 * lane + 1], srow_i);
+  }
+  __syncthreads();
+  const int64_t nfr = F;
+  const int32_t b_r = sums[2 * rb], b_i = sums[2 * rb + 1];
+  const int64_t kconst = 16384LL * 2 * nfr;
+  const int64_t tjs = (int64_t)b_r + b_i;
+  const uint32_t cj = 16 * tj + cl;
+  int64_t* const Rk = J.R + (size_t)k * (64 * 65 / 2) * 2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t ii = 16 * ti + 4 * grp + r;
+    const int32_t ir = sums[2 * ii], iim = sums[2 * ii + 1];
+    const int64_t tis = (int64_t)ir + iim, uis = (int64_t)iim - ir - nfr;
+    if (ii > cj) continue;
+    const size_t pair = (size_t)ii * 64 - (size_t)ii * (ii - 1) / 2 + (cj - ii);
+    Rk[2 * pair] = combine(acc[0], acc[1], acc[2], r) + 256 * (int64_t)acc[3][r] + 128 * (tis + tjs) - kconst;
+    Rk[2 * pair + 1] =
+        combine(acc[4], acc[5], acc[6], r) + 256 * (int64_t)acc[7][r] + 128 * (uis + tjs) - kconst + b_i;
+  }
+}
+
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R) {
   if (n_bins == 0) return hipSuccess;
+  if (cov_ring_fits(n_chan, n_frames, n_bins, Xa)) {
+    const size_t lds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
+    const CovJob J{Xa, R, n_chan, n_frames, n_bins, 0};
+    cov_ring_kernel<<<n_bins, 64 * 11, lds, s>>>(J);
+    return hipGetLastError();
+  }
   const uint32_t chp = (n_chan + 15) / 16 * 16;
   const CovGeo G(chp);
   uint32_t fs_n = 16 / G.jobs;
