@@ -57,6 +57,7 @@ struct Shape {
                       // BF: columns per tile; COV: frame splits; CFAR: rows per tile
   uint32_t grid = 0;  // persistent grid size (0: one block per tile)
   uint32_t b = 0;     // CFAR: waves per row
+  uint32_t ring = 0;  // BF / COV: the loader-fed kernel (tiles of 128 columns / one bin)
 };
 
 struct Class {
@@ -195,6 +196,7 @@ rd_status upload(Class& c, const rd_batch& B, const Shape& sh) {
   c.L.tile_job = reinterpret_cast<const uint32_t*>(static_cast<uint8_t*>(c.dev) + jb);
   c.L.n_tiles = tiles;
   c.L.grid = (sh.grid && sh.grid < tiles) ? sh.grid : tiles;
+  if (sh.ring && c.stage == S_BF) c.L.grid = tiles < B.n_cu ? tiles : B.n_cu;  // what the launch uses
   return RD_OK;
 }
 
@@ -202,8 +204,10 @@ hipError_t launch_class(const rd_batch& B, const Class& c, hipStream_t s) {
   switch (c.stage) {
     case S_PFB: return rd::launch_pfb_batch(s, c.key & 0xFFu, (c.key >> 8) & 1u, c.L);
     case S_FFT: return rd::launch_fft_batch(s, B.ctx->tw, c.key & 15u, (c.key >> 4) & 15u, (c.key >> 8) & 1u, c.L);
-    case S_BF: return rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.out_mask, c.L);
-    case S_COV: return rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
+    case S_BF:
+      return c.shape.ring ? rd::launch_bf_ring_batch(s, c.key, c.max_bt, B.n_cu, c.out_mask, c.L)
+                          : rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.out_mask, c.L);
+    case S_COV: return c.shape.ring ? rd::launch_cov_ring_batch(s, c.L) : rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
     case S_CFAR_ROWS: return rd::launch_cfar_rows_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     case S_CFAR_GATHER: return rd::launch_cfar_gather_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     default: return hipErrorInvalidValue;
@@ -240,8 +244,32 @@ bool bf_fits(const Class& c, uint32_t tc) {
   return words * 4 <= 160 * 1024;
 }
 
+// Every CPI of the class fits the loader-fed kernel of its stage.
+bool ring_fits(const rd_batch& B, const Class& c) {
+  for (uint32_t i : c.cpis) {
+    const rd_cpi_desc& d = B.d[i];
+    const rd_cpi_bufs& b = B.b[i];
+    if (c.stage == S_BF) {
+      const uint64_t cols = (uint64_t)d.n_frames * d.n_fft;
+      if ((c.key != 3 && c.key != 4) || d.n_chan != 16 * c.key || d.n_beams % 16 != 0 || cols % 128 != 0 ||
+          ((uintptr_t)b.X & 15u))
+        return false;
+    } else if (c.stage == S_COV) {
+      if (c.key != 64 || d.n_chan != 64 || ((uintptr_t)b.Xa & 15u)) return false;
+    } else {
+      return false;
+    }
+  }
+  return !c.cpis.empty();
+}
+
 Shape default_shape(const rd_batch& B, const Class& c) {
   Shape s;
+  if (ring_fits(B, c)) {
+    s.a = c.stage == S_BF ? 128 : 1;
+    s.ring = 1;
+    return s;
+  }
   switch (c.stage) {
     case S_PFB: s.a = 262144; break;
     case S_FFT: break;
@@ -277,11 +305,13 @@ std::vector<Shape> candidates(const rd_batch& B, const Class& c) {
       v.push_back(Shape{0, 0});
       break;
     case S_BF:
+      if (ring_fits(B, c)) v.push_back(Shape{128, 0, 0, 1});
       for (uint32_t tc : {64u, 128u, 256u})
         if (bf_fits(c, tc))
           for (uint32_t g : {0u, 4 * cu, 8 * cu}) v.push_back(Shape{tc, g});
       break;
     case S_COV:
+      if (ring_fits(B, c)) v.push_back(Shape{1, 0, 0, 1});
       for (uint32_t f = 1; f <= rd::cov_max_splits(c.key); f *= 2) v.push_back(Shape{f, 0});
       break;
     case S_CFAR_ROWS:
@@ -369,6 +399,7 @@ std::string describe(const rd_batch& B) {
       snprintf(line, sizeof line, " waves_per_row=%u", c.shape.b);
       s += line;
     }
+    if (c.shape.ring) s += " loader-fed";
     if (c.ms >= 0) {
       snprintf(line, sizeof line, " tuned %.4f ms (%u candidates)", c.ms, c.tried);
       s += line;

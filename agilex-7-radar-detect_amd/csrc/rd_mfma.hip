@@ -476,29 +476,43 @@ bool bf_ring_fits(const BfJob& J, uint32_t steps) {
          J.cols / kBrTC > 0 && !J.acc && (reinterpret_cast<uintptr_t>(J.X) & 15u) == 0;
 }
 
-template <int STEPS, int OUT>
-__global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
+// The tiles of a block: t_k = blockIdx.x + k * gridDim.x over the launch's
+// tile space. BATCH: tile t belongs to jobs[tile_job[t]] (its tiles are the
+// contiguous range from its tile0); a block that moves to another CPI's tiles
+// derives that CPI's weight digits between the tile's barrier and its
+// products, behind one extra barrier that the loader also takes.
+template <int STEPS, int OUT, bool BATCH>
+__device__ __forceinline__ void bf_ring_body(const BfJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                             uint64_t n_tiles, const BfJob& J0) {
   using G = BrGeo<STEPS>;
   constexpr uint32_t CHP = G::CHP;
   extern __shared__ uint32_t lds[];
+  auto job_of = [&](uint64_t t) -> uint32_t {
+    return BATCH ? __builtin_amdgcn_readfirstlane(tile_job[t]) : 0u;
+  };
+  auto job = [&](uint32_t jid) -> const BfJob& { return BATCH ? jobs[jid] : J0; };
+  const uint32_t max_bt = J0.bt_n;  // the LDS layout's beam tiles (batch: the class maximum)
   unsigned char* const raw = reinterpret_cast<unsigned char*>(lds);
   uint32_t* frag = reinterpret_cast<uint32_t*>(raw + kBrRing * G::SLOT);
-  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(J.bt_n, STEPS));
-  const int out = OUT >= 0 ? OUT : bf_out_mask(J);
+  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(max_bt, STEPS));
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool loader = wave == 8;
-  const uint64_t cols = J.cols;
-  const uint64_t n_tiles = cols / kBrTC;
   // this block's tiles: blockIdx.x + k * gridDim.x, k < nt
   const uint32_t nt = (uint32_t)((n_tiles - blockIdx.x + gridDim.x - 1) / gridDim.x);
-  if (!loader) bf_weights_to_lds(J.W, CHP, J.B, CHP, STEPS, J.bt_n, frag, tail, 512);
+  uint32_t cur = job_of(blockIdx.x);
+  if (!loader) {
+    const BfJob& J = job(cur);
+    bf_weights_to_lds(J.W, CHP, J.B, CHP, STEPS, J.bt_n, frag, tail, 512);
+  }
 
   if (loader) {
     // lane l: channel 2j + l / 32 of block j, columns 4 (l % 32) .. + 3
     const uint32_t half = lane >> 5, lc = (lane & 31) * 4;
     auto issue = [&](uint32_t k) {
       unsigned char* slot = raw + (k % kBrRing) * G::SLOT;
-      const uint64_t c0 = ((uint64_t)blockIdx.x + (uint64_t)k * gridDim.x) * kBrTC;
+      const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)k * gridDim.x;
+      const BfJob& J = job(job_of(t));
+      const uint64_t cols = J.cols, c0 = (t - (BATCH ? J.tile0 : 0u)) * kBrTC;
       const uint32_t* src = J.X + (size_t)half * cols + c0 + lc;
 #pragma unroll
       for (uint32_t j = 0; j < G::NBLK; ++j) lds_dma16(src + (size_t)(2 * j) * cols, slot + j * kBrBlk);
@@ -514,6 +528,13 @@ __global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
       __builtin_amdgcn_s_barrier();  // the compute waves are done with slot k % kBrRing
       __asm__ volatile("" ::: "memory");
       if (next < nt) issue(next++);
+      if (BATCH) {
+        const uint32_t jid = job_of((uint64_t)blockIdx.x + (uint64_t)k * gridDim.x);
+        if (jid != cur) {
+          __builtin_amdgcn_s_barrier();  // the compute waves' weight barrier
+          cur = jid;
+        }
+      }
     }
     return;
   }
@@ -522,10 +543,13 @@ __global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const int32_t kconst = 16384 * 2 * (int32_t)CHP;
   const uint32_t j = 16 * wave + cl;  // this lane's column of the tile
-  const size_t lane_row = (size_t)(4 * grp) * cols;
   for (uint32_t k = 0; k < nt; ++k) {
     const unsigned char* slot = raw + (k % kBrRing) * G::SLOT;
-    const uint64_t c0 = ((uint64_t)blockIdx.x + (uint64_t)k * gridDim.x) * kBrTC;
+    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)k * gridDim.x;
+    const uint32_t jid = job_of(t);
+    const BfJob& J = job(jid);
+    const uint64_t cols = J.cols;
+    const uint64_t c0 = (t - (BATCH ? J.tile0 : 0u)) * kBrTC;
     if (J.xa_out) {
       // the aligned copy: 16-byte stores of four consecutive channels of one
       // column, consecutive threads on consecutive quads
@@ -579,9 +603,16 @@ __global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
     ps_i += __shfl_xor(ps_i, 32);
     const int32_t sb_im = ps_r + ps_i;
     const int32_t sb_re = ps_r - ps_i - (int32_t)CHP;
-    lds_barrier();  // every wave is done with this slot: the loader refills it
+    lds_barrier();  // every wave is done with this slot (and with the previous tile's products)
+    if (BATCH && jid != cur) {  // another CPI: its weight digits replace the previous ones
+      bf_weights_to_lds(J.W, CHP, J.B, CHP, STEPS, J.bt_n, frag, tail, 512);
+      __syncthreads();
+      cur = jid;
+    }
 
+    const int out = OUT >= 0 ? OUT : bf_out_mask(J);
     const uint64_t col = c0 + j;
+    const size_t lane_row = (size_t)(4 * grp) * cols;
     for (uint32_t bt = 0; bt < J.bt_n; ++bt) {
       v4i rhh = {0, 0, 0, 0}, rx = {0, 0, 0, 0}, rll = {0, 0, 0, 0};
       v4i ihh = {0, 0, 0, 0}, ix = {0, 0, 0, 0}, ill = {0, 0, 0, 0};
@@ -615,6 +646,19 @@ __global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
   }
 }
 
+template <int STEPS, int OUT>
+__global__ __launch_bounds__(64 * 9) void bf_ring_kernel(BfJob J) {
+  bf_ring_body<STEPS, OUT, false>(nullptr, nullptr, J.cols / kBrTC, J);
+}
+
+// Batched: `cls` carries the class's largest beam-tile count (the LDS layout).
+template <int STEPS, int OUT>
+__global__ __launch_bounds__(64 * 9) void bf_ring_batch_kernel(const BfJob* __restrict__ jobs,
+                                                              const uint32_t* __restrict__ tile_job,
+                                                              uint32_t n_tiles, BfJob cls) {
+  bf_ring_body<STEPS, OUT, true>(jobs, tile_job, n_tiles, cls);
+}
+
 template <int STEPS>
 static hipError_t launch_bf_ring(hipStream_t s, const BfJob& J) {
   const uint64_t tiles = J.cols / kBrTC;
@@ -627,6 +671,34 @@ static hipError_t launch_bf_ring(hipStream_t s, const BfJob& J) {
     bf_ring_kernel<STEPS, kBfOutY | kBfOutF | kBfOutP><<<grid, 64 * 9, lds, s>>>(J);
   else
     bf_ring_kernel<STEPS, -1><<<grid, 64 * 9, lds, s>>>(J);
+  return hipGetLastError();
+}
+
+// The class's tiles (128 columns each) on one resident block per CU.
+hipError_t launch_bf_ring_batch(hipStream_t s, uint32_t steps, uint32_t max_bt, uint32_t n_cu, int out_mask,
+                                const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  if (steps != 3 && steps != 4) return hipErrorInvalidValue;
+  const dim3 grid(L.n_tiles < n_cu ? L.n_tiles : n_cu);
+  BfJob cls{};
+  cls.bt_n = max_bt;
+  const BfJob* jobs = static_cast<const BfJob*>(L.jobs);
+#define RD_BR(S)                                                                                           \
+  {                                                                                                        \
+    const size_t lds = BrGeo<S>::lds(max_bt);                                                              \
+    if (out_mask == kBfOutP)                                                                               \
+      bf_ring_batch_kernel<S, kBfOutP><<<grid, 64 * 9, lds, s>>>(jobs, L.tile_job, L.n_tiles, cls);        \
+    else if (out_mask == (kBfOutY | kBfOutF | kBfOutP))                                                    \
+      bf_ring_batch_kernel<S, kBfOutY | kBfOutF | kBfOutP><<<grid, 64 * 9, lds, s>>>(jobs, L.tile_job,     \
+                                                                                      L.n_tiles, cls);     \
+    else                                                                                                   \
+      bf_ring_batch_kernel<S, -1><<<grid, 64 * 9, lds, s>>>(jobs, L.tile_job, L.n_tiles, cls);             \
+  }
+  if (steps == 3)
+    RD_BR(3)
+  else
+    RD_BR(4)
+#undef RD_BR
   return hipGetLastError();
 }
 
@@ -964,14 +1036,12 @@ bool cov_ring_fits(uint32_t n_chan, uint32_t n_frames, uint32_t n_bins, const ui
   return n_chan == 64 && n_frames >= 1 && n_bins >= 1 && (reinterpret_cast<uintptr_t>(Xa) & 15u) == 0;
 }
 
-__global__ __launch_bounds__(64 * 11) void cov_ring_kernel(CovJob J) {
-  extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint32_t* smem) {
   unsigned char* const lds = reinterpret_cast<unsigned char*>(smem);
   unsigned char* const raw = lds;                                   // [kCrRing][kCrSlot]
   unsigned char* const planes0 = raw + kCrRing * kCrSlot;          // [2][6 planes]
   int32_t* const sums = reinterpret_cast<int32_t*>(planes0 + 2 * kCrPlanes);  // [64 rows][2]
-  const uint32_t F = J.F, N = J.N;
-  const uint32_t k = blockIdx.x;  // the block's bin
+  const uint32_t F = J.F, N = J.N;  // k: the block's bin
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool loader = wave == 10;
   const uint32_t nsteps = (F + kCovFS - 1) / kCovFS;
@@ -1101,13 +1171,34 @@ __global__ __launch_bounds__(64 * 11) void cov_ring_kernel(CovJob J) {
   }
 }
 
+// Single CPI: one block per bin.
+__global__ __launch_bounds__(64 * 11) void cov_ring_kernel(CovJob J) {
+  extern __shared__ uint32_t smem[];
+  cov_ring_body(J, blockIdx.x, smem);
+}
+
+// Batched: one block per (CPI, bin) of a class of 64-channel CPIs.
+__global__ __launch_bounds__(64 * 11) void cov_ring_batch_kernel(const CovJob* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ tile_job) {
+  extern __shared__ uint32_t smem[];
+  const CovJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[blockIdx.x])];
+  cov_ring_body(J, blockIdx.x - J.tile0, smem);
+}
+
+constexpr size_t kCrLds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
+
+hipError_t launch_cov_ring_batch(hipStream_t s, const ClassLaunch& L) {
+  if (L.n_tiles == 0) return hipSuccess;
+  cov_ring_batch_kernel<<<L.n_tiles, 64 * 11, kCrLds, s>>>(static_cast<const CovJob*>(L.jobs), L.tile_job);
+  return hipGetLastError();
+}
+
 hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32_t n_frames, uint32_t n_bins,
                       int64_t* R) {
   if (n_bins == 0) return hipSuccess;
   if (cov_ring_fits(n_chan, n_frames, n_bins, Xa)) {
-    const size_t lds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
     const CovJob J{Xa, R, n_chan, n_frames, n_bins, 0};
-    cov_ring_kernel<<<n_bins, 64 * 11, lds, s>>>(J);
+    cov_ring_kernel<<<n_bins, 64 * 11, kCrLds, s>>>(J);
     return hipGetLastError();
   }
   const uint32_t chp = (n_chan + 15) / 16 * 16;
