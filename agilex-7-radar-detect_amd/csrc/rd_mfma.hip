@@ -1364,16 +1364,25 @@ bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_
 
 // T1: the beams fill exactly one 16-beam tile (B = 16): no beam bounds, and a
 // lane's four beam constants stay in registers over the whole frame loop.
+// XCD-aware tile order: consecutive blocks are dealt to the 8 XCDs in turn,
+// so the blocks one XCD runs together take adjacent tiles and share the L2
+// lines of their loads and stores.
+__device__ __forceinline__ uint32_t xcd_tile() {
+  const uint32_t nblk = gridDim.x;
+  return nblk % 8 == 0 ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+}
+
+// Bins [4 tile, +4) of J; lay_bt: the beam tiles the LDS layout is sized for.
 template <int WAVES, int OUT, bool T1>
-__global__ __launch_bounds__(64 * (WAVES + 1)) void bfcov_kernel(BfCovJob J) {
+__device__ __forceinline__ void bfcov_body(const BfCovJob& J, uint32_t tile, uint32_t lay_bt) {
   using G = FbGeo<WAVES>;
   constexpr uint32_t FS = G::FS;
   extern __shared__ uint32_t lds[];
   unsigned char* const sm = reinterpret_cast<unsigned char*>(lds);
   unsigned char* const raw = sm + G::STAGE;  // the loader's ring of raw steps
   uint32_t* frag = reinterpret_cast<uint32_t*>(raw + G::RAW);
-  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(J.bf.bt_n, 1));
-  int32_t* sums = tail + J.bf.bt_n * 32;  // [bin][16 rows][4] digit sums of the covariance A rows
+  int32_t* tail = reinterpret_cast<int32_t*>(frag + bf_frag_words(lay_bt, 1));
+  int32_t* sums = tail + lay_bt * 32;  // [bin][16 rows][4] digit sums of the covariance A rows
   const int out = OUT >= 0 ? OUT : bf_out_mask(J.bf);
   const uint32_t CH = J.bf.CH, B = J.bf.B, N = J.N, F = J.F;
   const uint64_t cols = (uint64_t)F * N;
@@ -1381,11 +1390,6 @@ __global__ __launch_bounds__(64 * (WAVES + 1)) void bfcov_kernel(BfCovJob J) {
   const uint32_t grp = lane >> 4, cl = lane & 15;
   const bool loader = wave == WAVES;  // the last wave only loads
 
-  // XCD-aware tile order: consecutive blocks are dealt to the 8 XCDs in turn,
-  // so the blocks one XCD runs together take adjacent bins and share the L2
-  // lines of their loads and stores
-  const uint32_t nblk = gridDim.x;
-  const uint32_t tile = nblk % 8 == 0 ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
   const uint32_t k0 = tile * kFbNB;
 
   // conversion unit of a compute thread: channel quad q, bin b, frame f of the step
@@ -1641,6 +1645,11 @@ __global__ __launch_bounds__(64 * (WAVES + 1)) void bfcov_kernel(BfCovJob J) {
     o[0] = re;
     o[1] = im;
   }
+}
+
+template <int WAVES, int OUT, bool T1>
+__global__ __launch_bounds__(64 * (WAVES + 1)) void bfcov_kernel(BfCovJob J) {
+  bfcov_body<WAVES, OUT, T1>(J, xcd_tile(), J.bf.bt_n);
 }
 
 template <int WAVES, bool T1>
