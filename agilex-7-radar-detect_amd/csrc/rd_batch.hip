@@ -207,7 +207,8 @@ hipError_t launch_class(const rd_batch& B, const Class& c, hipStream_t s) {
     case S_BF:
       return c.shape.ring ? rd::launch_bf_ring_batch(s, c.key, c.max_bt, B.n_cu, c.out_mask, c.L)
                           : rd::launch_bf_batch(s, c.key, c.shape.a, c.max_ch, c.max_bt, c.out_mask, c.L);
-    case S_COV: return c.shape.ring ? rd::launch_cov_ring_batch(s, c.L) : rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
+    case S_COV:
+      return c.shape.ring ? rd::launch_cov_ring_batch(s, c.key, c.L) : rd::launch_cov_batch(s, c.key, c.shape.a, c.L);
     case S_CFAR_ROWS: return rd::launch_cfar_rows_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     case S_CFAR_GATHER: return rd::launch_cfar_gather_batch(s, c.key, c.shape.b, c.shape.a, c.L);
     default: return hipErrorInvalidValue;
@@ -255,7 +256,9 @@ bool ring_fits(const rd_batch& B, const Class& c) {
           ((uintptr_t)b.X & 15u))
         return false;
     } else if (c.stage == S_COV) {
-      if (c.key != 64 || d.n_chan != 64 || ((uintptr_t)b.Xa & 15u)) return false;
+      if ((c.key != 32 && c.key != 64) || d.n_chan != c.key || d.n_fft % (64 / c.key) != 0 ||
+          ((uintptr_t)b.Xa & 15u))
+        return false;
     } else {
       return false;
     }

@@ -236,11 +236,11 @@ hipError_t launch_cov_batch(hipStream_t s, uint32_t chp, uint32_t fs_n, const Cl
 // Loader-fed forms for classes whose every CPI fits them: the beamformer of
 // 48/64-channel CPIs with beams a multiple of 16 and columns a multiple of 128
 // (tiles of 128 columns, X 16-byte aligned), on min(tiles, n_cu) resident
-// blocks; the covariance of 64-channel CPIs (one bin per tile, Xa 16-byte
-// aligned).
+// blocks; the covariance of CPIs of exactly chp = 32 or 64 channels (tiles of
+// 64 / chp bins, the bin count a multiple of that, Xa 16-byte aligned).
 hipError_t launch_bf_ring_batch(hipStream_t s, uint32_t steps, uint32_t max_bt, uint32_t n_cu, int out_mask,
                                 const ClassLaunch& L);
-hipError_t launch_cov_ring_batch(hipStream_t s, const ClassLaunch& L);
+hipError_t launch_cov_ring_batch(hipStream_t s, uint32_t chp, const ClassLaunch& L);
 // CFAR over power rows (u32 cells) of n bins: W waves per row (a power of two
 // from cfar_min_waves(n) to 8), rpb rows per tile.
 uint32_t cfar_min_waves(uint32_t n);

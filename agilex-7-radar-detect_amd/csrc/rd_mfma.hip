@@ -441,9 +441,10 @@ __global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_batch_kernel(const
 
 // ============================================================================
 // Beamformer over the FFT layout for CPIs whose channels fill 3 or 4 K steps
-// (48 or 64 channels), beams a multiple of 16 and columns a multiple of 128:
+// (48 or 64 channels), beams a multiple of 16 and columns a multiple of 128
+// (at 32 channels the tiled kernel below measured faster):
 // a persistent block of 8 compute waves and one loader wave. The loader moves
-// each 128-column tile of X (64 channel rows of 512 bytes) into an LDS ring of
+// each 128-column tile of X (one 512-byte row per channel) into an LDS ring of
 // two slots with global_load_lds_dwordx4 (one instruction per channel pair)
 // and counts its own transfers; the compute waves issue no vector-memory load
 // in the tile loop, so their power and aligned-copy stores are never waited
@@ -1019,36 +1020,49 @@ uint32_t cov_bins_per_tile(uint32_t chp) { return CovGeo(chp).nb; }
 uint32_t cov_max_splits(uint32_t chp) { return 16 / CovGeo(chp).jobs; }
 
 // ============================================================================
-// Covariance of 64-channel CPIs (one bin per block, its 10 upper-triangle
-// 16x16 channel tiles on 10 compute waves) fed by an 11th, loader wave: each
-// 32-frame step of the bin (32 runs of 256 bytes in Xa[F][N][64]) is moved into
-// a ring of raw LDS slots by global_load_lds_dwordx4 (eight instructions a
-// step) kCrRing steps ahead, with counted waits; the compute waves convert a
-// slot into digit planes (double-buffered, so one barrier a step orders both
-// the planes and the slot's release) and run the same MFMA products and exact
-// int64 recombination as cov_tile.
+// Covariance of CPIs of 32 or 64 channels (CHP): a block owns NB = 64/CHP
+// adjacent bins (64 channel rows), their upper-triangle 16x16 channel tiles
+// on JOBS compute waves (6 or 10), and one loader wave. Each 32-frame step
+// of the block (32 runs of 256 bytes in Xa[F][N][CHP]: the NB bins' channels
+// of one frame) is moved into a ring of raw LDS slots by
+// global_load_lds_dwordx4 (eight instructions a step) kCrRing steps ahead,
+// with counted waits; the compute waves convert a slot into digit planes
+// (double-buffered, so one barrier a step orders both the planes and the
+// slot's release) and run the same MFMA products and exact int64
+// recombination as cov_tile.
 // ============================================================================
 constexpr uint32_t kCrRing = 3;
-constexpr uint32_t kCrSlot = kCovFS * 64 * 4;  // one step: [32 frames][64 channels] words
+constexpr uint32_t kCrSlot = kCovFS * 64 * 4;  // one step: [32 frames][64 rows] words
 constexpr uint32_t kCrPlanes = 6 * kCovPlane;
+constexpr size_t kCrLds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
+
+template <uint32_t CHP>
+struct CrGeo {
+  static constexpr uint32_t NB = 64 / CHP, NT = CHP / 16, UT = NT * (NT + 1) / 2, JOBS = NB * UT;
+};
 
 bool cov_ring_fits(uint32_t n_chan, uint32_t n_frames, uint32_t n_bins, const uint32_t* Xa) {
-  return n_chan == 64 && n_frames >= 1 && n_bins >= 1 && (reinterpret_cast<uintptr_t>(Xa) & 15u) == 0;
+  return (n_chan == 32 || n_chan == 64) && n_frames >= 1 && n_bins >= 1 &&
+         n_bins % (64 / n_chan) == 0 && (reinterpret_cast<uintptr_t>(Xa) & 15u) == 0;
 }
 
-__device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint32_t* smem) {
+// Bins [NB tile, +NB) of J.
+template <uint32_t CHP>
+__device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t tile, uint32_t* smem) {
+  using G = CrGeo<CHP>;
   unsigned char* const lds = reinterpret_cast<unsigned char*>(smem);
   unsigned char* const raw = lds;                                   // [kCrRing][kCrSlot]
   unsigned char* const planes0 = raw + kCrRing * kCrSlot;          // [2][6 planes]
   int32_t* const sums = reinterpret_cast<int32_t*>(planes0 + 2 * kCrPlanes);  // [64 rows][2]
-  const uint32_t F = J.F, N = J.N;  // k: the block's bin
+  const uint32_t F = J.F, N = J.N;
+  const uint32_t k_base = tile * G::NB;
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool loader = wave == 10;
+  const bool loader = wave == G::JOBS;
   const uint32_t nsteps = (F + kCovFS - 1) / kCovFS;
   if (threadIdx.x < 128) sums[threadIdx.x] = 0;
 
   if (loader) {
-    // lane l of instruction j: frame 4 j + l / 16 of the step, channels
+    // lane l of instruction j: frame 4 j + l / 16 of the step, rows
     // 4 (l % 16) .. + 3; frames past F re-read the last one (masked later)
     const uint32_t fl = lane >> 4, c4 = (lane & 15) * 4;
     auto issue = [&](uint32_t st) {
@@ -1056,7 +1070,7 @@ __device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint3
 #pragma unroll
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t m = min(st * kCovFS + 4 * j + fl, F - 1);
-        lds_dma16(J.Xa + ((size_t)m * N + k) * 64 + c4, slot + j * 1024);
+        lds_dma16(J.Xa + ((size_t)m * N + k_base) * CHP + c4, slot + j * 1024);
       }
     };
     uint32_t next = 0;
@@ -1076,26 +1090,26 @@ __device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint3
   }
 
   const uint32_t grp = lane >> 4, cl = lane & 15;
-  uint32_t u = wave, ti = 0;  // upper-triangle tile (ti, tj) of this wave
-  while (u >= 4 - ti) {
-    u -= 4 - ti;
+  const uint32_t bin = wave / G::UT;
+  uint32_t u = wave % G::UT, ti = 0;  // upper-triangle tile (ti, tj) of this wave
+  while (u >= G::NT - ti) {
+    u -= G::NT - ti;
     ++ti;
   }
   const uint32_t tj = ti + u;
-  const uint32_t ra = 16 * ti + cl, rb = 16 * tj + cl;  // operand rows of this lane
+  const uint32_t ra = bin * CHP + 16 * ti + cl, rb = bin * CHP + 16 * tj + cl;  // operand rows of this lane
   v4i acc[8];
 #pragma unroll
   for (int z = 0; z < 8; ++z) acc[z] = v4i{0, 0, 0, 0};
-  // conversion task of this thread (threads 0..511): row = lane (channel),
-  // frame group g = wave (4 frames)
-  const bool task = wave < 8;
-  const uint32_t g = wave;
+  // conversion tasks (row = lane, frame group g of 4 frames), g = 0..7, dealt
+  // to the compute waves; the frame sums of a row accumulate per task
   int32_t srow_r = 0, srow_i = 0;
   __syncthreads();  // step 0 is in the ring (and sums are zero)
   for (uint32_t st = 0; st < nsteps; ++st) {
     unsigned char* planes = planes0 + (st & 1) * kCrPlanes;
     const uint32_t m0 = st * kCovFS;
-    if (task) {
+#pragma unroll
+    for (uint32_t g = wave; g < 8; g += G::JOBS) {
       const unsigned char* rs = raw + (st % kCrRing) * kCrSlot + (4 * g * 64 + lane) * 4;
       const uint32_t w[4] = {*reinterpret_cast<const uint32_t*>(rs), *reinterpret_cast<const uint32_t*>(rs + 256),
                              *reinterpret_cast<const uint32_t*>(rs + 512),
@@ -1147,7 +1161,7 @@ __device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint3
     }
   }
   // per-row frame sums (the eight tasks of a row add up)
-  if (task && (srow_r | srow_i)) {
+  if (wave < 8 && (srow_r | srow_i)) {
     atomicAdd(&sums[2 * lane], srow_r);
     atomicAdd(&sums[2 * lane + 1], srow_i);
   }
@@ -1157,39 +1171,47 @@ __device__ __forceinline__ void cov_ring_body(const CovJob& J, uint32_t k, uint3
   const int64_t kconst = 16384LL * 2 * nfr;
   const int64_t tjs = (int64_t)b_r + b_i;
   const uint32_t cj = 16 * tj + cl;
-  int64_t* const Rk = J.R + (size_t)k * (64 * 65 / 2) * 2;
+  constexpr uint32_t NPAIRS = CHP * (CHP + 1) / 2;
+  int64_t* const Rk = J.R + (size_t)(k_base + bin) * NPAIRS * 2;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const uint32_t ii = 16 * ti + 4 * grp + r;
-    const int32_t ir = sums[2 * ii], iim = sums[2 * ii + 1];
+    const uint32_t il = bin * CHP + ii;  // its row
+    const int32_t ir = sums[2 * il], iim = sums[2 * il + 1];
     const int64_t tis = (int64_t)ir + iim, uis = (int64_t)iim - ir - nfr;
     if (ii > cj) continue;
-    const size_t pair = (size_t)ii * 64 - (size_t)ii * (ii - 1) / 2 + (cj - ii);
+    const size_t pair = (size_t)ii * CHP - (size_t)ii * (ii - 1) / 2 + (cj - ii);
     Rk[2 * pair] = combine(acc[0], acc[1], acc[2], r) + 256 * (int64_t)acc[3][r] + 128 * (tis + tjs) - kconst;
     Rk[2 * pair + 1] =
         combine(acc[4], acc[5], acc[6], r) + 256 * (int64_t)acc[7][r] + 128 * (uis + tjs) - kconst + b_i;
   }
 }
 
-// Single CPI: one block per bin.
-__global__ __launch_bounds__(64 * 11) void cov_ring_kernel(CovJob J) {
+// Single CPI: one block per NB bins.
+template <uint32_t CHP>
+__global__ __launch_bounds__(64 * (CrGeo<CHP>::JOBS + 1)) void cov_ring_kernel(CovJob J) {
   extern __shared__ uint32_t smem[];
-  cov_ring_body(J, blockIdx.x, smem);
+  cov_ring_body<CHP>(J, blockIdx.x, smem);
 }
 
-// Batched: one block per (CPI, bin) of a class of 64-channel CPIs.
-__global__ __launch_bounds__(64 * 11) void cov_ring_batch_kernel(const CovJob* __restrict__ jobs,
-                                                                const uint32_t* __restrict__ tile_job) {
+// Batched: one block per (CPI, NB bins) of a class of CHP-channel CPIs.
+template <uint32_t CHP>
+__global__ __launch_bounds__(64 * (CrGeo<CHP>::JOBS + 1)) void cov_ring_batch_kernel(
+    const CovJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
   extern __shared__ uint32_t smem[];
   const CovJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[blockIdx.x])];
-  cov_ring_body(J, blockIdx.x - J.tile0, smem);
+  cov_ring_body<CHP>(J, blockIdx.x - J.tile0, smem);
 }
 
-constexpr size_t kCrLds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
-
-hipError_t launch_cov_ring_batch(hipStream_t s, const ClassLaunch& L) {
+hipError_t launch_cov_ring_batch(hipStream_t s, uint32_t chp, const ClassLaunch& L) {
   if (L.n_tiles == 0) return hipSuccess;
-  cov_ring_batch_kernel<<<L.n_tiles, 64 * 11, kCrLds, s>>>(static_cast<const CovJob*>(L.jobs), L.tile_job);
+  const CovJob* jobs = static_cast<const CovJob*>(L.jobs);
+  if (chp == 32)
+    cov_ring_batch_kernel<32><<<L.n_tiles, 64 * (CrGeo<32>::JOBS + 1), kCrLds, s>>>(jobs, L.tile_job);
+  else if (chp == 64)
+    cov_ring_batch_kernel<64><<<L.n_tiles, 64 * (CrGeo<64>::JOBS + 1), kCrLds, s>>>(jobs, L.tile_job);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -1198,7 +1220,11 @@ hipError_t launch_cov(hipStream_t s, const uint32_t* Xa, uint32_t n_chan, uint32
   if (n_bins == 0) return hipSuccess;
   if (cov_ring_fits(n_chan, n_frames, n_bins, Xa)) {
     const CovJob J{Xa, R, n_chan, n_frames, n_bins, 0};
-    cov_ring_kernel<<<n_bins, 64 * 11, kCrLds, s>>>(J);
+    const uint32_t tiles = n_bins / (64 / n_chan);
+    if (n_chan == 32)
+      cov_ring_kernel<32><<<tiles, 64 * (CrGeo<32>::JOBS + 1), kCrLds, s>>>(J);
+    else
+      cov_ring_kernel<64><<<tiles, 64 * (CrGeo<64>::JOBS + 1), kCrLds, s>>>(J);
     return hipGetLastError();
   }
   const uint32_t chp = (n_chan + 15) / 16 * 16;

@@ -312,7 +312,7 @@ def test_bf_unit_weights_passthrough(gpu):
 # columns a multiple of 128): one tile, a few, and eight tiles per block
 @pytest.mark.parametrize("CH,B,F,N", [(16, 16, 5, 1024), (64, 64, 2, 4096), (4, 4, 16, 64), (7, 3, 3, 100),
                                       (33, 20, 2, 512), (48, 17, 1, 256), (32, 24, 3, 256), (16, 16, 2, 64),
-                                      (48, 32, 3, 256), (64, 16, 1, 128), (64, 16, 64, 4096)])
+                                      (48, 32, 3, 256), (64, 16, 1, 128), (64, 16, 64, 4096), (32, 32, 4, 256)])
 def test_bf_stage_matches_align_then_bf(gpu, orc, CH, B, F, N):
     rng = np.random.default_rng(CH + B + F + N)
     for X, W in [(stim.rand_cx(rng, CH * F * N), stim.rand_cx(rng, B * CH)),
@@ -329,8 +329,10 @@ def test_bf_stage_matches_align_then_bf(gpu, orc, CH, B, F, N):
 
 
 # ---------------------------------------------------------------------------- covariance
+# 32 and 64 channels with the bin count a multiple of 64 / CH take the
+# loader-fed kernel: (64, 20, 5), (64, 512, 2), (32, 40, 6), (32, 1, 2)
 @pytest.mark.parametrize("CH,F,N", [(4, 16, 64), (16, 40, 33), (64, 20, 5), (1, 7, 9), (20, 33, 8), (33, 17, 6),
-                                    (64, 512, 2), (16, 256, 12)])
+                                    (64, 512, 2), (16, 256, 12), (32, 40, 6), (16, 33, 4), (32, 1, 2)])
 def test_cov_random_and_extreme(gpu, orc, CH, F, N):
     rng = np.random.default_rng(CH * 7 + F + N)
     for Xa in (stim.rand_cx(rng, F * N * CH), stim.rand_extreme(rng, F * N * CH),
