@@ -462,6 +462,7 @@ struct BrGeo {
   static constexpr uint32_t CHP = STEPS * 16;
   static constexpr uint32_t NBLK = CHP / 2;  // loader instructions per tile
   static constexpr uint32_t SLOT = NBLK * kBrBlk;
+  static_assert((kBrRing - 1) * NBLK <= 40, "wait_vmcnt covers the loader's counts");
   static size_t lds(uint32_t bt_n) {
     return (size_t)kBrRing * SLOT + bf_frag_words(bt_n, STEPS) * 4 + (size_t)bt_n * 32 * 4;
   }
@@ -1032,6 +1033,7 @@ uint32_t cov_max_splits(uint32_t chp) { return 16 / CovGeo(chp).jobs; }
 // recombination as cov_tile.
 // ============================================================================
 constexpr uint32_t kCrRing = 3;
+static_assert((kCrRing - 1) * 8 <= 40, "wait_vmcnt covers the loader's counts");
 constexpr uint32_t kCrSlot = kCovFS * 64 * 4;  // one step: [32 frames][64 rows] words
 constexpr uint32_t kCrPlanes = 6 * kCovPlane;
 constexpr size_t kCrLds = (size_t)kCrRing * kCrSlot + 2 * kCrPlanes + 128 * 4;
