@@ -225,6 +225,29 @@ __device__ __forceinline__ bool ca_hit_inner(const CaRuns& r, uint32_t alpha, ui
   return lhs > (uint64_t)alpha * sum;
 }
 
+// CA decision (no dense output) of a bin in an edge chunk: the window indices
+// clamped into [0, n] so every read stays in the row's prefix, and a bin whose
+// window does not fit never detects (decide() gives it eval = false). For a
+// bin that fits, the clamps are no-ops and the sums are the unclamped ones;
+// with no lead (lag) reference cells the clamped pair reads one entry twice,
+// a zero sum, as decide() has it. n8 = (ref_lead + ref_lag) << 8.
+template <typename Cell, uint32_t SLOG>
+__device__ __forceinline__ bool ca_hit_edge(const uint64_t* P, uint32_t n, uint32_t i, const CfarGeom& g,
+                                            uint32_t n8) {
+  const uint32_t a1 = min(i + g.gl + g.rl + 1, n), a0 = min(i + g.gl + 1, n);
+  const uint32_t b1 = i >= g.gg ? i - g.gg : 0u, b0 = i >= g.gg + g.rg ? i - g.gg - g.rg : 0u;
+  const bool fit = (g.rl == 0 || i + g.gl + g.rl < n) && (g.rg == 0 || i >= g.gg + g.rg);
+  uint64_t lhs;
+  if constexpr (sizeof(Cell) == 4) {
+    const uint32_t cut = (uint32_t)P[pidx<SLOG>(i + 1)] - (uint32_t)P[pidx<SLOG>(i)];
+    lhs = (uint64_t)cut * n8;
+  } else {
+    lhs = (P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)]) * (uint64_t)n8;
+  }
+  const uint64_t sum = (P[pidx<SLOG>(a1)] - P[pidx<SLOG>(a0)]) + (P[pidx<SLOG>(b1)] - P[pidx<SLOG>(b0)]);
+  return fit && lhs > (uint64_t)g.alpha * sum;
+}
+
 // Bins of a row of n that are not evaluated (window does not fit, or the
 // detector is off): a closed form of the reference's per-bin count.
 __device__ __forceinline__ uint32_t suppressed(uint32_t n, const CfarGeom& g) {
@@ -352,6 +375,10 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
       const bool hit = ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8);
       em[q] = __ballot(hit || g.dense);
       if (g.dense) det += (uint32_t)__popcll(__ballot(hit));
+    } else if (ca && !g.dense) {  // an edge chunk of a CA row: only detections emit
+      const uint32_t i = c0 + q * 64 + lane;
+      const bool in = work && (full || i < n);
+      em[q] = __ballot(in && ca_hit_edge<Cell, SLOG>(P, n, i, g, n_ref << 8));
     } else {
       const uint32_t i = c0 + q * 64 + lane;
       const bool in = work && (full || i < n);
