@@ -372,9 +372,13 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
   static_for<S>([&](auto qc) {
     constexpr uint32_t q = decltype(qc)::value;
     if (q >= qa && q < qb) {  // every bin of the chunk is in the row
-      const bool hit = ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8);
-      em[q] = __ballot(hit || g.dense);
-      if (g.dense) det += (uint32_t)__popcll(__ballot(hit));
+      const uint64_t hm = __ballot(ca_hit_inner<Cell, q * kStep>(runs, g.alpha, n_ref << 8));
+      if (g.dense) {  // every bin emits
+        em[q] = __ballot(true);
+        det += (uint32_t)__popcll(hm);
+      } else {
+        em[q] = hm;
+      }
     } else if (ca && !g.dense) {  // an edge chunk of a CA row: only detections emit
       const uint32_t i = c0 + q * 64 + lane;
       const bool in = work && (full || i < n);
