@@ -1374,7 +1374,7 @@ struct FbGeo {
   static constexpr uint32_t COLS = FS * kFbNB;                  // columns per step
   static constexpr uint32_t PLANE = COLS * 16;                  // bytes per plane
   static constexpr uint32_t BUF = kFbPlanes * PLANE;            // one step's planes
-  static constexpr uint32_t RED = kFbNB * 36 * 64 * 4;          // covariance partials [bin][36][64] int32
+  static constexpr uint32_t RED = (WAVES / kFbNB - 1) * kFbNB * 36 * 64 * 4;  // partials [K-step wave - 1][bin][36][64]
   static constexpr uint32_t STAGE = 2 * BUF > RED ? 2 * BUF : RED;  // double-buffered planes, aliased by RED
   static constexpr uint32_t RAW = kFbRing * kFbSlot;               // the loader's ring
   static_assert(FS == 32, "a raw block holds 32 frames of a channel pair");
@@ -1398,6 +1398,48 @@ bool bfcov_fits(uint32_t n_chan, uint32_t n_bins, uint32_t n_frames, uint32_t n_
 __device__ __forceinline__ uint32_t xcd_tile() {
   const uint32_t nblk = gridDim.x;
   return nblk % 8 == 0 ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+}
+
+// The weight digits of a one-K-step beamformer (at most 16 channels) and the
+// per-beam sums Sa = sum (wr + wi), Wis = sum wi from the same loads: the lanes
+// of the real K rows (lane groups 0 and 1) hold channels 0-7 and 8-15 of one
+// beam, so one shuffle adds a beam's halves (bf_weights_to_lds reads W again
+// for the sums, a second dependent load round).
+__device__ __forceinline__ void bf1_weights_to_lds(const uint32_t* __restrict__ W, uint32_t CH, uint32_t B,
+                                                   uint32_t bt_n, uint32_t* frag, int32_t* tail, uint32_t nthr) {
+  for (uint32_t f = threadIdx.x; f < bt_n * 64; f += nthr) {
+    const uint32_t lane = f & 63, bt = f >> 6;
+    const uint32_t b = bt * 16 + (lane & 15);
+    const uint32_t k0 = 8 * (lane >> 4);
+    const bool imag = k0 >= 16;
+    const uint32_t a0 = imag ? k0 - 16 : k0;
+    uint32_t p[8];
+    uint64_t vmask = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const bool ok = b < B && a0 + t < CH;
+      p[t] = ok ? W[(size_t)b * CH + a0 + t] : 0u;
+      vmask |= ok ? (0xFFull << (8 * t)) : 0ull;
+    }
+    const Dig8 d = digits8(p, vmask);
+    uint64_t* o = reinterpret_cast<uint64_t*>(frag) + (size_t)bt * 256 + lane;
+    o[0] = imag ? d.hi : d.hr;
+    o[64] = imag ? d.li : d.lr;
+    o[128] = imag ? d.hr : d.hi;
+    o[192] = imag ? d.lr : d.li;
+    int32_t sa = 0, wis = 0;  // padded entries are zero
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      sa += cre(p[t]) + cim(p[t]);
+      wis += cim(p[t]);
+    }
+    sa += __shfl_xor(sa, 16);
+    wis += __shfl_xor(wis, 16);
+    if (lane < 16) {
+      tail[2 * b] = sa;
+      tail[2 * b + 1] = wis;
+    }
+  }
 }
 
 // Bins [4 tile, +4) of J; lay_bt: the beam tiles the LDS layout is sized for.
@@ -1437,7 +1479,7 @@ __device__ __forceinline__ void bfcov_body(const BfCovJob& J, uint32_t tile, uin
   const uint32_t nsteps = (F + FS - 1) / FS;
   const uint32_t* const X0 = J.bf.X + (xb ? (size_t)k0 * F : (size_t)k0);  // k0 is a multiple of 4
   // the compute waves derive the weight digits while the first steps load
-  if (!loader) bf_weights_to_lds(J.bf.W, CH, B, 16, 1, J.bf.bt_n, frag, tail, 64 * WAVES);
+  if (!loader) bf1_weights_to_lds(J.bf.W, CH, B, J.bf.bt_n, frag, tail, 64 * WAVES);
 
   if (loader) {
     // lane l of block j moves channel 2j + l / 32, frame l % 32 of the step
@@ -1619,29 +1661,27 @@ __device__ __forceinline__ void bfcov_body(const BfCovJob& J, uint32_t tile, uin
     }
   }
 
-  // the waves of a bin add their partial sums: the first K step's wave
-  // stores, the others add with LDS atomics, then the first forms the results
+  // the waves of a bin add their partial sums: each later K step's wave
+  // stores its own copy, and the first K step's wave adds them in registers
+  constexpr uint32_t KS = WAVES / kFbNB;  // K-step waves per bin
   __syncthreads();  // every wave is done with the planes (the area is reused)
-  int32_t* red = reinterpret_cast<int32_t*>(sm) + (size_t)cbin * 36 * 64 + lane;
-  if (cks == 0) {
+  if (cks != 0 && !loader) {
+    int32_t* red = reinterpret_cast<int32_t*>(sm) + ((size_t)(cks - 1) * kFbNB + cbin) * 36 * 64 + lane;
 #pragma unroll
     for (int z = 0; z < 9; ++z)
 #pragma unroll
       for (int r = 0; r < 4; ++r) red[(4 * z + r) * 64] = acc[z][r];
   }
   __syncthreads();
-  if (cks != 0 && !loader) {
+  if (cks != 0) return;
+#pragma unroll
+  for (uint32_t c = 1; c < KS; ++c) {
+    const int32_t* red = reinterpret_cast<const int32_t*>(sm) + ((size_t)(c - 1) * kFbNB + cbin) * 36 * 64 + lane;
 #pragma unroll
     for (int z = 0; z < 9; ++z)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) atomicAdd(&red[(4 * z + r) * 64], acc[z][r]);
+      for (int r = 0; r < 4; ++r) acc[z][r] += red[(4 * z + r) * 64];
   }
-  __syncthreads();
-  if (cks != 0) return;
-#pragma unroll
-  for (int z = 0; z < 9; ++z)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[z][r] = red[(4 * z + r) * 64];
 
   // frame sums of row i = 4 grp + r: T = sum_m (xr_i + xi_i) over the Re
   // A row [Xr | Xi], U = sum_m (xi_i + ~xr_i) = sum xi - sum xr - F over the
