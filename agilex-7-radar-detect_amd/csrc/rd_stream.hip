@@ -943,8 +943,11 @@ constexpr int fft_ppt(int log2n, bool all1) { return (all1 && log2n >= 6) ? 16 :
 // (exponent r * {0, 2, 1, 3}[q2]); the arithmetic of every sub-stage is the
 // radix-4 pass's. Leftover sub-stages run as one radix-4 pass over the
 // thread's four groups, then the final pass as in fft_tile.
+// tw4096: the twiddle table to stage into LDS here, after the frame loads are
+// issued, so the two global latencies overlap (nullptr: already staged).
 template <int LOG2N, int SPC, bool FL>
-__device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32_t* smem) {
+__device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32_t* smem,
+                                           const uint32_t* __restrict__ tw4096) {
   using G = FftGeo<LOG2N, 16>;
   using PkFlags = PkFlagsT<FL>;
   constexpr uint32_t N = G::N;
@@ -976,6 +979,13 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
   };
 
   if (g == 0) sfl[lf] = 0;
+  uint4 q4v[4];
+  if (active) {
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t)  // four consecutive samples 4c .. 4c+3, c = g + t*TPF
+      q4v[t] = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * (g + t * TPF));
+  }
+  if (tw4096) fft_stage_twiddles<LOG2N, TPF * FPB>(tw4096, smem);
   if (active) {
     // Sample x = 4c + r goes to lane slot (x & 1) * M + x / 2 (spc 2) or x.
     // With c = g + t*TPF: 2c and 4c are even, so the small offsets stay in
@@ -985,8 +995,8 @@ __device__ __forceinline__ void fft_tile16(const FftJob& J, uint32_t grp, uint32
     const uint32_t pb = pad(CS * g);
 #pragma unroll
     for (uint32_t t = 0; t < 4; ++t) {
-      const uint32_t c = g + t * TPF;  // four consecutive samples 4c .. 4c+3
-      const uint4 q4 = *reinterpret_cast<const uint4*>(in + (size_t)frame * N + 4 * c);
+      const uint32_t c = g + t * TPF;
+      const uint4 q4 = q4v[t];
       const uint32_t o = FOLD ? pb + t * (CS * TPF + CS * TPF / 32) : pad(CS * c);
       if constexpr (SPC == 2) {
         v[o] = q4.x;
@@ -1268,12 +1278,17 @@ using FftGeoOf = FftGeo<LOG2N, fft_ppt(LOG2N, ALL1)>;
 
 // FL: the stage flags are formed (the packed all-ones path forms them only
 // when a caller keeps them; the general path always does).
+// The twiddle table is staged into LDS first, except by the radix-16 tile,
+// which stages it after issuing its frame loads.
 template <int LOG2N, int SPC, bool ALL1, bool FL = true>
-__device__ __forceinline__ void fft_tile_any(const FftJob& J, uint32_t grp, uint32_t* smem) {
-  if constexpr (fft_ppt(LOG2N, ALL1) == 16)
-    fft_tile16<LOG2N, SPC, FL>(J, grp, smem);
-  else
+__device__ __forceinline__ void fft_tile_any(const FftJob& J, uint32_t grp, uint32_t* smem,
+                                             const uint32_t* __restrict__ tw4096) {
+  if constexpr (fft_ppt(LOG2N, ALL1) == 16) {
+    fft_tile16<LOG2N, SPC, FL>(J, grp, smem, tw4096);
+  } else {
+    fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
     fft_tile<LOG2N, SPC, ALL1, FL>(J, grp, smem);
+  }
 }
 
 // Single CPI: one block per tile.
@@ -1281,12 +1296,11 @@ template <int LOG2N, int SPC, bool ALL1, bool FL>
 __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB)) void fft_kernel(
     FftJob J, const uint32_t* __restrict__ tw4096) {
   extern __shared__ uint32_t smem[];
-  fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
   // blocked bins: consecutive blocks are dealt to the 8 XCDs in turn, so
   // adjacent frames (which share the output's cache lines) go to one XCD
   const uint32_t nb = gridDim.x;
   const uint32_t grp = J.bins_fpc && nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
-  fft_tile_any<LOG2N, SPC, ALL1, FL>(J, grp, smem);
+  fft_tile_any<LOG2N, SPC, ALL1, FL>(J, grp, smem, tw4096);
 }
 
 // Batched: one block per frame group of a class of CPIs (same length and lane
@@ -1299,8 +1313,7 @@ __global__ __launch_bounds__((FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>
   const uint32_t t = blockIdx.x;
   if (t >= n_tiles) return;
   const FftJob& J = jobs[__builtin_amdgcn_readfirstlane(tile_job[t])];
-  fft_stage_twiddles<LOG2N, FftGeoOf<LOG2N, ALL1>::TPF * FftGeoOf<LOG2N, ALL1>::FPB>(tw4096, smem);
-  fft_tile_any<LOG2N, SPC, ALL1>(J, t - J.tile0, smem);
+  fft_tile_any<LOG2N, SPC, ALL1>(J, t - J.tile0, smem, tw4096);
 }
 
 template <int LOG2N, int SPC, bool ALL1, bool FL>
