@@ -442,7 +442,7 @@ __global__ __launch_bounds__(256, STEPS >= 3 ? 2 : 3) void bf_batch_kernel(const
 // ============================================================================
 // Beamformer over the FFT layout for CPIs whose channels fill 3 or 4 K steps
 // (48 or 64 channels), beams a multiple of 16 and columns a multiple of 128
-// (at 32 channels the tiled kernel below measured faster):
+// (at 32 channels the tiled bf_kernel measured faster):
 // a persistent block of 8 compute waves and one loader wave. The loader moves
 // each 128-column tile of X (one 512-byte row per channel) into an LDS ring of
 // two slots with global_load_lds_dwordx4 (one instruction per channel pair)
