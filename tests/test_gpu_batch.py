@@ -130,6 +130,21 @@ def test_batch_equals_single_cpi_launches(gpu):
         b.close()
 
 
+def test_batch_loader_fed_classes_cross_cpis(gpu, orc):
+    """Loader-fed batch classes where the persistent beamformer's blocks move
+    from one CPI's tiles to the next (two 48-channel CPIs of 512 tiles of 128
+    columns each, more tiles than resident blocks) and the 32-channel
+    covariance class: every stage of every CPI against the oracle."""
+    specs = [dict(CH=48, N=1024, T=16, F=64, B=16), dict(CH=48, N=1024, T=16, F=64, B=32),
+             dict(CH=32, N=256, T=16, F=20, B=32)]
+    chains, exp = _build(gpu, specs)
+    b = rd_amd.Batch.of_chains(gpu, chains)
+    assert b.plan().count("loader-fed") == 2, b.plan()
+    b.launch()
+    _check(orc, chains, exp, b.events())
+    b.close()
+
+
 def test_batch_capacity_reported_per_cpi(gpu, orc):
     sc = rd_scenario.config("tiny")
     chains = []

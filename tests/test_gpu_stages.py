@@ -307,12 +307,14 @@ def test_bf_unit_weights_passthrough(gpu):
     assert not fl.any()
 
 
-# (64, 64, 2, 4096), (48, 32, 3, 256), (64, 16, 1, 128) and (64, 16, 64, 4096)
-# take the loader-fed kernel (48 or 64 channels, beams a multiple of 16,
-# columns a multiple of 128): one tile, a few, and eight tiles per block
+# (64, 64, 2, 4096), (48, 32, 3, 256), (64, 16, 1, 128), (64, 16, 64, 4096)
+# and (64, 128, 2, 512) take the loader-fed kernel (48 or 64 channels, beams a
+# multiple of 16, columns a multiple of 128): one tile, a few, eight tiles per
+# block, and 128 beams (eight beam tiles held in LDS)
 @pytest.mark.parametrize("CH,B,F,N", [(16, 16, 5, 1024), (64, 64, 2, 4096), (4, 4, 16, 64), (7, 3, 3, 100),
                                       (33, 20, 2, 512), (48, 17, 1, 256), (32, 24, 3, 256), (16, 16, 2, 64),
-                                      (48, 32, 3, 256), (64, 16, 1, 128), (64, 16, 64, 4096), (32, 32, 4, 256)])
+                                      (48, 32, 3, 256), (64, 16, 1, 128), (64, 16, 64, 4096), (32, 32, 4, 256),
+                                      (64, 128, 2, 512)])
 def test_bf_stage_matches_align_then_bf(gpu, orc, CH, B, F, N):
     rng = np.random.default_rng(CH + B + F + N)
     for X, W in [(stim.rand_cx(rng, CH * F * N), stim.rand_cx(rng, B * CH)),
