@@ -4,8 +4,10 @@ The covariance's int8 digit GEMMs accumulate in int32; a frame adds at most
 2^15 to one digit accumulator, so the largest frame count (65535) is exact only
 if no accumulator sums two digit kinds. These tests drive every accumulator to
 its bound: all-(-32768) and all-(-32767) inputs over 32768 and 65535 frames, for
-each padded channel count (16, 32, 64: four, two and one frame split per block
-in the single-CPI launch), and through the batch with one frame split.
+each padded channel count (16 and 32 with an odd bin count: four and two frame
+splits per block in the single-CPI launch; 32 and 64 with the bins a multiple
+of 64 / CH: the loader-fed kernel, every frame in one block), and through the
+batch with one frame split.
 Semantics: covar::cross_power summed by an acc_w = 64 block-mode Integrator
 (model/cpp/covariance/covar_model.hpp:137-151, 186-262).
 """
@@ -24,10 +26,9 @@ def _eq(a, b, what):
     assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}"
 
 
-@pytest.mark.parametrize("CH", [16, 32, 64])
+@pytest.mark.parametrize("CH,N", [(16, 3), (32, 3), (32, 2), (64, 1)])
 @pytest.mark.parametrize("F", [32768, 65535])
-def test_cov_full_scale_long_frames(gpu, orc, CH, F):
-    N = 1 if CH == 64 else 3
+def test_cov_full_scale_long_frames(gpu, orc, CH, N, F):
     print(f"cov CH={CH} F={F} N={N}", flush=True)
     for val in (vecio.pack(-32768, -32768), vecio.pack(-32767, -32767), vecio.pack(-32768, 32767)):
         Xa = np.full(F * N * CH, val, np.uint32)
@@ -49,8 +50,9 @@ def test_batch_cov_full_scale_long_frames(gpu, orc, autotune):
     """CPIs of 65535 frames whose bin 0 stays near -32767 in every channel and
     frame (a constant full-scale input through identity taps), so every digit
     accumulator of the covariance runs to its bound; the batch runs one frame
-    split for 64 channels and the autotuner also tries one for 16."""
-    specs = [(64, 8, 65535), (16, 8, 65535)]
+    split for 64 and 32 channels (loader-fed) and the autotuner also tries one
+    for 16."""
+    specs = [(64, 8, 65535), (32, 8, 65535), (16, 8, 65535)]
     chains, exp = [], []
     for CH, N, F in specs:
         T, B = 16, 1
