@@ -73,8 +73,8 @@ CfarGeom cfar_geom(const rd_cfar_cfg& c, uint32_t max_guard, uint32_t max_ref) {
 // S entries keeps the strided segment writes of the prefix pass free of bank
 // conflicts, and puts a lane's S entries at fixed offsets from one base.
 // LDS per row (u64 words): P, then the per-wave statistics [W][4] = {prefix
-// total, events, detections, -}, then per wave a list of its emitting bins
-// (64 * S u32 entries).
+// total, events, detections, -}, then per wave a list of up to 64 of its
+// emitting bins (u32 entries; a wave with more emits chunk by chunk).
 // ----------------------------------------------------------------------------
 struct CfarRowGeo {
   uint32_t W, slog, words;
@@ -95,7 +95,7 @@ __host__ __device__ inline CfarRowGeo cfar_row_geo(uint32_t n, uint32_t W) {
   G.slog = 0;
   while ((1u << G.slog) < per) ++G.slog;
   const uint32_t pmax = n + (G.slog ? (n >> G.slog) : 0);
-  G.words = pmax + 1 + W * 4 + (W * (64u << G.slog) + 1) / 2;
+  G.words = pmax + 1 + W * 4 + W * 32;
   return G;
 }
 
@@ -412,11 +412,32 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
     }
   }
 
-  // second pass: the wave's emitting bins, compacted in bin order into its
-  // LDS list, are decided again 64 at a time and written at their offsets
+  // second pass: the wave's emitting bins are decided again and written at
+  // their offsets. At most 64 (a sparse row's usual case): compacted in bin
+  // order into the wave's LDS list, one pass with the lanes full; more: chunk
+  // by chunk, the emitting lanes of each chunk at once.
   const uint32_t frame = frame_of(r, J.frame_ids, J.frame_mod);
-  uint32_t* lst = reinterpret_cast<uint32_t*>(stats + W * 4) + wr * 64 * S;
-  {
+  auto emit = [&](uint32_t i, uint64_t pos) {
+    Decision d;
+    if (ca && !g.dense) {
+      // a CA event without the dense mode is a detection, so its window
+      // fits the row: the sums need no clamping
+      d.cut = P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)];
+      const uint64_t lead = g.rl ? P[pidx<SLOG>(i + g.gl + g.rl + 1)] - P[pidx<SLOG>(i + g.gl + 1)] : 0ull;
+      const uint64_t lag = g.rg ? P[pidx<SLOG>(i - g.gg)] - P[pidx<SLOG>(i - g.gg - g.rg)] : 0ull;
+      d.sum = lead + lag;
+      d.n = n_ref;
+      d.eval = d.hit = true;
+    } else {
+      d = decide<SLOG>(P, n, i, g);
+    }
+    const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
+    if (pos < room)
+      put_event(dst + 3 * pos, kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut, d.eval ? d.sum : 0ull,
+                pos == 0, false);
+  };
+  if (cnt <= 64) {
+    uint32_t* lst = reinterpret_cast<uint32_t*>(stats + W * 4) + wr * 64;
     uint32_t kq = 0;
 #pragma unroll
     for (uint32_t q = 0; q < S; ++q) {
@@ -424,30 +445,15 @@ __device__ __forceinline__ uint64_t cfar_row_run(const CfarJob& J, uint32_t W, u
       if ((em[q] >> lane) & 1ull) lst[kq + (uint32_t)__popcll(em[q] & below)] = c0 + q * 64 + lane;
       kq += (uint32_t)__popcll(em[q]);
     }
-  }
-  wave_lds_sync();
-  for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
-    const uint32_t e = e0 + lane;
-    if (e < cnt) {
-      const uint32_t i = lst[e];
-      Decision d;
-      if (ca && !g.dense) {
-        // a CA event without the dense mode is a detection, so its window
-        // fits the row: the sums need no clamping
-        d.cut = P[pidx<SLOG>(i + 1)] - P[pidx<SLOG>(i)];
-        const uint64_t lead = g.rl ? P[pidx<SLOG>(i + g.gl + g.rl + 1)] - P[pidx<SLOG>(i + g.gl + 1)] : 0ull;
-        const uint64_t lag = g.rg ? P[pidx<SLOG>(i - g.gg)] - P[pidx<SLOG>(i - g.gg - g.rg)] : 0ull;
-        d.sum = lead + lag;
-        d.n = n_ref;
-        d.eval = d.hit = true;
-      } else {
-        d = decide<SLOG>(P, n, i, g);
-      }
-      const uint64_t pos = k + e;
-      const uint32_t kind = d.hit ? 0u : (!d.eval ? 2u : 1u);
-      if (pos < room)
-        put_event(dst + 3 * pos, kind, i, frame, d.eval ? d.n : 0u, g.alpha, 0, 0, d.cut, d.eval ? d.sum : 0ull,
-                  pos == 0, false);
+    wave_lds_sync();
+    if (lane < cnt) emit(lst[lane], k + lane);
+  } else {
+    uint64_t kq = k;
+#pragma unroll
+    for (uint32_t q = 0; q < S; ++q) {
+      if (em[q] == 0) continue;
+      if ((em[q] >> lane) & 1ull) emit(c0 + q * 64 + lane, kq + (uint32_t)__popcll(em[q] & below));
+      kq += (uint32_t)__popcll(em[q]);
     }
   }
   k += cnt;

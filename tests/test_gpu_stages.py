@@ -353,6 +353,10 @@ CFAR_CFGS = [
     [1, 0, 0, 4, 4, 32, 32, 0x0800],
     [1, 0, 0, 0, 0, 0, 0, 0x0800],
     [1, 0, 1, 4, 4, 16, 16, 0x0800],
+    # alpha 1.0 in the events mode: about a third of the bins detect, so waves
+    # of 128 bins and more emit over 64 events (the chunk-by-chunk emission)
+    [1, 0, 0, 1, 1, 4, 4, 0x100],
+    [1, 1, 0, 1, 1, 4, 4, 0x100],
 ]
 
 
@@ -387,7 +391,7 @@ def test_cfar_many_rows_across_calls(gpu, orc):
         _eq(eg, eo, f"events rows={rows} n={n}")
 
 
-@pytest.mark.parametrize("cfg", [CFAR_CFGS[0], CFAR_CFGS[5], CFAR_CFGS[6], CFAR_CFGS[8]])
+@pytest.mark.parametrize("cfg", [CFAR_CFGS[0], CFAR_CFGS[5], CFAR_CFGS[6], CFAR_CFGS[8], CFAR_CFGS[9]])
 @pytest.mark.parametrize("n", [64, 1024, 4096, 100, 5000, 2])
 def test_cfar_u32_rows_either_side_of_2_32(gpu, orc, cfg, n):
     """u32 cells in rows whose total is below 2^32, exactly 2^32 - 1, exactly
